@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Throughput of the hyper-vlasov test-particle step loop on one MI355X.
+
+Workload (default): A=208 (Z=82, N=126), 5,000 test particles per nucleon
+(1,040,000 tp), 128^3 grid with dx = 0.25 fm, dt = 0.25 fm/c, dp = 15 MeV/c,
+SOR omega 1.8 / tol 1e-6 / 512 iterations.  A step is one pass of the loop
+(gather + kick + drift + deposit + smearing + Skyrme + SOR Coulomb solve +
+diagnostics) over all test particles with inputs resident in HBM.
+
+Prints ONE JSON line.  `value` is the mixed-precision form (FP32 particles and
+grid, FP64 Coulomb potential / SOR / reductions); the plain FP64 form is timed
+the same way and reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "hyper-vlasov_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+
+
+def sor_bytes_per_launch(w, prec: str) -> float:
+    """Algorithmic bytes of one red or black SOR sweep: an iteration reads phi
+    (8 B) and the proton density (8 B FP64 / 4 B FP32) and writes phi (8 B) once
+    per interior node; one launch is half an iteration."""
+    n = w.grid
+    interior = (n - 2) ** 3
+    ws = 8 if prec == "fp64" else 4
+    return interior * (8 + ws + 8) / 2.0
+
+
+def run_gpu(w, prec: str, steps: int, warmup: int):
+    import hvamd
+    p = hvamd.HV_PREC_F64 if prec == "fp64" else hvamd.HV_PREC_MIXED
+    x, y, z, kx, ky, kz = w.initial_state()
+    sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, p)
+    sim.upload_particles(x, y, z, kx, ky, kz)
+    prm = w.params(warmup + steps)
+    sk = w.sky()
+    sim.run_init(sk, prm)
+    if warmup:
+        sim.run_steps(sk, prm, 0, warmup)
+    sim.sync()
+    sim.profile_enable(True)
+    t0 = time.perf_counter()
+    stats, its = sim.run_steps(sk, prm, warmup, steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    prof = sim.profile_read()
+    sim.profile_enable(False)
+    state = None
+    if prec == "fp64":
+        state = sim.download_particles()
+        state["phi"] = sim.download_field(hvamd.HV_FIELD_COULOMB)
+    sim.close()
+    return dict(wall=wall, its=its, prof=prof, stats=stats, state=state)
+
+
+def cpu_baseline(w, state, budget_s: float):
+    """The CPU restatement (oracle/liboracle.so, OpenMP) from the same warm
+    mid-run state, on this host's cores; a bounded number of steps."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import OParams, OSkyrme, Oracle, oworld, p
+    o = Oracle()
+    L = o.L
+    cores = len(os.sched_getaffinity(0))
+    n, d = w.world().n, w.world().d_max
+    prm = w.params()
+    sk = w.sky()
+    world = oworld(n, d)
+    s = OSkyrme(sk.A, sk.B, sk.C, sk.gamma)
+
+    def run(nsteps):
+        st = {k: state[k].copy() for k in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz")}
+        phi = state["phi"].copy()
+        op = OParams(prm.dt, nsteps, 1e30, 0.0, prm.boost_mode, w.z, w.n, prm.sigma_r, prm.part_per_nucleon,
+                     prm.sor_omega, prm.sor_tol, prm.sor_max_it)
+        stats = np.zeros(4 * nsteps)
+        its = np.zeros(nsteps + 1, dtype=np.int32)
+        t0 = time.perf_counter()
+        L.hvo_simulate_loop(*[p(st[k]) for k in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz")],
+                            w.n_protons, w.n_neutrons, ctypes.byref(world), ctypes.byref(s), ctypes.byref(op),
+                            p(phi), p(stats), p(its))
+        return time.perf_counter() - t0, its
+
+    t1, _ = run(1)
+    nsteps = int(max(2, min(40, budget_s / max(t1, 1e-3))))
+    t, its = run(nsteps)
+    return {
+        "value": w.total * nsteps / t,
+        "unit": "particle-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"{nsteps} steps of the {w.name} loop from the GPU's warm mid-run state "
+                   f"(mean {its[1:].mean():.0f} SOR iterations/step), oracle/liboracle.so, "
+                   f"OpenMP {cores} threads, -O3 -march=x86-64-v3"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--fp64", type=int, default=1, help="also time the plain FP64 form")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the single-GPU number is the "
+              "figure of record for this loop", file=sys.stderr)
+        sys.exit(2)
+
+    from hvamd import configs
+    w = configs.WORKLOADS[args.workload]()
+    mixed = run_gpu(w, "mixed", args.steps, args.warmup)
+    fp64 = run_gpu(w, "fp64", args.steps, args.warmup) if args.fp64 else None
+
+    value = w.total * args.steps / mixed["wall"]
+    sor_ms, sor_launches = mixed["prof"]["sor_sweep"]
+    launch_ms = sor_ms / max(sor_launches, 1)
+    achieved = sor_bytes_per_launch(w, "mixed") / (launch_ms * 1e-3) / 1e9
+    out = {
+        "metric": "particle-steps/s at 1.04M tp, 128^3 (mixed FP32/FP64 form; plain FP64 beside it)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded uniform-sphere nucleus)",
+        "config": {"workload": w.name, "test_particles": w.total, "grid": w.grid, "dx_fm": w.dx,
+                   "dt_fm_c": w.dt, "sor": [w.omega, w.tol, w.max_it], "parallelism": "single GPU"},
+        "sor_iters_per_step": float(np.mean(mixed["its"])),
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in mixed["prof"].items()},
+        "roofline": {"bound": "hbm", "kernel": "k_sor_colour (one red/black SOR sweep)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": 1e3 * launch_ms, "bytes_per_launch": sor_bytes_per_launch(w, "mixed")},
+    }
+    if fp64 is not None:
+        v64 = w.total * args.steps / fp64["wall"]
+        s_ms, s_n = fp64["prof"]["sor_sweep"]
+        l_ms = s_ms / max(s_n, 1)
+        out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["wall"] / args.steps,
+                       "sor_iters_per_step": float(np.mean(fp64["its"])),
+                       "kernel_ms_per_step": {k: v[0] / args.steps for k, v in fp64["prof"].items()},
+                       "sor_achieved_GBs": sor_bytes_per_launch(w, "fp64") / (l_ms * 1e-3) / 1e9,
+                       "sor_avg_launch_us": 1e3 * l_ms}
+        out["mixed_over_fp64"] = value / v64
+        if args.cpu_budget > 0:
+            out["cpu_baseline"] = cpu_baseline(w, fp64["state"], args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,940 @@
+// hv_abi.hip -- context, buffers and the extern "C" entry points of
+// libhvamd.so (declared in include/hv_abi.h).
+//
+// The host side owns one HIP stream per context and every device buffer of
+// the loop.  It launches the kernels of hv_kernels.h in the order of the
+// reference's cpu_simulate (src/hyper_vlasov.cpp:53-104) and drives the SOR
+// iteration count exactly as the reference's stop rule does
+// (include/physics.hpp:104, 152-153).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <cstdarg>
+#include <new>
+#include <vector>
+
+#include "hv_abi.h"
+#include "hv_kernels.h"
+
+#define HV_XSTR(x) HV_STR(x)
+#define HV_STR(x) #x
+#define HIP_VERSION_STR HV_XSTR(HIP_VERSION_MAJOR) "." HV_XSTR(HIP_VERSION_MINOR) "." HV_XSTR(HIP_VERSION_PATCH)
+
+using namespace hv;
+
+struct hv_ctx {
+    int precision = HV_PREC_F64;
+    hv_world world{};
+    int np = 0, nn = 0, total = 0;
+    long long G = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    char err[512] = {0};
+
+    // particle SoA (T = double or float): x y z kx ky kz fx fy fz
+    void *part[9] = {};
+    unsigned long long *acc = nullptr;  // 2G fixed-point CIC counts
+    void *density = nullptr;            // 2G T
+    void *U = nullptr;                  // 2G T
+    double *phi = nullptr;              // G
+    void *H = nullptr, *D = nullptr;    // smearing factor channels
+    size_t H_cap = 0, D_cap = 0;        // bytes
+    void *F[3] = {};                    // 2G T each (per-kernel API only)
+    void *smear_tab = nullptr;          // SmearTab<T> on device
+    double tab_sigma = -1.0;
+    int tab_ppn = -1;
+    int tab_warn = 0;
+    unsigned long long *maxd = nullptr; // per-iteration max |dphi| (double bits)
+    int maxd_cap = 0;
+    SorCtl *ctl = nullptr;
+    StatSlot *slots = nullptr;
+    int slot_cap = 0;
+    unsigned *status = nullptr;
+    double *pinned = nullptr;           // host staging for small read-backs
+    size_t pinned_cap = 0;              // doubles
+    int sor_pred = 16;
+    int excited = 0;
+    int last_outside = 0;
+
+    // kernel-class profiling
+    bool prof = false;
+    double prof_ms[HV_KCLASS_COUNT] = {};
+    long long prof_launches[HV_KCLASS_COUNT] = {};
+    struct Pending {
+        int cls;
+        hipEvent_t a, b;
+        long long launches;
+    };
+    std::vector<hipEvent_t> ev_free;
+    std::vector<Pending> pending;
+};
+
+namespace {
+
+constexpr int kThreads = 256;
+
+int set_err(hv_ctx *c, int code, const char *fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HV_HIP(c, expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_err((c), HV_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                                \
+    } while (0)
+
+#define HV_CHECK_LAUNCH(c) HV_HIP(c, hipGetLastError())
+
+inline unsigned blocks_for(long long n, int t = kThreads) {
+    return (unsigned)((n + t - 1) / t);
+}
+
+template <typename T>
+Geom<T> make_geom(const hv_ctx *c) {
+    Geom<T> g;
+    g.nx = c->world.n[0];
+    g.ny = c->world.n[1];
+    g.nz = c->world.n[2];
+    g.G = c->G;
+    g.dmx = (T)c->world.d_max[0];
+    g.dmy = (T)c->world.d_max[1];
+    g.dmz = (T)c->world.d_max[2];
+    g.dx = T(2.0) * g.dmx / g.nx;
+    g.dy = T(2.0) * g.dmy / g.ny;
+    g.dz = T(2.0) * g.dmz / g.nz;
+    g.two_dx = T(2.0) * g.dx;
+    g.two_dy = T(2.0) * g.dy;
+    g.two_dz = T(2.0) * g.dz;
+    return g;
+}
+
+template <typename T>
+Particles<T> particles(const hv_ctx *c) {
+    Particles<T> p;
+    p.x = (T *)c->part[0]; p.y = (T *)c->part[1]; p.z = (T *)c->part[2];
+    p.kx = (T *)c->part[3]; p.ky = (T *)c->part[4]; p.kz = (T *)c->part[5];
+    p.fx = (T *)c->part[6]; p.fy = (T *)c->part[7]; p.fz = (T *)c->part[8];
+    return p;
+}
+
+size_t tsize(const hv_ctx *c) { return c->precision == HV_PREC_F64 ? sizeof(double) : sizeof(float); }
+
+template <typename V>
+int fill(hv_ctx *c, V *p, V v, long long n) {
+    if (n <= 0) return HV_OK;
+    unsigned nb = blocks_for(n);
+    if (nb > 4096) nb = 4096;
+    k_fill<V><<<nb, kThreads, 0, c->stream>>>(p, v, n);
+    HV_CHECK_LAUNCH(c);
+    return HV_OK;
+}
+
+int ensure_pinned(hv_ctx *c, size_t doubles) {
+    if (c->pinned_cap >= doubles) return HV_OK;
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    c->pinned_cap = 0;
+    HV_HIP(c, hipHostMalloc((void **)&c->pinned, doubles * sizeof(double), hipHostMallocDefault));
+    c->pinned_cap = doubles;
+    return HV_OK;
+}
+
+int ensure_buf(hv_ctx *c, void **p, size_t *cap, size_t bytes) {
+    if (*cap >= bytes) return HV_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return set_err(c, HV_ERR_ALLOC, "device allocation of %zu bytes failed", bytes);
+    }
+    *cap = bytes;
+    return HV_OK;
+}
+
+// ---- kernel-class profiling ------------------------------------------------------
+hipEvent_t prof_event(hv_ctx *c) {
+    hipEvent_t e = nullptr;
+    if (!c->ev_free.empty()) {
+        e = c->ev_free.back();
+        c->ev_free.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+// record the start event of a bracket (nullptr when profiling is off)
+hipEvent_t prof_begin(hv_ctx *c) {
+    if (!c->prof) return nullptr;
+    hipEvent_t a = prof_event(c);
+    if (a) (void)hipEventRecord(a, c->stream);
+    return a;
+}
+
+void prof_end(hv_ctx *c, int cls, hipEvent_t a, long long launches) {
+    if (!c->prof || !a) return;
+    hipEvent_t b = prof_event(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({cls, a, b, launches});
+}
+
+// fold completed brackets into the counters (call at a host sync point)
+void prof_flush(hv_ctx *c) {
+    for (auto &p : c->pending) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            c->prof_ms[p.cls] += ms;
+            c->prof_launches[p.cls] += p.launches;
+        }
+        c->ev_free.push_back(p.a);
+        c->ev_free.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+// ---- smearing ----------------------------------------------------------------
+template <typename T>
+int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
+    if (!(sigma_r > 0.0) || ppn <= 0) return set_err(c, HV_ERR_ARG, "smearing needs sigma_r > 0 and ppn > 0");
+    if (c->smear_tab && c->tab_sigma == sigma_r && c->tab_ppn == ppn) return HV_OK;
+    hv_smear_table tb;
+    if (hv_smear_table_build(&tb, c->world.n, c->world.d_max, sigma_r, ppn) != 0)
+        return set_err(c, HV_ERR_ARG, "smearing radius exceeds %d nodes", HV_SMEAR_MAXR);
+    SmearTab<T> h;
+    memset(&h, 0, sizeof h);
+    h.rx = tb.rx; h.ry = tb.ry; h.rz = tb.rz;
+    for (int a = 0; a <= HV_SMEAR_MAXR; a++) {
+        h.kb[a] = tb.kb[a];
+        for (int b = 0; b <= HV_SMEAR_MAXR; b++) h.kc[a][b] = tb.kc[a][b];
+        h.gx[a] = (T)tb.gx[a];
+        h.gy[a] = (T)tb.gy[a];
+        h.gz[a] = (T)tb.gz[a];
+    }
+    h.term = (T)tb.term;
+    if (!c->smear_tab) HV_HIP(c, hipMalloc(&c->smear_tab, sizeof(SmearTab<T>)));
+    HV_HIP(c, hipMemcpyAsync(c->smear_tab, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    size_t n2 = (size_t)(2 * c->G);
+    int rc = ensure_buf(c, &c->H, &c->H_cap, (size_t)(tb.rz + 1) * n2 * sizeof(T));
+    if (rc) return rc;
+    rc = ensure_buf(c, &c->D, &c->D_cap, (size_t)(tb.rx + 1) * n2 * sizeof(T));
+    if (rc) return rc;
+    c->tab_sigma = sigma_r;
+    c->tab_ppn = ppn;
+    c->tab_warn = (tb.min_margin < 1e-9) ? HV_WARN_SMEAR_MARGIN : 0;
+    return HV_OK;
+}
+
+// counts (fixed point in c->acc when from_acc, else T counts in c->density) ->
+// smeared density in c->density
+template <typename T>
+int smear(hv_ctx *c, bool from_acc, double sigma_r, int ppn) {
+    int rc = ensure_smear<T>(c, sigma_r, ppn);
+    if (rc) return rc;
+    const long long n2 = 2 * c->G;
+    const SmearTab<T> *tb = (const SmearTab<T> *)c->smear_tab;
+    T *H = (T *)c->H, *D = (T *)c->D, *rho = (T *)c->density;
+    hipEvent_t pa = prof_begin(c);
+    if (from_acc)
+        k_smear_z<T, unsigned long long><<<blocks_for(n2), kThreads, 0, c->stream>>>(c->acc, H, tb, c->world.n[2], n2);
+    else
+        k_smear_z<T, T><<<blocks_for(n2), kThreads, 0, c->stream>>>(rho, H, tb, c->world.n[2], n2);
+    HV_CHECK_LAUNCH(c);
+    k_smear_y<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(H, D, tb, c->world.n[1], c->world.n[2], n2);
+    HV_CHECK_LAUNCH(c);
+    k_smear_x<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, tb, c->world.n[0], c->world.n[1],
+                                                              c->world.n[2], n2);
+    HV_CHECK_LAUNCH(c);
+    prof_end(c, HV_KCLASS_SMEAR, pa, 1);
+    return c->tab_warn;
+}
+
+// ---- SOR -------------------------------------------------------------------
+int ensure_maxd(hv_ctx *c, int max_it) {
+    if (c->maxd_cap >= max_it) return HV_OK;
+    if (c->maxd) (void)hipFree(c->maxd);
+    c->maxd = nullptr;
+    c->maxd_cap = 0;
+    HV_HIP(c, hipMalloc((void **)&c->maxd, sizeof(unsigned long long) * (size_t)max_it));
+    c->maxd_cap = max_it;
+    return HV_OK;
+}
+
+// Red-black SOR on c->phi with the proton density (first G values of `rho`).
+// The host launches iterations in chunks and reads back each chunk's
+// per-iteration max |dphi|; kernels launched past the converged iteration
+// return immediately (SorCtl), so the result is exactly the reference's
+// iterate: stop after the first iteration whose max |dphi| < tol.
+template <typename RT>
+int sor_solve(hv_ctx *c, const RT *rho, double omega, double tol, int max_it, int *iters_out,
+              double *maxdiff_out) {
+    if (max_it <= 0) return set_err(c, HV_ERR_ARG, "sor max_it must be positive");
+    const int nx = c->world.n[0], ny = c->world.n[1], nz = c->world.n[2];
+    int rc = ensure_maxd(c, max_it);
+    if (rc) return rc;
+    rc = ensure_pinned(c, (size_t)max_it + 8);
+    if (rc) return rc;
+    SorCoef cf;
+    double dx = 2.0 * c->world.d_max[0] / nx, dy = 2.0 * c->world.d_max[1] / ny, dz = 2.0 * c->world.d_max[2] / nz;
+    cf.dx2 = dx * dx;
+    cf.dy2 = dy * dy;
+    cf.dz2 = dz * dz;
+    cf.inv_dx2 = 1.0 / (2.0 / (dx * dx) + 2.0 / (dy * dy) + 2.0 / (dz * dz));
+    cf.src = 4.0 * HV_PI * HV_E2;
+    cf.omega = omega;
+    cf.one_minus_omega = 1.0 - omega;
+    cf.tol = tol;
+    rc = fill<unsigned long long>(c, c->maxd, 0ULL, max_it);
+    if (rc) return rc;
+    rc = fill<int>(c, (int *)c->ctl, 0, 2);
+    if (rc) return rc;
+    dim3 blk(64, 4, 1);
+    dim3 grd((unsigned)((nz - 2 + 63) / 64), (unsigned)((ny - 2 + 3) / 4), (unsigned)(nx - 2));
+    int it = 0, iters = -1, start = 0;
+    int chunk = c->sor_pred > 2 ? c->sor_pred - 1 : 1;
+    while (it < max_it) {
+        start = it;
+        int end = it + chunk < max_it ? it + chunk : max_it;
+        hipEvent_t pa = prof_begin(c), pb = nullptr;
+        for (; it < end; it++) {
+            k_sor_colour<RT><<<grd, blk, 0, c->stream>>>(c->phi, rho, nx, ny, nz, cf, 0, it, c->maxd, c->ctl);
+            k_sor_colour<RT><<<grd, blk, 0, c->stream>>>(c->phi, rho, nx, ny, nz, cf, 1, it, c->maxd, c->ctl);
+        }
+        HV_CHECK_LAUNCH(c);
+        if (pa) {
+            pb = prof_event(c);
+            if (pb) (void)hipEventRecord(pb, c->stream);
+        }
+        HV_HIP(c, hipMemcpyAsync(c->pinned, c->maxd + start, sizeof(double) * (size_t)(end - start),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HV_HIP(c, hipStreamSynchronize(c->stream));
+        for (int m = 0; m < end - start; m++) {
+            if (c->pinned[m] < tol) {
+                iters = start + m + 1;
+                break;
+            }
+        }
+        if (pa && pb) {
+            int real = (iters > 0 ? iters : end) - start;
+            c->pending.push_back({HV_KCLASS_SOR, pa, pb, 2LL * real});
+        }
+        prof_flush(c);
+        if (iters > 0) break;
+        chunk = 4;
+    }
+    if (iters < 0) iters = max_it;
+    // max |dphi| of the last iteration done (it lies in the last chunk read back)
+    const double md = c->pinned[iters - 1 - start];
+    c->sor_pred = iters;
+    if (iters_out) *iters_out = iters;
+    if (maxdiff_out) *maxdiff_out = md;
+    return (md > tol) ? HV_WARN_SOR_NOT_CONVERGED : HV_OK;
+}
+
+// ---- diagnostics -------------------------------------------------------------
+int ensure_slots(hv_ctx *c, int n) {
+    if (c->slot_cap >= n) return HV_OK;
+    if (c->slots) (void)hipFree(c->slots);
+    c->slots = nullptr;
+    c->slot_cap = 0;
+    HV_HIP(c, hipMalloc((void **)&c->slots, sizeof(StatSlot) * (size_t)n));
+    c->slot_cap = n;
+    return HV_OK;
+}
+
+double slot_mean(const StatSlot &s, int c, long long count) {
+    __int128 v = (__int128)(((unsigned __int128)s.s[c][1] << 64) | (unsigned __int128)s.s[c][0]);
+    return hv_stat_mean(v, count);
+}
+
+void slot_stats(const StatSlot &s, int np, int nn, double out[4]) {
+    out[0] = slot_mean(s, 0, np);
+    out[1] = slot_mean(s, 1, nn);
+    out[2] = slot_mean(s, 2, (long long)s.inbox[0]);
+    out[3] = slot_mean(s, 3, (long long)s.inbox[1]);
+}
+
+template <typename T>
+int stats_now(hv_ctx *c, double out[4]) {
+    int rc = ensure_slots(c, 1);
+    if (rc) return rc;
+    rc = fill<unsigned long long>(c, (unsigned long long *)c->slots, 0ULL, sizeof(StatSlot) / 8);
+    if (rc) return rc;
+    k_deposit<T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total, make_geom<T>(c),
+                                                                  nullptr, c->slots, c->status);
+    HV_CHECK_LAUNCH(c);
+    StatSlot s;
+    HV_HIP(c, hipMemcpyAsync(&s, c->slots, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    slot_stats(s, c->np, c->nn, out);
+    return HV_OK;
+}
+
+// ---- status word -------------------------------------------------------------
+int take_status(hv_ctx *c, int *bits) {
+    unsigned s = 0;
+    HV_HIP(c, hipMemcpyAsync(&s, c->status, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    unsigned zero = 0;
+    HV_HIP(c, hipMemcpyAsync(c->status, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    *bits = (int)s;
+    return HV_OK;
+}
+
+// ---- the fused loop ------------------------------------------------------------
+template <typename T>
+int potential_from_density(hv_ctx *c, const hv_skyrme *s, bool with_phi) {
+    hipEvent_t pa = prof_begin(c);
+    k_skyrme<T><<<blocks_for(c->G), kThreads, 0, c->stream>>>((const T *)c->density, (T *)c->U,
+                                                             with_phi ? c->phi : nullptr, s->A, s->B, s->C,
+                                                             s->gamma, c->G);
+    HV_CHECK_LAUNCH(c);
+    prof_end(c, HV_KCLASS_POTENTIAL, pa, 1);
+    return HV_OK;
+}
+
+// deposit already in c->acc -> rho -> SOR -> U (src/hyper_vlasov.cpp:94-98)
+template <typename T>
+int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, int *warn) {
+    int rc = smear<T>(c, true, p->sigma_r, p->part_per_nucleon);
+    if (rc < 0) return rc;
+    *warn |= rc;
+    rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
+    if (rc) return rc;
+    double md;
+    rc = sor_solve<T>(c, (const T *)c->density, p->sor_omega, p->sor_tol, p->sor_max_it, iters, &md);
+    if (rc < 0) return rc;
+    *warn |= rc;
+    return potential_from_density<T>(c, s, true);
+}
+
+int check_params(hv_ctx *c, const hv_skyrme *s, const hv_params *p) {
+    if (!s || !p) return set_err(c, HV_ERR_ARG, "null skyrme or params");
+    if (p->steps <= 0 || !(p->t_f > 0.0)) return set_err(c, HV_ERR_ARG, "need steps > 0 and t_f > 0");
+    if (p->sor_max_it <= 0 || !(p->sor_tol > 0.0)) return set_err(c, HV_ERR_ARG, "need sor_max_it > 0 and sor_tol > 0");
+    if (!(p->sigma_r > 0.0) || p->part_per_nucleon <= 0) return set_err(c, HV_ERR_ARG, "need sigma_r > 0 and ppn > 0");
+    return HV_OK;
+}
+
+template <typename T>
+int run_init(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *sor_iters) {
+    int rc = check_params(c, s, p);
+    if (rc) return rc;
+    // phi: zero interior, Dirichlet faces (src/hyper_vlasov.cpp:48,56)
+    std::vector<double> h((size_t)c->G, 0.0);
+    hv_coulomb_faces(h.data(), c->world.n, c->world.d_max, p->z);
+    HV_HIP(c, hipMemcpyAsync(c->phi, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
+    if (rc) return rc;
+    k_deposit<T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total, make_geom<T>(c),
+                                                                  c->acc, nullptr, c->status);
+    HV_CHECK_LAUNCH(c);
+    int warn = 0, it = 0;
+    rc = field_solve<T>(c, s, p, &it, &warn);
+    if (rc < 0) return rc;
+    if (sor_iters) *sor_iters = it;
+    k_gather_u<T, T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total,
+                                                                      make_geom<T>(c), (const T *)c->U, T(0), 0);
+    HV_CHECK_LAUNCH(c);
+    c->excited = 0;
+    int st = 0;
+    rc = take_status(c, &st);
+    if (rc) return rc;
+    return warn | (st & HV_WARN_PARTICLE_OUTSIDE);
+}
+
+template <typename T>
+int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int nsteps, double *stats,
+              int *sor_iters) {
+    int rc = check_params(c, s, p);
+    if (rc) return rc;
+    if (nsteps <= 0) return HV_OK;
+    rc = ensure_slots(c, nsteps + 1);
+    if (rc) return rc;
+    rc = fill<unsigned long long>(c, (unsigned long long *)c->slots, 0ULL,
+                                  (long long)(sizeof(StatSlot) / 8) * (nsteps + 1));
+    if (rc) return rc;
+    const Geom<T> g = make_geom<T>(c);
+    const Particles<T> pp = particles<T>(c);
+    const double dt = p->t_f / p->steps;
+    const T kick_fac = (T)(dt / (2.0 * HV_HBARC));
+    const T drift_fac = (T)((dt * HV_HBARC) / HV_MC2);
+    double bp, bn;
+    if (p->boost_mode == HV_BOOST_SYMMETRIC) {
+        bp = p->eta_exc;
+        bn = p->eta_exc;
+    } else {
+        double zz = p->z, nnuc = p->n;
+        bp = p->eta_exc * nnuc / (zz + nnuc);
+        bn = p->eta_exc * zz / (zz + nnuc);
+    }
+    // diagnostics of the positions at the start of the first step
+    k_deposit<T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(pp, c->np, c->total, g, nullptr, c->slots,
+                                                                  c->status);
+    HV_CHECK_LAUNCH(c);
+    int warn = 0;
+    for (int n = 0; n < nsteps; n++) {
+        const int step = first + n;
+        int boost = 0;
+        if (step * dt >= p->t_exc && !c->excited) {
+            c->excited = 1;
+            boost = 1;
+        }
+        hipEvent_t pa = prof_begin(c);
+        k_particle_step<T, T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(
+            pp, c->np, c->total, g, (const T *)c->U, c->acc, kick_fac, drift_fac, n > 0 ? 1 : 0, boost, (T)bp,
+            (T)bn, c->slots + (n + 1), c->status);
+        HV_CHECK_LAUNCH(c);
+        prof_end(c, HV_KCLASS_PARTICLE, pa, 1);
+        int it = 0;
+        rc = field_solve<T>(c, s, p, &it, &warn);
+        if (rc < 0) return rc;
+        if (sor_iters) sor_iters[n] = it;
+    }
+    // end-of-step half kick of the last step, with the force stored (:101-103)
+    k_gather_u<T, T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(pp, c->np, c->total, g, (const T *)c->U,
+                                                                      kick_fac, 1);
+    HV_CHECK_LAUNCH(c);
+    if (stats) {
+        std::vector<StatSlot> h((size_t)nsteps + 1);
+        HV_HIP(c, hipMemcpyAsync(h.data(), c->slots, sizeof(StatSlot) * h.size(), hipMemcpyDeviceToHost, c->stream));
+        HV_HIP(c, hipStreamSynchronize(c->stream));
+        for (int n = 0; n < nsteps; n++) slot_stats(h[n], c->np, c->nn, stats + 4 * n);
+    }
+    int st = 0;
+    rc = take_status(c, &st);
+    if (rc) return rc;
+    prof_flush(c);
+    return warn | (st & HV_WARN_PARTICLE_OUTSIDE);
+}
+
+// dispatch on precision
+#define HV_DISPATCH(c, call_f64, call_mixed) ((c)->precision == HV_PREC_F64 ? (call_f64) : (call_mixed))
+
+int field_ptr(hv_ctx *c, int which, void **p, long long *n, bool *is_f64) {
+    *is_f64 = (c->precision == HV_PREC_F64);
+    switch (which) {
+    case HV_FIELD_DENSITY: *p = c->density; *n = 2 * c->G; return HV_OK;
+    case HV_FIELD_POTENTIALS: *p = c->U; *n = 2 * c->G; return HV_OK;
+    case HV_FIELD_COULOMB: *p = c->phi; *n = c->G; *is_f64 = true; return HV_OK;
+    case HV_FIELD_FORCES_X:
+    case HV_FIELD_FORCES_Y:
+    case HV_FIELD_FORCES_Z: {
+        int a = which - HV_FIELD_FORCES_X;
+        if (!c->F[a]) {
+            for (int q = 0; q < 3; q++) {
+                if (c->F[q]) continue;
+                if (hipMalloc(&c->F[q], tsize(c) * (size_t)(2 * c->G)) != hipSuccess)
+                    return set_err(c, HV_ERR_ALLOC, "force field allocation failed");
+                HV_HIP(c, hipMemsetAsync(c->F[q], 0, tsize(c) * (size_t)(2 * c->G), c->stream));
+            }
+        }
+        *p = c->F[a];
+        *n = 2 * c->G;
+        return HV_OK;
+    }
+    default: return set_err(c, HV_ERR_ARG, "unknown field %d", which);
+    }
+}
+
+int copy_in(hv_ctx *c, void *dst, const double *src, long long n, bool f64) {
+    if (f64) {
+        HV_HIP(c, hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        HV_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp((size_t)n);
+        for (long long i = 0; i < n; i++) tmp[(size_t)i] = (float)src[i];
+        HV_HIP(c, hipMemcpyAsync(dst, tmp.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        HV_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return HV_OK;
+}
+
+int copy_out(hv_ctx *c, double *dst, const void *src, long long n, bool f64) {
+    if (f64) {
+        HV_HIP(c, hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        HV_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp((size_t)n);
+        HV_HIP(c, hipMemcpyAsync(tmp.data(), src, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        HV_HIP(c, hipStreamSynchronize(c->stream));
+        for (long long i = 0; i < n; i++) dst[i] = (double)tmp[(size_t)i];
+    }
+    return HV_OK;
+}
+
+}  // namespace
+
+// =============================================================================
+extern "C" {
+
+int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int precision, hv_ctx **out) {
+    if (!out) return HV_ERR_ARG;
+    *out = nullptr;
+    if (!world) return HV_ERR_ARG;
+    if (precision != HV_PREC_F64 && precision != HV_PREC_MIXED) return HV_ERR_ARG;
+    for (int a = 0; a < 3; a++)
+        if (world->n[a] < 4 || !(world->d_max[a] > 0.0)) return HV_ERR_ARG;
+    if (n_protons < 0 || n_neutrons < 0 || n_protons > HV_MAX_SPECIES_PARTICLES ||
+        n_neutrons > HV_MAX_SPECIES_PARTICLES || n_protons + n_neutrons == 0)
+        return HV_ERR_ARG;
+    hv_ctx *c = new (std::nothrow) hv_ctx();
+    if (!c) return HV_ERR_ALLOC;
+    c->precision = precision;
+    c->world = *world;
+    c->np = n_protons;
+    c->nn = n_neutrons;
+    c->total = n_protons + n_neutrons;
+    c->G = (long long)world->n[0] * world->n[1] * world->n[2];
+    const size_t ts = tsize(c);
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+    for (int q = 0; q < 9 && ok; q++) ok = hipMalloc(&c->part[q], ts * (size_t)c->total) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->acc, sizeof(unsigned long long) * (size_t)(2 * c->G)) == hipSuccess;
+    ok = ok && hipMalloc(&c->density, ts * (size_t)(2 * c->G)) == hipSuccess;
+    ok = ok && hipMalloc(&c->U, ts * (size_t)(2 * c->G)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->phi, sizeof(double) * (size_t)c->G) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->ctl, sizeof(SorCtl)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->status, sizeof(unsigned)) == hipSuccess;
+    if (!ok) {
+        hv_ctx_destroy(c);
+        return HV_ERR_ALLOC;
+    }
+    // zero state like the reference's allocators (include/init.hpp:97-99, 111, 124-128)
+    for (int q = 0; q < 9 && ok; q++) ok = hipMemsetAsync(c->part[q], 0, ts * (size_t)c->total, c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->acc, 0, sizeof(unsigned long long) * (size_t)(2 * c->G), c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->density, 0, ts * (size_t)(2 * c->G), c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->U, 0, ts * (size_t)(2 * c->G), c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->phi, 0, sizeof(double) * (size_t)c->G, c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->ctl, 0, sizeof(SorCtl), c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->status, 0, sizeof(unsigned), c->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!ok) {
+        hv_ctx_destroy(c);
+        return HV_ERR_HIP;
+    }
+    *out = c;
+    return HV_OK;
+}
+
+int hv_ctx_destroy(hv_ctx *c) {
+    if (!c) return HV_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int q = 0; q < 9; q++)
+        if (c->part[q]) (void)hipFree(c->part[q]);
+    void *bufs[] = {c->acc, c->density, c->U, c->phi, c->H, c->D, c->F[0], c->F[1], c->F[2],
+                    c->smear_tab, c->maxd, c->ctl, c->slots, c->status};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    prof_flush(c);
+    for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HV_OK;
+}
+
+const char *hv_last_error(const hv_ctx *c) { return c ? c->err : "null context"; }
+
+int hv_sync(hv_ctx *c) {
+    if (!c) return HV_ERR_ARG;
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    return HV_OK;
+}
+
+void *hv_stream(hv_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int hv_last_outside(hv_ctx *c, int *count) {
+    if (!c || !count) return HV_ERR_ARG;
+    *count = c->last_outside;
+    return HV_OK;
+}
+
+int hv_upload_particles(hv_ctx *c, const double *x, const double *y, const double *z, const double *kx,
+                        const double *ky, const double *kz, const double *fx, const double *fy,
+                        const double *fz) {
+    if (!c) return HV_ERR_ARG;
+    const double *src[9] = {x, y, z, kx, ky, kz, fx, fy, fz};
+    for (int q = 0; q < 9; q++) {
+        if (!src[q]) continue;
+        int rc = copy_in(c, c->part[q], src[q], c->total, c->precision == HV_PREC_F64);
+        if (rc) return rc;
+    }
+    return HV_OK;
+}
+
+int hv_download_particles(hv_ctx *c, double *x, double *y, double *z, double *kx, double *ky, double *kz,
+                          double *fx, double *fy, double *fz) {
+    if (!c) return HV_ERR_ARG;
+    double *dst[9] = {x, y, z, kx, ky, kz, fx, fy, fz};
+    for (int q = 0; q < 9; q++) {
+        if (!dst[q]) continue;
+        int rc = copy_out(c, dst[q], c->part[q], c->total, c->precision == HV_PREC_F64);
+        if (rc) return rc;
+    }
+    return HV_OK;
+}
+
+int hv_upload_field(hv_ctx *c, int which, const double *host) {
+    if (!c || !host) return HV_ERR_ARG;
+    void *p;
+    long long n;
+    bool f64;
+    int rc = field_ptr(c, which, &p, &n, &f64);
+    if (rc) return rc;
+    return copy_in(c, p, host, n, f64);
+}
+
+int hv_download_field(hv_ctx *c, int which, double *host) {
+    if (!c || !host) return HV_ERR_ARG;
+    void *p;
+    long long n;
+    bool f64;
+    int rc = field_ptr(c, which, &p, &n, &f64);
+    if (rc) return rc;
+    return copy_out(c, host, p, n, f64);
+}
+
+// ---- per-kernel entry points --------------------------------------------------
+int hv_cic_deposit(hv_ctx *c) {
+    if (!c) return HV_ERR_ARG;
+    int rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
+    if (rc) return rc;
+    if (c->precision == HV_PREC_F64) {
+        k_deposit<double><<<blocks_for(c->total), kThreads, 0, c->stream>>>(
+            particles<double>(c), c->np, c->total, make_geom<double>(c), c->acc, nullptr, c->status);
+        HV_CHECK_LAUNCH(c);
+        k_counts_to_field<double><<<blocks_for(2 * c->G), kThreads, 0, c->stream>>>(c->acc, (double *)c->density,
+                                                                                   2 * c->G);
+    } else {
+        k_deposit<float><<<blocks_for(c->total), kThreads, 0, c->stream>>>(
+            particles<float>(c), c->np, c->total, make_geom<float>(c), c->acc, nullptr, c->status);
+        HV_CHECK_LAUNCH(c);
+        k_counts_to_field<float><<<blocks_for(2 * c->G), kThreads, 0, c->stream>>>(c->acc, (float *)c->density,
+                                                                                  2 * c->G);
+    }
+    HV_CHECK_LAUNCH(c);
+    int st = 0;
+    rc = take_status(c, &st);
+    if (rc) return rc;
+    return st & HV_WARN_PARTICLE_OUTSIDE;
+}
+
+int hv_smear_density(hv_ctx *c, double sigma_r, int ppn) {
+    if (!c) return HV_ERR_ARG;
+    return HV_DISPATCH(c, smear<double>(c, false, sigma_r, ppn), smear<float>(c, false, sigma_r, ppn));
+}
+
+int hv_skyrme_potentials(hv_ctx *c, const hv_skyrme *s) {
+    if (!c || !s) return HV_ERR_ARG;
+    return HV_DISPATCH(c, potential_from_density<double>(c, s, false), potential_from_density<float>(c, s, false));
+}
+
+int hv_set_coulomb_bc(hv_ctx *c, int z) {
+    if (!c) return HV_ERR_ARG;
+    std::vector<double> h((size_t)c->G);
+    HV_HIP(c, hipMemcpyAsync(h.data(), c->phi, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    hv_coulomb_faces(h.data(), c->world.n, c->world.d_max, z);
+    HV_HIP(c, hipMemcpyAsync(c->phi, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    return HV_OK;
+}
+
+int hv_coulomb_sor(hv_ctx *c, double omega, double tol, int max_it, int *iters, double *max_diff) {
+    if (!c) return HV_ERR_ARG;
+    if (!(tol > 0.0)) return set_err(c, HV_ERR_ARG, "tol must be positive");
+    return HV_DISPATCH(c, sor_solve<double>(c, (const double *)c->density, omega, tol, max_it, iters, max_diff),
+                       sor_solve<float>(c, (const float *)c->density, omega, tol, max_it, iters, max_diff));
+}
+
+int hv_merge_potentials(hv_ctx *c) {
+    if (!c) return HV_ERR_ARG;
+    if (c->precision == HV_PREC_F64)
+        k_merge<double><<<blocks_for(c->G), kThreads, 0, c->stream>>>((double *)c->U, c->phi, c->G);
+    else
+        k_merge<float><<<blocks_for(c->G), kThreads, 0, c->stream>>>((float *)c->U, c->phi, c->G);
+    HV_CHECK_LAUNCH(c);
+    return HV_OK;
+}
+
+int hv_forces_fdm(hv_ctx *c) {
+    if (!c) return HV_ERR_ARG;
+    void *p;
+    long long n;
+    bool f64;
+    int rc = field_ptr(c, HV_FIELD_FORCES_X, &p, &n, &f64);
+    if (rc) return rc;
+    if (c->precision == HV_PREC_F64)
+        k_fdm<double><<<blocks_for(2 * c->G), kThreads, 0, c->stream>>>(
+            (const double *)c->U, (double *)c->F[0], (double *)c->F[1], (double *)c->F[2], make_geom<double>(c));
+    else
+        k_fdm<float><<<blocks_for(2 * c->G), kThreads, 0, c->stream>>>(
+            (const float *)c->U, (float *)c->F[0], (float *)c->F[1], (float *)c->F[2], make_geom<float>(c));
+    HV_CHECK_LAUNCH(c);
+    return HV_OK;
+}
+
+int hv_cic_gather(hv_ctx *c) {
+    if (!c) return HV_ERR_ARG;
+    void *p;
+    long long n;
+    bool f64;
+    int rc = field_ptr(c, HV_FIELD_FORCES_X, &p, &n, &f64);
+    if (rc) return rc;
+    if (c->precision == HV_PREC_F64)
+        k_gather_f<double><<<blocks_for(c->total), kThreads, 0, c->stream>>>(
+            particles<double>(c), c->np, c->total, make_geom<double>(c), (const double *)c->F[0],
+            (const double *)c->F[1], (const double *)c->F[2]);
+    else
+        k_gather_f<float><<<blocks_for(c->total), kThreads, 0, c->stream>>>(
+            particles<float>(c), c->np, c->total, make_geom<float>(c), (const float *)c->F[0],
+            (const float *)c->F[1], (const float *)c->F[2]);
+    HV_CHECK_LAUNCH(c);
+    return HV_OK;
+}
+
+int hv_kick_half(hv_ctx *c, double dt) {
+    if (!c) return HV_ERR_ARG;
+    double fac = dt / (2.0 * HV_HBARC);
+    if (c->precision == HV_PREC_F64)
+        k_kick<double><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<double>(c), c->total, fac);
+    else
+        k_kick<float><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<float>(c), c->total, (float)fac);
+    HV_CHECK_LAUNCH(c);
+    return HV_OK;
+}
+
+int hv_drift(hv_ctx *c, double dt) {
+    if (!c) return HV_ERR_ARG;
+    double fac = (dt * HV_HBARC) / HV_MC2;
+    if (c->precision == HV_PREC_F64)
+        k_drift<double><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<double>(c), c->total, fac);
+    else
+        k_drift<float><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<float>(c), c->total, (float)fac);
+    HV_CHECK_LAUNCH(c);
+    return HV_OK;
+}
+
+int hv_excite(hv_ctx *c, double eta, int z, int n, int mode) {
+    if (!c) return HV_ERR_ARG;
+    double dp, dn;
+    if (mode == HV_BOOST_SYMMETRIC) {
+        dp = eta;
+        dn = eta;
+    } else if (mode == HV_BOOST_REF_WEIGHTED) {
+        double zz = z, nnuc = n;
+        dp = eta * nnuc / (zz + nnuc);
+        dn = eta * zz / (zz + nnuc);
+    } else {
+        return set_err(c, HV_ERR_ARG, "unknown boost mode %d", mode);
+    }
+    if (c->precision == HV_PREC_F64)
+        k_excite<double><<<blocks_for(c->total), kThreads, 0, c->stream>>>((double *)c->part[5], c->np, c->total, dp,
+                                                                          dn);
+    else
+        k_excite<float><<<blocks_for(c->total), kThreads, 0, c->stream>>>((float *)c->part[5], c->np, c->total,
+                                                                         (float)dp, (float)dn);
+    HV_CHECK_LAUNCH(c);
+    return HV_OK;
+}
+
+int hv_center_of_mass(hv_ctx *c, int type, double *out) {
+    if (!c || !out || (type != HV_PROTONS && type != HV_NEUTRONS)) return HV_ERR_ARG;
+    double st[4];
+    int rc = HV_DISPATCH(c, stats_now<double>(c, st), stats_now<float>(c, st));
+    if (rc) return rc;
+    *out = st[type];
+    return HV_OK;
+}
+
+int hv_mean_squared_radius(hv_ctx *c, int type, double *out) {
+    if (!c || !out || (type != HV_PROTONS && type != HV_NEUTRONS)) return HV_ERR_ARG;
+    double st[4];
+    int rc = HV_DISPATCH(c, stats_now<double>(c, st), stats_now<float>(c, st));
+    if (rc) return rc;
+    *out = st[2 + type];
+    return HV_OK;
+}
+
+int hv_run_init(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *sor_iters) {
+    if (!c) return HV_ERR_ARG;
+    return HV_DISPATCH(c, run_init<double>(c, s, p, sor_iters), run_init<float>(c, s, p, sor_iters));
+}
+
+int hv_run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first_step, int nsteps, double *stats,
+                 int *sor_iters) {
+    if (!c) return HV_ERR_ARG;
+    if (first_step < 0 || nsteps < 0) return set_err(c, HV_ERR_ARG, "negative step range");
+    return HV_DISPATCH(c, run_steps<double>(c, s, p, first_step, nsteps, stats, sor_iters),
+                       run_steps<float>(c, s, p, first_step, nsteps, stats, sor_iters));
+}
+
+int hv_simulate(hv_ctx *c, const hv_skyrme *s, const hv_params *p, double *stats, int *sor_iters) {
+    if (!c) return HV_ERR_ARG;
+    int it0 = 0;
+    int rc = hv_run_init(c, s, p, &it0);
+    if (rc < 0) return rc;
+    int warn = rc;
+    rc = hv_run_steps(c, s, p, 0, p->steps, stats, sor_iters);
+    if (rc < 0) return rc;
+    return warn | rc;
+}
+
+int hv_timer_start(hv_ctx *c) {
+    if (!c) return HV_ERR_ARG;
+    HV_HIP(c, hipEventRecord(c->ev0, c->stream));
+    return HV_OK;
+}
+
+int hv_timer_stop(hv_ctx *c, double *ms) {
+    if (!c || !ms) return HV_ERR_ARG;
+    HV_HIP(c, hipEventRecord(c->ev1, c->stream));
+    HV_HIP(c, hipEventSynchronize(c->ev1));
+    float f = 0.0f;
+    HV_HIP(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return HV_OK;
+}
+
+int hv_profile_enable(hv_ctx *c, int on) {
+    if (!c) return HV_ERR_ARG;
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    prof_flush(c);
+    c->prof = on != 0;
+    for (int k = 0; k < HV_KCLASS_COUNT; k++) {
+        c->prof_ms[k] = 0.0;
+        c->prof_launches[k] = 0;
+    }
+    return HV_OK;
+}
+
+int hv_profile_read(hv_ctx *c, int kclass, double *ms, long long *launches) {
+    if (!c || kclass < 0 || kclass >= HV_KCLASS_COUNT) return HV_ERR_ARG;
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    prof_flush(c);
+    if (ms) *ms = c->prof_ms[kclass];
+    if (launches) *launches = c->prof_launches[kclass];
+    return HV_OK;
+}
+
+const char *hv_build_info(void) {
+    return "libhvamd gfx950 -O3 -ffp-contract=off (HIP " HIP_VERSION_STR ")";
+}
+
+}  // extern "C"

@@ -1,0 +1,19 @@
+"""hvamd: Python layer of the MI355X (gfx950) hyper-vlasov test-particle loop.
+
+The compute lives in libhvamd.so (HIP kernels behind the C-ABI of
+include/hv_abi.h); this package binds it with ctypes and adds the workload
+builders and post-processing.
+"""
+from ._lib import (HV_BOOST_REF_WEIGHTED, HV_BOOST_SYMMETRIC, HV_FIELD_COULOMB, HV_FIELD_DENSITY,
+                   HV_FIELD_FORCES_X, HV_FIELD_FORCES_Y, HV_FIELD_FORCES_Z, HV_FIELD_POTENTIALS,
+                   HV_NEUTRONS, HV_PREC_F64, HV_PREC_MIXED, HV_PROTONS, HvError, load)
+from .sim import (HBARC, MC2, RHO0, Params, Simulation, Skyrme, World, dipole,
+                  sample_uniform_spheres, sigma_from_fwhm)
+
+__all__ = [
+    "HV_BOOST_REF_WEIGHTED", "HV_BOOST_SYMMETRIC", "HV_FIELD_COULOMB", "HV_FIELD_DENSITY",
+    "HV_FIELD_FORCES_X", "HV_FIELD_FORCES_Y", "HV_FIELD_FORCES_Z", "HV_FIELD_POTENTIALS",
+    "HV_NEUTRONS", "HV_PREC_F64", "HV_PREC_MIXED", "HV_PROTONS", "HvError", "load",
+    "HBARC", "MC2", "RHO0", "Params", "Simulation", "Skyrme", "World", "dipole",
+    "sample_uniform_spheres", "sigma_from_fwhm",
+]

@@ -1,0 +1,153 @@
+"""ctypes binding of libhvamd.so (C-ABI declared in include/hv_abi.h and
+include/hv_init.h).
+
+Only plain pointers and sizes cross the boundary.  Loading fails loudly when
+the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = Path(os.environ.get("HVAMD_LIB", PKG_ROOT / "libhvamd.so"))
+
+HV_OK = 0
+HV_ERR_ARG = -1
+HV_ERR_HIP = -2
+HV_ERR_ALLOC = -3
+HV_ERR_STATE = -4
+HV_WARN_SOR_NOT_CONVERGED = 1
+HV_WARN_PARTICLE_OUTSIDE = 2
+HV_WARN_SMEAR_MARGIN = 4
+
+HV_PREC_F64 = 0
+HV_PREC_MIXED = 1
+
+HV_PROTONS = 0
+HV_NEUTRONS = 1
+
+HV_BOOST_REF_WEIGHTED = 0
+HV_BOOST_SYMMETRIC = 1
+
+HV_FIELD_DENSITY = 0
+HV_FIELD_POTENTIALS = 1
+HV_FIELD_COULOMB = 2
+HV_FIELD_FORCES_X = 3
+HV_FIELD_FORCES_Y = 4
+HV_FIELD_FORCES_Z = 5
+
+HV_KCLASS_SOR = 0
+HV_KCLASS_PARTICLE = 1
+HV_KCLASS_SMEAR = 2
+HV_KCLASS_POTENTIAL = 3
+KCLASS_NAMES = ("sor_sweep", "particle_pass", "smear", "potential")
+
+
+class HvWorld(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3), ("d_max", ctypes.c_double * 3)]
+
+
+class HvSkyrme(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_double), ("B", ctypes.c_double), ("C", ctypes.c_double),
+                ("gamma", ctypes.c_double)]
+
+
+class HvParams(ctypes.Structure):
+    _fields_ = [
+        ("t_f", ctypes.c_double),
+        ("steps", ctypes.c_int),
+        ("substeps", ctypes.c_int),
+        ("t_exc", ctypes.c_double),
+        ("eta_exc", ctypes.c_double),
+        ("boost_mode", ctypes.c_int),
+        ("z", ctypes.c_int),
+        ("n", ctypes.c_int),
+        ("sigma_r", ctypes.c_double),
+        ("part_per_nucleon", ctypes.c_int),
+        ("sor_omega", ctypes.c_double),
+        ("sor_tol", ctypes.c_double),
+        ("sor_max_it", ctypes.c_int),
+    ]
+
+
+class HvError(RuntimeError):
+    """An entry point returned a negative status."""
+
+    def __init__(self, fn: str, code: int, message: str):
+        super().__init__(f"{fn} failed with status {code}: {message}")
+        self.code = code
+
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "hv_ctx_create": (_I, [ctypes.POINTER(HvWorld), _I, _I, _I, ctypes.POINTER(_P)]),
+    "hv_ctx_destroy": (_I, [_P]),
+    "hv_last_error": (ctypes.c_char_p, [_P]),
+    "hv_sync": (_I, [_P]),
+    "hv_stream": (_P, [_P]),
+    "hv_last_outside": (_I, [_P, _IP]),
+    "hv_upload_particles": (_I, [_P] + [_P] * 9),
+    "hv_download_particles": (_I, [_P] + [_P] * 9),
+    "hv_upload_field": (_I, [_P, _I, _P]),
+    "hv_download_field": (_I, [_P, _I, _P]),
+    "hv_cic_deposit": (_I, [_P]),
+    "hv_smear_density": (_I, [_P, _D, _I]),
+    "hv_skyrme_potentials": (_I, [_P, ctypes.POINTER(HvSkyrme)]),
+    "hv_set_coulomb_bc": (_I, [_P, _I]),
+    "hv_coulomb_sor": (_I, [_P, _D, _D, _I, _IP, _DP]),
+    "hv_merge_potentials": (_I, [_P]),
+    "hv_forces_fdm": (_I, [_P]),
+    "hv_cic_gather": (_I, [_P]),
+    "hv_kick_half": (_I, [_P, _D]),
+    "hv_drift": (_I, [_P, _D]),
+    "hv_excite": (_I, [_P, _D, _I, _I, _I]),
+    "hv_center_of_mass": (_I, [_P, _I, _DP]),
+    "hv_mean_squared_radius": (_I, [_P, _I, _DP]),
+    "hv_run_init": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _IP]),
+    "hv_run_steps": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _I, _I, _P, _P]),
+    "hv_simulate": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _P, _P]),
+    "hv_timer_start": (_I, [_P]),
+    "hv_timer_stop": (_I, [_P, _DP]),
+    "hv_build_info": (ctypes.c_char_p, []),
+    "hv_profile_enable": (_I, [_P, _I]),
+    "hv_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
+    "hv_fermi_momentum": (_D, [_D]),
+    "hv_sample_uniform_spheres": (_I, [ctypes.c_ulonglong, _I, _I, _D, _D] + [_P] * 6),
+    "hv_mix64": (ctypes.c_ulonglong, [ctypes.c_ulonglong]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libhvamd.so once; raise if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is not built: run `make -C hyper-vlasov_amd` or __graft_entry__.build(); "
+            "the step loop has no CPU fallback")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return ctypes.c_void_p(a.ctypes.data)

@@ -1,0 +1,203 @@
+/*
+ * hv_abi.h -- C-ABI of libhvamd.so, the MI355X (gfx950) implementation of the
+ * hyper-vlasov per-timestep test-particle loop.
+ *
+ * The reference runs this loop on the CPU as free C++ templates over raw host
+ * SoA pointers, called from cpu_simulate (src/hyper_vlasov.cpp:33-118), with a
+ * commented-out gpu_simulate stub of the same signature
+ * (src/hyper_vlasov.cpp:120-123) and a `use_gpu` input key
+ * (include/init.hpp:322-323) as its hook for a device path.  This library fills
+ * that hook: one entry point per reference kernel, with the same meaning and
+ * the same in-place semantics, plus a fused whole-loop fast path.
+ *
+ * Conventions
+ *   - Every call returns int: 0 ok, < 0 error (see HV_ERR_*), > 0 warning bit
+ *     mask (HV_WARN_*).  hv_last_error() gives a message.  Nothing calls exit().
+ *     (The reference returns void, prints "SOR COULOMB DID NOT CONVERGE!" on
+ *     non-convergence, include/physics.hpp:155-156, and exit(1)s on allocation
+ *     failure, include/init.hpp:94-95.)
+ *   - Device buffers belong to the context.  Host arrays stay caller-owned and
+ *     are touched only inside hv_upload_* / hv_download_* and the calls that
+ *     return host scalars.  (Reference: caller allocates with create_* and
+ *     frees with free_*, include/init.hpp:90-157, 257-273.)
+ *   - Particles are stored protons first, then neutrons; species is implied by
+ *     index, as in the reference (include/tools.hpp:62, 122).
+ *   - Grid arrays are row-major, z fastest: node (i,j,k) at (i*ny + j)*nz + k
+ *     (IDX, include/sim_structs.hpp:38).  "Double" fields hold [proton G |
+ *     neutron G] (include/init.hpp:102-111).
+ *   - One HIP stream per context; a context is not thread-safe (use one per
+ *     host thread).  Calls are stream-ordered; those returning host scalars
+ *     synchronise.
+ *   - Precision HV_PREC_F64 reproduces the CPU restatement bit for bit.
+ *     HV_PREC_MIXED stores particles and the smeared grid fields in FP32 and
+ *     keeps the Coulomb potential, its final SOR sweeps and all diagnostics
+ *     reductions in FP64.
+ */
+#ifndef HV_ABI_H
+#define HV_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define HV_OK 0
+#define HV_ERR_ARG (-1)     /* invalid argument or size */
+#define HV_ERR_HIP (-2)     /* HIP runtime error */
+#define HV_ERR_ALLOC (-3)   /* device allocation failed */
+#define HV_ERR_STATE (-4)   /* call not valid in the current context state */
+#define HV_WARN_SOR_NOT_CONVERGED 1  /* SOR hit max_it with max|dphi| > tol */
+#define HV_WARN_PARTICLE_OUTSIDE 2   /* some particles were outside the grid or non-finite */
+#define HV_WARN_SMEAR_MARGIN 4       /* a smearing tap lies within 1e-9 of the cutoff sphere */
+
+/* ---- precision ----------------------------------------------------------- */
+#define HV_PREC_F64 0
+#define HV_PREC_MIXED 1
+
+/* ---- species selectors (include/sim_structs.hpp:28-30) -------------------- */
+#define HV_PROTONS 0
+#define HV_NEUTRONS 1
+
+/* ---- boost modes --------------------------------------------------------- */
+#define HV_BOOST_REF_WEIGHTED 0 /* protons kz += eta N/A, neutrons kz -= eta Z/A (include/physics.hpp:237-246) */
+#define HV_BOOST_SYMMETRIC 1    /* protons kz += eta, neutrons kz -= eta */
+
+/* ---- device fields (for hv_upload_field / hv_download_field) -------------- */
+#define HV_FIELD_DENSITY 0    /* 2G: CIC counts after hv_cic_deposit, density after hv_smear_density */
+#define HV_FIELD_POTENTIALS 1 /* 2G: U_p | U_n */
+#define HV_FIELD_COULOMB 2    /* G : phi (persistent SOR warm-start state) */
+#define HV_FIELD_FORCES_X 3   /* 2G */
+#define HV_FIELD_FORCES_Y 4   /* 2G */
+#define HV_FIELD_FORCES_Z 5   /* 2G */
+
+/* World<T> (include/sim_structs.hpp:73-76): n nodes and half-width d_max [fm]
+ * per axis; spacing 2 d_max / n; node i at d_max (2 i / n - 1). */
+typedef struct hv_world {
+    int n[3];
+    double d_max[3];
+} hv_world;
+
+/* Skyrme<T> (include/sim_structs.hpp:84-86): U = A t + B t^gamma + 2 tau C (rho_n - rho_p)/rho0 */
+typedef struct hv_skyrme {
+    double A, B, C, gamma;
+} hv_skyrme;
+
+/* The loop parameters of Parameters<T> (include/sim_structs.hpp:49-53) plus
+ * the knobs the reference hard-codes (SOR omega/tolerance/cap,
+ * include/physics.hpp:103-104, include/sim_structs.hpp:33,44) and the boost
+ * convention. */
+typedef struct hv_params {
+    double t_f;           /* final time [fm/c]; dt = t_f / steps (src/hyper_vlasov.cpp:35) */
+    int steps;
+    int substeps;         /* stats cadence of the stats.txt writer */
+    double t_exc;         /* boost once at the first step with step*dt >= t_exc */
+    double eta_exc;       /* boost strength [fm^-1] */
+    int boost_mode;       /* HV_BOOST_* */
+    int z, n;             /* nucleus Z and N */
+    double sigma_r;       /* smearing width [fm] (calc_sigma(r_fwhm)) */
+    int part_per_nucleon; /* test particles per nucleon (density normalisation) */
+    double sor_omega;     /* reference: 1.5 */
+    double sor_tol;       /* reference: 1e-4, on max |dphi| of an iteration */
+    int sor_max_it;       /* reference: 512 */
+} hv_params;
+
+typedef struct hv_ctx hv_ctx;
+
+/* ---- context ------------------------------------------------------------- */
+/* Allocates every device buffer the loop needs (replaces create_* in
+ * include/init.hpp:90-157 and the allocations of src/hyper_vlasov.cpp:44-51).
+ * Each species may hold at most 2^23 - 1 test particles. */
+int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int precision, hv_ctx **out);
+int hv_ctx_destroy(hv_ctx *ctx);
+const char *hv_last_error(const hv_ctx *ctx);
+int hv_sync(hv_ctx *ctx);
+/* the HIP stream of the context (hipStream_t), for callers that interleave work */
+void *hv_stream(hv_ctx *ctx);
+/* number of particles skipped as out-of-grid or non-finite by the last deposit */
+int hv_last_outside(hv_ctx *ctx, int *count);
+
+/* ---- particle / field transfer (host FP64 <-> device) -------------------- */
+/* TestParticles<T> (include/sim_structs.hpp:55-60): x,y,z [fm], kx,ky,kz
+ * [fm^-1], fx,fy,fz [MeV/fm].  Any pointer may be NULL to skip that array. */
+int hv_upload_particles(hv_ctx *ctx, const double *x, const double *y, const double *z,
+                        const double *kx, const double *ky, const double *kz,
+                        const double *fx, const double *fy, const double *fz);
+int hv_download_particles(hv_ctx *ctx, double *x, double *y, double *z, double *kx, double *ky,
+                          double *kz, double *fx, double *fy, double *fz);
+int hv_upload_field(hv_ctx *ctx, int which, const double *host);
+int hv_download_field(hv_ctx *ctx, int which, double *host);
+
+/* ---- one call per reference kernel ---------------------------------------- */
+/* distribute_volumetric_particles_cic (include/tools.hpp:36-96): particles ->
+ * HV_FIELD_DENSITY as CIC counts (deterministic fixed-point accumulation). */
+int hv_cic_deposit(hv_ctx *ctx);
+/* compute_volumetric_densities (include/physics.hpp:203-234): counts ->
+ * smeared density, in place on HV_FIELD_DENSITY (no caller scratch needed). */
+int hv_smear_density(hv_ctx *ctx, double sigma_r, int part_per_nucleon);
+/* compute_volumetric_skyrme_potentials (include/physics.hpp:273-281) */
+int hv_skyrme_potentials(hv_ctx *ctx, const hv_skyrme *skyrme);
+/* set_initial_coulomb_boundaries (include/tools.hpp:284-300) */
+int hv_set_coulomb_bc(hv_ctx *ctx, int z);
+/* compute_volumetric_coulomb_potentials_sor (include/physics.hpp:100-157):
+ * red-black SOR on HV_FIELD_COULOMB with source = proton density.  iters and
+ * max_diff (may be NULL) receive the iterations done and the last max|dphi|. */
+int hv_coulomb_sor(hv_ctx *ctx, double omega, double tol, int max_it, int *iters, double *max_diff);
+/* merge_volumetric_potentials (include/tools.hpp:378-384): U_p += phi */
+int hv_merge_potentials(hv_ctx *ctx);
+/* compute_volumetric_forces_fdm (include/physics.hpp:160-200): U -> forces */
+int hv_forces_fdm(hv_ctx *ctx);
+/* distribute_forces_to_particles_cic (include/tools.hpp:99-168): forces -> particle f */
+int hv_cic_gather(hv_ctx *ctx);
+/* update_momenta_half (include/physics.hpp:249-258) */
+int hv_kick_half(hv_ctx *ctx, double dt);
+/* update_positions_full (include/physics.hpp:261-270) */
+int hv_drift(hv_ctx *ctx, double dt);
+/* nuclear_excitation (include/physics.hpp:237-246), mode HV_BOOST_* */
+int hv_excite(hv_ctx *ctx, double eta, int z, int n, int mode);
+/* center_of_mass (include/tools.hpp:328-338): mean z of a species */
+int hv_center_of_mass(hv_ctx *ctx, int type, double *out);
+/* mean_squared_radius (include/tools.hpp:303-325): <r^2> of in-box particles of a species */
+int hv_mean_squared_radius(hv_ctx *ctx, int type, double *out);
+
+/* ---- fused step loop (the body of cpu_simulate, src/hyper_vlasov.cpp:53-104) */
+/* hv_run_init: Coulomb faces + pre-loop field solve + gather (:53-62).
+ * hv_run_steps: steps [first_step, first_step + nsteps) of the loop (:63-104);
+ * the boost fires once, at the first step with step*dt >= t_exc.
+ * stats (may be NULL) receives 4 doubles per step: centre of mass z of protons
+ * and neutrons and <r^2> of protons and neutrons for the positions at the
+ * start of the step (the quantities of the stats.txt line, :65-70);
+ * sor_iters (may be NULL) the SOR iterations of each step's solve. */
+int hv_run_init(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, int *sor_iters);
+int hv_run_steps(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, int first_step,
+                 int nsteps, double *stats, int *sor_iters);
+
+/* gpu_simulate's body without file output: hv_run_init + hv_run_steps(0, steps). */
+int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, double *stats,
+                int *sor_iters);
+
+/* ---- timing / introspection ---------------------------------------------- */
+/* Milliseconds of device time between the last pair of hv_timer_start /
+ * hv_timer_stop calls (hipEvents on the context stream). */
+int hv_timer_start(hv_ctx *ctx);
+int hv_timer_stop(hv_ctx *ctx, double *ms);
+/* Library build string (arch, flags). */
+const char *hv_build_info(void);
+
+/* ---- kernel-class profiling (roofline accounting) -------------------------- */
+/* When enabled, the fused loop brackets each kernel class with hipEvents on the
+ * context stream and accumulates device milliseconds and the number of
+ * launches that did work (early-exit SOR launches past convergence are not
+ * counted as launches; their few microseconds are in the time). */
+#define HV_KCLASS_SOR 0       /* one red or black SOR sweep */
+#define HV_KCLASS_PARTICLE 1  /* fused gather + kick + drift + deposit + diagnostics pass */
+#define HV_KCLASS_SMEAR 2     /* the smearing passes of one step (one launch = one step's set) */
+#define HV_KCLASS_POTENTIAL 3 /* Skyrme + Coulomb merge */
+#define HV_KCLASS_COUNT 4
+int hv_profile_enable(hv_ctx *ctx, int on); /* also resets the counters */
+int hv_profile_read(hv_ctx *ctx, int kclass, double *ms, long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HV_ABI_H */

@@ -1,0 +1,283 @@
+"""GPU parity: libhvamd.so (through its C-ABI) against the CPU oracle.
+
+FP64 is held to bit equality: every kernel keeps the oracle's operation order,
+both sides are compiled with -ffp-contract=off, the CIC deposit and the
+diagnostics sums are exact integer sums, the Skyrme power is the shared
+deterministic hv_det_pow and the smearing weights / Coulomb faces come from the
+same host-side tables.  The mixed-precision path is held to the tolerances
+stated in each test.
+"""
+import numpy as np
+import pytest
+
+import hvamd
+from hvamd import configs
+from inputs import SIGMA_R, particles, smooth_field
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [
+    ((32, 32, 32), (8.0, 8.0, 8.0)),      # c0 grid
+    ((20, 16, 24), (6.0, 5.0, 7.5)),      # anisotropic, non-power-of-two
+]
+
+
+def _sim(n, d, np_, nn, prec=hvamd.HV_PREC_F64):
+    return hvamd.Simulation(hvamd.World(n, d), np_, nn, prec)
+
+
+def _eq(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape
+    if not np.array_equal(a, b):
+        bad = np.flatnonzero(a.ravel() != b.ravel())
+        i = bad[0]
+        raise AssertionError(f"{bad.size} of {a.size} differ; first at {i}: {a.ravel()[i]!r} vs {b.ravel()[i]!r}")
+
+
+@pytest.mark.parametrize("n,d", GRIDS)
+def test_deposit_bitwise(oracle, n, d):
+    print(f"deposit grid={n}", flush=True)
+    st = particles(11, 700, 900, 0.6 * min(d), spill=0.01)
+    sim = _sim(n, d, 700, 900)
+    sim.upload_particles(**st)
+    warn = sim.distribute_volumetric_particles_cic()
+    got = sim.download_field(hvamd.HV_FIELD_DENSITY)
+    want, _, skipped = oracle.deposit(st["x"], st["y"], st["z"], 700, 900, n, d)
+    _eq(got, want)
+    assert skipped > 0 and (warn & 2)
+
+
+@pytest.mark.parametrize("n,d", GRIDS)
+def test_smear_bitwise(oracle, n, d):
+    print(f"smear grid={n}", flush=True)
+    st = particles(12, 500, 600, 0.5 * min(d))
+    counts, _, _ = oracle.deposit(st["x"], st["y"], st["z"], 500, 600, n, d)
+    sim = _sim(n, d, 500, 600)
+    sim.upload_field(hvamd.HV_FIELD_DENSITY, counts)
+    sim.compute_volumetric_densities(SIGMA_R, 100)
+    got = sim.download_field(hvamd.HV_FIELD_DENSITY)
+    _eq(got, oracle.smear(counts, n, d, SIGMA_R, 100))
+
+
+@pytest.mark.parametrize("n,d", GRIDS)
+def test_skyrme_merge_fdm_bitwise(oracle, n, d):
+    print(f"skyrme/merge/fdm grid={n}", flush=True)
+    G = int(np.prod(n))
+    rho = np.concatenate([smooth_field(1, n, 0.09), smooth_field(2, n, 0.1)])
+    phi = smooth_field(3, n, 20.0)
+    sim = _sim(n, d, 10, 10)
+    sim.upload_field(hvamd.HV_FIELD_DENSITY, rho)
+    sim.upload_field(hvamd.HV_FIELD_COULOMB, phi)
+    sim.compute_volumetric_skyrme_potentials(hvamd.Skyrme(-356.0, 303.0, 32.0, 7.0 / 6.0))
+    U = sim.download_field(hvamd.HV_FIELD_POTENTIALS)
+    U_o = oracle.skyrme(rho, n, d, -356.0, 303.0, 32.0, 7.0 / 6.0)
+    _eq(U, U_o)
+    sim.merge_volumetric_potentials()
+    M_o = oracle.merge(U_o, phi, n, d)
+    _eq(sim.download_field(hvamd.HV_FIELD_POTENTIALS), M_o)
+    sim.compute_volumetric_forces_fdm()
+    F_o = oracle.fdm(M_o, n, d)
+    for which, want in zip((hvamd.HV_FIELD_FORCES_X, hvamd.HV_FIELD_FORCES_Y, hvamd.HV_FIELD_FORCES_Z), F_o):
+        _eq(sim.download_field(which), want)
+    assert G == len(phi)
+
+
+@pytest.mark.parametrize("n,d", GRIDS)
+@pytest.mark.parametrize("omega,tol", [(1.5, 1e-4), (1.8, 1e-6)])
+def test_sor_bitwise(oracle, n, d, omega, tol):
+    print(f"sor grid={n} omega={omega} tol={tol}", flush=True)
+    G = int(np.prod(n))
+    rho_p = smooth_field(4, n, 0.08)
+    phi0 = oracle.coulomb_bc(np.zeros(G), n, d, 50)
+    sim = _sim(n, d, 10, 10)
+    sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p, np.zeros(G)]))
+    sim.set_initial_coulomb_boundaries(50)
+    _eq(sim.download_field(hvamd.HV_FIELD_COULOMB), phi0)
+    it, md = sim.compute_volumetric_coulomb_potentials_sor(omega, tol, 512)
+    phi_o, it_o, md_o, _ = oracle.sor(phi0, rho_p, n, d, omega, tol, 512)
+    assert it == it_o and md == md_o
+    _eq(sim.download_field(hvamd.HV_FIELD_COULOMB), phi_o)
+    # warm start: a second solve after a small change converges in fewer steps
+    rho2 = rho_p * 1.001
+    sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho2, np.zeros(G)]))
+    it2, _ = sim.compute_volumetric_coulomb_potentials_sor(omega, tol, 512)
+    phi_o2, it_o2, _, _ = oracle.sor(phi_o, rho2, n, d, omega, tol, 512)
+    assert it2 == it_o2 and it2 < it_o
+    _eq(sim.download_field(hvamd.HV_FIELD_COULOMB), phi_o2)
+
+
+def test_sor_cap_warning(oracle):
+    print("sor cap", flush=True)
+    n, d = (24, 24, 24), (6.0, 6.0, 6.0)
+    G = int(np.prod(n))
+    sim = _sim(n, d, 10, 10)
+    sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([smooth_field(5, n, 0.08), np.zeros(G)]))
+    sim.set_initial_coulomb_boundaries(82)
+    it, md = sim.compute_volumetric_coulomb_potentials_sor(1.5, 1e-12, 7)
+    assert it == 7 and md > 1e-12
+    assert sim.last_warnings & hvamd._lib.HV_WARN_SOR_NOT_CONVERGED
+
+
+@pytest.mark.parametrize("n,d", GRIDS)
+def test_particle_kernels_bitwise(oracle, n, d):
+    print(f"gather/kick/drift/excite/stats grid={n}", flush=True)
+    np_, nn = 800, 1100
+    st = particles(13, np_, nn, 0.7 * min(d), spill=0.01)
+    F = [smooth_field(6 + a, n, 3.0, positive=False) for a in range(3)]
+    F = [np.concatenate([f, 0.5 * f]) for f in F]
+    sim = _sim(n, d, np_, nn)
+    sim.upload_particles(**st)
+    for which, f in zip((hvamd.HV_FIELD_FORCES_X, hvamd.HV_FIELD_FORCES_Y, hvamd.HV_FIELD_FORCES_Z), F):
+        sim.upload_field(which, f)
+    sim.distribute_forces_to_particles_cic()
+    got = sim.download_particles()
+    f_o = oracle.gather(st["x"], st["y"], st["z"], np_, nn, F, n, d)
+    for k, want in zip(("fx", "fy", "fz"), f_o):
+        _eq(got[k], want)
+    sim.update_momenta_half(0.5)
+    k_o = oracle.kick([st["kx"], st["ky"], st["kz"]], f_o, 0.5)
+    sim.update_positions_full(0.5)
+    x_o = oracle.drift([st["x"], st["y"], st["z"]], k_o, 0.5)
+    sim.nuclear_excitation(0.1014, 20, 28, hvamd.HV_BOOST_REF_WEIGHTED)
+    kz_o = oracle.excite(k_o[2], np_, nn, 0.1014, 20, 28, 0)
+    got = sim.download_particles()
+    for k, want in zip(("kx", "ky", "kz", "x", "y", "z"), (k_o[0], k_o[1], kz_o, *x_o)):
+        _eq(got[k], want)
+    st_o = oracle.stats(x_o[0], x_o[1], x_o[2], np_, nn, n, d)
+    got_st = [sim.center_of_mass(hvamd.HV_PROTONS), sim.center_of_mass(hvamd.HV_NEUTRONS),
+              sim.mean_squared_radius(hvamd.HV_PROTONS), sim.mean_squared_radius(hvamd.HV_NEUTRONS)]
+    _eq(np.array(got_st), st_o)
+
+
+def _oracle_run(oracle, w, st, steps):
+    p = w.params(steps)
+    s = w.sky()
+    n, d = w.world().n, w.world().d_max
+    return oracle.simulate(st, w.n_protons, w.n_neutrons, n, d, s.A, s.B, s.C, s.gamma, p.dt, steps, p.t_exc,
+                           p.eta_exc, p.boost_mode, w.z, w.n, p.sigma_r, p.part_per_nucleon, p.sor_omega,
+                           p.sor_tol, p.sor_max_it)
+
+
+@pytest.mark.parametrize("steps", [1, 12])
+def test_fused_loop_bitwise_c0(oracle, steps):
+    """The whole step loop of the A=40, 32^3 workload, FP64, bit for bit."""
+    print(f"fused loop c0 steps={steps}", flush=True)
+    w = configs.c0()
+    x, y, z, kx, ky, kz = w.initial_state()
+    st = dict(x=x, y=y, z=z, kx=kx, ky=ky, kz=kz)
+    sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, hvamd.HV_PREC_F64)
+    sim.upload_particles(**st)
+    stats, its = sim.simulate(w.sky(), w.params(steps))
+    got = sim.download_particles()
+    st_o, stats_o, its_o, _, _ = _oracle_run(oracle, w, st, steps)
+    assert list(its) == list(its_o[1:])
+    _eq(stats, stats_o)
+    for k in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz"):
+        _eq(got[k], st_o[k])
+
+
+def test_fused_loop_chunked_equals_single(oracle):
+    """run_steps in chunks gives the same bits as one call (the boost and the
+    split half kicks are carried across calls)."""
+    print("fused loop chunked", flush=True)
+    w = configs.c0()
+    x, y, z, kx, ky, kz = w.initial_state()
+    prm = w.params(9)
+    prm.t_exc = 2.0  # boost at step 4
+    outs = []
+    for chunks in ([9], [3, 1, 5]):
+        sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, hvamd.HV_PREC_F64)
+        sim.upload_particles(x, y, z, kx, ky, kz)
+        sim.run_init(w.sky(), prm)
+        first = 0
+        allst = []
+        for c in chunks:
+            s, _ = sim.run_steps(w.sky(), prm, first, c)
+            allst.append(s)
+            first += c
+        outs.append((np.concatenate(allst), sim.download_particles()))
+    _eq(outs[0][0], outs[1][0])
+    for k in ("x", "kz", "fz"):
+        _eq(outs[0][1][k], outs[1][1][k])
+
+
+def test_fused_loop_matches_per_kernel_sequence(oracle):
+    """hv_simulate equals the reference's call sequence driven kernel by kernel."""
+    print("fused vs per-kernel", flush=True)
+    w = configs.c0()
+    x, y, z, kx, ky, kz = w.initial_state()
+    prm = w.params(3)
+    sk = w.sky()
+    a = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons)
+    a.upload_particles(x, y, z, kx, ky, kz)
+    stats_a, _ = a.simulate(sk, prm)
+    b = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons)
+    b.upload_particles(x, y, z, kx, ky, kz)
+
+    def solve():
+        b.distribute_volumetric_particles_cic()
+        b.compute_volumetric_densities(prm.sigma_r, prm.part_per_nucleon)
+        b.compute_volumetric_skyrme_potentials(sk)
+        b.compute_volumetric_coulomb_potentials_sor(prm.sor_omega, prm.sor_tol, prm.sor_max_it)
+        b.merge_volumetric_potentials()
+        b.compute_volumetric_forces_fdm()
+        b.distribute_forces_to_particles_cic()
+
+    b.set_initial_coulomb_boundaries(prm.z)
+    solve()
+    excited = False
+    stats_b = []
+    for step in range(prm.steps):
+        stats_b.append([b.center_of_mass(0), b.center_of_mass(1), b.mean_squared_radius(0),
+                        b.mean_squared_radius(1)])
+        if step * prm.dt >= prm.t_exc and not excited:
+            excited = True
+            b.nuclear_excitation(prm.eta_exc, prm.z, prm.n, prm.boost_mode)
+        b.update_momenta_half(prm.dt)
+        b.update_positions_full(prm.dt)
+        solve()
+        b.update_momenta_half(prm.dt)
+    _eq(stats_a, np.array(stats_b))
+    pa, pb = a.download_particles(), b.download_particles()
+    for k in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz"):
+        _eq(pa[k], pb[k])
+    _eq(a.download_field(hvamd.HV_FIELD_COULOMB), b.download_field(hvamd.HV_FIELD_COULOMB))
+
+
+def test_mixed_precision_tracks_fp64():
+    """FP32 particles/grid with FP64 Coulomb and reductions stay close to the
+    FP64 path over a short horizon (statistical quantities)."""
+    print("mixed vs fp64", flush=True)
+    w = configs.c0()
+    x, y, z, kx, ky, kz = w.initial_state()
+    res = {}
+    for prec in (hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED):
+        sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, prec)
+        sim.upload_particles(x, y, z, kx, ky, kz)
+        stats, _ = sim.simulate(w.sky(), w.params(40))
+        res[prec] = hvamd.dipole(stats, w.z, w.n)
+    d64, d32 = res[hvamd.HV_PREC_F64], res[hvamd.HV_PREC_MIXED]
+    rel = np.abs(d32 - d64).max() / np.abs(d64).max()
+    print("mixed vs fp64 D(t) max rel", rel)
+    assert rel < 1e-3
+
+
+def test_determinism_large():
+    """Two runs of the 1.04M-particle, 128^3 workload give identical bits."""
+    print("determinism c2 2 steps", flush=True)
+    w = configs.c2()
+    x, y, z, kx, ky, kz = w.initial_state()
+    outs = []
+    for _ in range(2):
+        sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, hvamd.HV_PREC_F64)
+        sim.upload_particles(x, y, z, kx, ky, kz)
+        prm = w.params(2)
+        prm.sor_max_it = 64
+        stats, its = sim.simulate(w.sky(), prm)
+        outs.append((stats, sim.download_particles()["z"], sim.download_field(hvamd.HV_FIELD_COULOMB)))
+        sim.close()
+    _eq(outs[0][0], outs[1][0])
+    _eq(outs[0][1], outs[1][1])
+    _eq(outs[0][2], outs[1][2])
