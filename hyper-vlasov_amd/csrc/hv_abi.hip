@@ -15,6 +15,8 @@
 #include <new>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "hv_abi.h"
 #include "hv_kernels.h"
 
@@ -38,7 +40,9 @@ struct hv_ctx {
     unsigned long long *acc = nullptr;  // 2G fixed-point CIC counts
     void *density = nullptr;            // 2G T
     void *U = nullptr;                  // 2G T
-    double *phi = nullptr;              // G
+    double *phi = nullptr;              // 2 Gh, colour-split (hv::Split)
+    void *rho_s = nullptr;              // 2 Gh T: colour-split copy of the proton density
+    Split sp{};
     void *H = nullptr, *D = nullptr;    // smearing factor channels
     size_t H_cap = 0, D_cap = 0;        // bytes
     void *F[3] = {};                    // 2G T each (per-kernel API only)
@@ -46,8 +50,14 @@ struct hv_ctx {
     double tab_sigma = -1.0;
     int tab_ppn = -1;
     int tab_warn = 0;
-    unsigned long long *maxd = nullptr; // per-iteration max |dphi| (double bits)
+    SmearTiles stl{};
+    size_t stl_lds = 0;
+    unsigned char *sflags = nullptr;    // per-tile "has counts" flags of the smearing
+    size_t sflags_cap = 0;
+    unsigned long long *maxd = nullptr; // per-iteration slot maxima of |dphi| (double bits)
+    double *maxc = nullptr;             // per-iteration max |dphi|, collected for read-back
     int maxd_cap = 0;
+    long long *partials = nullptr;      // per-block diagnostics partial sums (6 per block)
     SorCtl *ctl = nullptr;
     StatSlot *slots = nullptr;
     int slot_cap = 0;
@@ -57,6 +67,18 @@ struct hv_ctx {
     int sor_pred = 16;
     int excited = 0;
     int last_outside = 0;
+
+    // particle sort by (species, brick): storage order differs from the
+    // caller's order once sorted; pid maps storage slot -> original index
+    int *pid = nullptr, *pid_alt = nullptr;
+    void *part_alt[9] = {};
+    unsigned *keys[2] = {};
+    int *ord[2] = {};
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int sort_every = 8;  // steps between sorts; 0 = never
+    int since_sort = 1 << 30;
+    bool permuted = false;
 
     // kernel-class profiling
     bool prof = false;
@@ -116,6 +138,28 @@ Geom<T> make_geom(const hv_ctx *c) {
     g.two_dy = T(2.0) * g.dy;
     g.two_dz = T(2.0) * g.dz;
     return g;
+}
+
+Split make_split(const hv_world &w) {
+    Split s;
+    s.nx = w.n[0];
+    s.ny = w.n[1];
+    s.nz = w.n[2];
+    s.nzh = (w.n[2] + 1) / 2;
+    s.Gh = (long long)s.nx * s.ny * s.nzh;
+    return s;
+}
+
+void natural_to_split(const Split &s, const double *nat, double *spl) {
+    for (int i = 0; i < s.nx; i++)
+        for (int j = 0; j < s.ny; j++)
+            for (int k = 0; k < s.nz; k++) spl[split_index(s, i, j, k)] = nat[((long long)i * s.ny + j) * s.nz + k];
+}
+
+void split_to_natural(const Split &s, const double *spl, double *nat) {
+    for (int i = 0; i < s.nx; i++)
+        for (int j = 0; j < s.ny; j++)
+            for (int k = 0; k < s.nz; k++) nat[((long long)i * s.ny + j) * s.nz + k] = spl[split_index(s, i, j, k)];
 }
 
 template <typename T>
@@ -227,7 +271,25 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
     HV_HIP(c, hipMemcpyAsync(c->smear_tab, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HV_HIP(c, hipStreamSynchronize(c->stream));
     size_t n2 = (size_t)(2 * c->G);
-    int rc = ensure_buf(c, &c->H, &c->H_cap, (size_t)(tb.rz + 1) * n2 * sizeof(T));
+    // LDS tiles: TJ = 8 rows, TK the widest of 64/32/16 columns that fits 64 KB
+    SmearTiles stl;
+    stl.TJ = 8;
+    size_t lds = 0;
+    for (int tk = 64; tk >= 8; tk /= 2) {
+        stl.TK = tk;
+        const size_t R = stl.TJ + 2 * tb.ry, W = tk + 2 * tb.rz;
+        lds = sizeof(T) * (R * W + (size_t)(tb.rz + 1) * R * tk);
+        if (lds <= 65536) break;
+    }
+    if (lds > 65536) return set_err(c, HV_ERR_ARG, "smearing radius too large for the LDS tiles");
+    stl.njt = (c->world.n[1] + stl.TJ - 1) / stl.TJ;
+    stl.nkt = (c->world.n[2] + stl.TK - 1) / stl.TK;
+    c->stl = stl;
+    c->stl_lds = lds;
+    const size_t ntiles = (size_t)2 * c->world.n[0] * stl.njt * stl.nkt;
+    int rc = ensure_buf(c, (void **)&c->sflags, &c->sflags_cap, ntiles);
+    if (rc) return rc;
+    rc = ensure_buf(c, &c->H, &c->H_cap, sizeof(T));
     if (rc) return rc;
     rc = ensure_buf(c, &c->D, &c->D_cap, (size_t)(tb.rx + 1) * n2 * sizeof(T));
     if (rc) return rc;
@@ -247,27 +309,33 @@ int smear(hv_ctx *c, bool from_acc, double sigma_r, int ppn) {
     const SmearTab<T> *tb = (const SmearTab<T> *)c->smear_tab;
     T *H = (T *)c->H, *D = (T *)c->D, *rho = (T *)c->density;
     hipEvent_t pa = prof_begin(c);
+    (void)H;
+    const int nx = c->world.n[0], ny = c->world.n[1], nz = c->world.n[2];
+    const unsigned ntiles = (unsigned)(2 * nx * c->stl.njt * c->stl.nkt);
     if (from_acc)
-        k_smear_z<T, unsigned long long><<<blocks_for(n2), kThreads, 0, c->stream>>>(c->acc, H, tb, c->world.n[2], n2);
+        k_smear_zy<T, unsigned long long><<<ntiles, 256, c->stl_lds, c->stream>>>(c->acc, D, c->sflags, tb, nx, ny,
+                                                                                 nz, c->stl);
     else
-        k_smear_z<T, T><<<blocks_for(n2), kThreads, 0, c->stream>>>(rho, H, tb, c->world.n[2], n2);
+        k_smear_zy<T, T><<<ntiles, 256, c->stl_lds, c->stream>>>(rho, D, c->sflags, tb, nx, ny, nz, c->stl);
     HV_CHECK_LAUNCH(c);
-    k_smear_y<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(H, D, tb, c->world.n[1], c->world.n[2], n2);
-    HV_CHECK_LAUNCH(c);
-    k_smear_x<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, tb, c->world.n[0], c->world.n[1],
-                                                              c->world.n[2], n2);
+    k_smear_xs<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, c->sflags, tb, nx, ny, nz, c->stl);
     HV_CHECK_LAUNCH(c);
     prof_end(c, HV_KCLASS_SMEAR, pa, 1);
     return c->tab_warn;
 }
 
 // ---- SOR -------------------------------------------------------------------
+inline size_t maxd_words(int iters) { return (size_t)iters * kMaxSlots * kSlotStride; }
+
 int ensure_maxd(hv_ctx *c, int max_it) {
     if (c->maxd_cap >= max_it) return HV_OK;
     if (c->maxd) (void)hipFree(c->maxd);
+    if (c->maxc) (void)hipFree(c->maxc);
     c->maxd = nullptr;
+    c->maxc = nullptr;
     c->maxd_cap = 0;
-    HV_HIP(c, hipMalloc((void **)&c->maxd, sizeof(unsigned long long) * (size_t)max_it));
+    HV_HIP(c, hipMalloc((void **)&c->maxd, sizeof(unsigned long long) * maxd_words(max_it)));
+    HV_HIP(c, hipMalloc((void **)&c->maxc, sizeof(double) * (size_t)max_it));
     c->maxd_cap = max_it;
     return HV_OK;
 }
@@ -278,10 +346,13 @@ int ensure_maxd(hv_ctx *c, int max_it) {
 // return immediately (SorCtl), so the result is exactly the reference's
 // iterate: stop after the first iteration whose max |dphi| < tol.
 template <typename RT>
-int sor_solve(hv_ctx *c, const RT *rho, double omega, double tol, int max_it, int *iters_out,
+int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it, int *iters_out,
               double *maxdiff_out) {
     if (max_it <= 0) return set_err(c, HV_ERR_ARG, "sor max_it must be positive");
     const int nx = c->world.n[0], ny = c->world.n[1], nz = c->world.n[2];
+    RT *rho = (RT *)c->rho_s;
+    k_rho_split<RT><<<blocks_for(c->G), kThreads, 0, c->stream>>>(rho_nat, rho, c->G, c->sp);
+    HV_CHECK_LAUNCH(c);
     int rc = ensure_maxd(c, max_it);
     if (rc) return rc;
     rc = ensure_pinned(c, (size_t)max_it + 8);
@@ -296,12 +367,19 @@ int sor_solve(hv_ctx *c, const RT *rho, double omega, double tol, int max_it, in
     cf.omega = omega;
     cf.one_minus_omega = 1.0 - omega;
     cf.tol = tol;
-    rc = fill<unsigned long long>(c, c->maxd, 0ULL, max_it);
+    rc = fill<unsigned long long>(c, c->maxd, 0ULL, (long long)maxd_words(max_it));
     if (rc) return rc;
     rc = fill<int>(c, (int *)c->ctl, 0, 2);
     if (rc) return rc;
-    dim3 blk(64, 4, 1);
-    dim3 grd((unsigned)((nz - 2 + 63) / 64), (unsigned)((ny - 2 + 3) / 4), (unsigned)(nx - 2));
+    SorTiles tl;
+    tl.n_ic = (nx - 2 + kSorPlanes - 1) / kSorPlanes;
+    tl.n_jb = (ny - 2 + kSorRows - 1) / kSorRows;
+    tl.n_hb = (c->sp.nzh + 63) / 64;
+    const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
+    tl.per_xcd = (ntiles + kXcds - 1) / kXcds;
+    dim3 blk(64, kSorRows, 1);
+    dim3 grd((unsigned)(tl.per_xcd * kXcds), 1, 1);
+    (void)nz;
     int it = 0, iters = -1, start = 0;
     int chunk = c->sor_pred > 2 ? c->sor_pred - 1 : 1;
     while (it < max_it) {
@@ -309,15 +387,17 @@ int sor_solve(hv_ctx *c, const RT *rho, double omega, double tol, int max_it, in
         int end = it + chunk < max_it ? it + chunk : max_it;
         hipEvent_t pa = prof_begin(c), pb = nullptr;
         for (; it < end; it++) {
-            k_sor_colour<RT><<<grd, blk, 0, c->stream>>>(c->phi, rho, nx, ny, nz, cf, 0, it, c->maxd, c->ctl);
-            k_sor_colour<RT><<<grd, blk, 0, c->stream>>>(c->phi, rho, nx, ny, nz, cf, 1, it, c->maxd, c->ctl);
+            k_sor_colour<RT><<<grd, blk, 0, c->stream>>>(c->phi, rho, c->sp, tl, cf, 0, it, c->maxd, c->ctl);
+            k_sor_colour<RT><<<grd, blk, 0, c->stream>>>(c->phi, rho, c->sp, tl, cf, 1, it, c->maxd, c->ctl);
         }
         HV_CHECK_LAUNCH(c);
         if (pa) {
             pb = prof_event(c);
             if (pb) (void)hipEventRecord(pb, c->stream);
         }
-        HV_HIP(c, hipMemcpyAsync(c->pinned, c->maxd + start, sizeof(double) * (size_t)(end - start),
+        k_sor_collect<<<(unsigned)(end - start), 64, 0, c->stream>>>(c->maxd, c->maxc, start, end);
+        HV_CHECK_LAUNCH(c);
+        HV_HIP(c, hipMemcpyAsync(c->pinned, c->maxc, sizeof(double) * (size_t)(end - start),
                                  hipMemcpyDeviceToHost, c->stream));
         HV_HIP(c, hipStreamSynchronize(c->stream));
         for (int m = 0; m < end - start; m++) {
@@ -366,20 +446,101 @@ void slot_stats(const StatSlot &s, int np, int nn, double out[4]) {
     out[3] = slot_mean(s, 3, (long long)s.inbox[1]);
 }
 
+// diagnostics of the current positions into slots[slot]
+template <typename T>
+int stats_into(hv_ctx *c, int slot) {
+    const unsigned nb = blocks_for(c->total);
+    k_deposit<T><<<nb, kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total, make_geom<T>(c), nullptr,
+                                                 c->partials, c->status);
+    HV_CHECK_LAUNCH(c);
+    k_stats_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, c->slots + slot);
+    HV_CHECK_LAUNCH(c);
+    return HV_OK;
+}
+
 template <typename T>
 int stats_now(hv_ctx *c, double out[4]) {
     int rc = ensure_slots(c, 1);
     if (rc) return rc;
-    rc = fill<unsigned long long>(c, (unsigned long long *)c->slots, 0ULL, sizeof(StatSlot) / 8);
+    rc = stats_into<T>(c, 0);
     if (rc) return rc;
-    k_deposit<T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total, make_geom<T>(c),
-                                                                  nullptr, c->slots, c->status);
-    HV_CHECK_LAUNCH(c);
     StatSlot s;
     HV_HIP(c, hipMemcpyAsync(&s, c->slots, sizeof s, hipMemcpyDeviceToHost, c->stream));
     HV_HIP(c, hipStreamSynchronize(c->stream));
     slot_stats(s, c->np, c->nn, out);
     return HV_OK;
+}
+
+// ---- particle sort -------------------------------------------------------------
+template <typename T>
+Particles<T> particles_alt(const hv_ctx *c) {
+    Particles<T> p;
+    p.x = (T *)c->part_alt[0]; p.y = (T *)c->part_alt[1]; p.z = (T *)c->part_alt[2];
+    p.kx = (T *)c->part_alt[3]; p.ky = (T *)c->part_alt[4]; p.kz = (T *)c->part_alt[5];
+    p.fx = (T *)c->part_alt[6]; p.fy = (T *)c->part_alt[7]; p.fz = (T *)c->part_alt[8];
+    return p;
+}
+
+int sort_key_bits(const hv_ctx *c) {
+    const long long nb = (long long)((c->world.n[0] + 3) >> kBrickShift) * ((c->world.n[1] + 3) >> kBrickShift) *
+                         ((c->world.n[2] + 3) >> kBrickShift);
+    const long long kmax = 2 * (nb + 1);
+    int bits = 1;
+    while ((1LL << bits) < kmax) bits++;
+    return bits;
+}
+
+// Reorders the particle arrays by (species, brick of the cell), stable.
+template <typename T>
+int sort_particles(hv_ctx *c) {
+    const int n = c->total;
+    k_sort_keys<T><<<blocks_for(n), kThreads, 0, c->stream>>>(particles<T>(c), c->np, n, make_geom<T>(c), c->keys[0],
+                                                              c->ord[0]);
+    HV_CHECK_LAUNCH(c);
+    size_t bytes = c->sort_tmp_bytes;
+    HV_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp, bytes, c->keys[0], c->keys[1], c->ord[0], c->ord[1],
+                                        (unsigned)n, 0, sort_key_bits(c), c->stream));
+    k_permute<T><<<blocks_for(n), kThreads, 0, c->stream>>>(particles<T>(c), particles_alt<T>(c), c->pid,
+                                                            c->pid_alt, c->ord[1], n);
+    HV_CHECK_LAUNCH(c);
+    for (int q = 0; q < 9; q++) std::swap(c->part[q], c->part_alt[q]);
+    std::swap(c->pid, c->pid_alt);
+    c->permuted = true;
+    c->since_sort = 0;
+    return HV_OK;
+}
+
+// Puts the particle arrays back into the caller's order (pid = identity).
+int restore_order(hv_ctx *c) {
+    if (!c->permuted) return HV_OK;
+    const int n = c->total;
+    for (int q = 0; q < 9; q++) {
+        if (c->precision == HV_PREC_F64)
+            k_unpermute<double><<<blocks_for(n), kThreads, 0, c->stream>>>((const double *)c->part[q],
+                                                                          (double *)c->part_alt[q], c->pid, n);
+        else
+            k_unpermute<float><<<blocks_for(n), kThreads, 0, c->stream>>>((const float *)c->part[q],
+                                                                         (float *)c->part_alt[q], c->pid, n);
+        HV_CHECK_LAUNCH(c);
+        std::swap(c->part[q], c->part_alt[q]);
+    }
+    k_iota<<<blocks_for(n), kThreads, 0, c->stream>>>(c->pid, n);
+    HV_CHECK_LAUNCH(c);
+    c->permuted = false;
+    c->since_sort = 1 << 30;
+    return HV_OK;
+}
+
+template <typename T>
+int maybe_sort(hv_ctx *c) {
+    if (c->sort_every <= 0) return HV_OK;
+    if (c->since_sort < c->sort_every) {
+        c->since_sort++;
+        return HV_OK;
+    }
+    int rc = sort_particles<T>(c);
+    c->since_sort = 1;
+    return rc;
 }
 
 // ---- status word -------------------------------------------------------------
@@ -400,7 +561,7 @@ int potential_from_density(hv_ctx *c, const hv_skyrme *s, bool with_phi) {
     hipEvent_t pa = prof_begin(c);
     k_skyrme<T><<<blocks_for(c->G), kThreads, 0, c->stream>>>((const T *)c->density, (T *)c->U,
                                                              with_phi ? c->phi : nullptr, s->A, s->B, s->C,
-                                                             s->gamma, c->G);
+                                                             s->gamma, c->G, c->sp);
     HV_CHECK_LAUNCH(c);
     prof_end(c, HV_KCLASS_POTENTIAL, pa, 1);
     return HV_OK;
@@ -434,11 +595,17 @@ int run_init(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *sor_iters) 
     int rc = check_params(c, s, p);
     if (rc) return rc;
     // phi: zero interior, Dirichlet faces (src/hyper_vlasov.cpp:48,56)
-    std::vector<double> h((size_t)c->G, 0.0);
+    std::vector<double> h((size_t)c->G, 0.0), hs((size_t)(2 * c->sp.Gh), 0.0);
     hv_coulomb_faces(h.data(), c->world.n, c->world.d_max, p->z);
-    HV_HIP(c, hipMemcpyAsync(c->phi, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    natural_to_split(c->sp, h.data(), hs.data());
+    HV_HIP(c, hipMemcpyAsync(c->phi, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
     rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
     if (rc) return rc;
+    if (c->sort_every > 0) {
+        rc = sort_particles<T>(c);
+        if (rc) return rc;
+    }
     k_deposit<T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total, make_geom<T>(c),
                                                                   c->acc, nullptr, c->status);
     HV_CHECK_LAUNCH(c);
@@ -464,11 +631,8 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
     if (nsteps <= 0) return HV_OK;
     rc = ensure_slots(c, nsteps + 1);
     if (rc) return rc;
-    rc = fill<unsigned long long>(c, (unsigned long long *)c->slots, 0ULL,
-                                  (long long)(sizeof(StatSlot) / 8) * (nsteps + 1));
-    if (rc) return rc;
+
     const Geom<T> g = make_geom<T>(c);
-    const Particles<T> pp = particles<T>(c);
     const double dt = p->t_f / p->steps;
     const T kick_fac = (T)(dt / (2.0 * HV_HBARC));
     const T drift_fac = (T)((dt * HV_HBARC) / HV_MC2);
@@ -482,9 +646,9 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
         bn = p->eta_exc * zz / (zz + nnuc);
     }
     // diagnostics of the positions at the start of the first step
-    k_deposit<T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(pp, c->np, c->total, g, nullptr, c->slots,
-                                                                  c->status);
-    HV_CHECK_LAUNCH(c);
+    rc = stats_into<T>(c, 0);
+    if (rc) return rc;
+    const unsigned nbp = blocks_for(c->total);
     int warn = 0;
     for (int n = 0; n < nsteps; n++) {
         const int step = first + n;
@@ -493,10 +657,14 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
             c->excited = 1;
             boost = 1;
         }
+        rc = maybe_sort<T>(c);
+        if (rc) return rc;
         hipEvent_t pa = prof_begin(c);
-        k_particle_step<T, T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(
-            pp, c->np, c->total, g, (const T *)c->U, c->acc, kick_fac, drift_fac, n > 0 ? 1 : 0, boost, (T)bp,
-            (T)bn, c->slots + (n + 1), c->status);
+        k_particle_step<T, T><<<nbp, kThreads, 0, c->stream>>>(
+            particles<T>(c), c->np, c->total, g, (const T *)c->U, c->acc, kick_fac, drift_fac, n > 0 ? 1 : 0, boost, (T)bp,
+            (T)bn, c->partials, c->status);
+        HV_CHECK_LAUNCH(c);
+        k_stats_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nbp, c->slots + (n + 1));
         HV_CHECK_LAUNCH(c);
         prof_end(c, HV_KCLASS_PARTICLE, pa, 1);
         int it = 0;
@@ -505,8 +673,8 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
         if (sor_iters) sor_iters[n] = it;
     }
     // end-of-step half kick of the last step, with the force stored (:101-103)
-    k_gather_u<T, T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(pp, c->np, c->total, g, (const T *)c->U,
-                                                                      kick_fac, 1);
+    k_gather_u<T, T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total, g,
+                                                                      (const T *)c->U, kick_fac, 1);
     HV_CHECK_LAUNCH(c);
     if (stats) {
         std::vector<StatSlot> h((size_t)nsteps + 1);
@@ -599,16 +767,33 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
     c->nn = n_neutrons;
     c->total = n_protons + n_neutrons;
     c->G = (long long)world->n[0] * world->n[1] * world->n[2];
+    c->sp = make_split(*world);
     const size_t ts = tsize(c);
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     for (int q = 0; q < 9 && ok; q++) ok = hipMalloc(&c->part[q], ts * (size_t)c->total) == hipSuccess;
+    for (int q = 0; q < 9 && ok; q++) ok = hipMalloc(&c->part_alt[q], ts * (size_t)c->total) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->pid, sizeof(int) * (size_t)c->total) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->pid_alt, sizeof(int) * (size_t)c->total) == hipSuccess;
+    for (int q = 0; q < 2 && ok; q++) {
+        ok = hipMalloc((void **)&c->keys[q], sizeof(unsigned) * (size_t)c->total) == hipSuccess;
+        ok = ok && hipMalloc((void **)&c->ord[q], sizeof(int) * (size_t)c->total) == hipSuccess;
+    }
+    if (ok) {
+        size_t bytes = 0;
+        ok = rocprim::radix_sort_pairs(nullptr, bytes, c->keys[0], c->keys[1], c->ord[0], c->ord[1],
+                                       (unsigned)c->total, 0, sort_key_bits(c), c->stream) == hipSuccess;
+        ok = ok && hipMalloc(&c->sort_tmp, bytes + 256) == hipSuccess;
+        c->sort_tmp_bytes = bytes + 256;
+    }
     ok = ok && hipMalloc((void **)&c->acc, sizeof(unsigned long long) * (size_t)(2 * c->G)) == hipSuccess;
     ok = ok && hipMalloc(&c->density, ts * (size_t)(2 * c->G)) == hipSuccess;
     ok = ok && hipMalloc(&c->U, ts * (size_t)(2 * c->G)) == hipSuccess;
-    ok = ok && hipMalloc((void **)&c->phi, sizeof(double) * (size_t)c->G) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->phi, sizeof(double) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
+    ok = ok && hipMalloc(&c->rho_s, ts * (size_t)(2 * c->sp.Gh)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->ctl, sizeof(SorCtl)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->status, sizeof(unsigned)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->partials, sizeof(long long) * 6 * (size_t)blocks_for(c->total)) == hipSuccess;
     if (!ok) {
         hv_ctx_destroy(c);
         return HV_ERR_ALLOC;
@@ -618,9 +803,14 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
     ok = ok && hipMemsetAsync(c->acc, 0, sizeof(unsigned long long) * (size_t)(2 * c->G), c->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(c->density, 0, ts * (size_t)(2 * c->G), c->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(c->U, 0, ts * (size_t)(2 * c->G), c->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(c->phi, 0, sizeof(double) * (size_t)c->G, c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->phi, 0, sizeof(double) * (size_t)(2 * c->sp.Gh), c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->rho_s, 0, ts * (size_t)(2 * c->sp.Gh), c->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(c->ctl, 0, sizeof(SorCtl), c->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(c->status, 0, sizeof(unsigned), c->stream) == hipSuccess;
+    if (ok) {
+        k_iota<<<blocks_for(c->total), kThreads, 0, c->stream>>>(c->pid, c->total);
+        ok = hipGetLastError() == hipSuccess;
+    }
     ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
     if (!ok) {
         hv_ctx_destroy(c);
@@ -633,10 +823,15 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
 int hv_ctx_destroy(hv_ctx *c) {
     if (!c) return HV_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (int q = 0; q < 9; q++)
+    for (int q = 0; q < 9; q++) {
         if (c->part[q]) (void)hipFree(c->part[q]);
-    void *bufs[] = {c->acc, c->density, c->U, c->phi, c->H, c->D, c->F[0], c->F[1], c->F[2],
-                    c->smear_tab, c->maxd, c->ctl, c->slots, c->status};
+        if (c->part_alt[q]) (void)hipFree(c->part_alt[q]);
+    }
+    void *sbufs[] = {c->pid, c->pid_alt, c->keys[0], c->keys[1], c->ord[0], c->ord[1], c->sort_tmp};
+    for (void *b : sbufs)
+        if (b) (void)hipFree(b);
+    void *bufs[] = {c->acc, c->density, c->U, c->phi, c->rho_s, c->H, c->D, c->F[0], c->F[1], c->F[2],
+                    c->smear_tab, c->maxd, c->maxc, c->ctl, c->slots, c->status, c->partials, c->sflags};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->pinned) (void)hipHostFree(c->pinned);
@@ -669,6 +864,8 @@ int hv_upload_particles(hv_ctx *c, const double *x, const double *y, const doubl
                         const double *ky, const double *kz, const double *fx, const double *fy,
                         const double *fz) {
     if (!c) return HV_ERR_ARG;
+    int rc0 = restore_order(c);
+    if (rc0) return rc0;
     const double *src[9] = {x, y, z, kx, ky, kz, fx, fy, fz};
     for (int q = 0; q < 9; q++) {
         if (!src[q]) continue;
@@ -684,7 +881,18 @@ int hv_download_particles(hv_ctx *c, double *x, double *y, double *z, double *kx
     double *dst[9] = {x, y, z, kx, ky, kz, fx, fy, fz};
     for (int q = 0; q < 9; q++) {
         if (!dst[q]) continue;
-        int rc = copy_out(c, dst[q], c->part[q], c->total, c->precision == HV_PREC_F64);
+        const void *src = c->part[q];
+        if (c->permuted) {
+            if (c->precision == HV_PREC_F64)
+                k_unpermute<double><<<blocks_for(c->total), kThreads, 0, c->stream>>>(
+                    (const double *)c->part[q], (double *)c->part_alt[q], c->pid, c->total);
+            else
+                k_unpermute<float><<<blocks_for(c->total), kThreads, 0, c->stream>>>(
+                    (const float *)c->part[q], (float *)c->part_alt[q], c->pid, c->total);
+            HV_CHECK_LAUNCH(c);
+            src = c->part_alt[q];
+        }
+        int rc = copy_out(c, dst[q], src, c->total, c->precision == HV_PREC_F64);
         if (rc) return rc;
     }
     return HV_OK;
@@ -692,6 +900,13 @@ int hv_download_particles(hv_ctx *c, double *x, double *y, double *z, double *kx
 
 int hv_upload_field(hv_ctx *c, int which, const double *host) {
     if (!c || !host) return HV_ERR_ARG;
+    if (which == HV_FIELD_COULOMB) {
+        std::vector<double> hs((size_t)(2 * c->sp.Gh), 0.0);
+        natural_to_split(c->sp, host, hs.data());
+        HV_HIP(c, hipMemcpyAsync(c->phi, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, c->stream));
+        HV_HIP(c, hipStreamSynchronize(c->stream));
+        return HV_OK;
+    }
     void *p;
     long long n;
     bool f64;
@@ -702,6 +917,13 @@ int hv_upload_field(hv_ctx *c, int which, const double *host) {
 
 int hv_download_field(hv_ctx *c, int which, double *host) {
     if (!c || !host) return HV_ERR_ARG;
+    if (which == HV_FIELD_COULOMB) {
+        std::vector<double> hs((size_t)(2 * c->sp.Gh));
+        HV_HIP(c, hipMemcpyAsync(hs.data(), c->phi, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, c->stream));
+        HV_HIP(c, hipStreamSynchronize(c->stream));
+        split_to_natural(c->sp, hs.data(), host);
+        return HV_OK;
+    }
     void *p;
     long long n;
     bool f64;
@@ -748,12 +970,10 @@ int hv_skyrme_potentials(hv_ctx *c, const hv_skyrme *s) {
 int hv_set_coulomb_bc(hv_ctx *c, int z) {
     if (!c) return HV_ERR_ARG;
     std::vector<double> h((size_t)c->G);
-    HV_HIP(c, hipMemcpyAsync(h.data(), c->phi, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
-    HV_HIP(c, hipStreamSynchronize(c->stream));
+    int rc = hv_download_field(c, HV_FIELD_COULOMB, h.data());
+    if (rc) return rc;
     hv_coulomb_faces(h.data(), c->world.n, c->world.d_max, z);
-    HV_HIP(c, hipMemcpyAsync(c->phi, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
-    HV_HIP(c, hipStreamSynchronize(c->stream));
-    return HV_OK;
+    return hv_upload_field(c, HV_FIELD_COULOMB, h.data());
 }
 
 int hv_coulomb_sor(hv_ctx *c, double omega, double tol, int max_it, int *iters, double *max_diff) {
@@ -766,9 +986,9 @@ int hv_coulomb_sor(hv_ctx *c, double omega, double tol, int max_it, int *iters, 
 int hv_merge_potentials(hv_ctx *c) {
     if (!c) return HV_ERR_ARG;
     if (c->precision == HV_PREC_F64)
-        k_merge<double><<<blocks_for(c->G), kThreads, 0, c->stream>>>((double *)c->U, c->phi, c->G);
+        k_merge<double><<<blocks_for(c->G), kThreads, 0, c->stream>>>((double *)c->U, c->phi, c->G, c->sp);
     else
-        k_merge<float><<<blocks_for(c->G), kThreads, 0, c->stream>>>((float *)c->U, c->phi, c->G);
+        k_merge<float><<<blocks_for(c->G), kThreads, 0, c->stream>>>((float *)c->U, c->phi, c->G, c->sp);
     HV_CHECK_LAUNCH(c);
     return HV_OK;
 }
@@ -910,6 +1130,26 @@ int hv_timer_stop(hv_ctx *c, double *ms) {
     HV_HIP(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
     *ms = f;
     return HV_OK;
+}
+
+int hv_set_option(hv_ctx *c, int option, int value) {
+    if (!c) return HV_ERR_ARG;
+    switch (option) {
+    case HV_OPT_SORT_EVERY:
+        if (value < 0) return set_err(c, HV_ERR_ARG, "sort_every must be >= 0");
+        c->sort_every = value;
+        c->since_sort = 1 << 30;
+        return HV_OK;
+    default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
+    }
+}
+
+int hv_get_option(hv_ctx *c, int option, int *value) {
+    if (!c || !value) return HV_ERR_ARG;
+    switch (option) {
+    case HV_OPT_SORT_EVERY: *value = c->sort_every; return HV_OK;
+    default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
+    }
 }
 
 int hv_profile_enable(hv_ctx *c, int on) {

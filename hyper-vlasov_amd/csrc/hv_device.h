@@ -32,6 +32,19 @@ __device__ __forceinline__ long long idx3(int i, int j, int k, int ny, int nz) {
     return ((long long)i * ny + j) * nz + k;
 }
 
+// Colour-split storage of the Coulomb potential (and of the SOR source): the
+// red nodes ((i+j+k) even) and the black nodes live in two arrays, each row
+// (i, j) holding its nodes of one colour at consecutive slots k >> 1.  A red
+// sweep then reads the black array and the red array with unit stride.
+struct Split {
+    int nx, ny, nz, nzh;  // nzh = ceil(nz / 2)
+    long long Gh;         // nx * ny * nzh: offset of the black array
+};
+
+__host__ __device__ __forceinline__ long long split_index(const Split &s, int i, int j, int k) {
+    return (((i + j + k) & 1) ? s.Gh : 0) + ((long long)i * s.ny + j) * s.nzh + (k >> 1);
+}
+
 // CIC cell of a particle (include/tools.hpp:50-60 and :108-120).  Returns false
 // when the particle is outside [0, n) on any axis or non-finite.
 template <typename T>
@@ -167,6 +180,20 @@ __device__ __forceinline__ double wave_max_f64(double v) {
     }
     return v;
 }
+
+// Exact 128-bit accumulation in registers: (lo, hi) += v (sign-extended).
+__device__ __forceinline__ void add_i128(unsigned long long &lo, unsigned long long &hi, long long v) {
+    unsigned long long nlo = lo + (unsigned long long)v;
+    hi += ((v < 0) ? ~0ULL : 0ULL) + (nlo < lo ? 1ULL : 0ULL);
+    lo = nlo;
+}
+
+// Same-address device-scope atomics from thousands of blocks serialise at the
+// coherence point (8 XCDs share it), so block maxima are folded into one of
+// kMaxSlots addresses, each on its own 128-byte line; readers take the max of
+// the slots.
+constexpr int kMaxSlots = 64;
+constexpr int kSlotStride = 16;  // unsigned long longs (128 B)
 
 // max over the block of non-negative doubles, folded into *dst (bit pattern of
 // a non-negative double orders like the unsigned integer).

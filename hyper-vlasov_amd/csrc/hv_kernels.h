@@ -45,13 +45,46 @@ __device__ __forceinline__ void stat_values(T x, T y, T z, bool proton, const Ge
     v[5] = (!proton && inbox) ? 1 : 0;
 }
 
-__device__ __forceinline__ void stat_commit(long long (&v)[6], StatSlot *slot) {
+// Block partial sums of the six diagnostics, written (not accumulated) to
+// partials[6 * blockIdx.x + c]; k_stats_finalize folds them into a StatSlot.
+// Each block value is exact: |v| <= 256 * 2^54 < 2^63.
+__device__ __forceinline__ void stat_commit(long long (&v)[6], long long *partials) {
     block_sum_i64<6>(v);
     if (threadIdx.x == 0) {
-        for (int c = 0; c < 4; c++)
-            if (v[c]) atomic_add_i128(slot->s[c], v[c]);
-        if (v[4]) atomicAdd(&slot->inbox[0], (unsigned long long)v[4]);
-        if (v[5]) atomicAdd(&slot->inbox[1], (unsigned long long)v[5]);
+#pragma unroll
+        for (int c = 0; c < 6; c++) partials[6 * blockIdx.x + c] = v[c];
+    }
+}
+
+// One block: exact sum of nblocks rows of partials into *slot (128-bit sums;
+// integer addition, so the order does not matter).
+__global__ void __launch_bounds__(256)
+k_stats_finalize(const long long *__restrict__ partials, int nblocks, StatSlot *__restrict__ slot) {
+    __shared__ unsigned long long lo[6][256], hi[6][256];
+    unsigned long long l[6] = {0, 0, 0, 0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+        for (int c = 0; c < 6; c++) add_i128(l[c], h[c], partials[6 * b + c]);
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        lo[c][threadIdx.x] = l[c];
+        hi[c][threadIdx.x] = h[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int c = threadIdx.x;
+        unsigned long long L = 0, H = 0;
+        for (int t = 0; t < (int)blockDim.x; t++) {
+            unsigned long long nl = L + lo[c][t];
+            H += hi[c][t] + (nl < L ? 1ULL : 0ULL);
+            L = nl;
+        }
+        if (c < 4) {
+            slot->s[c][0] = L;
+            slot->s[c][1] = H;
+        } else {
+            slot->inbox[c - 4] = L;
+        }
     }
 }
 
@@ -66,6 +99,143 @@ __device__ __forceinline__ void deposit_particle(const Cell<T> &c, long long off
         long long q = hv_fix_weight((double)corner_weight(c, corner));
         atomicAdd(&acc[off + idx3(i, j, k, g.ny, g.nz)], (unsigned long long)q);
     }
+}
+
+// ---- block-local CIC deposit ----------------------------------------------------
+// When the particles of a block are sorted by cell (see k_sort_keys), their
+// cells fill a small box; the block then accumulates the box of nodes in LDS
+// (exact int64 adds) and flushes only the non-zero nodes to HBM, one global
+// atomic per node instead of eight per particle.  Blocks whose box is too big
+// or that hold both species fall back to per-particle global atomics.  All
+// adds are integer adds, so the result is the same bits either way.
+constexpr int kBoxCap = 1536;  // nodes (12 KB of int64)
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int s = kWave / 2; s > 0; s >>= 1) v = min(v, __shfl_xor(v, s, kWave));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int s = kWave / 2; s > 0; s >>= 1) v = max(v, __shfl_xor(v, s, kWave));
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ void block_deposit(bool has, const Cell<T> &c, int species, const Geom<T> &g,
+                                              unsigned long long *__restrict__ acc) {
+    __shared__ unsigned long long box[kBoxCap];
+    __shared__ int red[8][16];
+    __shared__ int bb[8];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    int v[8];
+    v[0] = has ? c.x0 : 0x7fffffff;
+    v[1] = has ? c.y0 : 0x7fffffff;
+    v[2] = has ? c.z0 : 0x7fffffff;
+    v[3] = has ? species : 0x7fffffff;
+    v[4] = has ? c.x0 : -1;
+    v[5] = has ? c.y0 : -1;
+    v[6] = has ? c.z0 : -1;
+    v[7] = has ? species : -1;
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = wave_min_i(v[q]);
+#pragma unroll
+    for (int q = 4; q < 8; q++) v[q] = wave_max_i(v[q]);
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 8; q++) red[q][wave] = v[q];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int q = threadIdx.x;
+        int r = red[q][0];
+        for (int w = 1; w < nw; w++) r = (q < 4) ? min(r, red[q][w]) : max(r, red[q][w]);
+        bb[q] = r;
+    }
+    __syncthreads();
+    const int x0 = bb[0], y0 = bb[1], z0 = bb[2];
+    const int ex = bb[4] - x0 + 2, ey = bb[5] - y0 + 2, ez = bb[6] - z0 + 2;
+    if (bb[4] < 0) return;  // no particle of this block is inside the grid
+    const long long off = bb[3] ? g.G : 0;
+    const bool lds = (bb[3] == bb[7]) && (long long)ex * ey * ez <= kBoxCap;
+    if (!lds) {
+        if (has) {
+#pragma unroll
+            for (int corner = 0; corner < 8; corner++) {
+                int i = c.x0 + (corner & 1), j = c.y0 + ((corner >> 1) & 1), k = c.z0 + ((corner >> 2) & 1);
+                if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
+                long long q = hv_fix_weight((double)corner_weight(c, corner));
+                atomicAdd(&acc[(species ? g.G : 0) + idx3(i, j, k, g.ny, g.nz)], (unsigned long long)q);
+            }
+        }
+        return;
+    }
+    const int vol = ex * ey * ez;
+    for (int e = threadIdx.x; e < vol; e += blockDim.x) box[e] = 0ULL;
+    __syncthreads();
+    if (has) {
+#pragma unroll
+        for (int corner = 0; corner < 8; corner++) {
+            int i = c.x0 + (corner & 1), j = c.y0 + ((corner >> 1) & 1), k = c.z0 + ((corner >> 2) & 1);
+            if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
+            long long q = hv_fix_weight((double)corner_weight(c, corner));
+            atomicAdd(&box[((i - x0) * ey + (j - y0)) * ez + (k - z0)], (unsigned long long)q);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < vol; e += blockDim.x) {
+        const unsigned long long val = box[e];
+        if (!val) continue;
+        const int i = x0 + e / (ey * ez), j = y0 + (e / ez) % ey, k = z0 + e % ez;
+        if (i < g.nx && j < g.ny && k < g.nz) atomicAdd(&acc[off + idx3(i, j, k, g.ny, g.nz)], val);
+    }
+}
+
+// ---- particle sort by (species, brick) ------------------------------------------
+// Key = species * (NB + 1) + brick of the particle's cell (bricks of
+// 4 x 4 x 4 cells, row-major), NB + 1 for particles outside the grid; protons
+// keep [0, np) because the species is the leading part of the key.
+constexpr int kBrickShift = 2;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_sort_keys(Particles<T> p, int np, int total, Geom<T> g, unsigned *__restrict__ keys, int *__restrict__ idx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int bx = (g.nx + 3) >> kBrickShift, by = (g.ny + 3) >> kBrickShift, bz = (g.nz + 3) >> kBrickShift;
+    const unsigned nb = (unsigned)(bx * by * bz);
+    Cell<T> c;
+    unsigned b = nb;
+    if (cic_cell(p.x[i], p.y[i], p.z[i], g, c))
+        b = (unsigned)(((c.x0 >> kBrickShift) * by + (c.y0 >> kBrickShift)) * bz + (c.z0 >> kBrickShift));
+    keys[i] = (i < np ? 0u : nb + 1u) + b;
+    idx[i] = i;
+}
+
+// gather the particle arrays (and ids) in sorted order into the other buffer set
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_permute(Particles<T> src, Particles<T> dst, const int *__restrict__ pid_src, int *__restrict__ pid_dst,
+          const int *__restrict__ order, int total) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= total) return;
+    const int o = order[s];
+    dst.x[s] = src.x[o]; dst.y[s] = src.y[o]; dst.z[s] = src.z[o];
+    dst.kx[s] = src.kx[o]; dst.ky[s] = src.ky[o]; dst.kz[s] = src.kz[o];
+    dst.fx[s] = src.fx[o]; dst.fy[s] = src.fy[o]; dst.fz[s] = src.fz[o];
+    pid_dst[s] = pid_src[o];
+}
+
+// storage order -> original particle order
+template <typename T>
+__global__ void k_unpermute(const T *__restrict__ src, T *__restrict__ dst, const int *__restrict__ pid, int total) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < total) dst[pid[s]] = src[s];
+}
+
+__global__ void k_iota(int *__restrict__ a, int n) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) a[s] = s;
 }
 
 // ---- fused particle pass --------------------------------------------------
@@ -83,15 +253,16 @@ template <typename T, typename UT>
 __global__ void __launch_bounds__(kBlock)
 k_particle_step(Particles<T> p, int np, int total, Geom<T> g, const UT *__restrict__ U,
                 unsigned long long *__restrict__ acc, T kick_fac, T drift_fac, int end_kick,
-                int boost, T boost_p, T boost_n, StatSlot *slot, unsigned *status) {
+                int boost, T boost_p, T boost_n, long long *partials, unsigned *status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     long long v[6] = {0, 0, 0, 0, 0, 0};
+    Cell<T> c;
+    bool has = false;
     if (i < total) {
         const bool proton = i < np;
         const long long off = proton ? 0 : g.G;
         T x = p.x[i], y = p.y[i], z = p.z[i];
         T kx = p.kx[i], ky = p.ky[i], kz = p.kz[i];
-        Cell<T> c;
         T fx = T(0.0), fy = T(0.0), fz = T(0.0);
         if (cic_cell(x, y, z, g, c)) gather_force_from_u<T, UT>(U + off, c, g, fx, fy, fz);
         if (end_kick) {
@@ -111,31 +282,32 @@ k_particle_step(Particles<T> p, int np, int total, Geom<T> g, const UT *__restri
         z += drift_fac * kz;
         p.x[i] = x; p.y[i] = y; p.z[i] = z;
         p.kx[i] = kx; p.ky[i] = ky; p.kz[i] = kz;
-        if (cic_cell(x, y, z, g, c)) deposit_particle(c, off, g, acc);
-        else atomicOr(status, kStatusOutside);
+        has = cic_cell(x, y, z, g, c);
+        if (!has) atomicOr(status, kStatusOutside);
         stat_values(x, y, z, proton, g, v);
     }
-    stat_commit(v, slot);
+    block_deposit(has, c, i < np ? 0 : 1, g, acc);
+    stat_commit(v, partials);
 }
 
 // deposit (and optionally diagnostics) of the current positions, no motion
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_deposit(Particles<T> p, int np, int total, Geom<T> g, unsigned long long *__restrict__ acc,
-          StatSlot *slot, unsigned *status) {
+          long long *partials, unsigned *status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     long long v[6] = {0, 0, 0, 0, 0, 0};
+    Cell<T> c;
+    bool has = false;
     if (i < total) {
         const bool proton = i < np;
         T x = p.x[i], y = p.y[i], z = p.z[i];
-        Cell<T> c;
-        if (acc) {
-            if (cic_cell(x, y, z, g, c)) deposit_particle(c, proton ? 0 : g.G, g, acc);
-            else atomicOr(status, kStatusOutside);
-        }
+        has = cic_cell(x, y, z, g, c);
+        if (acc && !has) atomicOr(status, kStatusOutside);
         stat_values(x, y, z, proton, g, v);
     }
-    if (slot) stat_commit(v, slot);
+    if (acc) block_deposit(has, c, i < np ? 0 : 1, g, acc);
+    if (partials) stat_commit(v, partials);
 }
 
 // gather F(x) from U into p.f, optionally followed by the half kick
@@ -253,11 +425,115 @@ k_smear_x(const T *__restrict__ D, T *__restrict__ rho, const SmearTab<T> *__res
     rho[q] = acc * tb->term;
 }
 
+// ---- tiled smearing: z and y factors fused in LDS, sparse x pass --------------
+// Same factor passes and the same operation order as k_smear_z/y/x above
+// (bit-identical results), organised for the hardware:
+//   k_smear_zy: one block per (species, plane i, row tile of TJ rows, column
+//     tile of TK columns).  It stages the counts of the tile plus its
+//     (ry rows, rz columns) halo in LDS, builds every H_K of the (TJ + 2 ry) x
+//     TK columns in LDS, and writes the D_a channels of its TJ x TK nodes.  A
+//     tile whose halo holds no count writes nothing and flags itself empty:
+//     its H and D are exactly zero.
+//   k_smear_xs: the x factor, skipping planes whose tile is flagged empty
+//     (their terms are exact zeros, so skipping them changes no bit).
+struct SmearTiles {
+    int TJ, TK;     // tile rows, tile columns
+    int njt, nkt;   // tiles per plane along j and k
+};
+
+template <typename T, typename CT>
+__global__ void __launch_bounds__(256)
+k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned char *__restrict__ flags,
+           const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
+    extern __shared__ unsigned char smem[];
+    const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
+    const int TJ = st.TJ, TK = st.TK;
+    const int R = TJ + 2 * ry, W = TK + 2 * rz;
+    T *C = (T *)smem;          // R x W counts
+    T *Hs = C + R * W;         // (rz + 1) x R x TK
+    int t = blockIdx.x;
+    const int kt = t % st.nkt;
+    t /= st.nkt;
+    const int jt = t % st.njt;
+    t /= st.njt;
+    const int i = t % nx;
+    const int sp = t / nx;
+    const long long G = (long long)nx * ny * nz, n2 = 2 * G;
+    const long long base = sp * G + (long long)i * ny * nz;
+    const int j0 = jt * TJ, k0 = kt * TK;
+    int nonzero = 0;
+    for (int e = threadIdx.x; e < R * W; e += blockDim.x) {
+        const int r = e / W, cc = e - r * W;
+        const int j = j0 - ry + r, k = k0 - rz + cc;
+        T v = T(0.0);
+        if (j >= 0 && j < ny && k >= 0 && k < nz) v = count_value<T, CT>(cnt, base + (long long)j * nz + k);
+        C[e] = v;
+        nonzero |= (v != T(0.0));
+    }
+    const long long tile = blockIdx.x;
+    if (!__syncthreads_or(nonzero)) {
+        if (threadIdx.x == 0) flags[tile] = 0;
+        return;
+    }
+    for (int e = threadIdx.x; e < R * TK; e += blockDim.x) {
+        const int r = e / TK, cc = e - r * TK;
+        const T *cp = C + r * W + cc + rz;
+        T h = tb->gz[0] * cp[0];
+        Hs[r * TK + cc] = h;
+        for (int K = 1; K <= rz; K++) {
+            h = (h + tb->gz[K] * cp[-K]) + tb->gz[K] * cp[K];
+            Hs[(K * R + r) * TK + cc] = h;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < TJ * TK; e += blockDim.x) {
+        const int r = e / TK, cc = e - r * TK;
+        const int j = j0 + r, k = k0 + cc;
+        if (j >= ny || k >= nz) continue;
+        const long long q = base + (long long)j * nz + k;
+        for (int a = 0; a <= rx; a++) {
+            const int kb = tb->kb[a];
+            T acc = T(0.0);
+            for (int b = -kb; b <= kb; b++) {
+                const int ab = b < 0 ? -b : b;
+                acc = acc + tb->gy[ab] * Hs[(tb->kc[a][ab] * R + (r + ry + b)) * TK + cc];
+            }
+            D[(long long)a * n2 + q] = acc;
+        }
+    }
+    if (threadIdx.x == 0) flags[tile] = 1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_smear_xs(const T *__restrict__ D, T *__restrict__ rho, const unsigned char *__restrict__ flags,
+           const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
+    const long long G = (long long)nx * ny * nz, n2 = 2 * G;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n2) return;
+    const long long plane = (long long)ny * nz;
+    const int sp = q >= G ? 1 : 0;
+    const long long node = q - sp * G;
+    const int i = (int)(node / plane), j = (int)((node / nz) % ny), k = (int)(node % nz);
+    const long long trow = ((long long)sp * nx) * st.njt * st.nkt + (long long)(j / st.TJ) * st.nkt + k / st.TK;
+    const long long tstride = (long long)st.njt * st.nkt;
+    T acc = T(0.0);
+    for (int a = -tb->rx; a <= tb->rx; a++) {
+        const int aa = a < 0 ? -a : a;
+        if (tb->kb[aa] < 0) continue;
+        const int ia = i + a;
+        if (ia < 0 || ia >= nx) continue;
+        if (!flags[trow + ia * tstride]) continue;
+        acc = acc + tb->gx[aa] * D[(long long)aa * n2 + q + (long long)a * plane];
+    }
+    rho[q] = acc * tb->term;
+}
+
 // ---- Skyrme potentials (include/physics.hpp:273-281), optionally + phi ------
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_skyrme(const T *__restrict__ rho, T *__restrict__ U, const double *__restrict__ phi, double A, double B,
-         double C, double gamma, long long G) {
+         double C, double gamma, long long G, Split sp) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G) return;
     double rp = (double)rho[i], rn = (double)rho[i + G];
@@ -266,7 +542,8 @@ k_skyrme(const T *__restrict__ rho, T *__restrict__ U, const double *__restrict_
     if (phi) {
         // merge_volumetric_potentials (include/tools.hpp:378-384), in the
         // grid precision like the separate pass
-        U[i] = (T)up + (T)phi[i];
+        const int k = (int)(i % sp.nz), j = (int)((i / sp.nz) % sp.ny), ii = (int)(i / ((long long)sp.ny * sp.nz));
+        U[i] = (T)up + (T)phi[split_index(sp, ii, j, k)];
     } else {
         U[i] = (T)up;
     }
@@ -275,9 +552,20 @@ k_skyrme(const T *__restrict__ rho, T *__restrict__ U, const double *__restrict_
 
 // U_p += phi (include/tools.hpp:378-384)
 template <typename T>
-__global__ void k_merge(T *__restrict__ U, const double *__restrict__ phi, long long G) {
+__global__ void k_merge(T *__restrict__ U, const double *__restrict__ phi, long long G, Split sp) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < G) U[i] += (T)phi[i];
+    if (i >= G) return;
+    const int k = (int)(i % sp.nz), j = (int)((i / sp.nz) % sp.ny), ii = (int)(i / ((long long)sp.ny * sp.nz));
+    U[i] += (T)phi[split_index(sp, ii, j, k)];
+}
+
+// proton density (first G values of rho, natural order) -> colour-split copy
+template <typename T>
+__global__ void k_rho_split(const T *__restrict__ rho, T *__restrict__ rs, long long G, Split sp) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) return;
+    const int k = (int)(i % sp.nz), j = (int)((i / sp.nz) % sp.ny), ii = (int)(i / ((long long)sp.ny * sp.nz));
+    rs[split_index(sp, ii, j, k)] = rho[i];
 }
 
 // F = -grad U on the grid (include/physics.hpp:160-200)
@@ -366,42 +654,93 @@ struct SorCtl {
     int iters;
 };
 
-// One colour of one iteration, in place.  Block (64, 4): x runs along k.
-// Iteration `it` first looks at the previous iteration's max |dphi|: once it
-// is below tol the solve is over and every later launch returns at once.
+// Max |dphi| of iteration it: max over the kMaxSlots slots of row it.
+// Called by whole waves: lane l reads slot l (kMaxSlots == wave width).
+__device__ __forceinline__ double sor_row_max(const unsigned long long *__restrict__ maxd, int it) {
+    const int lane = threadIdx.x & (kWave - 1);
+    double v = __longlong_as_double((long long)maxd[((long long)it * kMaxSlots + lane) * kSlotStride]);
+    return wave_max_f64(v);
+}
+
+// One colour of one iteration, in place, on the colour-split arrays
+// (include/physics.hpp:106-151).  Each block is 64 k-slots x 4 rows and walks
+// kSorPlanes planes.  Blocks are dispatched round-robin over the 8 XCDs, so
+// block b runs on XCD b % 8; tiles are numbered so that each XCD always gets the
+// same contiguous slab of planes, keeping its slab in that XCD's L2 from one
+// sweep to the next.  Iteration `it` first checks iteration it-1's max |dphi|:
+// once it is below tol the solve is over, and this and every later launch
+// return at once (iteration count = the reference's stop rule).
+constexpr int kSorPlanes = 2;
+constexpr int kSorRows = 4;
+constexpr int kXcds = 8;
+
+struct SorTiles {
+    int n_ic, n_jb, n_hb;  // plane chunks, row blocks, k-slot blocks
+    int per_xcd;           // tiles per XCD (grid = kXcds * per_xcd blocks)
+};
+
 template <typename RT>
 __global__ void __launch_bounds__(256)
-k_sor_colour(double *__restrict__ phi, const RT *__restrict__ rho, int nx, int ny, int nz, SorCoef cf,
+k_sor_colour(double *__restrict__ phi, const RT *__restrict__ rho, Split sp, SorTiles tl, SorCoef cf,
              int colour, int it, unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
     if (ctl->done) return;
     if (colour == 0 && it > 0) {
-        double prev = __longlong_as_double((long long)maxd[it - 1]);
+        const double prev = sor_row_max(maxd, it - 1);
         if (prev < cf.tol) {
-            if (threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) {
+            if (threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x == 0) {
                 ctl->done = 1;
                 ctl->iters = it;
             }
             return;
         }
     }
-    const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = 1 + blockIdx.z;
+    const int b = blockIdx.x;
+    const int t = (b % kXcds) * tl.per_xcd + b / kXcds;
+    const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
     double diff = 0.0;
-    if (k < nz - 1 && j < ny - 1 && ((i + j + k) & 1) == colour) {
-        const long long idx = idx3(i, j, k, ny, nz);
-        const long long sx = (long long)ny * nz;
-        double r = (double)rho[idx];
-        double phi_x = (phi[idx + sx] + phi[idx - sx]) / cf.dx2;
-        double phi_y = (phi[idx + nz] + phi[idx - nz]) / cf.dy2;
-        double phi_z = (phi[idx + 1] + phi[idx - 1]) / cf.dz2;
-        double phi_star = (phi_x + phi_y + phi_z + cf.src * r) * cf.inv_dx2;
-        double phi_old = phi[idx];
-        double nv = cf.one_minus_omega * phi_old + cf.omega * phi_star;
-        phi[idx] = nv;
-        diff = fabs(nv - phi_old);
+    if (t < ntiles) {
+        const int ic = t / (tl.n_jb * tl.n_hb);
+        const int rem = t - ic * tl.n_jb * tl.n_hb;
+        const int jb = rem / tl.n_hb, hb = rem - jb * tl.n_hb;
+        const int j = 1 + jb * kSorRows + threadIdx.y;
+        const int h = hb * 64 + threadIdx.x;
+        const int ny = sp.ny, nz = sp.nz, nzh = sp.nzh;
+        double *dst = phi + (colour ? sp.Gh : 0);
+        const double *oth = phi + (colour ? 0 : sp.Gh);
+        const RT *src = rho + (colour ? sp.Gh : 0);
+        if (j < ny - 1) {
+            for (int di = 0; di < kSorPlanes; di++) {
+                const int i = 1 + ic * kSorPlanes + di;
+                if (i >= sp.nx - 1) break;
+                const int off = (i + j + colour) & 1;
+                const int k = 2 * h + off;
+                if (k < 1 || k > nz - 2) continue;
+                const long long row = ((long long)i * ny + j) * nzh;
+                const long long q = row + h;
+                const long long sx = (long long)ny * nzh;
+                double r = (double)src[q];
+                double phi_x = (oth[q + sx] + oth[q - sx]) / cf.dx2;
+                double phi_y = (oth[q + nzh] + oth[q - nzh]) / cf.dy2;
+                double phi_z = (oth[q + off] + oth[q + off - 1]) / cf.dz2;
+                double phi_star = (phi_x + phi_y + phi_z + cf.src * r) * cf.inv_dx2;
+                double phi_old = dst[q];
+                double nv = cf.one_minus_omega * phi_old + cf.omega * phi_star;
+                dst[q] = nv;
+                double d = fabs(nv - phi_old);
+                diff = d > diff ? d : diff;
+            }
+        }
     }
-    block_max_atomic(diff, &maxd[it]);
+    block_max_atomic(diff, &maxd[((long long)it * kMaxSlots + (b % kMaxSlots)) * kSlotStride]);
+}
+
+// Folds the slot maxima of iterations [first, last) into out[0..last-first).
+__global__ void k_sor_collect(const unsigned long long *__restrict__ maxd, double *__restrict__ out, int first,
+                              int last) {
+    const int it = first + blockIdx.x;
+    if (it >= last) return;
+    const double m = sor_row_max(maxd, it);
+    if (threadIdx.x == 0) out[blockIdx.x] = m;
 }
 
 }  // namespace hv

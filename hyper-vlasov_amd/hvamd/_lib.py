@@ -40,6 +40,8 @@ HV_FIELD_FORCES_X = 3
 HV_FIELD_FORCES_Y = 4
 HV_FIELD_FORCES_Z = 5
 
+HV_OPT_SORT_EVERY = 0
+
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1
 HV_KCLASS_SMEAR = 2
@@ -120,6 +122,8 @@ SIGNATURES = {
     "hv_timer_stop": (_I, [_P, _DP]),
     "hv_build_info": (ctypes.c_char_p, []),
     "hv_profile_enable": (_I, [_P, _I]),
+    "hv_set_option": (_I, [_P, _I, _I]),
+    "hv_get_option": (_I, [_P, _I, _IP]),
     "hv_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
     "hv_fermi_momentum": (_D, [_D]),
     "hv_sample_uniform_spheres": (_I, [ctypes.c_ulonglong, _I, _I, _D, _D] + [_P] * 6),

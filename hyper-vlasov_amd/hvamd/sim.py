@@ -272,6 +272,14 @@ class Simulation:
     def sync(self):
         self._rc("hv_sync", self.lib.hv_sync(self.ctx))
 
+    def set_option(self, option: int, value: int):
+        self._rc("hv_set_option", self.lib.hv_set_option(self.ctx, option, value))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int()
+        self._rc("hv_get_option", self.lib.hv_get_option(self.ctx, option, ctypes.byref(v)))
+        return v.value
+
     def profile_enable(self, on: bool = True):
         self._rc("hv_profile_enable", self.lib.hv_profile_enable(self.ctx, 1 if on else 0))
 

@@ -175,6 +175,15 @@ int hv_run_steps(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, 
 int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, double *stats,
                 int *sor_iters);
 
+/* ---- tuning options --------------------------------------------------------- */
+/* HV_OPT_SORT_EVERY: steps between re-sorts of the particle arrays by
+ * (species, 4^3-cell brick) (default 8; 0 disables sorting).  Sorting never
+ * changes results -- deposits and diagnostics are exact integer sums and
+ * hv_download_particles returns the caller's order -- only speed. */
+#define HV_OPT_SORT_EVERY 0
+int hv_set_option(hv_ctx *ctx, int option, int value);
+int hv_get_option(hv_ctx *ctx, int option, int *value);
+
 /* ---- timing / introspection ---------------------------------------------- */
 /* Milliseconds of device time between the last pair of hv_timer_start /
  * hv_timer_stop calls (hipEvents on the context stream). */
