@@ -19,6 +19,7 @@
 
 #include "hv_abi.h"
 #include "hv_kernels.h"
+#include "hv_sor_resident.h"
 
 #define HV_XSTR(x) HV_STR(x)
 #define HV_STR(x) #x
@@ -77,6 +78,18 @@ struct hv_ctx {
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int sort_every = 8;  // steps between sorts; 0 = never
+    // LDS-resident SOR (hv_sor_resident.h)
+    int sor_resident = 0;     // option: use it when the grid fits (off by default: slower than per-sweep launches at 64^3..128^3)
+    int res_state = 0;        // 0 unknown, 1 available, -1 not available
+    ResTiles rtl{};
+    size_t res_lds = 0;
+    double *res_x = nullptr;  // exchange buffers (2 colours)
+    int *res_flag = nullptr;
+    double *res_slots = nullptr;
+    int res_slots_cap = 0;
+    int *res_err = nullptr;
+    double *res_result = nullptr;
+    unsigned long long *res_trace = nullptr;  // phase timestamps (HV_RES_TRACE builds only)
     int since_sort = 1 << 30;
     bool permuted = false;
 
@@ -340,6 +353,129 @@ int ensure_maxd(hv_ctx *c, int max_it) {
     return HV_OK;
 }
 
+// ---- LDS-resident SOR --------------------------------------------------------
+template <typename RT>
+size_t res_lds_bytes(const ResTiles &t) {
+    const size_t tsz = (size_t)t.TI * t.TJ * t.nzh;
+    return sizeof(double) * (2 * tsz + 2 * (size_t)t.side_i + 2 * (size_t)t.side_j + 32) + sizeof(RT) * 2 * tsz;
+}
+
+// Picks the tiling and checks that every tile block can be resident at once.
+template <typename RT>
+int res_setup(hv_ctx *c) {
+    if (c->res_state) return c->res_state > 0 ? HV_OK : 1;
+    c->res_state = -1;
+    int dev = 0, ncu = 0, maxlds = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 1;
+    int coop = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return 1;
+    const int nx = c->world.n[0], ny = c->world.n[1];
+    const long long rows = (long long)nx * ny;
+    long long target = rows / 32;
+    if (target < 1) target = 1;
+    if (target > ncu) target = ncu;
+    if (target > kSlotPoll * kWave) target = kSlotPoll * kWave;
+    ResTiles best{};
+    long long best_cost = -1;
+    for (int PI = 1; PI <= nx && PI <= target; PI++) {
+        int PJ = (int)(target / PI);
+        if (PJ < 1) break;
+        if (PJ > ny) PJ = ny;
+        ResTiles t;
+        t.TI = (nx + PI - 1) / PI;
+        t.TJ = (ny + PJ - 1) / PJ;
+        t.PI = (nx + t.TI - 1) / t.TI;
+        t.PJ = (ny + t.TJ - 1) / t.TJ;
+        t.nzh = c->sp.nzh;
+        t.side_i = t.TJ * t.nzh;
+        t.side_j = t.TI * t.nzh;
+        t.xstride = 2 * t.side_i + 2 * t.side_j;
+        if (res_lds_bytes<RT>(t) > (size_t)maxlds) continue;
+        const long long cost = (long long)t.TI * t.TJ * 4 + t.TI + t.TJ;
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = t;
+        }
+    }
+    if (best_cost < 0) return 1;
+    const size_t lds = res_lds_bytes<RT>(best);
+    if (hipFuncSetAttribute((const void *)k_sor_resident<RT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess)
+        return 1;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sor_resident<RT>, kResThreads, lds) != hipSuccess ||
+        per_cu < 1 || (long long)best.PI * best.PJ > (long long)per_cu * ncu)
+        return 1;
+    const size_t nT = (size_t)best.PI * best.PJ;
+    if (hipMalloc((void **)&c->res_x, sizeof(double) * 2 * nT * best.xstride) != hipSuccess) return 1;
+    if (hipMalloc((void **)&c->res_flag, sizeof(int) * nT) != hipSuccess) return 1;
+    if (hipMalloc((void **)&c->res_err, sizeof(int) * 4) != hipSuccess) return 1;
+    if (hipMalloc((void **)&c->res_result, sizeof(double) * 4) != hipSuccess) return 1;
+    c->rtl = best;
+    c->res_lds = lds;
+    c->res_state = 1;
+    return HV_OK;
+}
+
+template <typename RT>
+int sor_resident(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, int *iters_out, double *md_out) {
+    const int nT = c->rtl.PI * c->rtl.PJ;
+    if (c->res_slots_cap < max_it) {
+        if (c->res_slots) (void)hipFree(c->res_slots);
+        c->res_slots = nullptr;
+        c->res_slots_cap = 0;
+        HV_HIP(c, hipMalloc((void **)&c->res_slots, sizeof(double) * (size_t)max_it * nT));
+        c->res_slots_cap = max_it;
+    }
+    int rc = fill<double>(c, c->res_slots, -1.0, (long long)max_it * nT);
+    if (rc) return rc;
+    rc = fill<int>(c, c->res_flag, 0, nT);
+    if (rc) return rc;
+    rc = fill<int>(c, c->res_err, 0, 4);
+    if (rc) return rc;
+    ResSync sy;
+#ifdef HV_RES_TRACE
+    static unsigned long long *trace = nullptr;
+    if (!trace) (void)hipMalloc((void **)&trace, 16 * 16 * sizeof(unsigned long long));
+    sy.trace = trace;
+    c->res_trace = trace;
+#endif
+    sy.xr = c->res_x;
+    sy.xb = c->res_x + (size_t)nT * c->rtl.xstride;
+    sy.flag_red = c->res_flag;
+    sy.slots = c->res_slots;
+    sy.err = c->res_err;
+    sy.result = c->res_result;
+    double *phi = c->phi;
+    Split sp = c->sp;
+    ResTiles tl = c->rtl;
+    SorCoef cfv = cf;
+    int mi = max_it;
+    void *args[] = {&phi, (void *)&rho, &sp, &tl, &cfv, &mi, &sy};
+    hipEvent_t pa = prof_begin(c);
+    HV_HIP(c, hipLaunchCooperativeKernel((const void *)k_sor_resident<RT>, dim3((unsigned)nT), dim3(kResThreads), args,
+                                         (unsigned)c->res_lds, c->stream));
+    hipEvent_t pb = nullptr;
+    if (pa) {
+        pb = prof_event(c);
+        if (pb) (void)hipEventRecord(pb, c->stream);
+    }
+    double res[2];
+    int err = 0;
+    HV_HIP(c, hipMemcpyAsync(res, c->res_result, sizeof res, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(c, hipMemcpyAsync(&err, c->res_err, sizeof err, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    if (err) return set_err(c, HV_ERR_HIP, "LDS-resident SOR: a tile wait exceeded its spin limit");
+    const int iters = (int)res[0];
+    if (pa && pb) c->pending.push_back({HV_KCLASS_SOR, pa, pb, 2LL * iters});
+    prof_flush(c);
+    *iters_out = iters;
+    *md_out = res[1];
+    return HV_OK;
+}
+
 // Red-black SOR on c->phi with the proton density (first G values of `rho`).
 // The host launches iterations in chunks and reads back each chunk's
 // per-iteration max |dphi|; kernels launched past the converged iteration
@@ -362,11 +498,24 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
     cf.dx2 = dx * dx;
     cf.dy2 = dy * dy;
     cf.dz2 = dz * dz;
+    cf.ydx2 = 1.0 / cf.dx2;
+    cf.ydy2 = 1.0 / cf.dy2;
+    cf.ydz2 = 1.0 / cf.dz2;
     cf.inv_dx2 = 1.0 / (2.0 / (dx * dx) + 2.0 / (dy * dy) + 2.0 / (dz * dz));
     cf.src = 4.0 * HV_PI * HV_E2;
     cf.omega = omega;
     cf.one_minus_omega = 1.0 - omega;
     cf.tol = tol;
+    if (c->sor_resident && res_setup<RT>(c) == HV_OK) {
+        int iters = 0;
+        double md = 0.0;
+        rc = sor_resident<RT>(c, rho, cf, max_it, &iters, &md);
+        if (rc) return rc;
+        c->sor_pred = iters;
+        if (iters_out) *iters_out = iters;
+        if (maxdiff_out) *maxdiff_out = md;
+        return (md > tol) ? HV_WARN_SOR_NOT_CONVERGED : HV_OK;
+    }
     rc = fill<unsigned long long>(c, c->maxd, 0ULL, (long long)maxd_words(max_it));
     if (rc) return rc;
     rc = fill<int>(c, (int *)c->ctl, 0, 2);
@@ -827,7 +976,8 @@ int hv_ctx_destroy(hv_ctx *c) {
         if (c->part[q]) (void)hipFree(c->part[q]);
         if (c->part_alt[q]) (void)hipFree(c->part_alt[q]);
     }
-    void *sbufs[] = {c->pid, c->pid_alt, c->keys[0], c->keys[1], c->ord[0], c->ord[1], c->sort_tmp};
+    void *sbufs[] = {c->pid, c->pid_alt, c->keys[0], c->keys[1], c->ord[0], c->ord[1], c->sort_tmp,
+                     c->res_x, c->res_flag, c->res_slots, c->res_err, c->res_result};
     for (void *b : sbufs)
         if (b) (void)hipFree(b);
     void *bufs[] = {c->acc, c->density, c->U, c->phi, c->rho_s, c->H, c->D, c->F[0], c->F[1], c->F[2],
@@ -1140,6 +1290,9 @@ int hv_set_option(hv_ctx *c, int option, int value) {
         c->sort_every = value;
         c->since_sort = 1 << 30;
         return HV_OK;
+    case HV_OPT_SOR_RESIDENT:
+        c->sor_resident = value ? 1 : 0;
+        return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -1148,6 +1301,7 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
     if (!c || !value) return HV_ERR_ARG;
     switch (option) {
     case HV_OPT_SORT_EVERY: *value = c->sort_every; return HV_OK;
+    case HV_OPT_SOR_RESIDENT: *value = c->sor_resident && c->res_state >= 0; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -1172,6 +1326,15 @@ int hv_profile_read(hv_ctx *c, int kclass, double *ms, long long *launches) {
     if (launches) *launches = c->prof_launches[kclass];
     return HV_OK;
 }
+
+#ifdef HV_RES_TRACE
+// trace builds only: the phase timestamps of the last resident SOR solve
+int hv_debug_trace(hv_ctx *c, unsigned long long *host256) {
+    if (!c || !c->res_trace) return HV_ERR_STATE;
+    HV_HIP(c, hipMemcpy(host256, c->res_trace, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    return HV_OK;
+}
+#endif
 
 const char *hv_build_info(void) {
     return "libhvamd gfx950 -O3 -ffp-contract=off (HIP " HIP_VERSION_STR ")";

@@ -190,6 +190,20 @@ HV_HD HV_INLINE double hv_det_pow(double x, double y) {
     return hv_det_exp(y * hv_det_log(x));
 }
 
+/* ---- correctly rounded division by a fixed divisor ------------------------- */
+/* a / b for a divisor b known in advance, with y = 1.0 / b (correctly rounded,
+ * computed once): q = RN(a y) is within one ulp of a / b, r = a - q b is exact
+ * (one fma), and RN(q + r y) is the correctly rounded quotient (Markstein's
+ * theorem, for results away from overflow and underflow).  Three
+ * instructions instead of a full IEEE division sequence; the same bits as
+ * a / b, which tests/test_oracle_closed_forms.py checks on random operands for
+ * the grid spacings used here.  Needs a fused multiply-add (fma()). */
+HV_HD HV_INLINE double hv_div_fixed(double a, double b, double y) {
+    double q = a * y;
+    double r = fma(-q, b, a);
+    return fma(r, y, q);
+}
+
 /* ---- Skyrme mean field (include/physics_formulas.hpp:71-77) --------------- */
 /* U_q = A t + B t^gamma + 2 tau_q C (rho_n - rho_p)/rho0, t = (rho_p+rho_n)/rho0,
  * tau_p = -1, tau_n = +1.  Same operation order as the reference expression;

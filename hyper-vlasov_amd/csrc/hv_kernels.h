@@ -643,6 +643,7 @@ __global__ void k_excite(T *__restrict__ kz, int np, int total, T dp, T dn) {
 // ---- red-black SOR (include/physics.hpp:100-157) ---------------------------
 struct SorCoef {
     double dx2, dy2, dz2;   // dx*dx etc.
+    double ydx2, ydy2, ydz2;  // 1 / dx2 etc. (for hv_div_fixed)
     double inv_dx2;         // 1 / (2/dx^2 + 2/dy^2 + 2/dz^2)
     double src;             // 4 pi 1.44
     double omega, one_minus_omega;
@@ -719,9 +720,9 @@ k_sor_colour(double *__restrict__ phi, const RT *__restrict__ rho, Split sp, Sor
                 const long long q = row + h;
                 const long long sx = (long long)ny * nzh;
                 double r = (double)src[q];
-                double phi_x = (oth[q + sx] + oth[q - sx]) / cf.dx2;
-                double phi_y = (oth[q + nzh] + oth[q - nzh]) / cf.dy2;
-                double phi_z = (oth[q + off] + oth[q + off - 1]) / cf.dz2;
+                double phi_x = hv_div_fixed(oth[q + sx] + oth[q - sx], cf.dx2, cf.ydx2);
+                double phi_y = hv_div_fixed(oth[q + nzh] + oth[q - nzh], cf.dy2, cf.ydy2);
+                double phi_z = hv_div_fixed(oth[q + off] + oth[q + off - 1], cf.dz2, cf.ydz2);
                 double phi_star = (phi_x + phi_y + phi_z + cf.src * r) * cf.inv_dx2;
                 double phi_old = dst[q];
                 double nv = cf.one_minus_omega * phi_old + cf.omega * phi_star;

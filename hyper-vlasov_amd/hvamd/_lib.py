@@ -41,6 +41,7 @@ HV_FIELD_FORCES_Y = 4
 HV_FIELD_FORCES_Z = 5
 
 HV_OPT_SORT_EVERY = 0
+HV_OPT_SOR_RESIDENT = 1
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1

@@ -181,6 +181,11 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * changes results -- deposits and diagnostics are exact integer sums and
  * hv_download_particles returns the caller's order -- only speed. */
 #define HV_OPT_SORT_EVERY 0
+/* HV_OPT_SOR_RESIDENT: 1 runs each SOR solve as one cooperative launch with
+ * the potential held in LDS (when the grid fits the CUs' LDS); 0 (default)
+ * launches one kernel per red or black sweep.  Same bits either way.  Reading
+ * it back gives 0 when the resident form is off or unavailable. */
+#define HV_OPT_SOR_RESIDENT 1
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 

@@ -522,3 +522,20 @@ int hvo_simulate_loop(double *x, double *y, double *z, double *kx, double *ky, d
     if (!phi_inout) return -1;
     return simulate_impl(x, y, z, kx, ky, kz, fx, fy, fz, np, nn, w, s, p, phi_inout, stats, sor_iters, 0);
 }
+
+/* Counts operands a (drawn uniformly in [lo, hi) from a xorshift stream) for
+ * which hv_div_fixed(a, b, 1/b) differs from the IEEE quotient a / b. */
+long long hvo_div_fixed_mismatches(double b, double lo, double hi, long long n, unsigned long long seed) {
+    double y = 1.0 / b;
+    long long bad = 0;
+    unsigned long long s = seed | 1ULL;
+    for (long long i = 0; i < n; i++) {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        double u = (double)(s >> 11) * (1.0 / 9007199254740992.0);
+        double a = lo + (hi - lo) * u;
+        if (hv_div_fixed(a, b, y) != a / b) bad++;
+    }
+    return bad;
+}

@@ -115,6 +115,9 @@ int hvo_simulate_loop(double *x, double *y, double *z, double *kx, double *ky, d
                       const hvo_skyrme *s, const hvo_params *p, double *phi_inout, double *stats,
                       int *sor_iters);
 
+/* mismatches of the fixed-divisor division (hv_detmath.h) against a / b */
+long long hvo_div_fixed_mismatches(double b, double lo, double hi, long long n, unsigned long long seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -307,3 +307,33 @@ def test_sort_cadence_does_not_change_bits(oracle, sort_every):
     again = sim.download_particles()
     _eq(again["fx"], np.arange(w.total, dtype=np.float64))
     _eq(again["x"], st_o["x"])
+
+
+@pytest.mark.parametrize("n,d", GRIDS + [((64, 64, 64), (12.8, 12.8, 12.8)), ((128, 128, 128), (16.0, 16.0, 16.0))])
+def test_sor_resident_equals_per_sweep_launches(oracle, n, d):
+    """The LDS-resident cooperative SOR gives the same iterate and iteration
+    count as one launch per red/black sweep."""
+    print(f"sor resident vs launches grid={n}", flush=True)
+    G = int(np.prod(n))
+    rho_p = smooth_field(14, n, 0.09)
+    outs = []
+    for resident in (1, 0):
+        sim = _sim(n, d, 10, 10)
+        sim.set_option(hvamd._lib.HV_OPT_SOR_RESIDENT, resident)
+        sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p, np.zeros(G)]))
+        sim.set_initial_coulomb_boundaries(82)
+        it, md = sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 60)
+        it2, md2 = sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 200)
+        used = sim.get_option(hvamd._lib.HV_OPT_SOR_RESIDENT)
+        print(f"  resident option={resident} active={used} iters={it},{it2}", flush=True)
+        outs.append((it, md, it2, md2, sim.download_field(hvamd.HV_FIELD_COULOMB), used))
+        sim.close()
+    assert outs[0][5] == 1, "resident SOR not available on this device"
+    assert outs[0][:4] == outs[1][:4]
+    _eq(outs[0][4], outs[1][4])
+    if G <= 64 ** 3:
+        phi0 = oracle.coulomb_bc(np.zeros(G), n, d, 82)
+        phi_o, it_o, md_o, _ = oracle.sor(phi0, rho_p, n, d, 1.8, 1e-6, 60)
+        phi_o, it_o2, md_o2, _ = oracle.sor(phi_o, rho_p, n, d, 1.8, 1e-6, 200)
+        assert (it_o, md_o, it_o2, md_o2) == outs[0][:4]
+        _eq(outs[0][4], phi_o)

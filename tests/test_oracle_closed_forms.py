@@ -306,3 +306,16 @@ def test_stats_exact_on_lattice(oracle):
     r2 = x * x + y * y + z * z
     assert st[2] == math.fsum(r2[:np_][inb[:np_]]) / inb[:np_].sum()
     assert st[3] == math.fsum(r2[np_:][inb[np_:]]) / inb[np_:].sum()
+
+
+@pytest.mark.parametrize("b", [0.5 ** 2, 0.4 ** 2, 0.25 ** 2, (2 * 12.0 / 96) ** 2, (2 * 7.5 / 24) ** 2,
+                               (2 * 5.0 / 16) ** 2, (24.0 / 96) ** 2, 0.3 ** 2, 1.0 / 3.0])
+def test_fixed_divisor_division_is_ieee(oracle, b):
+    """The SOR kernels divide by dx^2, dy^2, dz^2 through hv_div_fixed
+    (q = a y, r = fma(-q, b, a), q + r y): it must give a / b bit for bit."""
+    import ctypes
+    fn = oracle.L.hvo_div_fixed_mismatches
+    fn.restype = ctypes.c_longlong
+    fn.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_longlong, ctypes.c_ulonglong]
+    for lo, hi in ((0.0, 1.0), (1.0, 200.0), (-50.0, 50.0), (1e-6, 1e-3)):
+        assert fn(b, lo, hi, 2_000_000, 12345) == 0
