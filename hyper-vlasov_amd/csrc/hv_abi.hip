@@ -43,6 +43,13 @@ struct hv_ctx {
     void *U = nullptr;                  // 2G T
     double *phi = nullptr;              // 2 Gh, colour-split (hv::Split)
     void *rho_s = nullptr;              // 2 Gh T: colour-split copy of the proton density
+    float *sor_R = nullptr;             // mixed SOR: FP32 residual and correction (2 Gh each)
+    float *sor_delta = nullptr;
+    int mixed_fp64_sweeps = 0;          // FP64 iterations of the last mixed solve
+    bool phi_warm = false;              // phi holds a previous solve (mixed SOR corrects it in FP32)
+    int sor_fused = 0;                  // option: one launch per SOR iteration (red + black fused; slower at 128^3)
+    double *phi_alt = nullptr;          // second potential buffer of the fused SOR
+    float *sor_delta_alt = nullptr;
     Split sp{};
     void *H = nullptr, *D = nullptr;    // smearing factor channels
     size_t H_cap = 0, D_cap = 0;        // bytes
@@ -246,6 +253,29 @@ void prof_end(hv_ctx *c, int cls, hipEvent_t a, long long launches) {
     (void)hipEventRecord(b, c->stream);
     c->pending.push_back({cls, a, b, launches});
 }
+
+// Per-kernel brackets for the SOR sweeps: one launch in kSorSample is
+// bracketed by its own event pair (the launch gaps stay out of the figure).
+constexpr int kSorSample = 8;
+
+struct KBracket {
+    hv_ctx *c;
+    hipEvent_t a = nullptr;
+    int kclass;
+    KBracket(hv_ctx *cc, int it, int colour, int kc = HV_KCLASS_SOR) : c(cc), kclass(kc) {
+        if (c->prof && ((it * 2 + colour) % kSorSample) == 3) {
+            a = prof_event(c);
+            if (a) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~KBracket() {
+        if (!a) return;
+        hipEvent_t b = prof_event(c);
+        if (!b) return;
+        (void)hipEventRecord(b, c->stream);
+        c->pending.push_back({kclass, a, b, 1});
+    }
+};
 
 // fold completed brackets into the counters (call at a host sync point)
 void prof_flush(hv_ctx *c) {
@@ -476,6 +506,71 @@ int sor_resident(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, int *i
     return HV_OK;
 }
 
+// Runs red/black iterations it = 0, 1, ... until the first iteration whose
+// max |update| is below tol (or max_it), launching them in chunks and reading
+// back each chunk's per-iteration maxima.  launch(it) enqueues iteration it;
+// launches past the converged iteration return at once.
+template <typename Launch>
+int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double *md_out, Launch launch) {
+    const double tol = cf.tol;
+    int rc = fill<unsigned long long>(c, c->maxd, 0ULL, (long long)maxd_words(max_it));
+    if (rc) return rc;
+    rc = fill<int>(c, (int *)c->ctl, 0, 2);
+    if (rc) return rc;
+    int it = 0, iters = -1, start = 0;
+    int chunk = c->sor_pred > 2 ? c->sor_pred - 1 : 1;
+    while (it < max_it) {
+        start = it;
+        int end = it + chunk < max_it ? it + chunk : max_it;
+        for (; it < end; it++) launch(it);
+        HV_CHECK_LAUNCH(c);
+        k_sor_collect<<<(unsigned)(end - start), 64, 0, c->stream>>>(c->maxd, c->maxc, start, end);
+        HV_CHECK_LAUNCH(c);
+        HV_HIP(c, hipMemcpyAsync(c->pinned, c->maxc, sizeof(double) * (size_t)(end - start),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HV_HIP(c, hipStreamSynchronize(c->stream));
+        for (int m = 0; m < end - start; m++) {
+            if (c->pinned[m] < tol) {
+                iters = start + m + 1;
+                break;
+            }
+        }
+        prof_flush(c);
+        if (iters > 0) break;
+        chunk = 4;
+    }
+    if (iters < 0) iters = max_it;
+    *iters_out = iters;
+    *md_out = c->pinned[iters - 1 - start];
+    return HV_OK;
+}
+
+// The reference's SOR on c->phi (FP64), one launch per iteration (fused) or
+// per sweep.  The fused form alternates phi and phi_alt; the result is
+// left in c->phi.
+template <typename RT>
+int sor_exact(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, bool fused, dim3 sgrd, dim3 sblk,
+              const SorTiles &tl, dim3 fgrd, dim3 fblk, const FusedTiles &ft, size_t rows_lds, int *iters,
+              double *md) {
+    if (fused) {
+        double *pbuf[2] = {c->phi, c->phi_alt};
+        int rc = sor_iterate(c, cf, max_it, iters, md, [&](int it) {
+            KBracket kb(c, it, 1);
+            k_sor_fused<true, double, RT><<<fgrd, fblk, rows_lds * sizeof(double), c->stream>>>(
+                pbuf[it & 1], pbuf[(it + 1) & 1], rho, c->sp, ft, cf, it, c->maxd, c->ctl);
+        });
+        if (rc) return rc;
+        if (*iters & 1) std::swap(c->phi, c->phi_alt);
+        return HV_OK;
+    }
+    return sor_iterate(c, cf, max_it, iters, md, [&](int it) {
+        for (int colour = 0; colour < 2; colour++) {
+            KBracket kb(c, it, colour);
+            k_sor_colour<RT><<<sgrd, sblk, 0, c->stream>>>(c->phi, rho, c->sp, tl, cf, colour, it, c->maxd, c->ctl);
+        }
+    });
+}
+
 // Red-black SOR on c->phi with the proton density (first G values of `rho`).
 // The host launches iterations in chunks and reads back each chunk's
 // per-iteration max |dphi|; kernels launched past the converged iteration
@@ -516,57 +611,63 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         if (maxdiff_out) *maxdiff_out = md;
         return (md > tol) ? HV_WARN_SOR_NOT_CONVERGED : HV_OK;
     }
-    rc = fill<unsigned long long>(c, c->maxd, 0ULL, (long long)maxd_words(max_it));
-    if (rc) return rc;
-    rc = fill<int>(c, (int *)c->ctl, 0, 2);
-    if (rc) return rc;
     SorTiles tl;
     tl.n_ic = (nx - 2 + kSorPlanes - 1) / kSorPlanes;
     tl.n_jb = (ny - 2 + kSorRows - 1) / kSorRows;
     tl.n_hb = (c->sp.nzh + 63) / 64;
     const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
     tl.per_xcd = (ntiles + kXcds - 1) / kXcds;
-    dim3 blk(64, kSorRows, 1);
-    dim3 grd((unsigned)(tl.per_xcd * kXcds), 1, 1);
+    const dim3 sblk(64, kSorRows, 1), sgrd((unsigned)(tl.per_xcd * kXcds), 1, 1);
+    FusedTiles ft;
+    ft.ti = (nx + kFT - 1) / kFT;
+    ft.tj = (ny + kFT - 1) / kFT;
+    ft.per_xcd = (ft.ti * ft.tj + kXcds - 1) / kXcds;
+    const dim3 fgrd((unsigned)(ft.per_xcd * kXcds), 1, 1), fblk(kFThreads, 1, 1);
+    const size_t rows_lds = (size_t)(kFT + 2) * (kFT + 2) * c->sp.nzh;
+    const bool fused = c->sor_fused && rows_lds * sizeof(double) <= 65536;
     (void)nz;
-    int it = 0, iters = -1, start = 0;
-    int chunk = c->sor_pred > 2 ? c->sor_pred - 1 : 1;
-    while (it < max_it) {
-        start = it;
-        int end = it + chunk < max_it ? it + chunk : max_it;
-        hipEvent_t pa = prof_begin(c), pb = nullptr;
-        for (; it < end; it++) {
-            k_sor_colour<RT><<<grd, blk, 0, c->stream>>>(c->phi, rho, c->sp, tl, cf, 0, it, c->maxd, c->ctl);
-            k_sor_colour<RT><<<grd, blk, 0, c->stream>>>(c->phi, rho, c->sp, tl, cf, 1, it, c->maxd, c->ctl);
-        }
+    int iters = 0;
+    double md = 0.0;
+    if (c->precision == HV_PREC_MIXED && c->phi_warm) {
+        // FP32 defect-correction iterations on the small correction of a warm
+        // potential, then FP64 sweeps until the stop rule holds in FP64
+        k_sor_defect<RT><<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, rho, c->sor_R, c->sor_delta,
+                                                                              c->sp, cf);
         HV_CHECK_LAUNCH(c);
-        if (pa) {
-            pb = prof_event(c);
-            if (pb) (void)hipEventRecord(pb, c->stream);
+        float *dbuf[2] = {c->sor_delta, c->sor_delta_alt};
+        if (fused) {
+            rc = sor_iterate(c, cf, max_it, &iters, &md, [&](int it) {
+                KBracket kb(c, it, 1, HV_KCLASS_SOR_F32);
+                k_sor_fused<false, float, float><<<fgrd, fblk, rows_lds * sizeof(float), c->stream>>>(
+                    dbuf[it & 1], dbuf[(it + 1) & 1], c->sor_R, c->sp, ft, cf, it, c->maxd, c->ctl);
+            });
+        } else {
+            rc = sor_iterate(c, cf, max_it, &iters, &md, [&](int it) {
+                for (int colour = 0; colour < 2; colour++) {
+                    KBracket kb(c, it, colour, HV_KCLASS_SOR_F32);
+                    k_sor_colour_f32<RT><<<sgrd, sblk, 0, c->stream>>>(c->sor_delta, c->sor_R, c->sp, tl, cf, colour,
+                                                                      it, c->maxd, c->ctl);
+                }
+            });
         }
-        k_sor_collect<<<(unsigned)(end - start), 64, 0, c->stream>>>(c->maxd, c->maxc, start, end);
+        if (rc) return rc;
+        const float *dres = fused ? dbuf[iters & 1] : c->sor_delta;
+        k_sor_fold<<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, dres, 2 * c->sp.Gh);
         HV_CHECK_LAUNCH(c);
-        HV_HIP(c, hipMemcpyAsync(c->pinned, c->maxc, sizeof(double) * (size_t)(end - start),
-                                 hipMemcpyDeviceToHost, c->stream));
-        HV_HIP(c, hipStreamSynchronize(c->stream));
-        for (int m = 0; m < end - start; m++) {
-            if (c->pinned[m] < tol) {
-                iters = start + m + 1;
-                break;
-            }
-        }
-        if (pa && pb) {
-            int real = (iters > 0 ? iters : end) - start;
-            c->pending.push_back({HV_KCLASS_SOR, pa, pb, 2LL * real});
-        }
-        prof_flush(c);
-        if (iters > 0) break;
-        chunk = 4;
+        int it64 = 0;
+        const int pred = iters;
+        c->sor_pred = 2;
+        rc = sor_exact(c, rho, cf, max_it, fused, sgrd, sblk, tl, fgrd, fblk, ft, rows_lds, &it64, &md);
+        if (rc) return rc;
+        c->mixed_fp64_sweeps = it64;
+        c->sor_pred = pred;
+        iters += it64;
+    } else {
+        rc = sor_exact(c, rho, cf, max_it, fused, sgrd, sblk, tl, fgrd, fblk, ft, rows_lds, &iters, &md);
+        if (rc) return rc;
+        c->sor_pred = iters;
     }
-    if (iters < 0) iters = max_it;
-    // max |dphi| of the last iteration done (it lies in the last chunk read back)
-    const double md = c->pinned[iters - 1 - start];
-    c->sor_pred = iters;
+    c->phi_warm = true;
     if (iters_out) *iters_out = iters;
     if (maxdiff_out) *maxdiff_out = md;
     return (md > tol) ? HV_WARN_SOR_NOT_CONVERGED : HV_OK;
@@ -749,6 +850,7 @@ int run_init(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *sor_iters) 
     natural_to_split(c->sp, h.data(), hs.data());
     HV_HIP(c, hipMemcpyAsync(c->phi, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, c->stream));
     HV_HIP(c, hipStreamSynchronize(c->stream));
+    c->phi_warm = false;
     rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
     if (rc) return rc;
     if (c->sort_every > 0) {
@@ -940,6 +1042,12 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
     ok = ok && hipMalloc(&c->U, ts * (size_t)(2 * c->G)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->phi, sizeof(double) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
     ok = ok && hipMalloc(&c->rho_s, ts * (size_t)(2 * c->sp.Gh)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->phi_alt, sizeof(double) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
+    if (precision == HV_PREC_MIXED) {
+        ok = ok && hipMalloc((void **)&c->sor_delta_alt, sizeof(float) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
+        ok = ok && hipMalloc((void **)&c->sor_R, sizeof(float) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
+        ok = ok && hipMalloc((void **)&c->sor_delta, sizeof(float) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
+    }
     ok = ok && hipMalloc((void **)&c->ctl, sizeof(SorCtl)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->status, sizeof(unsigned)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->partials, sizeof(long long) * 6 * (size_t)blocks_for(c->total)) == hipSuccess;
@@ -980,7 +1088,8 @@ int hv_ctx_destroy(hv_ctx *c) {
                      c->res_x, c->res_flag, c->res_slots, c->res_err, c->res_result};
     for (void *b : sbufs)
         if (b) (void)hipFree(b);
-    void *bufs[] = {c->acc, c->density, c->U, c->phi, c->rho_s, c->H, c->D, c->F[0], c->F[1], c->F[2],
+    void *bufs[] = {c->acc, c->density, c->U, c->phi, c->phi_alt, c->rho_s, c->sor_R, c->sor_delta,
+                    c->sor_delta_alt, c->H, c->D, c->F[0], c->F[1], c->F[2],
                     c->smear_tab, c->maxd, c->maxc, c->ctl, c->slots, c->status, c->partials, c->sflags};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1055,6 +1164,7 @@ int hv_upload_field(hv_ctx *c, int which, const double *host) {
         natural_to_split(c->sp, host, hs.data());
         HV_HIP(c, hipMemcpyAsync(c->phi, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, c->stream));
         HV_HIP(c, hipStreamSynchronize(c->stream));
+        c->phi_warm = false;
         return HV_OK;
     }
     void *p;
@@ -1293,6 +1403,9 @@ int hv_set_option(hv_ctx *c, int option, int value) {
     case HV_OPT_SOR_RESIDENT:
         c->sor_resident = value ? 1 : 0;
         return HV_OK;
+    case HV_OPT_SOR_FUSED:
+        c->sor_fused = value ? 1 : 0;
+        return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -1302,6 +1415,7 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
     switch (option) {
     case HV_OPT_SORT_EVERY: *value = c->sort_every; return HV_OK;
     case HV_OPT_SOR_RESIDENT: *value = c->sor_resident && c->res_state >= 0; return HV_OK;
+    case HV_OPT_SOR_FUSED: *value = c->sor_fused; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }

@@ -735,6 +735,225 @@ k_sor_colour(double *__restrict__ phi, const RT *__restrict__ rho, Split sp, Sor
     block_max_atomic(diff, &maxd[((long long)it * kMaxSlots + (b % kMaxSlots)) * kSlotStride]);
 }
 
+// ---- mixed precision SOR: defect correction --------------------------------
+// With the FP64 potential phi held fixed, the red-black iteration for
+// phi + delta, delta starting at 0, is
+//   delta* = (sum_axes (delta_+ + delta_-)/h^2 + R) inv,
+//   delta  = (1 - omega) delta + omega delta*,
+// where R = sum_axes (phi_+ + phi_-)/h^2 + 4 pi e^2 rho - phi / inv is the
+// FP64 residual of phi: the same iterates as the SOR on phi itself and the same
+// max |update|, carried in FP32 on the small correction delta (half the bytes
+// per sweep).  R is computed once per solve in FP64 and stored in FP32.
+
+// R (FP32, split) for the interior nodes, zero elsewhere; delta set to zero.
+template <typename RT>
+__global__ void __launch_bounds__(kBlock)
+k_sor_defect(const double *__restrict__ phi, const RT *__restrict__ rho, float *__restrict__ R,
+             float *__restrict__ delta, Split sp, SorCoef cf) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 2 * sp.Gh) return;
+    const int colour = q >= sp.Gh ? 1 : 0;
+    const long long qq = q - (colour ? sp.Gh : 0);
+    const int h = (int)(qq % sp.nzh);
+    const long long row = qq / sp.nzh;
+    const int j = (int)(row % sp.ny), i = (int)(row / sp.ny);
+    const int off = (i + j + colour) & 1;
+    const int k = 2 * h + off;
+    float res = 0.0f;
+    if (i >= 1 && i <= sp.nx - 2 && j >= 1 && j <= sp.ny - 2 && k >= 1 && k <= sp.nz - 2) {
+        const double *oth = phi + (colour ? 0 : sp.Gh);
+        const long long sx = (long long)sp.ny * sp.nzh;
+        const double ax = (oth[qq + sx] + oth[qq - sx]) * cf.ydx2;
+        const double ay = (oth[qq + sp.nzh] + oth[qq - sp.nzh]) * cf.ydy2;
+        const double az = (oth[qq + off] + oth[qq + off - 1]) * cf.ydz2;
+        res = (float)(ax + ay + az + cf.src * (double)rho[q] - phi[q] / cf.inv_dx2);
+    }
+    R[q] = res;
+    delta[q] = 0.0f;
+}
+
+template <typename RT>
+__global__ void __launch_bounds__(256)
+k_sor_colour_f32(float *__restrict__ delta, const float *__restrict__ R, Split sp, SorTiles tl, SorCoef cf,
+                 int colour, int it, unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
+    if (ctl->done) return;
+    if (colour == 0 && it > 0) {
+        const double prev = sor_row_max(maxd, it - 1);
+        if (prev < cf.tol) {
+            if (threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x == 0) {
+                ctl->done = 1;
+                ctl->iters = it;
+            }
+            return;
+        }
+    }
+    const int b = blockIdx.x;
+    const int t = (b % kXcds) * tl.per_xcd + b / kXcds;
+    const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
+    float diff = 0.0f;
+    if (t < ntiles) {
+        const int ic = t / (tl.n_jb * tl.n_hb);
+        const int rem = t - ic * tl.n_jb * tl.n_hb;
+        const int jb = rem / tl.n_hb, hb = rem - jb * tl.n_hb;
+        const int j = 1 + jb * kSorRows + threadIdx.y;
+        const int h = hb * 64 + threadIdx.x;
+        const int ny = sp.ny, nz = sp.nz, nzh = sp.nzh;
+        float *dst = delta + (colour ? sp.Gh : 0);
+        const float *oth = delta + (colour ? 0 : sp.Gh);
+        const float *src = R + (colour ? sp.Gh : 0);
+        const float yx = (float)cf.ydx2, yy = (float)cf.ydy2, yz = (float)cf.ydz2, inv = (float)cf.inv_dx2;
+        const float om = (float)cf.omega, om1 = (float)cf.one_minus_omega;
+        if (j < ny - 1) {
+            for (int di = 0; di < kSorPlanes; di++) {
+                const int i = 1 + ic * kSorPlanes + di;
+                if (i >= sp.nx - 1) break;
+                const int off = (i + j + colour) & 1;
+                const int k = 2 * h + off;
+                if (k < 1 || k > nz - 2) continue;
+                const long long q = ((long long)i * ny + j) * nzh + h;
+                const long long sx = (long long)ny * nzh;
+                const float s = (oth[q + sx] + oth[q - sx]) * yx + (oth[q + nzh] + oth[q - nzh]) * yy +
+                                (oth[q + off] + oth[q + off - 1]) * yz + src[q];
+                const float old = dst[q];
+                const float nv = om1 * old + om * (s * inv);
+                dst[q] = nv;
+                const float d = fabsf(nv - old);
+                diff = d > diff ? d : diff;
+            }
+        }
+    }
+    block_max_atomic((double)diff, &maxd[((long long)it * kMaxSlots + (b % kMaxSlots)) * kSlotStride]);
+}
+
+// phi += delta (both colours; delta is zero off the interior)
+__global__ void k_sor_fold(double *__restrict__ phi, const float *__restrict__ delta, long long n) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) phi[q] += (double)delta[q];
+}
+
+// ---- one launch per SOR iteration: red and black fused ------------------------
+// Out of place (in -> out).  A block owns FT x FT grid rows (all k): it first
+// computes the new red values of its rows and of the one-row ring around them
+// into LDS (the ring's are the same values the neighbouring blocks compute for
+// their own rows: same operands, same operations), then the new black values
+// of its rows from them, and writes both colours of its rows to `out`.
+// Boundary nodes are copied, so `out` is complete.  Per iteration it reads the
+// potential and the source once and writes the potential once (plus ring
+// re-reads served by L2), and pays one launch instead of two.
+//
+// EXACT (FP64 potential): the reference's update on phi, with the division by
+// dx^2 as hv_div_fixed (bit-identical).  !EXACT (FP32 correction of the mixed
+// form): the delta iteration with the FP64 residual R as source.
+constexpr int kFT = 8;            // tile side in rows
+constexpr int kFThreads = 512;
+
+struct FusedTiles {
+    int ti, tj;      // tiles along i and j
+    int per_xcd;     // tiles per XCD slice of the grid
+};
+
+template <bool EXACT, typename VT, typename ST>
+__device__ __forceinline__ VT sor_update(VT xm, VT xp, VT ym, VT yp, VT zm, VT zp, VT old, ST srcv,
+                                         const SorCoef &cf) {
+    if constexpr (EXACT) {
+        const double phi_x = hv_div_fixed(xp + xm, cf.dx2, cf.ydx2);
+        const double phi_y = hv_div_fixed(yp + ym, cf.dy2, cf.ydy2);
+        const double phi_z = hv_div_fixed(zp + zm, cf.dz2, cf.ydz2);
+        const double phi_star = (phi_x + phi_y + phi_z + cf.src * (double)srcv) * cf.inv_dx2;
+        return cf.one_minus_omega * old + cf.omega * phi_star;
+    } else {
+        const float s = (xp + xm) * (float)cf.ydx2 + (yp + ym) * (float)cf.ydy2 + (zp + zm) * (float)cf.ydz2 +
+                        (float)srcv;
+        return (float)cf.one_minus_omega * old + (float)cf.omega * (s * (float)cf.inv_dx2);
+    }
+}
+
+template <bool EXACT, typename VT, typename ST>
+__global__ void __launch_bounds__(kFThreads)
+k_sor_fused(const VT *__restrict__ in, VT *__restrict__ out, const ST *__restrict__ src, Split sp, FusedTiles ft,
+            SorCoef cf, int it, unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
+    extern __shared__ unsigned char smem[];
+    VT *rn = (VT *)smem;  // new red values, (kFT + 2)^2 rows x nzh
+    if (ctl->done) return;
+    if (it > 0) {
+        const double prev = sor_row_max(maxd, it - 1);
+        if (prev < cf.tol) {
+            if (threadIdx.x == 0 && blockIdx.x == 0) {
+                ctl->done = 1;
+                ctl->iters = it;
+            }
+            return;
+        }
+    }
+    const int b = blockIdx.x;
+    const int t = (b % kXcds) * ft.per_xcd + b / kXcds;
+    const int nx = sp.nx, ny = sp.ny, nz = sp.nz, nzh = sp.nzh;
+    const long long sx = (long long)ny * nzh;
+    const VT *in_r = in, *in_b = in + sp.Gh;
+    const ST *src_r = src, *src_b = src + sp.Gh;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
+    const int RW = kFT + 2;
+    double diff = 0.0;
+    const bool active = t < ft.ti * ft.tj;
+    const int i0 = active ? (t / ft.tj) * kFT : 0, j0 = active ? (t % ft.tj) * kFT : 0;
+    if (active) {
+        // phase A: red on own rows + ring
+        for (int r = wave; r < RW * RW; r += nwaves) {
+            const int li = r / RW - 1, lj = r % RW - 1;
+            const int i = i0 + li, j = j0 + lj;
+            if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
+            const int off = (i + j) & 1;
+            const long long row = ((long long)i * ny + j) * nzh;
+            const bool inner_row = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+            for (int h = lane; h < nzh; h += kWave) {
+                const long long q = row + h;
+                const int k = 2 * h + off;
+                VT v = in_r[q];
+                if (inner_row && k >= 1 && k <= nz - 2) {
+                    const VT nv = sor_update<EXACT, VT, ST>(in_b[q - sx], in_b[q + sx], in_b[q - nzh], in_b[q + nzh],
+                                                           in_b[q + off - 1], in_b[q + off], v, src_r[q], cf);
+                    if (li >= 0 && li < kFT && lj >= 0 && lj < kFT) {
+                        const double d = fabs((double)nv - (double)v);
+                        diff = d > diff ? d : diff;
+                    }
+                    v = nv;
+                }
+                rn[r * nzh + h] = v;
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+        // phase B: black on own rows; write both colours
+        for (int r = wave; r < kFT * kFT; r += nwaves) {
+            const int li = r / kFT, lj = r % kFT;
+            const int i = i0 + li, j = j0 + lj;
+            if (i >= nx || j >= ny) continue;
+            const int off = (i + j + 1) & 1;
+            const long long row = ((long long)i * ny + j) * nzh;
+            const int rr = (li + 1) * RW + (lj + 1);  // LDS row of (i, j)
+            const bool inner_row = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+            for (int h = lane; h < nzh; h += kWave) {
+                const long long q = row + h;
+                const int k = 2 * h + off;
+                VT v = in_b[q];
+                if (inner_row && k >= 1 && k <= nz - 2) {
+                    const VT *c0 = rn + rr * nzh;
+                    const VT nv = sor_update<EXACT, VT, ST>(rn[(rr - RW) * nzh + h], rn[(rr + RW) * nzh + h],
+                                                           rn[(rr - 1) * nzh + h], rn[(rr + 1) * nzh + h],
+                                                           c0[h + off - 1], c0[h + off], v, src_b[q], cf);
+                    const double d = fabs((double)nv - (double)v);
+                    diff = d > diff ? d : diff;
+                    v = nv;
+                }
+                out[sp.Gh + q] = v;
+                out[q] = rn[rr * nzh + h];
+            }
+        }
+    }
+    block_max_atomic(diff, &maxd[((long long)it * kMaxSlots + (b % kMaxSlots)) * kSlotStride]);
+}
+
 // Folds the slot maxima of iterations [first, last) into out[0..last-first).
 __global__ void k_sor_collect(const unsigned long long *__restrict__ maxd, double *__restrict__ out, int first,
                               int last) {

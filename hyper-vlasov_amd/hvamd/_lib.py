@@ -42,12 +42,14 @@ HV_FIELD_FORCES_Z = 5
 
 HV_OPT_SORT_EVERY = 0
 HV_OPT_SOR_RESIDENT = 1
+HV_OPT_SOR_FUSED = 2
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1
 HV_KCLASS_SMEAR = 2
 HV_KCLASS_POTENTIAL = 3
-KCLASS_NAMES = ("sor_sweep", "particle_pass", "smear", "potential")
+HV_KCLASS_SOR_F32 = 4
+KCLASS_NAMES = ("sor_sweep", "particle_pass", "smear", "potential", "sor_sweep_f32")
 
 
 class HvWorld(ctypes.Structure):

@@ -186,6 +186,10 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * launches one kernel per red or black sweep.  Same bits either way.  Reading
  * it back gives 0 when the resident form is off or unavailable. */
 #define HV_OPT_SOR_RESIDENT 1
+/* HV_OPT_SOR_FUSED: 1 runs one launch per SOR iteration (red and black
+ * sweeps fused, out of place, the red ring recomputed per block); 0 (default)
+ * one launch per sweep.  Same bits. */
+#define HV_OPT_SOR_FUSED 2
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
@@ -198,15 +202,17 @@ int hv_timer_stop(hv_ctx *ctx, double *ms);
 const char *hv_build_info(void);
 
 /* ---- kernel-class profiling (roofline accounting) -------------------------- */
-/* When enabled, the fused loop brackets each kernel class with hipEvents on the
- * context stream and accumulates device milliseconds and the number of
- * launches that did work (early-exit SOR launches past convergence are not
- * counted as launches; their few microseconds are in the time). */
+/* When enabled, the loop brackets kernels with hipEvent pairs on the context
+ * stream and accumulates device milliseconds and bracketed launches per class:
+ * every particle pass, smearing and potential launch, and one SOR sweep
+ * launch in eight (each bracketed alone, so launch gaps stay out; the sampled
+ * average is the kernel's average duration). */
 #define HV_KCLASS_SOR 0       /* one red or black SOR sweep */
 #define HV_KCLASS_PARTICLE 1  /* fused gather + kick + drift + deposit + diagnostics pass */
 #define HV_KCLASS_SMEAR 2     /* the smearing passes of one step (one launch = one step's set) */
 #define HV_KCLASS_POTENTIAL 3 /* Skyrme + Coulomb merge */
-#define HV_KCLASS_COUNT 4
+#define HV_KCLASS_SOR_F32 4   /* one sweep of the mixed form's FP32 SOR correction */
+#define HV_KCLASS_COUNT 5
 int hv_profile_enable(hv_ctx *ctx, int on); /* also resets the counters */
 int hv_profile_read(hv_ctx *ctx, int kclass, double *ms, long long *launches);
 

@@ -337,3 +337,50 @@ def test_sor_resident_equals_per_sweep_launches(oracle, n, d):
         phi_o, it_o2, md_o2, _ = oracle.sor(phi_o, rho_p, n, d, 1.8, 1e-6, 200)
         assert (it_o, md_o, it_o2, md_o2) == outs[0][:4]
         _eq(outs[0][4], phi_o)
+
+
+@pytest.mark.parametrize("n,d", GRIDS + [((24, 40, 34), (6.0, 8.0, 7.0)), ((128, 128, 128), (16.0, 16.0, 16.0))])
+def test_sor_fused_equals_per_sweep(oracle, n, d):
+    """One launch per iteration (red + black fused, out of place, ring
+    recomputed) gives the same bits as one launch per sweep."""
+    print(f"sor fused vs per-sweep grid={n}", flush=True)
+    G = int(np.prod(n))
+    rho_p = smooth_field(15, n, 0.09)
+    outs = []
+    for fused in (1, 0):
+        sim = _sim(n, d, 10, 10)
+        sim.set_option(hvamd._lib.HV_OPT_SOR_FUSED, fused)
+        sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p, np.zeros(G)]))
+        sim.set_initial_coulomb_boundaries(82)
+        r1 = sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 37)
+        r2 = sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 300)
+        outs.append((r1, r2, sim.download_field(hvamd.HV_FIELD_COULOMB)))
+        sim.close()
+    assert outs[0][:2] == outs[1][:2]
+    _eq(outs[0][2], outs[1][2])
+
+
+def test_mixed_sor_matches_fp64_iterations():
+    """The mixed form's FP32 defect-correction SOR follows the FP64 iteration:
+    the FP32 correction carries the same iterates up to rounding, so the count
+    matches the FP64 solve within a few final FP64 sweeps, and the potential
+    agrees within the stop tolerance's reach."""
+    print("mixed SOR vs fp64", flush=True)
+    n, d = (48, 48, 48), (9.6, 9.6, 9.6)
+    G = int(np.prod(n))
+    rho_p = smooth_field(16, n, 0.09)
+    res = {}
+    for prec in (hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED):
+        sim = _sim(n, d, 10, 10, prec)
+        sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p, np.zeros(G)]))
+        sim.set_initial_coulomb_boundaries(82)
+        it0, _ = sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 512)  # cold: FP64 in both
+        sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p * 1.01, np.zeros(G)]))
+        it1, md1 = sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 512)  # warm: FP32 correction
+        res[prec] = (it0, it1, md1, sim.download_field(hvamd.HV_FIELD_COULOMB))
+        sim.close()
+    f, m = res[hvamd.HV_PREC_F64], res[hvamd.HV_PREC_MIXED]
+    print("iters fp64", f[:2], "mixed", m[:2])
+    assert f[0] == m[0]
+    assert abs(f[1] - m[1]) <= 12 and m[2] < 1e-6
+    assert np.abs(f[3] - m[3]).max() < 2e-5
