@@ -29,15 +29,31 @@ sys.path.insert(0, str(ROOT / "hyper-vlasov_amd"))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
 
 
-def sor_bytes_per_launch(w, prec: str) -> float:
-    """Algorithmic bytes of one red or black SOR sweep: an iteration reads the
-    iterate and its source and writes the iterate once per interior node
-    (FP64: phi, rho, phi = 24 B; mixed form: the FP32 correction delta, its
-    FP32 residual source and delta = 12 B); one launch is half an iteration."""
-    n = w.grid
-    interior = (n - 2) ** 3
-    per_node = 24.0 if prec == "fp64" else 12.0
-    return interior * per_node / 2.0
+def sor_roofline(w, res):
+    """Roofline entry for the SOR kernel that dominates a run's device time.
+
+    Algorithmic bytes of one SOR iteration per interior node: read the iterate
+    and its source, write the iterate (FP64 potential: 8 + 8 + 8 = 24 B; the
+    mixed form's FP32 correction and FP32 residual source: 4 + 4 + 4 = 12 B).
+    A per-sweep launch is half an iteration; one launch of the LDS-resident
+    FP32 kernel runs a whole correction solve (its iteration count is read
+    back per solve), so its bytes per launch are iterations x 12 B x nodes."""
+    interior = (w.grid - 2) ** 3
+    prof, cnt = res["prof"], res["sor"]
+    ms, n = prof.get("sor_resident_f32", (0.0, 0))
+    if n > 0:
+        it = cnt["iters_f32"] / n
+        return {"kernel": "k_sor_resident_f32 (the mixed form's FP32 SOR correction, one LDS-resident launch per solve)",
+                "bytes_per_launch": it * 12.0 * interior, "avg_launch_us": 1e3 * ms / n,
+                "iterations_per_launch": it, "launches": n}
+    ms, n = prof.get("sor_sweep_f32", (0.0, 0))
+    if n > 0:
+        return {"kernel": "k_sor_colour_f32 (one red or black sweep of the FP32 SOR correction)",
+                "bytes_per_launch": 6.0 * interior, "avg_launch_us": 1e3 * ms / n, "iterations_per_launch": 0.5,
+                "launches": n}
+    ms, n = prof["sor_sweep"]
+    return {"kernel": "k_sor_colour (one red or black FP64 SOR sweep)", "bytes_per_launch": 12.0 * interior,
+            "avg_launch_us": 1e3 * ms / max(n, 1), "iterations_per_launch": 0.5, "launches": n}
 
 
 def copy_calibration(mib: int):
@@ -71,13 +87,14 @@ def run_gpu(w, prec: str, steps: int, warmup: int):
     sim.sync()
     wall = time.perf_counter() - t0
     prof = sim.profile_read()
+    sor = sim.sor_counts()
     sim.profile_enable(False)
     state = None
     if prec == "fp64":
         state = sim.download_particles()
         state["phi"] = sim.download_field(hvamd.HV_FIELD_COULOMB)
     sim.close()
-    return dict(wall=wall, its=its, prof=prof, stats=stats, state=state)
+    return dict(wall=wall, its=its, prof=prof, sor=sor, stats=stats, state=state)
 
 
 def cpu_baseline(w, state, budget_s: float):
@@ -145,9 +162,8 @@ def main():
     fp64 = run_gpu(w, "fp64", args.steps, args.warmup) if args.fp64 else None
 
     value = w.total * args.steps / mixed["wall"]
-    sor_ms, sor_launches = mixed["prof"]["sor_sweep_f32"]
-    launch_ms = sor_ms / max(sor_launches, 1)
-    achieved = sor_bytes_per_launch(w, "mixed") / (launch_ms * 1e-3) / 1e9
+    rl = sor_roofline(w, mixed)
+    achieved = rl["bytes_per_launch"] / (rl["avg_launch_us"] * 1e-6) / 1e9
     out = {
         "metric": "particle-steps/s at 1.04M tp, 128^3 (mixed FP32/FP64 form; plain FP64 beside it)",
         "value": value,
@@ -164,21 +180,19 @@ def main():
         "config": {"workload": w.name, "test_particles": w.total, "grid": w.grid, "dx_fm": w.dx,
                    "dt_fm_c": w.dt, "sor": [w.omega, w.tol, w.max_it], "parallelism": "single GPU"},
         "sor_iters_per_step": float(np.mean(mixed["its"])),
+        "sor_counts": mixed["sor"],
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in mixed["prof"].items()},
-        "roofline": {"bound": "hbm", "kernel": "k_sor_colour_f32 (one red or black sweep of the FP32 SOR correction)",
-                     "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_us": 1e3 * launch_ms, "bytes_per_launch": sor_bytes_per_launch(w, "mixed")},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, **rl},
     }
     if fp64 is not None:
         v64 = w.total * args.steps / fp64["wall"]
-        s_ms, s_n = fp64["prof"]["sor_sweep"]
-        l_ms = s_ms / max(s_n, 1)
+        r64 = sor_roofline(w, fp64)
         out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["wall"] / args.steps,
                        "sor_iters_per_step": float(np.mean(fp64["its"])),
                        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in fp64["prof"].items()},
-                       "sor_achieved_GBs": sor_bytes_per_launch(w, "fp64") / (l_ms * 1e-3) / 1e9,
-                       "sor_avg_launch_us": 1e3 * l_ms}
+                       "sor_achieved_GBs": r64["bytes_per_launch"] / (r64["avg_launch_us"] * 1e-6) / 1e9,
+                       "sor_avg_launch_us": r64["avg_launch_us"]}
         out["mixed_over_fp64"] = value / v64
         if args.cpu_budget > 0:
             out["cpu_baseline"] = cpu_baseline(w, fp64["state"], args.cpu_budget)

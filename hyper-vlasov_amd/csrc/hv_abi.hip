@@ -7,6 +7,7 @@
 // iteration count exactly as the reference's stop rule does
 // (include/physics.hpp:104, 152-153).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <cstdio>
@@ -46,6 +47,7 @@ struct hv_ctx {
     float *sor_R = nullptr;             // mixed SOR: FP32 residual and correction (2 Gh each)
     float *sor_delta = nullptr;
     int mixed_fp64_sweeps = 0;          // FP64 iterations of the last mixed solve
+    long long cnt_solves = 0, cnt_it32 = 0, cnt_it64 = 0;  // SOR work since hv_profile_enable
     bool phi_warm = false;              // phi holds a previous solve (mixed SOR corrects it in FP32)
     int sor_fused = 0;                  // option: one launch per SOR iteration (red + black fused; slower at 128^3)
     double *phi_alt = nullptr;          // second potential buffer of the fused SOR
@@ -97,6 +99,18 @@ struct hv_ctx {
     int *res_err = nullptr;
     double *res_result = nullptr;
     unsigned long long *res_trace = nullptr;  // phase timestamps (HV_RES_TRACE builds only)
+    // LDS-resident FP32 correction iteration of the mixed form
+    int sor_res32 = 1;        // option: use it when the grid fits
+    int res32_lag = 2;        // option: iterations between a max and its convergence test
+    int res32_state = 0;
+    ResTiles rtl32{};
+    size_t res32_lds = 0;
+    unsigned long long *res32_x = nullptr;  // tagged exchange words (2 colours)
+    const void *res32_kernel = nullptr;     // instance for the chosen tile shape
+    float *res32_slots = nullptr;
+    int res32_slots_cap = 0;
+    int *res32_err = nullptr;
+    float *res32_result = nullptr;
     int since_sort = 1 << 30;
     bool permuted = false;
 
@@ -254,28 +268,26 @@ void prof_end(hv_ctx *c, int cls, hipEvent_t a, long long launches) {
     c->pending.push_back({cls, a, b, launches});
 }
 
-// Per-kernel brackets for the SOR sweeps: one launch in kSorSample is
-// bracketed by its own event pair (the launch gaps stay out of the figure).
+// Per-kernel timing of the SOR sweeps: one launch in kSorSample is issued
+// through hipExtLaunchKernelGGL with a start/stop event pair, which the runtime
+// stamps at the dispatch packet's own begin and end (the interval a kernel
+// trace reports), so launch gaps stay out of the figure.
 constexpr int kSorSample = 8;
 
-struct KBracket {
-    hv_ctx *c;
-    hipEvent_t a = nullptr;
-    int kclass;
-    KBracket(hv_ctx *cc, int it, int colour, int kc = HV_KCLASS_SOR) : c(cc), kclass(kc) {
-        if (c->prof && ((it * 2 + colour) % kSorSample) == 3) {
-            a = prof_event(c);
-            if (a) (void)hipEventRecord(a, c->stream);
+template <typename K, typename... A>
+void sor_launch(hv_ctx *c, int it, int colour, int kclass, K kernel, dim3 grid, dim3 block, size_t shm,
+                A... args) {
+    if (c->prof && ((it * 2 + colour) % kSorSample) == 3) {
+        hipEvent_t a = prof_event(c), b = a ? prof_event(c) : nullptr;
+        if (a && b) {
+            hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)shm, c->stream, a, b, 0u, args...);
+            c->pending.push_back({kclass, a, b, 1});
+            return;
         }
+        if (a) c->ev_free.push_back(a);
     }
-    ~KBracket() {
-        if (!a) return;
-        hipEvent_t b = prof_event(c);
-        if (!b) return;
-        (void)hipEventRecord(b, c->stream);
-        c->pending.push_back({kclass, a, b, 1});
-    }
-};
+    hipLaunchKernelGGL(kernel, grid, block, shm, c->stream, args...);
+}
 
 // fold completed brackets into the counters (call at a host sync point)
 void prof_flush(hv_ctx *c) {
@@ -390,11 +402,17 @@ size_t res_lds_bytes(const ResTiles &t) {
     return sizeof(double) * (2 * tsz + 2 * (size_t)t.side_i + 2 * (size_t)t.side_j + 32) + sizeof(RT) * 2 * tsz;
 }
 
-// Picks the tiling and checks that every tile block can be resident at once.
-template <typename RT>
-int res_setup(hv_ctx *c) {
-    if (c->res_state) return c->res_state > 0 ? HV_OK : 1;
-    c->res_state = -1;
+size_t res32_lds_bytes(const ResTiles &t) {
+    const size_t tsz = (size_t)t.TI * t.TJ * t.nzh;
+    return sizeof(float) * (4 * tsz + (size_t)t.xstride + 32);
+}
+
+// Picks the tiling (one block per tile, at most one tile per CU and 256
+// tiles) with the least exchange per node that fits the LDS, and checks that
+// every tile block can be resident at once.
+template <typename LdsFn>
+int pick_res_tiles(hv_ctx *c, const void *kernel, int threads, LdsFn lds_bytes, ResTiles *out, size_t *lds_out,
+                   int max_xstride = 1 << 30) {
     int dev = 0, ncu = 0, maxlds = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
@@ -422,7 +440,7 @@ int res_setup(hv_ctx *c) {
         t.side_i = t.TJ * t.nzh;
         t.side_j = t.TI * t.nzh;
         t.xstride = 2 * t.side_i + 2 * t.side_j;
-        if (res_lds_bytes<RT>(t) > (size_t)maxlds) continue;
+        if (lds_bytes(t) > (size_t)maxlds || t.xstride > max_xstride) continue;
         const long long cost = (long long)t.TI * t.TJ * 4 + t.TI + t.TJ;
         if (best_cost < 0 || cost < best_cost) {
             best_cost = cost;
@@ -430,13 +448,24 @@ int res_setup(hv_ctx *c) {
         }
     }
     if (best_cost < 0) return 1;
-    const size_t lds = res_lds_bytes<RT>(best);
-    if (hipFuncSetAttribute((const void *)k_sor_resident<RT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-        hipSuccess)
-        return 1;
+    const size_t lds = lds_bytes(best);
+    if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return 1;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sor_resident<RT>, kResThreads, lds) != hipSuccess ||
-        per_cu < 1 || (long long)best.PI * best.PJ > (long long)per_cu * ncu)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu < 1 ||
+        (long long)best.PI * best.PJ > (long long)per_cu * ncu)
+        return 1;
+    *out = best;
+    *lds_out = lds;
+    return HV_OK;
+}
+
+template <typename RT>
+int res_setup(hv_ctx *c) {
+    if (c->res_state) return c->res_state > 0 ? HV_OK : 1;
+    c->res_state = -1;
+    ResTiles best{};
+    size_t lds = 0;
+    if (pick_res_tiles(c, (const void *)k_sor_resident<RT>, kResThreads, res_lds_bytes<RT>, &best, &lds) != HV_OK)
         return 1;
     const size_t nT = (size_t)best.PI * best.PJ;
     if (hipMalloc((void **)&c->res_x, sizeof(double) * 2 * nT * best.xstride) != hipSuccess) return 1;
@@ -446,6 +475,89 @@ int res_setup(hv_ctx *c) {
     c->rtl = best;
     c->res_lds = lds;
     c->res_state = 1;
+    return HV_OK;
+}
+
+int res32_setup(hv_ctx *c) {
+    if (c->res32_state) return c->res32_state > 0 ? HV_OK : 1;
+    c->res32_state = -1;
+    ResTiles best{};
+    size_t lds = 0;
+    if (pick_res_tiles(c, (const void *)k_sor_resident_f32<0, 0, 0>, kRes32Threads, res32_lds_bytes, &best, &lds,
+                       kFetchMax * kRes32Threads) != HV_OK)
+        return 1;
+    // a fixed-shape instance for the common tilings (same LDS size and block)
+    c->res32_kernel = (const void *)k_sor_resident_f32<0, 0, 0>;
+    if (best.TI == 8 && best.TJ == 8 && best.nzh == 64) c->res32_kernel = (const void *)k_sor_resident_f32<8, 8, 64>;
+    if (best.TI == 8 && best.TJ == 4 && best.nzh == 32) c->res32_kernel = (const void *)k_sor_resident_f32<8, 4, 32>;
+    if (best.TI == 4 && best.TJ == 8 && best.nzh == 32) c->res32_kernel = (const void *)k_sor_resident_f32<4, 8, 32>;
+    if (c->res32_kernel != (const void *)k_sor_resident_f32<0, 0, 0> &&
+        hipFuncSetAttribute(c->res32_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        c->res32_kernel = (const void *)k_sor_resident_f32<0, 0, 0>;
+    const size_t nT = (size_t)best.PI * best.PJ;
+    if (hipMalloc((void **)&c->res32_x, sizeof(unsigned long long) * 2 * nT * best.xstride) != hipSuccess) return 1;
+    if (hipMalloc((void **)&c->res32_err, sizeof(int) * 4) != hipSuccess) return 1;
+    if (hipMalloc((void **)&c->res32_result, sizeof(float) * 4) != hipSuccess) return 1;
+    c->rtl32 = best;
+    c->res32_lds = lds;
+    c->res32_state = 1;
+    return HV_OK;
+}
+
+// The mixed form's FP32 correction iteration on c->sor_delta with source
+// c->sor_R, in one cooperative launch; returns the iterations done.
+int sor_resident_f32(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out) {
+    const int nT = c->rtl32.PI * c->rtl32.PJ;
+    if (c->res32_slots_cap < max_it) {
+        if (c->res32_slots) (void)hipFree(c->res32_slots);
+        c->res32_slots = nullptr;
+        c->res32_slots_cap = 0;
+        HV_HIP(c, hipMalloc((void **)&c->res32_slots, sizeof(float) * (size_t)max_it * nT));
+        c->res32_slots_cap = max_it;
+    }
+    int rc = fill<float>(c, c->res32_slots, -1.0f, (long long)max_it * nT);
+    if (rc) return rc;
+    rc = fill<unsigned long long>(c, c->res32_x, 0ULL, 2LL * nT * c->rtl32.xstride);
+    if (rc) return rc;
+    rc = fill<int>(c, c->res32_err, 0, 4);
+    if (rc) return rc;
+    ResSync32 sy;
+#ifdef HV_RES_TRACE
+    static unsigned long long *trace32 = nullptr;
+    if (!trace32) (void)hipMalloc((void **)&trace32, 16 * 16 * sizeof(unsigned long long));
+    sy.trace = trace32;
+    c->res_trace = trace32;
+#endif
+    sy.xr = c->res32_x;
+    sy.xb = c->res32_x + (size_t)nT * c->rtl32.xstride;
+    sy.slots = c->res32_slots;
+    sy.err = c->res32_err;
+    sy.result = c->res32_result;
+    float *delta = c->sor_delta;
+    const float *R = c->sor_R;
+    Split sp = c->sp;
+    ResTiles tl = c->rtl32;
+    SorCoef cfv = cf;
+    int mi = max_it, lag = c->res32_lag;
+    void *args[] = {&delta, (void *)&R, &sp, &tl, &cfv, &mi, &lag, &sy};
+    hipEvent_t pa = prof_begin(c);
+    HV_HIP(c, hipLaunchCooperativeKernel(c->res32_kernel, dim3((unsigned)nT), dim3(kRes32Threads), args,
+                                         (unsigned)c->res32_lds, c->stream));
+    hipEvent_t pb = nullptr;
+    if (pa) {
+        pb = prof_event(c);
+        if (pb) (void)hipEventRecord(pb, c->stream);
+    }
+    float res[2];
+    int err = 0;
+    HV_HIP(c, hipMemcpyAsync(res, c->res32_result, sizeof res, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(c, hipMemcpyAsync(&err, c->res32_err, sizeof err, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    if (err) return set_err(c, HV_ERR_HIP, "LDS-resident FP32 SOR: a tile wait exceeded its spin limit");
+    const int iters = (int)res[0];
+    if (pa && pb) c->pending.push_back({HV_KCLASS_SOR_RES_F32, pa, pb, 1});
+    prof_flush(c);
+    *iters_out = iters;
     return HV_OK;
 }
 
@@ -555,19 +667,17 @@ int sor_exact(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, bool fuse
     if (fused) {
         double *pbuf[2] = {c->phi, c->phi_alt};
         int rc = sor_iterate(c, cf, max_it, iters, md, [&](int it) {
-            KBracket kb(c, it, 1);
-            k_sor_fused<true, double, RT><<<fgrd, fblk, rows_lds * sizeof(double), c->stream>>>(
-                pbuf[it & 1], pbuf[(it + 1) & 1], rho, c->sp, ft, cf, it, c->maxd, c->ctl);
+            sor_launch(c, it, 1, HV_KCLASS_SOR, k_sor_fused<true, double, RT>, fgrd, fblk, rows_lds * sizeof(double),
+                       pbuf[it & 1], pbuf[(it + 1) & 1], (const RT *)rho, c->sp, ft, cf, it, c->maxd, c->ctl);
         });
         if (rc) return rc;
         if (*iters & 1) std::swap(c->phi, c->phi_alt);
         return HV_OK;
     }
     return sor_iterate(c, cf, max_it, iters, md, [&](int it) {
-        for (int colour = 0; colour < 2; colour++) {
-            KBracket kb(c, it, colour);
-            k_sor_colour<RT><<<sgrd, sblk, 0, c->stream>>>(c->phi, rho, c->sp, tl, cf, colour, it, c->maxd, c->ctl);
-        }
+        for (int colour = 0; colour < 2; colour++)
+            sor_launch(c, it, colour, HV_KCLASS_SOR, k_sor_colour<RT>, sgrd, sblk, 0, c->phi, (const RT *)rho, c->sp,
+                       tl, cf, colour, it, c->maxd, c->ctl);
     });
 }
 
@@ -607,6 +717,8 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         rc = sor_resident<RT>(c, rho, cf, max_it, &iters, &md);
         if (rc) return rc;
         c->sor_pred = iters;
+        c->cnt_it64 += iters;
+        c->cnt_solves++;
         if (iters_out) *iters_out = iters;
         if (maxdiff_out) *maxdiff_out = md;
         return (md > tol) ? HV_WARN_SOR_NOT_CONVERGED : HV_OK;
@@ -635,23 +747,25 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
                                                                               c->sp, cf);
         HV_CHECK_LAUNCH(c);
         float *dbuf[2] = {c->sor_delta, c->sor_delta_alt};
-        if (fused) {
+        bool res32 = false;
+        if (c->sor_res32 && res32_setup(c) == HV_OK) {
+            rc = sor_resident_f32(c, cf, max_it, &iters);
+            res32 = true;
+        } else if (fused) {
             rc = sor_iterate(c, cf, max_it, &iters, &md, [&](int it) {
-                KBracket kb(c, it, 1, HV_KCLASS_SOR_F32);
-                k_sor_fused<false, float, float><<<fgrd, fblk, rows_lds * sizeof(float), c->stream>>>(
-                    dbuf[it & 1], dbuf[(it + 1) & 1], c->sor_R, c->sp, ft, cf, it, c->maxd, c->ctl);
+                sor_launch(c, it, 1, HV_KCLASS_SOR_F32, k_sor_fused<false, float, float>, fgrd, fblk,
+                           rows_lds * sizeof(float), dbuf[it & 1], dbuf[(it + 1) & 1], (const float *)c->sor_R, c->sp,
+                           ft, cf, it, c->maxd, c->ctl);
             });
         } else {
             rc = sor_iterate(c, cf, max_it, &iters, &md, [&](int it) {
-                for (int colour = 0; colour < 2; colour++) {
-                    KBracket kb(c, it, colour, HV_KCLASS_SOR_F32);
-                    k_sor_colour_f32<RT><<<sgrd, sblk, 0, c->stream>>>(c->sor_delta, c->sor_R, c->sp, tl, cf, colour,
-                                                                      it, c->maxd, c->ctl);
-                }
+                for (int colour = 0; colour < 2; colour++)
+                    sor_launch(c, it, colour, HV_KCLASS_SOR_F32, k_sor_colour_f32<RT>, sgrd, sblk, 0, c->sor_delta,
+                               (const float *)c->sor_R, c->sp, tl, cf, colour, it, c->maxd, c->ctl);
             });
         }
         if (rc) return rc;
-        const float *dres = fused ? dbuf[iters & 1] : c->sor_delta;
+        const float *dres = (fused && !res32) ? dbuf[iters & 1] : c->sor_delta;
         k_sor_fold<<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, dres, 2 * c->sp.Gh);
         HV_CHECK_LAUNCH(c);
         int it64 = 0;
@@ -661,12 +775,16 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         if (rc) return rc;
         c->mixed_fp64_sweeps = it64;
         c->sor_pred = pred;
+        c->cnt_it32 += iters;
+        c->cnt_it64 += it64;
         iters += it64;
     } else {
         rc = sor_exact(c, rho, cf, max_it, fused, sgrd, sblk, tl, fgrd, fblk, ft, rows_lds, &iters, &md);
         if (rc) return rc;
         c->sor_pred = iters;
+        c->cnt_it64 += iters;
     }
+    c->cnt_solves++;
     c->phi_warm = true;
     if (iters_out) *iters_out = iters;
     if (maxdiff_out) *maxdiff_out = md;
@@ -1085,7 +1203,8 @@ int hv_ctx_destroy(hv_ctx *c) {
         if (c->part_alt[q]) (void)hipFree(c->part_alt[q]);
     }
     void *sbufs[] = {c->pid, c->pid_alt, c->keys[0], c->keys[1], c->ord[0], c->ord[1], c->sort_tmp,
-                     c->res_x, c->res_flag, c->res_slots, c->res_err, c->res_result};
+                     c->res_x, c->res_flag, c->res_slots, c->res_err, c->res_result,
+                     c->res32_x, c->res32_slots, c->res32_err, c->res32_result};
     for (void *b : sbufs)
         if (b) (void)hipFree(b);
     void *bufs[] = {c->acc, c->density, c->U, c->phi, c->phi_alt, c->rho_s, c->sor_R, c->sor_delta,
@@ -1406,6 +1525,15 @@ int hv_set_option(hv_ctx *c, int option, int value) {
     case HV_OPT_SOR_FUSED:
         c->sor_fused = value ? 1 : 0;
         return HV_OK;
+    case HV_OPT_SOR_RESIDENT_F32:
+        c->sor_res32 = value ? 1 : 0;
+        return HV_OK;
+    case HV_OPT_SOR_F32_LAG:
+        // the test of row it - lag runs before this iteration's black sweep,
+        // so row it itself is never complete then: lag >= 1
+        if (value < 1 || value > 64) return set_err(c, HV_ERR_ARG, "the FP32 SOR lag must be in [1, 64]");
+        c->res32_lag = value;
+        return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -1416,6 +1544,8 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
     case HV_OPT_SORT_EVERY: *value = c->sort_every; return HV_OK;
     case HV_OPT_SOR_RESIDENT: *value = c->sor_resident && c->res_state >= 0; return HV_OK;
     case HV_OPT_SOR_FUSED: *value = c->sor_fused; return HV_OK;
+    case HV_OPT_SOR_RESIDENT_F32: *value = c->sor_res32 && c->res32_state >= 0; return HV_OK;
+    case HV_OPT_SOR_F32_LAG: *value = c->res32_lag; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -1429,6 +1559,15 @@ int hv_profile_enable(hv_ctx *c, int on) {
         c->prof_ms[k] = 0.0;
         c->prof_launches[k] = 0;
     }
+    c->cnt_solves = c->cnt_it32 = c->cnt_it64 = 0;
+    return HV_OK;
+}
+
+int hv_sor_counts(hv_ctx *c, long long *solves, long long *iters_f32, long long *iters_f64) {
+    if (!c) return HV_ERR_ARG;
+    if (solves) *solves = c->cnt_solves;
+    if (iters_f32) *iters_f32 = c->cnt_it32;
+    if (iters_f64) *iters_f64 = c->cnt_it64;
     return HV_OK;
 }
 

@@ -43,13 +43,16 @@ HV_FIELD_FORCES_Z = 5
 HV_OPT_SORT_EVERY = 0
 HV_OPT_SOR_RESIDENT = 1
 HV_OPT_SOR_FUSED = 2
+HV_OPT_SOR_RESIDENT_F32 = 3
+HV_OPT_SOR_F32_LAG = 4
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1
 HV_KCLASS_SMEAR = 2
 HV_KCLASS_POTENTIAL = 3
 HV_KCLASS_SOR_F32 = 4
-KCLASS_NAMES = ("sor_sweep", "particle_pass", "smear", "potential", "sor_sweep_f32")
+HV_KCLASS_SOR_RES_F32 = 5
+KCLASS_NAMES = ("sor_sweep", "particle_pass", "smear", "potential", "sor_sweep_f32", "sor_resident_f32")
 
 
 class HvWorld(ctypes.Structure):
@@ -128,6 +131,8 @@ SIGNATURES = {
     "hv_set_option": (_I, [_P, _I, _I]),
     "hv_get_option": (_I, [_P, _I, _IP]),
     "hv_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
+    "hv_sor_counts": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                           ctypes.POINTER(ctypes.c_longlong)]),
     "hv_fermi_momentum": (_D, [_D]),
     "hv_sample_uniform_spheres": (_I, [ctypes.c_ulonglong, _I, _I, _D, _D] + [_P] * 6),
     "hv_mix64": (ctypes.c_ulonglong, [ctypes.c_ulonglong]),

@@ -293,6 +293,13 @@ class Simulation:
             out[name] = (ms.value, n.value)
         return out
 
+    def sor_counts(self) -> dict:
+        """SOR solves, FP32 correction iterations and FP64 iterations since
+        profile_enable."""
+        v = [ctypes.c_longlong() for _ in range(3)]
+        self._rc("hv_sor_counts", self.lib.hv_sor_counts(self.ctx, *[ctypes.byref(x) for x in v]))
+        return {"solves": v[0].value, "iters_f32": v[1].value, "iters_f64": v[2].value}
+
 
 def dipole(stats: np.ndarray, z: int, n: int) -> np.ndarray:
     """D(t) = N Z / A (<z>_p - <z>_n)  (scripts/programs.py:99)."""

@@ -190,6 +190,16 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * sweeps fused, out of place, the red ring recomputed per block); 0 (default)
  * one launch per sweep.  Same bits. */
 #define HV_OPT_SOR_FUSED 2
+/* HV_OPT_SOR_RESIDENT_F32: 1 (default) runs the mixed form's FP32 correction
+ * iteration as one cooperative launch with the correction and its source held
+ * in LDS, when the grid fits; 0 uses one launch per sweep.  The FP64 sweeps
+ * that end every mixed solve apply the reference's stop rule either way. */
+#define HV_OPT_SOR_RESIDENT_F32 3
+/* HV_OPT_SOR_F32_LAG: the LDS-resident FP32 iteration stops after iteration
+ * it once the max |update| of iteration it - lag is below tol (default 2,
+ * range 1..64), so that its tiles never wait on a whole-grid reduction of the
+ * current one. */
+#define HV_OPT_SOR_F32_LAG 4
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
@@ -212,9 +222,13 @@ const char *hv_build_info(void);
 #define HV_KCLASS_SMEAR 2     /* the smearing passes of one step (one launch = one step's set) */
 #define HV_KCLASS_POTENTIAL 3 /* Skyrme + Coulomb merge */
 #define HV_KCLASS_SOR_F32 4   /* one sweep of the mixed form's FP32 SOR correction */
-#define HV_KCLASS_COUNT 5
+#define HV_KCLASS_SOR_RES_F32 5 /* one LDS-resident launch of the FP32 correction (a whole solve) */
+#define HV_KCLASS_COUNT 6
 int hv_profile_enable(hv_ctx *ctx, int on); /* also resets the counters */
 int hv_profile_read(hv_ctx *ctx, int kclass, double *ms, long long *launches);
+/* SOR work since the last hv_profile_enable: solves, FP32 correction
+ * iterations (mixed form) and FP64 iterations. */
+int hv_sor_counts(hv_ctx *ctx, long long *solves, long long *iters_f32, long long *iters_f64);
 
 #ifdef __cplusplus
 }

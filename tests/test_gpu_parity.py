@@ -360,27 +360,32 @@ def test_sor_fused_equals_per_sweep(oracle, n, d):
     _eq(outs[0][2], outs[1][2])
 
 
-def test_mixed_sor_matches_fp64_iterations():
+@pytest.mark.parametrize("res32", [0, 1])
+def test_mixed_sor_matches_fp64_iterations(res32):
     """The mixed form's FP32 defect-correction SOR follows the FP64 iteration:
     the FP32 correction carries the same iterates up to rounding, so the count
-    matches the FP64 solve within a few final FP64 sweeps, and the potential
-    agrees within the stop tolerance's reach."""
-    print("mixed SOR vs fp64", flush=True)
+    matches the FP64 solve within a few final FP64 sweeps (plus the lag of the
+    LDS-resident FP32 iteration's convergence test), and the potential agrees
+    within the stop tolerance's reach."""
+    print("mixed SOR vs fp64, resident FP32", res32, flush=True)
     n, d = (48, 48, 48), (9.6, 9.6, 9.6)
     G = int(np.prod(n))
     rho_p = smooth_field(16, n, 0.09)
     res = {}
     for prec in (hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED):
         sim = _sim(n, d, 10, 10, prec)
+        sim.set_option(hvamd.HV_OPT_SOR_RESIDENT_F32, res32)
         sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p, np.zeros(G)]))
         sim.set_initial_coulomb_boundaries(82)
         it0, _ = sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 512)  # cold: FP64 in both
         sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p * 1.01, np.zeros(G)]))
         it1, md1 = sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 512)  # warm: FP32 correction
+        used = sim.get_option(hvamd.HV_OPT_SOR_RESIDENT_F32)
         res[prec] = (it0, it1, md1, sim.download_field(hvamd.HV_FIELD_COULOMB))
         sim.close()
     f, m = res[hvamd.HV_PREC_F64], res[hvamd.HV_PREC_MIXED]
-    print("iters fp64", f[:2], "mixed", m[:2])
+    print("iters fp64", f[:2], "mixed", m[:2], "resident FP32 used", used)
+    assert used == res32
     assert f[0] == m[0]
-    assert abs(f[1] - m[1]) <= 12 and m[2] < 1e-6
+    assert abs(f[1] - m[1]) <= 12 + 2 * res32 and m[2] < 1e-6
     assert np.abs(f[3] - m[3]).max() < 2e-5
