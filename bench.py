@@ -27,6 +27,24 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "hyper-vlasov_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_c2.json"
+
+
+def measured_traffic(rl, workload):
+    """HBM bytes per launch of the roofline kernel from the committed counter
+    profile (FETCH_SIZE / WRITE_SIZE per dispatch, scaled by a plain copy of
+    known size), for the 128^3 workload only; None when not profiled."""
+    if workload != "c2" or not TRAFFIC_FILE.exists():
+        return None
+    d = json.loads(TRAFFIC_FILE.read_text())
+    if rl["kernel"].startswith("k_sor_resident_f32"):
+        return d["k_sor_resident_f32_bytes_per_iteration"] * rl["iterations_per_launch"]
+    for name, v in d["kernels"].items():
+        if rl["kernel"].startswith("k_sor_colour_f32") and name.startswith("k_sor_colour_f32"):
+            return v["hbm_bytes"]
+        if rl["kernel"].startswith("k_sor_colour (") and name == "k_sor_colour<double>":
+            return v["hbm_bytes"]
+    return None
 
 
 def sor_roofline(w, res):
@@ -183,7 +201,7 @@ def main():
         "sor_counts": mixed["sor"],
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in mixed["prof"].items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, **rl},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(rl, args.workload), **rl},
     }
     if fp64 is not None:
         v64 = w.total * args.steps / fp64["wall"]

@@ -212,11 +212,12 @@ int hv_timer_stop(hv_ctx *ctx, double *ms);
 const char *hv_build_info(void);
 
 /* ---- kernel-class profiling (roofline accounting) -------------------------- */
-/* When enabled, the loop brackets kernels with hipEvent pairs on the context
- * stream and accumulates device milliseconds and bracketed launches per class:
- * every particle pass, smearing and potential launch, and one SOR sweep
- * launch in eight (each bracketed alone, so launch gaps stay out; the sampled
- * average is the kernel's average duration). */
+/* When enabled, the loop times kernels with hipEvent pairs on the context
+ * stream and accumulates device milliseconds and timed launches per class:
+ * every particle pass, smearing and potential launch, every LDS-resident SOR
+ * launch, and one SOR sweep launch in eight (issued with start/stop events
+ * that the runtime stamps at that dispatch's own begin and end, so launch gaps
+ * stay out; the sampled average is the kernel's average duration). */
 #define HV_KCLASS_SOR 0       /* one red or black SOR sweep */
 #define HV_KCLASS_PARTICLE 1  /* fused gather + kick + drift + deposit + diagnostics pass */
 #define HV_KCLASS_SMEAR 2     /* the smearing passes of one step (one launch = one step's set) */
