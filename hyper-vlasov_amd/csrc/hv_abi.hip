@@ -68,6 +68,8 @@ struct hv_ctx {
     double *maxc = nullptr;             // per-iteration max |dphi|, collected for read-back
     int maxd_cap = 0;
     long long *partials = nullptr;      // per-block diagnostics partial sums (6 per block)
+    long long *en_partials = nullptr;   // total energy: per-block partial sums
+    unsigned long long *en_out = nullptr;  // total energy: exact 128-bit sums (3 components)
     SorCtl *ctl = nullptr;
     StatSlot *slots = nullptr;
     int slot_cap = 0;
@@ -1058,6 +1060,44 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
     return warn | (st & HV_WARN_PARTICLE_OUTSIDE);
 }
 
+// ---- total energy (no counterpart in the reference) -------------------------------
+constexpr int kEnergyBlocks = 1024;
+
+template <typename T>
+int total_energy(hv_ctx *c, const hv_skyrme *s, int ppn, double out[4]) {
+    if (!s || !out || ppn <= 0) return set_err(c, HV_ERR_ARG, "total energy needs skyrme, out and ppn > 0");
+    if (!c->en_partials) {
+        HV_HIP(c, hipMalloc((void **)&c->en_partials, sizeof(long long) * 3 * kEnergyBlocks));
+        HV_HIP(c, hipMalloc((void **)&c->en_out, sizeof(unsigned long long) * 6));
+    }
+    const Particles<T> pp = particles<T>(c);
+    k_energy_kin<T><<<kEnergyBlocks, kThreads, 0, c->stream>>>(pp.kx, pp.ky, pp.kz, c->total, c->en_partials);
+    HV_CHECK_LAUNCH(c);
+    k_sum_partials<<<1, 64, 0, c->stream>>>(c->en_partials, kEnergyBlocks, 1, c->en_out);
+    HV_CHECK_LAUNCH(c);
+    const double dV = hv_spacing(c->world.d_max[0], c->world.n[0]) * hv_spacing(c->world.d_max[1], c->world.n[1]) *
+                      hv_spacing(c->world.d_max[2], c->world.n[2]);
+    k_energy_grid<T><<<kEnergyBlocks, kThreads, 0, c->stream>>>((const T *)c->density, c->phi, c->G, c->sp, s->A,
+                                                                s->B, s->C, s->gamma, dV,
+                                                                c->en_partials + kEnergyBlocks);
+    HV_CHECK_LAUNCH(c);
+    k_sum_partials<<<1, 64, 0, c->stream>>>(c->en_partials + kEnergyBlocks, kEnergyBlocks, 2, c->en_out + 2);
+    HV_CHECK_LAUNCH(c);
+    unsigned long long h[6];
+    HV_HIP(c, hipMemcpyAsync(h, c->en_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    double v[3];
+    for (int q = 0; q < 3; q++) {
+        const __int128 S = (__int128)(((unsigned __int128)h[2 * q + 1] << 64) | (unsigned __int128)h[2 * q]);
+        v[q] = (double)S * HV_STAT_INV;
+    }
+    out[1] = HV_KIN_FAC * v[0] / (double)ppn;
+    out[2] = v[1];
+    out[3] = v[2];
+    out[0] = out[1] + out[2] + out[3];
+    return HV_OK;
+}
+
 // dispatch on precision
 #define HV_DISPATCH(c, call_f64, call_mixed) ((c)->precision == HV_PREC_F64 ? (call_f64) : (call_mixed))
 
@@ -1209,7 +1249,8 @@ int hv_ctx_destroy(hv_ctx *c) {
         if (b) (void)hipFree(b);
     void *bufs[] = {c->acc, c->density, c->U, c->phi, c->phi_alt, c->rho_s, c->sor_R, c->sor_delta,
                     c->sor_delta_alt, c->H, c->D, c->F[0], c->F[1], c->F[2],
-                    c->smear_tab, c->maxd, c->maxc, c->ctl, c->slots, c->status, c->partials, c->sflags};
+                    c->smear_tab, c->maxd, c->maxc, c->ctl, c->slots, c->status, c->partials, c->sflags,
+                    c->en_partials, c->en_out};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->pinned) (void)hipHostFree(c->pinned);
@@ -1469,6 +1510,12 @@ int hv_mean_squared_radius(hv_ctx *c, int type, double *out) {
     if (rc) return rc;
     *out = st[2 + type];
     return HV_OK;
+}
+
+int hv_total_energy(hv_ctx *c, const hv_skyrme *s, int part_per_nucleon, double out[4]) {
+    if (!c) return HV_ERR_ARG;
+    return HV_DISPATCH(c, total_energy<double>(c, s, part_per_nucleon, out),
+                       total_energy<float>(c, s, part_per_nucleon, out));
 }
 
 int hv_run_init(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *sor_iters) {

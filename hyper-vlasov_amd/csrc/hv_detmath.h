@@ -216,6 +216,29 @@ HV_HD HV_INLINE double hv_skyrme_f64(double A, double B, double C, double gamma,
     return A * t + B * hv_det_pow(t, gamma) + 2.0 * tau * C * ((rho_n - rho_p) / HV_RHO0);
 }
 
+/* ---- total energy (no counterpart in the reference) ----------------------- */
+/* Energy density [MeV fm^-3] of the mean field, split into its Skyrme and
+ * Coulomb parts:
+ *   e_sky  = A rho^2 / (2 rho0) + B rho0 t^(gamma+1) / (gamma+1) + C (rho_n - rho_p)^2 / rho0
+ *   e_coul = rho_p phi / 2
+ * with rho = rho_p + rho_n and t = rho / rho0.  Its functional derivative in
+ * rho_q is skyrme_potential (include/physics_formulas.hpp:71-77) plus, for
+ * protons, phi (phi linear in rho_p through the symmetric discrete Laplacian).
+ * t^(gamma+1) is taken as t * t^gamma with the same hv_det_pow the potential
+ * uses. */
+HV_HD HV_INLINE void hv_energy_density(double A, double B, double C, double gamma, double rho_p, double rho_n,
+                                       double phi, double *e_sky, double *e_coul) {
+    double rho = rho_p + rho_n;
+    double t = rho / HV_RHO0;
+    double d = rho_n - rho_p;
+    *e_sky = A * rho * rho / (2.0 * HV_RHO0) + B * HV_RHO0 * (t * hv_det_pow(t, gamma)) / (gamma + 1.0) +
+             C * d * d / HV_RHO0;
+    *e_coul = 0.5 * rho_p * phi;
+}
+
+/* kinetic_energy() (include/physics_formulas.hpp:45-49): (hbar c)^2 / (2 m c^2) [MeV fm^2] */
+#define HV_KIN_FAC (HV_HBARC * HV_HBARC / (2.0 * HV_MC2))
+
 /* ---- grid geometry (include/tools.hpp:395-401, include/physics.hpp:102) ---- */
 /* node i of n along an axis of half-width d_max sits at d_max (2 i / n - 1) */
 HV_HD HV_INLINE double hv_node_pos(double d_max, int n, int i) {

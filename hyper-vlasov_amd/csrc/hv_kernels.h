@@ -550,6 +550,54 @@ k_skyrme(const T *__restrict__ rho, T *__restrict__ U, const double *__restrict_
     U[i + G] = (T)un;
 }
 
+// ---- total energy (no counterpart in the reference) -----------------------------
+// Grid part: per node dV * e_sky and dV * e_coul (hv_energy_density) of the
+// smeared density and phi, quantised to 2^-30 and summed exactly; partials
+// [2 * block + {0, 1}].
+template <typename T>
+__global__ void k_energy_grid(const T *__restrict__ rho, const double *__restrict__ phi, long long G, Split sp,
+                              double A, double B, double C, double gamma, double dV, long long *__restrict__ partials) {
+    long long v[2] = {0, 0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < G; i += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(i % sp.nz), j = (int)((i / sp.nz) % sp.ny), ii = (int)(i / ((long long)sp.ny * sp.nz));
+        double es, ec;
+        hv_energy_density(A, B, C, gamma, (double)rho[i], (double)rho[i + G], phi[split_index(sp, ii, j, k)], &es, &ec);
+        v[0] += hv_stat_fix(es * dV);
+        v[1] += hv_stat_fix(ec * dV);
+    }
+    block_sum_i64<2>(v);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = v[0];
+        partials[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// Kinetic part: sum of |k|^2 over all test particles, quantised to 2^-30;
+// partials[block].
+template <typename T>
+__global__ void k_energy_kin(const T *__restrict__ kx, const T *__restrict__ ky, const T *__restrict__ kz, int total,
+                             long long *__restrict__ partials) {
+    long long v[1] = {0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const double a = (double)kx[i], b = (double)ky[i], c = (double)kz[i];
+        v[0] += hv_stat_fix(a * a + b * b + c * c);
+    }
+    block_sum_i64<1>(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+}
+
+// exact 128-bit sums of nv interleaved components over nblocks partials:
+// out[2 * c] = low word, out[2 * c + 1] = high word of component c
+__global__ void k_sum_partials(const long long *__restrict__ partials, int nblocks, int nv,
+                               unsigned long long *__restrict__ out) {
+    const int c = threadIdx.x;
+    if (c >= nv) return;
+    unsigned long long lo = 0, hi = 0;
+    for (int b = 0; b < nblocks; b++) add_i128(lo, hi, partials[(long long)nv * b + c]);
+    out[2 * c] = lo;
+    out[2 * c + 1] = hi;
+}
+
 // U_p += phi (include/tools.hpp:378-384)
 template <typename T>
 __global__ void k_merge(T *__restrict__ U, const double *__restrict__ phi, long long G, Split sp) {

@@ -131,6 +131,7 @@ SIGNATURES = {
     "hv_set_option": (_I, [_P, _I, _I]),
     "hv_get_option": (_I, [_P, _I, _IP]),
     "hv_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
+    "hv_total_energy": (_I, [_P, ctypes.POINTER(HvSkyrme), _I, _DP]),
     "hv_sor_counts": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                            ctypes.POINTER(ctypes.c_longlong)]),
     "hv_fermi_momentum": (_D, [_D]),

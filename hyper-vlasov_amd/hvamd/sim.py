@@ -225,6 +225,14 @@ class Simulation:
     def nuclear_excitation(self, eta: float, z: int, n: int, mode: int = HV_BOOST_REF_WEIGHTED) -> int:
         return self._rc("hv_excite", self.lib.hv_excite(self.ctx, eta, z, n, mode))
 
+    def total_energy(self, skyrme: Skyrme, part_per_nucleon: int) -> np.ndarray:
+        """hv_total_energy: [E, E_kin, E_sky, E_coul] of the current state [MeV]."""
+        out = np.zeros(4)
+        s = skyrme.c()
+        self._rc("hv_total_energy", self.lib.hv_total_energy(self.ctx, ctypes.byref(s), part_per_nucleon,
+                                                             out.ctypes.data_as(_lib._DP)))
+        return out
+
     def center_of_mass(self, type_: int) -> float:
         out = ctypes.c_double()
         self._rc("hv_center_of_mass", self.lib.hv_center_of_mass(self.ctx, type_, ctypes.byref(out)))

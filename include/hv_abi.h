@@ -159,6 +159,22 @@ int hv_center_of_mass(hv_ctx *ctx, int type, double *out);
 /* mean_squared_radius (include/tools.hpp:303-325): <r^2> of in-box particles of a species */
 int hv_mean_squared_radius(hv_ctx *ctx, int type, double *out);
 
+/* ---- total energy (no counterpart in the reference) ---------------------- */
+/* Total energy of the current state [MeV], the conserved quantity whose drift
+ * measures the integration error:
+ *   E_kin  = (hbar c)^2 / (2 m c^2) sum_i |k_i|^2 / ppn   (the prefactor is
+ *            kinetic_energy(), include/physics_formulas.hpp:45-49)
+ *   E_sky  = dV sum_cells [A rho^2/(2 rho0) + B rho0 t^(gamma+1)/(gamma+1)
+ *            + C (rho_n - rho_p)^2 / rho0],   t = (rho_p + rho_n) / rho0
+ *   E_coul = dV sum_cells rho_p phi / 2
+ * whose derivative in rho_q is skyrme_potential (include/physics_formulas.hpp:71-77)
+ * plus phi for protons.  Evaluated on the particles' momenta and on the
+ * smeared density and phi held by the context, which hv_run_init and
+ * hv_run_steps leave consistent with the positions (momenta synchronous after
+ * the closing half kick).  Each |k_i|^2 and dV*e is quantised to 2^-30 and
+ * summed exactly (order-free).  out = {E_kin + E_sky + E_coul, E_kin, E_sky, E_coul}. */
+int hv_total_energy(hv_ctx *ctx, const hv_skyrme *skyrme, int part_per_nucleon, double out[4]);
+
 /* ---- fused step loop (the body of cpu_simulate, src/hyper_vlasov.cpp:53-104) */
 /* hv_run_init: Coulomb faces + pre-loop field solve + gather (:53-62).
  * hv_run_steps: steps [first_step, first_step + nsteps) of the loop (:63-104);

@@ -539,3 +539,28 @@ long long hvo_div_fixed_mismatches(double b, double lo, double hi, long long n, 
     }
     return bad;
 }
+
+/* ---- total energy (no reference counterpart) ------------------------------ */
+void hvo_total_energy(const double *kx, const double *ky, const double *kz, int total, const double *rho,
+                      const double *phi, const hvo_world *w, const hvo_skyrme *s, int ppn, double out[4]) {
+    __int128 sk = 0, ss = 0, sc = 0;
+    for (int i = 0; i < total; i++) {
+        double k2 = kx[i] * kx[i] + ky[i] * ky[i] + kz[i] * kz[i];
+        sk += hv_stat_fix(k2);
+    }
+    const long long G = (long long)w->n[0] * w->n[1] * w->n[2];
+    const double dV = hv_spacing(w->d_max[0], w->n[0]) * hv_spacing(w->d_max[1], w->n[1]) *
+                      hv_spacing(w->d_max[2], w->n[2]);
+    for (long long q = 0; q < G; q++) {
+        double es, ec;
+        hv_energy_density(s->A, s->B, s->C, s->gamma, rho[q], rho[q + G], phi[q], &es, &ec);
+        ss += hv_stat_fix(es * dV);
+        sc += hv_stat_fix(ec * dV);
+    }
+    const double ekin = HV_KIN_FAC * ((double)sk * HV_STAT_INV) / (double)ppn;
+    const double esky = (double)ss * HV_STAT_INV, ecoul = (double)sc * HV_STAT_INV;
+    out[1] = ekin;
+    out[2] = esky;
+    out[3] = ecoul;
+    out[0] = ekin + esky + ecoul;
+}

@@ -115,6 +115,15 @@ int hvo_simulate_loop(double *x, double *y, double *z, double *kx, double *ky, d
                       const hvo_skyrme *s, const hvo_params *p, double *phi_inout, double *stats,
                       int *sor_iters);
 
+/* Total energy of a state (no reference counterpart; hv_energy_density in
+ * hv_detmath.h): out = {E_kin + E_sky + E_coul, E_kin, E_sky, E_coul} [MeV],
+ * E_kin = (hbar c)^2/(2 m c^2) sum_i |k_i|^2 / ppn, the grid parts
+ * dV sum_cells e_sky / e_coul over the smeared density rho (2G) and phi (G);
+ * every per-particle |k|^2 and per-cell dV*e is quantised to 2^-30 and summed
+ * exactly, like the diagnostics sums. */
+void hvo_total_energy(const double *kx, const double *ky, const double *kz, int total, const double *rho,
+                      const double *phi, const hvo_world *w, const hvo_skyrme *s, int ppn, double out[4]);
+
 /* mismatches of the fixed-divisor division (hv_detmath.h) against a / b */
 long long hvo_div_fixed_mismatches(double b, double lo, double hi, long long n, unsigned long long seed);
 

@@ -57,6 +57,7 @@ class Oracle:
         L.hvo_sor.argtypes = [_P, _P, _P, _D, _D, _I, _P, _P]
         L.hvo_center_of_mass.argtypes = [_P, _I, _I, _I]
         L.hvo_mean_squared_radius.argtypes = [_P, _P, _P, _I, _I, _P, _I]
+        L.hvo_total_energy.argtypes = [_P, _P, _P, _I, _P, _P, _P, _P, _I, _P]
         self.L = L
 
     # each returns new arrays; inputs are float64 numpy arrays
@@ -130,6 +131,15 @@ class Oracle:
         kz = kz.copy()
         self.L.hvo_excite(p(kz), np_, nn, eta, z, n, mode)
         return kz
+
+    def total_energy(self, k, rho, phi, n, d_max, A, B, C, gamma, ppn):
+        """[E, E_kin, E_sky, E_coul] (hvo_total_energy)."""
+        out = np.zeros(4)
+        w = oworld(n, d_max)
+        s = OSkyrme(A, B, C, gamma)
+        self.L.hvo_total_energy(p(k[0]), p(k[1]), p(k[2]), len(k[0]), p(rho), p(phi), ctypes.byref(w),
+                                ctypes.byref(s), ppn, p(out))
+        return out
 
     def stats(self, x, y, z, np_, nn, n, d_max):
         st = np.zeros(4)

@@ -178,6 +178,35 @@ def test_fused_loop_bitwise_c0(oracle, steps):
         _eq(got[k], st_o[k])
 
 
+@pytest.mark.parametrize("prec", [hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED])
+def test_total_energy_matches_oracle(oracle, prec):
+    """hv_total_energy on the loop's state equals the oracle's exact fixed-point
+    evaluation of the same (downloaded) momenta, density and phi, bit for bit
+    in both forms (the mixed form's FP32 values widen exactly)."""
+    print(f"total energy prec={prec}", flush=True)
+    w = configs.c0()
+    x, y, z, kx, ky, kz = w.initial_state()
+    sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, prec)
+    sim.upload_particles(x, y, z, kx, ky, kz)
+    prm = w.params(6)
+    sk = w.sky()
+    sim.run_init(sk, prm)
+    e0 = sim.total_energy(sk, prm.part_per_nucleon)
+    sim.run_steps(sk, prm, 0, 6)
+    e1 = sim.total_energy(sk, prm.part_per_nucleon)
+    st = sim.download_particles()
+    rho = sim.download_field(hvamd.HV_FIELD_DENSITY)
+    phi = sim.download_field(hvamd.HV_FIELD_COULOMB)
+    sim.close()
+    n, d = w.world().n, w.world().d_max
+    want = oracle.total_energy((st["kx"], st["ky"], st["kz"]), rho, phi, n, d, sk.A, sk.B, sk.C, sk.gamma,
+                               prm.part_per_nucleon)
+    print("E init", e0, "after 6 steps", e1)
+    _eq(e1, want)
+    assert e1[1] > 0 and e1[2] < 0 and e1[3] > 0
+    assert abs(e1[0] - e0[0]) < 0.05 * abs(e0[1])
+
+
 def test_fused_loop_chunked_equals_single(oracle):
     """run_steps in chunks gives the same bits as one call (the boost and the
     split half kicks are carried across calls)."""
