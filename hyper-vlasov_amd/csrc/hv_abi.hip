@@ -89,6 +89,8 @@ struct hv_ctx {
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int sort_every = 8;  // steps between sorts; 0 = never
+    int smear_tk_max = 64;  // widest smearing tile (columns) tried
+    bool own_stream = true; // false for the instances of a batch (the batch's stream)
     // LDS-resident SOR (hv_sor_resident.h)
     int sor_resident = 0;     // option: use it when the grid fits (off by default: slower than per-sweep launches at 64^3..128^3)
     int res_state = 0;        // 0 unknown, 1 available, -1 not available
@@ -328,11 +330,11 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
     HV_HIP(c, hipMemcpyAsync(c->smear_tab, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HV_HIP(c, hipStreamSynchronize(c->stream));
     size_t n2 = (size_t)(2 * c->G);
-    // LDS tiles: TJ = 8 rows, TK the widest of 64/32/16 columns that fits 64 KB
+    // LDS tiles: TJ = 8 rows, TK the widest of smear_tk_max, /2, ... columns that fits 64 KB
     SmearTiles stl;
     stl.TJ = 8;
     size_t lds = 0;
-    for (int tk = 64; tk >= 8; tk /= 2) {
+    for (int tk = c->smear_tk_max; tk >= 8; tk /= 2) {
         stl.TK = tk;
         const size_t R = stl.TJ + 2 * tb.ry, W = tk + 2 * tb.rz;
         lds = sizeof(T) * (R * W + (size_t)(tb.rz + 1) * R * tk);
@@ -683,6 +685,37 @@ int sor_exact(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, bool fuse
     });
 }
 
+// SOR coefficients of the context's grid (include/physics.hpp:101-103)
+SorCoef make_sor_coef(const hv_ctx *c, double omega, double tol) {
+    const int nx = c->world.n[0], ny = c->world.n[1], nz = c->world.n[2];
+    SorCoef cf;
+    double dx = 2.0 * c->world.d_max[0] / nx, dy = 2.0 * c->world.d_max[1] / ny, dz = 2.0 * c->world.d_max[2] / nz;
+    cf.dx2 = dx * dx;
+    cf.dy2 = dy * dy;
+    cf.dz2 = dz * dz;
+    cf.ydx2 = 1.0 / cf.dx2;
+    cf.ydy2 = 1.0 / cf.dy2;
+    cf.ydz2 = 1.0 / cf.dz2;
+    cf.inv_dx2 = 1.0 / (2.0 / (dx * dx) + 2.0 / (dy * dy) + 2.0 / (dz * dz));
+    cf.src = 4.0 * HV_PI * HV_E2;
+    cf.omega = omega;
+    cf.one_minus_omega = 1.0 - omega;
+    cf.tol = tol;
+    return cf;
+}
+
+// tiles of the per-sweep SOR kernels (kSorPlanes planes x kSorRows rows x 64
+// k-slots per block, XCD slabs)
+SorTiles make_sor_tiles(const hv_ctx *c) {
+    SorTiles tl;
+    tl.n_ic = (c->world.n[0] - 2 + kSorPlanes - 1) / kSorPlanes;
+    tl.n_jb = (c->world.n[1] - 2 + kSorRows - 1) / kSorRows;
+    tl.n_hb = (c->sp.nzh + 63) / 64;
+    const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
+    tl.per_xcd = (ntiles + kXcds - 1) / kXcds;
+    return tl;
+}
+
 // Red-black SOR on c->phi with the proton density (first G values of `rho`).
 // The host launches iterations in chunks and reads back each chunk's
 // per-iteration max |dphi|; kernels launched past the converged iteration
@@ -700,19 +733,7 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
     if (rc) return rc;
     rc = ensure_pinned(c, (size_t)max_it + 8);
     if (rc) return rc;
-    SorCoef cf;
-    double dx = 2.0 * c->world.d_max[0] / nx, dy = 2.0 * c->world.d_max[1] / ny, dz = 2.0 * c->world.d_max[2] / nz;
-    cf.dx2 = dx * dx;
-    cf.dy2 = dy * dy;
-    cf.dz2 = dz * dz;
-    cf.ydx2 = 1.0 / cf.dx2;
-    cf.ydy2 = 1.0 / cf.dy2;
-    cf.ydz2 = 1.0 / cf.dz2;
-    cf.inv_dx2 = 1.0 / (2.0 / (dx * dx) + 2.0 / (dy * dy) + 2.0 / (dz * dz));
-    cf.src = 4.0 * HV_PI * HV_E2;
-    cf.omega = omega;
-    cf.one_minus_omega = 1.0 - omega;
-    cf.tol = tol;
+    const SorCoef cf = make_sor_coef(c, omega, tol);
     if (c->sor_resident && res_setup<RT>(c) == HV_OK) {
         int iters = 0;
         double md = 0.0;
@@ -725,12 +746,7 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         if (maxdiff_out) *maxdiff_out = md;
         return (md > tol) ? HV_WARN_SOR_NOT_CONVERGED : HV_OK;
     }
-    SorTiles tl;
-    tl.n_ic = (nx - 2 + kSorPlanes - 1) / kSorPlanes;
-    tl.n_jb = (ny - 2 + kSorRows - 1) / kSorRows;
-    tl.n_hb = (c->sp.nzh + 63) / 64;
-    const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
-    tl.per_xcd = (ntiles + kXcds - 1) / kXcds;
+    const SorTiles tl = make_sor_tiles(c);
     const dim3 sblk(64, kSorRows, 1), sgrd((unsigned)(tl.per_xcd * kXcds), 1, 1);
     FusedTiles ft;
     ft.ti = (nx + kFT - 1) / kFT;
@@ -1258,7 +1274,7 @@ int hv_ctx_destroy(hv_ctx *c) {
     for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HV_OK;
 }
@@ -1575,6 +1591,12 @@ int hv_set_option(hv_ctx *c, int option, int value) {
     case HV_OPT_SOR_RESIDENT_F32:
         c->sor_res32 = value ? 1 : 0;
         return HV_OK;
+    case HV_OPT_SMEAR_TILE:
+        if (value != 8 && value != 16 && value != 32 && value != 64)
+            return set_err(c, HV_ERR_ARG, "smearing tile width must be 8, 16, 32 or 64");
+        c->smear_tk_max = value;
+        c->tab_sigma = -1.0;  // rebuild the tiles at the next smearing
+        return HV_OK;
     case HV_OPT_SOR_F32_LAG:
         // the test of row it - lag runs before this iteration's black sweep,
         // so row it itself is never complete then: lag >= 1
@@ -1593,6 +1615,7 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
     case HV_OPT_SOR_FUSED: *value = c->sor_fused; return HV_OK;
     case HV_OPT_SOR_RESIDENT_F32: *value = c->sor_res32 && c->res32_state >= 0; return HV_OK;
     case HV_OPT_SOR_F32_LAG: *value = c->res32_lag; return HV_OK;
+    case HV_OPT_SMEAR_TILE: *value = c->smear_tk_max; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -1641,3 +1664,6 @@ const char *hv_build_info(void) {
 }
 
 }  // extern "C"
+
+// ---- ensembles (hv_batch_*) -------------------------------------------------------
+#include "hv_batch_impl.h"

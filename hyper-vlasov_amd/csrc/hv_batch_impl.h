@@ -1,0 +1,590 @@
+// hv_batch_impl.h -- ensembles of nuclei stepped together (included at the end
+// of hv_abi.hip: it uses that file's context and helpers).
+//
+// A batch owns one context per instance (its own grid, particle counts,
+// buffers), all on the batch's stream.  hv_batch_run_steps runs the step loop
+// of every instance in lock step with one launch per loop stage for the whole
+// ensemble (hv_batch_kernels.h): particle pass, diagnostics, smearing, the
+// SOR sweeps (instances converge independently; a converged instance's blocks
+// return at once), the Skyrme + Coulomb potential.  Sorting, the pre-loop
+// solve and the per-instance read-backs stay per instance.
+//
+// Per-instance results equal a solo hv_run_steps of the same instance with the
+// same options bit for bit: the same block bodies run on the same operands and
+// the SOR stop rule is evaluated per instance.  Batch instances use the
+// per-sweep SOR launches (the LDS-resident and fused SOR forms are
+// single-instance only).
+
+#include "hv_batch_kernels.h"
+
+enum { kBPart, kBZy, kBN2, kBG, kBGh2, kBSor, kBKinds };
+
+struct hv_batch {
+    int n = 0;
+    int precision = HV_PREC_F64;
+    std::vector<hv_ctx *> inst;
+    hipStream_t stream = nullptr;
+    void *args = nullptr;                 // BInst<T>[n]
+    std::vector<unsigned char> hargs;     // host image of args
+    int *first = nullptr;                 // kBKinds prefix tables of n + 1 ints
+    unsigned blocks[kBKinds] = {};
+    size_t smear_lds = 0;
+    double *maxc = nullptr;               // collected SOR maxima, n x maxc_width
+    int maxc_width = 0;
+    double *pinned = nullptr;
+    size_t pinned_cap = 0;
+    std::vector<int> pred32, pred64;      // per-instance SOR iteration predictions
+    std::vector<int> boost;               // per-instance boost flag of the current step
+    bool ready = false;
+    int sort_every = 8;
+    // stage timing (hipEvent pairs per stage, when enabled)
+    bool prof = false;
+    double prof_ms[HV_KCLASS_COUNT] = {};
+    long long prof_launches[HV_KCLASS_COUNT] = {};
+    std::vector<hipEvent_t> ev;
+    struct Sample {
+        int cls;
+        hipEvent_t a, e;
+    };
+    std::vector<Sample> samples;  // sampled sweep launches, folded at the next read-back
+    char err[512] = {0};
+};
+
+namespace {
+
+int batch_err(hv_batch *b, int code, const char *fmt, ...) {
+    if (b) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(b->err, sizeof b->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+// a failing instance call: copy its message
+int batch_fwd(hv_batch *b, hv_ctx *c, int rc) {
+    if (rc < 0) batch_err(b, rc, "%s", c->err);
+    return rc;
+}
+
+#define HV_BHIP(b, expr)                                                                       \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return batch_err((b), HV_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                             __FILE__, __LINE__);                                              \
+    } while (0)
+
+int batch_pinned(hv_batch *b, size_t doubles) {
+    if (b->pinned_cap >= doubles) return HV_OK;
+    if (b->pinned) (void)hipHostFree(b->pinned);
+    b->pinned = nullptr;
+    b->pinned_cap = 0;
+    HV_BHIP(b, hipHostMalloc((void **)&b->pinned, doubles * sizeof(double), hipHostMallocDefault));
+    b->pinned_cap = doubles;
+    return HV_OK;
+}
+
+double step_dt(const hv_params &p) { return p.t_f / p.steps; }
+
+// Per-instance argument image (host) for the current buffers and parameters.
+template <typename T>
+void batch_fill_args(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
+    b->hargs.assign(sizeof(BInst<T>) * (size_t)b->n, 0);
+    BInst<T> *h = (BInst<T> *)b->hargs.data();
+    for (int i = 0; i < b->n; i++) {
+        hv_ctx *c = b->inst[i];
+        const hv_params &p = prm[i];
+        BInst<T> &A = h[i];
+        A.p = particles<T>(c);
+        A.np = c->np;
+        A.total = c->total;
+        A.g = make_geom<T>(c);
+        A.U = (T *)c->U;
+        A.acc = c->acc;
+        // the factors of run_steps, bit for bit
+        const double dt = step_dt(p);
+        A.kick_fac = (T)(dt / (2.0 * HV_HBARC));
+        A.drift_fac = (T)((dt * HV_HBARC) / HV_MC2);
+        double bp, bn;
+        if (p.boost_mode == HV_BOOST_SYMMETRIC) {
+            bp = p.eta_exc;
+            bn = p.eta_exc;
+        } else {
+            double zz = p.z, nnuc = p.n;
+            bp = p.eta_exc * nnuc / (zz + nnuc);
+            bn = p.eta_exc * zz / (zz + nnuc);
+        }
+        A.boost_p = (T)bp;
+        A.boost_n = (T)bn;
+        A.boost = b->boost[i];
+        A.partials = c->partials;
+        A.status = c->status;
+        A.slots = c->slots;
+        A.part_blocks = (int)blocks_for(c->total);
+        A.tab = (const SmearTab<T> *)c->smear_tab;
+        A.D = (T *)c->D;
+        A.rho = (T *)c->density;
+        A.flags = c->sflags;
+        A.st = c->stl;
+        A.G = c->G;
+        A.sp = c->sp;
+        A.phi = c->phi;
+        A.rho_s = (T *)c->rho_s;
+        A.R = c->sor_R;
+        A.delta = c->sor_delta;
+        A.tl = make_sor_tiles(c);
+        A.cf = make_sor_coef(c, p.sor_omega, p.sor_tol);
+        A.maxd = c->maxd;
+        A.ctl = c->ctl;
+        A.A = sk[i].A;
+        A.B = sk[i].B;
+        A.C = sk[i].C;
+        A.gamma = sk[i].gamma;
+    }
+}
+
+template <typename T>
+int batch_upload_args(hv_batch *b) {
+    HV_BHIP(b, hipMemcpyAsync(b->args, b->hargs.data(), b->hargs.size(), hipMemcpyHostToDevice, b->stream));
+    HV_BHIP(b, hipStreamSynchronize(b->stream));
+    return HV_OK;
+}
+
+// Buffers, block tables and the argument array for a run with these params.
+template <typename T>
+int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslots) {
+    std::vector<int> hf((size_t)kBKinds * (b->n + 1), 0);
+    size_t lds = 0;
+    for (int i = 0; i < b->n; i++) {
+        hv_ctx *c = b->inst[i];
+        int rc = ensure_smear<T>(c, prm[i].sigma_r, prm[i].part_per_nucleon);
+        if (rc) return batch_fwd(b, c, rc);
+        rc = ensure_maxd(c, prm[i].sor_max_it);
+        if (rc) return batch_fwd(b, c, rc);
+        rc = ensure_slots(c, nslots);
+        if (rc) return batch_fwd(b, c, rc);
+        const SorTiles tl = make_sor_tiles(c);
+        const long long cnt[kBKinds] = {
+            (long long)blocks_for(c->total),
+            2LL * c->world.n[0] * c->stl.njt * c->stl.nkt,
+            (long long)blocks_for(2 * c->G),
+            (long long)blocks_for(c->G),
+            (long long)blocks_for(2 * c->sp.Gh),
+            (long long)tl.per_xcd * kXcds,  // a multiple of 8: local block b runs on XCD b % 8
+        };
+        for (int k = 0; k < kBKinds; k++) {
+            const long long nx = hf[(size_t)k * (b->n + 1) + i] + cnt[k];
+            if (nx > 0x7fffffffLL) return batch_err(b, HV_ERR_ARG, "batch too large for one launch");
+            hf[(size_t)k * (b->n + 1) + i + 1] = (int)nx;
+        }
+        lds = c->stl_lds > lds ? c->stl_lds : lds;
+    }
+    for (int k = 0; k < kBKinds; k++) b->blocks[k] = (unsigned)hf[(size_t)k * (b->n + 1) + b->n];
+    b->smear_lds = lds;
+    if (!b->first) HV_BHIP(b, hipMalloc((void **)&b->first, sizeof(int) * hf.size()));
+    HV_BHIP(b, hipMemcpyAsync(b->first, hf.data(), sizeof(int) * hf.size(), hipMemcpyHostToDevice, b->stream));
+    if (!b->args) HV_BHIP(b, hipMalloc(&b->args, sizeof(BInst<T>) * (size_t)b->n));
+    batch_fill_args<T>(b, sk, prm);
+    return batch_upload_args<T>(b);
+}
+
+const int *btab(const hv_batch *b, int kind) { return b->first + (size_t)kind * (b->n + 1); }
+
+hipEvent_t batch_event(hv_batch *b) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    b->ev.push_back(e);
+    return e;
+}
+
+struct BStage {
+    hv_batch *b;
+    int cls;
+    hipEvent_t a = nullptr, e = nullptr;
+    BStage(hv_batch *bb, int k) : b(bb), cls(k) {
+        if (!b->prof) return;
+        a = batch_event(b);
+        if (a) (void)hipEventRecord(a, b->stream);
+    }
+    ~BStage() {
+        if (!a) return;
+        e = batch_event(b);
+        if (!e) return;
+        (void)hipEventRecord(e, b->stream);
+        float ms = 0.0f;
+        if (hipEventSynchronize(e) == hipSuccess && hipEventElapsedTime(&ms, a, e) == hipSuccess) {
+            b->prof_ms[cls] += ms;
+            b->prof_launches[cls] += 1;
+        }
+        for (hipEvent_t x : b->ev) (void)hipEventDestroy(x);
+        b->ev.clear();
+    }
+};
+
+// SOR iterations of every instance in chunks of batched sweeps; iters[i]
+// receives the reference stop rule's count of instance i.
+template <typename T>
+int batch_sor_iterate(hv_batch *b, const hv_params *prm, bool f32, const std::vector<int> &pred,
+                      std::vector<int> &iters, std::vector<double> &lastmax) {
+    const int n = b->n;
+    const BInst<T> *a = (const BInst<T> *)b->args;
+    const int max_it = prm[0].sor_max_it;
+    kb_sor_reset<T><<<n, 64, 0, b->stream>>>(a);
+    HV_BHIP(b, hipGetLastError());
+    iters.assign(n, -1);
+    lastmax.assign(n, 0.0);
+    int remaining = n, it = 0, chunk = 1;
+    for (int i = 0; i < n; i++) chunk = pred[i] - 1 > chunk ? pred[i] - 1 : chunk;
+    const dim3 sblk(64, kSorRows, 1), sgrd(b->blocks[kBSor], 1, 1);
+    while (it < max_it && remaining > 0) {
+        const int start = it, end = it + chunk < max_it ? it + chunk : max_it;
+        for (; it < end; it++)
+            for (int colour = 0; colour < 2; colour++) {
+                // one launch in kSorSample is timed by start/stop events of its own dispatch
+                hipEvent_t ea = nullptr, ee = nullptr;
+                if (b->prof && ((it * 2 + colour) % kSorSample) == 3 &&
+                    hipEventCreate(&ea) == hipSuccess && hipEventCreate(&ee) == hipSuccess) {
+                    b->samples.push_back({f32 ? HV_KCLASS_SOR_F32 : HV_KCLASS_SOR, ea, ee});
+                } else if (ea) {
+                    (void)hipEventDestroy(ea);
+                    ea = nullptr;
+                }
+                if (f32)
+                    hipExtLaunchKernelGGL(kb_sor_colour_f32<T>, sgrd, sblk, 0u, b->stream, ea, ee, 0u,
+                                          (const BInst<T> *)a, btab(b, kBSor), n, colour, it, max_it);
+                else
+                    hipExtLaunchKernelGGL(kb_sor_colour<T>, sgrd, sblk, 0u, b->stream, ea, ee, 0u,
+                                          (const BInst<T> *)a, btab(b, kBSor), n, colour, it, max_it);
+            }
+        HV_BHIP(b, hipGetLastError());
+        const int width = end - start;
+        if (b->maxc_width < width) {
+            if (b->maxc) (void)hipFree(b->maxc);
+            b->maxc = nullptr;
+            b->maxc_width = 0;
+            HV_BHIP(b, hipMalloc((void **)&b->maxc, sizeof(double) * (size_t)n * width));
+            b->maxc_width = width;
+        }
+        int rc = batch_pinned(b, (size_t)n * width);
+        if (rc) return rc;
+        kb_sor_collect<T><<<(unsigned)(n * width), 64, 0, b->stream>>>(a, b->maxc, width, start, end);
+        HV_BHIP(b, hipGetLastError());
+        HV_BHIP(b, hipMemcpyAsync(b->pinned, b->maxc, sizeof(double) * (size_t)n * width, hipMemcpyDeviceToHost,
+                                  b->stream));
+        HV_BHIP(b, hipStreamSynchronize(b->stream));
+        for (auto &smp : b->samples) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, smp.a, smp.e) == hipSuccess) {
+                b->prof_ms[smp.cls] += ms;
+                b->prof_launches[smp.cls] += 1;
+            }
+            (void)hipEventDestroy(smp.a);
+            (void)hipEventDestroy(smp.e);
+        }
+        b->samples.clear();
+        for (int i = 0; i < n; i++) {
+            if (iters[i] >= 0) continue;
+            const double *row = b->pinned + (size_t)i * width;
+            for (int m = 0; m < width; m++) {
+                if (row[m] < prm[i].sor_tol) {
+                    iters[i] = start + m + 1;
+                    lastmax[i] = row[m];
+                    remaining--;
+                    break;
+                }
+            }
+            if (iters[i] < 0) lastmax[i] = row[width - 1];
+        }
+        chunk = 4;
+    }
+    for (int i = 0; i < n; i++)
+        if (iters[i] < 0) iters[i] = max_it;
+    return HV_OK;
+}
+
+template <typename T>
+int batch_field_solve(hv_batch *b, const hv_params *prm, int *iters_out, int *warn) {
+    const int n = b->n;
+    const BInst<T> *a = (const BInst<T> *)b->args;
+    {
+        BStage st(b, HV_KCLASS_SMEAR);
+        kb_smear_zy<T><<<b->blocks[kBZy], 256, b->smear_lds, b->stream>>>(a, btab(b, kBZy), n);
+        kb_smear_xs<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
+        kb_zero_acc<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
+        HV_BHIP(b, hipGetLastError());
+    }
+    std::vector<int> it1, it2;
+    std::vector<double> md1, md2;
+    {
+        BStage st(b, HV_KCLASS_SOR_STAGE);
+        kb_rho_split<T><<<b->blocks[kBG], kThreads, 0, b->stream>>>(a, btab(b, kBG), n);
+        HV_BHIP(b, hipGetLastError());
+        if (b->precision == HV_PREC_MIXED) {
+            kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
+            HV_BHIP(b, hipGetLastError());
+            int rc = batch_sor_iterate<T>(b, prm, true, b->pred32, it1, md1);
+            if (rc) return rc;
+            kb_sor_fold<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
+            HV_BHIP(b, hipGetLastError());
+            std::vector<int> two(n, 2);
+            rc = batch_sor_iterate<T>(b, prm, false, two, it2, md2);
+            if (rc) return rc;
+            for (int i = 0; i < n; i++) {
+                b->pred32[i] = it1[i];
+                b->inst[i]->cnt_it32 += it1[i];
+                b->inst[i]->cnt_it64 += it2[i];
+                b->inst[i]->mixed_fp64_sweeps = it2[i];
+                iters_out[i] = it1[i] + it2[i];
+                if (md2[i] > prm[i].sor_tol) *warn |= HV_WARN_SOR_NOT_CONVERGED;
+            }
+        } else {
+            int rc = batch_sor_iterate<T>(b, prm, false, b->pred64, it1, md1);
+            if (rc) return rc;
+            for (int i = 0; i < n; i++) {
+                b->pred64[i] = it1[i];
+                b->inst[i]->cnt_it64 += it1[i];
+                iters_out[i] = it1[i];
+                if (md1[i] > prm[i].sor_tol) *warn |= HV_WARN_SOR_NOT_CONVERGED;
+            }
+        }
+        for (int i = 0; i < n; i++) b->inst[i]->cnt_solves++;
+    }
+    {
+        BStage st(b, HV_KCLASS_POTENTIAL);
+        kb_skyrme<T><<<b->blocks[kBG], kThreads, 0, b->stream>>>(a, btab(b, kBG), n);
+        HV_BHIP(b, hipGetLastError());
+    }
+    return HV_OK;
+}
+
+int batch_check(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
+    if (!sk || !prm) return batch_err(b, HV_ERR_ARG, "null skyrme or params");
+    for (int i = 0; i < b->n; i++) {
+        int rc = check_params(b->inst[i], &sk[i], &prm[i]);
+        if (rc) return batch_fwd(b, b->inst[i], rc);
+        if (prm[i].sor_max_it != prm[0].sor_max_it)
+            return batch_err(b, HV_ERR_ARG, "the instances of a batch share sor_max_it");
+    }
+    return HV_OK;
+}
+
+template <typename T>
+int batch_run_init(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int *sor_iters) {
+    int rc = batch_check(b, sk, prm);
+    if (rc) return rc;
+    int warn = 0;
+    for (int i = 0; i < b->n; i++) {
+        int it = 0;
+        rc = run_init<T>(b->inst[i], &sk[i], &prm[i], &it);
+        if (rc < 0) return batch_fwd(b, b->inst[i], rc);
+        warn |= rc;
+        if (sor_iters) sor_iters[i] = it;
+        b->pred32[i] = 16;
+        b->pred64[i] = it;
+        b->boost[i] = 0;
+    }
+    b->ready = true;
+    return warn;
+}
+
+template <typename T>
+int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int first, int nsteps, double *stats,
+                    int *sor_iters) {
+    if (!b->ready) return batch_err(b, HV_ERR_STATE, "hv_batch_run_init first");
+    int rc = batch_check(b, sk, prm);
+    if (rc) return rc;
+    if (nsteps <= 0) return HV_OK;
+    const int n = b->n;
+    for (int i = 0; i < n; i++) {
+        hv_ctx *c = b->inst[i];
+        rc = ensure_slots(c, nsteps + 1);
+        if (rc) return batch_fwd(b, c, rc);
+        rc = stats_into<T>(c, 0);  // diagnostics of the positions at the start of the first step
+        if (rc) return batch_fwd(b, c, rc);
+        b->boost[i] = 0;
+    }
+    rc = batch_build<T>(b, sk, prm, nsteps + 1);
+    if (rc) return rc;
+    const BInst<T> *a = (const BInst<T> *)b->args;
+    std::vector<int> its(n);
+    std::vector<std::vector<int>> iters_all;
+    int warn = 0;
+    for (int s = 0; s < nsteps; s++) {
+        const int step = first + s;
+        bool dirty = false;
+        for (int i = 0; i < n; i++) {
+            hv_ctx *c = b->inst[i];
+            void *before = c->part[0];
+            rc = maybe_sort<T>(c);
+            if (rc) return batch_fwd(b, c, rc);
+            if (c->part[0] != before) dirty = true;
+            int bo = 0;
+            if (step * step_dt(prm[i]) >= prm[i].t_exc && !c->excited) {
+                c->excited = 1;
+                bo = 1;
+            }
+            if (bo != b->boost[i]) dirty = true;
+            b->boost[i] = bo;
+        }
+        if (dirty) {
+            batch_fill_args<T>(b, sk, prm);
+            rc = batch_upload_args<T>(b);
+            if (rc) return rc;
+        }
+        {
+            BStage st(b, HV_KCLASS_PARTICLE);
+            kb_particle_step<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n, s > 0 ? 1 : 0);
+            kb_stats_finalize<T><<<n, 256, 0, b->stream>>>(a, s + 1);
+            HV_BHIP(b, hipGetLastError());
+        }
+        rc = batch_field_solve<T>(b, prm, its.data(), &warn);
+        if (rc) return rc;
+        if (sor_iters)
+            for (int i = 0; i < n; i++) sor_iters[(size_t)i * nsteps + s] = its[i];
+    }
+    // closing half kick of the last step, with the force stored (src/hyper_vlasov.cpp:101-103)
+    kb_gather_kick<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n);
+    HV_BHIP(b, hipGetLastError());
+    bool clear = false;
+    for (int i = 0; i < n; i++) {
+        clear = clear || b->boost[i];
+        b->boost[i] = 0;
+    }
+    if (clear) {
+        batch_fill_args<T>(b, sk, prm);
+        rc = batch_upload_args<T>(b);
+        if (rc) return rc;
+    }
+    for (int i = 0; i < n; i++) {
+        hv_ctx *c = b->inst[i];
+        if (stats) {
+            std::vector<StatSlot> h((size_t)nsteps + 1);
+            HV_BHIP(b, hipMemcpyAsync(h.data(), c->slots, sizeof(StatSlot) * h.size(), hipMemcpyDeviceToHost,
+                                      b->stream));
+            HV_BHIP(b, hipStreamSynchronize(b->stream));
+            for (int s = 0; s < nsteps; s++) slot_stats(h[s], c->np, c->nn, stats + ((size_t)i * nsteps + s) * 4);
+        }
+        int st = 0;
+        rc = take_status(c, &st);
+        if (rc) return batch_fwd(b, c, rc);
+        warn |= st & HV_WARN_PARTICLE_OUTSIDE;
+    }
+    return warn;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hv_batch_create(int n, const hv_world *worlds, const int *n_protons, const int *n_neutrons, int precision,
+                    hv_batch **out) {
+    if (!out) return HV_ERR_ARG;
+    *out = nullptr;
+    if (n <= 0 || !worlds || !n_protons || !n_neutrons) return HV_ERR_ARG;
+    if (precision != HV_PREC_F64 && precision != HV_PREC_MIXED) return HV_ERR_ARG;
+    hv_batch *b = new (std::nothrow) hv_batch();
+    if (!b) return HV_ERR_ALLOC;
+    b->n = n;
+    b->precision = precision;
+    b->pred32.assign(n, 16);
+    b->pred64.assign(n, 16);
+    b->boost.assign(n, 0);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        hv_batch_destroy(b);
+        return HV_ERR_HIP;
+    }
+    for (int i = 0; i < n; i++) {
+        hv_ctx *c = nullptr;
+        int rc = hv_ctx_create(&worlds[i], n_protons[i], n_neutrons[i], precision, &c);
+        if (rc) {
+            hv_batch_destroy(b);
+            return rc;
+        }
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        c->stream = b->stream;
+        c->own_stream = false;
+        // batched SOR is the per-sweep form
+        c->sor_res32 = 0;
+        c->sor_fused = 0;
+        c->sor_resident = 0;
+        c->sort_every = b->sort_every;
+        b->inst.push_back(c);
+    }
+    *out = b;
+    return HV_OK;
+}
+
+int hv_batch_destroy(hv_batch *b) {
+    if (!b) return HV_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (hv_ctx *c : b->inst) hv_ctx_destroy(c);
+    if (b->args) (void)hipFree(b->args);
+    if (b->first) (void)hipFree(b->first);
+    if (b->maxc) (void)hipFree(b->maxc);
+    if (b->pinned) (void)hipHostFree(b->pinned);
+    for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return HV_OK;
+}
+
+const char *hv_batch_last_error(const hv_batch *b) { return b ? b->err : "null batch"; }
+
+int hv_batch_size(const hv_batch *b) { return b ? b->n : 0; }
+
+hv_ctx *hv_batch_instance(hv_batch *b, int i) {
+    if (!b || i < 0 || i >= b->n) return nullptr;
+    return b->inst[i];
+}
+
+int hv_batch_set_option(hv_batch *b, int option, int value) {
+    if (!b) return HV_ERR_ARG;
+    if (option != HV_OPT_SORT_EVERY && option != HV_OPT_SMEAR_TILE)
+        return batch_err(b, HV_ERR_ARG, "option %d is not a batch option", option);
+    for (hv_ctx *c : b->inst) {
+        int rc = hv_set_option(c, option, value);
+        if (rc) return batch_fwd(b, c, rc);
+    }
+    if (option == HV_OPT_SORT_EVERY) b->sort_every = value;
+    return HV_OK;
+}
+
+int hv_batch_run_init(hv_batch *b, const hv_skyrme *skyrme, const hv_params *params, int *sor_iters) {
+    if (!b) return HV_ERR_ARG;
+    return b->precision == HV_PREC_F64 ? batch_run_init<double>(b, skyrme, params, sor_iters)
+                                       : batch_run_init<float>(b, skyrme, params, sor_iters);
+}
+
+int hv_batch_run_steps(hv_batch *b, const hv_skyrme *skyrme, const hv_params *params, int first_step, int nsteps,
+                       double *stats, int *sor_iters) {
+    if (!b) return HV_ERR_ARG;
+    if (first_step < 0 || nsteps < 0) return batch_err(b, HV_ERR_ARG, "negative step range");
+    return b->precision == HV_PREC_F64
+               ? batch_run_steps<double>(b, skyrme, params, first_step, nsteps, stats, sor_iters)
+               : batch_run_steps<float>(b, skyrme, params, first_step, nsteps, stats, sor_iters);
+}
+
+int hv_batch_profile_enable(hv_batch *b, int on) {
+    if (!b) return HV_ERR_ARG;
+    HV_BHIP(b, hipStreamSynchronize(b->stream));
+    b->prof = on != 0;
+    for (int k = 0; k < HV_KCLASS_COUNT; k++) {
+        b->prof_ms[k] = 0.0;
+        b->prof_launches[k] = 0;
+    }
+    for (hv_ctx *c : b->inst) c->cnt_solves = c->cnt_it32 = c->cnt_it64 = 0;
+    return HV_OK;
+}
+
+int hv_batch_profile_read(hv_batch *b, int kclass, double *ms, long long *launches) {
+    if (!b || kclass < 0 || kclass >= HV_KCLASS_COUNT) return HV_ERR_ARG;
+    HV_BHIP(b, hipStreamSynchronize(b->stream));
+    if (ms) *ms = b->prof_ms[kclass];
+    if (launches) *launches = b->prof_launches[kclass];
+    return HV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+// hv_batch_kernels.h -- one launch per loop stage for a whole ensemble of
+// nuclei (each with its own grid, particle count, Skyrme set and boost).
+//
+// Every batched kernel runs the same block body as the single-instance kernel
+// (hv_kernels.h): a launch covers the blocks of all instances back to back,
+// a block finds its instance in a prefix table of per-instance block counts
+// (first[i] = blocks of instances before i) and runs the body with its local
+// block index and the instance's arguments from a device array.  The node and
+// particle arithmetic is therefore the single-instance arithmetic, and an
+// instance of a batch produces the bits of a solo run of it.
+#pragma once
+
+#include "hv_kernels.h"
+
+namespace hv {
+
+// All per-instance arguments of the batched stages.
+template <typename T>
+struct BInst {
+    // particle pass
+    Particles<T> p;
+    int np, total;
+    Geom<T> g;
+    T *U;
+    unsigned long long *acc;
+    T kick_fac, drift_fac, boost_p, boost_n;
+    int boost;  // 1 during the step that applies the boost
+    long long *partials;
+    unsigned *status;
+    StatSlot *slots;
+    int part_blocks;
+    // smearing
+    const SmearTab<T> *tab;
+    T *D;
+    T *rho;
+    unsigned char *flags;
+    SmearTiles st;
+    // grid, SOR, potentials
+    long long G;
+    Split sp;
+    double *phi;
+    T *rho_s;
+    float *R, *delta;
+    SorTiles tl;
+    SorCoef cf;
+    unsigned long long *maxd;
+    SorCtl *ctl;
+    double A, B, C, gamma;
+};
+
+// instance of block b: first[i] <= b < first[i + 1] (wave-uniform)
+__device__ __forceinline__ int batch_instance(const int *__restrict__ first, int n, int b) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+#define HV_BATCH_LOCATE()                                     \
+    const int inst = batch_instance(first, n, blockIdx.x);    \
+    const int bid = (int)blockIdx.x - first[inst];            \
+    const BInst<T> &A = a[inst]
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_particle_step(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, int end_kick) {
+    HV_BATCH_LOCATE();
+    particle_step_body<T, T>(bid, A.p, A.np, A.total, A.g, A.U, A.acc, A.kick_fac, A.drift_fac, end_kick, A.boost,
+                             A.boost_p, A.boost_n, A.partials, A.status);
+}
+
+// one block per instance
+template <typename T>
+__global__ void __launch_bounds__(256) kb_stats_finalize(const BInst<T> *__restrict__ a, int slot) {
+    const BInst<T> &A = a[blockIdx.x];
+    stats_finalize_block(A.partials, A.part_blocks, A.slots + slot);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+kb_smear_zy(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.flags, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_smear_xs(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    smear_xs_body<T>(bid, A.D, A.rho, A.flags, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st);
+}
+
+// acc = 0 (2G words), blocks of the 2G table
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_zero_acc(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    const long long q = (long long)bid * blockDim.x + threadIdx.x;
+    if (q < 2 * A.G) A.acc[q] = 0ULL;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_rho_split(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    rho_split_body<T>(bid, A.rho, A.rho_s, A.G, A.sp);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_skyrme(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    skyrme_body<T>(bid, A.rho, A.U, A.phi, A.A, A.B, A.C, A.gamma, A.G, A.sp);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_sor_defect(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    sor_defect_body<T>(bid, A.phi, A.rho_s, A.R, A.delta, A.sp, A.cf);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_sor_fold(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    sor_fold_body(bid, A.phi, A.delta, 2 * A.sp.Gh);
+}
+
+// Start of a solve: SorCtl cleared and row 0 of the slot maxima zeroed (later
+// rows are zeroed one iteration ahead by the black sweep); one block per
+// instance.
+template <typename T>
+__global__ void kb_sor_reset(const BInst<T> *__restrict__ a) {
+    const BInst<T> &A = a[blockIdx.x];
+    if (threadIdx.x < kMaxSlots) A.maxd[(long long)threadIdx.x * kSlotStride] = 0ULL;
+    if (threadIdx.x == 0) {
+        A.ctl->done = 0;
+        A.ctl->iters = 0;
+    }
+}
+
+// zero row it + 1 of the slot maxima (no block of this launch touches it; the
+// next iteration's launches accumulate into it)
+__device__ __forceinline__ void zero_next_row(unsigned long long *maxd, int colour, int bid, int it, int max_it) {
+    if (colour == 1 && bid == 0 && threadIdx.y == 0 && it + 1 < max_it)
+        maxd[((long long)(it + 1) * kMaxSlots + threadIdx.x) * kSlotStride] = 0ULL;
+}
+
+// one colour of FP64 SOR (source: the colour-split proton density)
+template <typename T>
+__global__ void __launch_bounds__(256)
+kb_sor_colour(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, int colour, int it, int max_it) {
+    HV_BATCH_LOCATE();
+    zero_next_row(A.maxd, colour, bid, it, max_it);
+    sor_colour_body<T>(bid, A.phi, A.rho_s, A.sp, A.tl, A.cf, colour, it, A.maxd, A.ctl);
+}
+
+// one colour of the mixed form's FP32 correction iteration
+template <typename T>
+__global__ void __launch_bounds__(256)
+kb_sor_colour_f32(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, int colour, int it,
+                  int max_it) {
+    HV_BATCH_LOCATE();
+    zero_next_row(A.maxd, colour, bid, it, max_it);
+    sor_colour_f32_body<T>(bid, A.delta, A.R, A.sp, A.tl, A.cf, colour, it, A.maxd, A.ctl);
+}
+
+// per-iteration maxima of iterations [it0, it1) of every instance into
+// out[inst * width + (it - it0)]; grid = n * (it1 - it0)
+template <typename T>
+__global__ void kb_sor_collect(const BInst<T> *__restrict__ a, double *__restrict__ out, int width, int it0,
+                               int it1) {
+    const int m = it1 - it0;
+    const int inst = blockIdx.x / m, r = blockIdx.x - inst * m;
+    sor_collect_body(r, a[inst].maxd, out + (long long)inst * width, it0, it1);
+}
+
+// closing half kick of a run of steps (gather of the final U)
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_gather_kick(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    gather_u_body<T, T>(bid, A.p, A.np, A.total, A.g, A.U, A.kick_fac, 1);
+}
+
+#undef HV_BATCH_LOCATE
+
+}  // namespace hv

@@ -46,20 +46,21 @@ __device__ __forceinline__ void stat_values(T x, T y, T z, bool proton, const Ge
 }
 
 // Block partial sums of the six diagnostics, written (not accumulated) to
-// partials[6 * blockIdx.x + c]; k_stats_finalize folds them into a StatSlot.
+// partials[6 * bid + c] (bid: the block index within the pass); k_stats_finalize
+// folds them into a StatSlot.
 // Each block value is exact: |v| <= 256 * 2^54 < 2^63.
-__device__ __forceinline__ void stat_commit(long long (&v)[6], long long *partials) {
+__device__ __forceinline__ void stat_commit(long long (&v)[6], long long *partials, int bid) {
     block_sum_i64<6>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) partials[6 * blockIdx.x + c] = v[c];
+        for (int c = 0; c < 6; c++) partials[6 * bid + c] = v[c];
     }
 }
 
 // One block: exact sum of nblocks rows of partials into *slot (128-bit sums;
 // integer addition, so the order does not matter).
-__global__ void __launch_bounds__(256)
-k_stats_finalize(const long long *__restrict__ partials, int nblocks, StatSlot *__restrict__ slot) {
+__device__ __forceinline__ void stats_finalize_block(const long long *__restrict__ partials, int nblocks,
+                                                     StatSlot *__restrict__ slot) {
     __shared__ unsigned long long lo[6][256], hi[6][256];
     unsigned long long l[6] = {0, 0, 0, 0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
@@ -86,6 +87,11 @@ k_stats_finalize(const long long *__restrict__ partials, int nblocks, StatSlot *
             slot->inbox[c - 4] = L;
         }
     }
+}
+
+__global__ void __launch_bounds__(256)
+k_stats_finalize(const long long *__restrict__ partials, int nblocks, StatSlot *__restrict__ slot) {
+    stats_finalize_block(partials, nblocks, slot);
 }
 
 // ---- CIC deposit of one particle (include/tools.hpp:62-94) ------------------
@@ -250,11 +256,11 @@ __global__ void k_iota(int *__restrict__ a, int n) {
 // The intermediate k of the two half kicks is rounded exactly as in the
 // reference's two separate updates.
 template <typename T, typename UT>
-__global__ void __launch_bounds__(kBlock)
-k_particle_step(Particles<T> p, int np, int total, Geom<T> g, const UT *__restrict__ U,
+__device__ __forceinline__ void particle_step_body(const int bid, Particles<T> p, int np, int total, Geom<T> g,
+                                                   const UT *__restrict__ U,
                 unsigned long long *__restrict__ acc, T kick_fac, T drift_fac, int end_kick,
                 int boost, T boost_p, T boost_n, long long *partials, unsigned *status) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * blockDim.x + threadIdx.x;
     long long v[6] = {0, 0, 0, 0, 0, 0};
     Cell<T> c;
     bool has = false;
@@ -287,7 +293,16 @@ k_particle_step(Particles<T> p, int np, int total, Geom<T> g, const UT *__restri
         stat_values(x, y, z, proton, g, v);
     }
     block_deposit(has, c, i < np ? 0 : 1, g, acc);
-    stat_commit(v, partials);
+    stat_commit(v, partials, bid);
+}
+
+template <typename T, typename UT>
+__global__ void __launch_bounds__(kBlock)
+k_particle_step(Particles<T> p, int np, int total, Geom<T> g, const UT *__restrict__ U,
+                unsigned long long *__restrict__ acc, T kick_fac, T drift_fac, int end_kick,
+                int boost, T boost_p, T boost_n, long long *partials, unsigned *status) {
+    particle_step_body<T, UT>(blockIdx.x, p, np, total, g, U, acc, kick_fac, drift_fac, end_kick, boost, boost_p,
+                              boost_n, partials, status);
 }
 
 // deposit (and optionally diagnostics) of the current positions, no motion
@@ -307,14 +322,14 @@ k_deposit(Particles<T> p, int np, int total, Geom<T> g, unsigned long long *__re
         stat_values(x, y, z, proton, g, v);
     }
     if (acc) block_deposit(has, c, i < np ? 0 : 1, g, acc);
-    if (partials) stat_commit(v, partials);
+    if (partials) stat_commit(v, partials, blockIdx.x);
 }
 
 // gather F(x) from U into p.f, optionally followed by the half kick
 template <typename T, typename UT>
-__global__ void __launch_bounds__(kBlock)
-k_gather_u(Particles<T> p, int np, int total, Geom<T> g, const UT *__restrict__ U, T kick_fac, int kick) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void gather_u_body(const int bid, Particles<T> p, int np, int total, Geom<T> g,
+                                              const UT *__restrict__ U, T kick_fac, int kick) {
+    const int i = bid * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const long long off = (i < np) ? 0 : g.G;
     Cell<T> c;
@@ -328,6 +343,12 @@ k_gather_u(Particles<T> p, int np, int total, Geom<T> g, const UT *__restrict__ 
         p.ky[i] += kick_fac * fy;
         p.kz[i] += kick_fac * fz;
     }
+}
+
+template <typename T, typename UT>
+__global__ void __launch_bounds__(kBlock)
+k_gather_u(Particles<T> p, int np, int total, Geom<T> g, const UT *__restrict__ U, T kick_fac, int kick) {
+    gather_u_body<T, UT>(blockIdx.x, p, np, total, g, U, kick_fac, kick);
 }
 
 // fixed-point counts -> T (per-reference-kernel path)
@@ -442,8 +463,8 @@ struct SmearTiles {
 };
 
 template <typename T, typename CT>
-__global__ void __launch_bounds__(256)
-k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned char *__restrict__ flags,
+__device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restrict__ cnt, T *__restrict__ D,
+                                              unsigned char *__restrict__ flags,
            const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
     extern __shared__ unsigned char smem[];
     const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
@@ -451,7 +472,7 @@ k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned char *__restr
     const int R = TJ + 2 * ry, W = TK + 2 * rz;
     T *C = (T *)smem;          // R x W counts
     T *Hs = C + R * W;         // (rz + 1) x R x TK
-    int t = blockIdx.x;
+    int t = bid;
     const int kt = t % st.nkt;
     t /= st.nkt;
     const int jt = t % st.njt;
@@ -470,7 +491,7 @@ k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned char *__restr
         C[e] = v;
         nonzero |= (v != T(0.0));
     }
-    const long long tile = blockIdx.x;
+    const long long tile = bid;
     if (!__syncthreads_or(nonzero)) {
         if (threadIdx.x == 0) flags[tile] = 0;
         return;
@@ -504,12 +525,19 @@ k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned char *__restr
     if (threadIdx.x == 0) flags[tile] = 1;
 }
 
+template <typename T, typename CT>
+__global__ void __launch_bounds__(256)
+k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned char *__restrict__ flags,
+           const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
+    smear_zy_body<T, CT>(blockIdx.x, cnt, D, flags, tb, nx, ny, nz, st);
+}
+
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_smear_xs(const T *__restrict__ D, T *__restrict__ rho, const unsigned char *__restrict__ flags,
+__device__ __forceinline__ void smear_xs_body(const int bid, const T *__restrict__ D, T *__restrict__ rho,
+                                              const unsigned char *__restrict__ flags,
            const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
     const long long G = (long long)nx * ny * nz, n2 = 2 * G;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)bid * blockDim.x + threadIdx.x;
     if (q >= n2) return;
     const long long plane = (long long)ny * nz;
     const int sp = q >= G ? 1 : 0;
@@ -529,12 +557,19 @@ k_smear_xs(const T *__restrict__ D, T *__restrict__ rho, const unsigned char *__
     rho[q] = acc * tb->term;
 }
 
-// ---- Skyrme potentials (include/physics.hpp:273-281), optionally + phi ------
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_skyrme(const T *__restrict__ rho, T *__restrict__ U, const double *__restrict__ phi, double A, double B,
+k_smear_xs(const T *__restrict__ D, T *__restrict__ rho, const unsigned char *__restrict__ flags,
+           const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
+    smear_xs_body<T>(blockIdx.x, D, rho, flags, tb, nx, ny, nz, st);
+}
+
+// ---- Skyrme potentials (include/physics.hpp:273-281), optionally + phi ------
+template <typename T>
+__device__ __forceinline__ void skyrme_body(const int bid, const T *__restrict__ rho, T *__restrict__ U,
+                                            const double *__restrict__ phi, double A, double B,
          double C, double gamma, long long G, Split sp) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long i = (long long)bid * blockDim.x + threadIdx.x;
     if (i >= G) return;
     double rp = (double)rho[i], rn = (double)rho[i + G];
     double up = hv_skyrme_f64(A, B, C, gamma, rp, rn, 0);
@@ -550,6 +585,13 @@ k_skyrme(const T *__restrict__ rho, T *__restrict__ U, const double *__restrict_
     U[i + G] = (T)un;
 }
 
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_skyrme(const T *__restrict__ rho, T *__restrict__ U, const double *__restrict__ phi, double A, double B,
+         double C, double gamma, long long G, Split sp) {
+    skyrme_body<T>(blockIdx.x, rho, U, phi, A, B, C, gamma, G, sp);
+}
+
 // ---- total energy (no counterpart in the reference) -----------------------------
 // Grid part: per node dV * e_sky and dV * e_coul (hv_energy_density) of the
 // smeared density and phi, quantised to 2^-30 and summed exactly; partials
@@ -558,7 +600,8 @@ template <typename T>
 __global__ void k_energy_grid(const T *__restrict__ rho, const double *__restrict__ phi, long long G, Split sp,
                               double A, double B, double C, double gamma, double dV, long long *__restrict__ partials) {
     long long v[2] = {0, 0};
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < G; i += (long long)gridDim.x * blockDim.x) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < G; i += stride) {
         const int k = (int)(i % sp.nz), j = (int)((i / sp.nz) % sp.ny), ii = (int)(i / ((long long)sp.ny * sp.nz));
         double es, ec;
         hv_energy_density(A, B, C, gamma, (double)rho[i], (double)rho[i + G], phi[split_index(sp, ii, j, k)], &es, &ec);
@@ -609,11 +652,17 @@ __global__ void k_merge(T *__restrict__ U, const double *__restrict__ phi, long 
 
 // proton density (first G values of rho, natural order) -> colour-split copy
 template <typename T>
-__global__ void k_rho_split(const T *__restrict__ rho, T *__restrict__ rs, long long G, Split sp) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rho_split_body(const int bid, const T *__restrict__ rho, T *__restrict__ rs,
+                                               long long G, Split sp) {
+    long long i = (long long)bid * blockDim.x + threadIdx.x;
     if (i >= G) return;
     const int k = (int)(i % sp.nz), j = (int)((i / sp.nz) % sp.ny), ii = (int)(i / ((long long)sp.ny * sp.nz));
     rs[split_index(sp, ii, j, k)] = rho[i];
+}
+
+template <typename T>
+__global__ void k_rho_split(const T *__restrict__ rho, T *__restrict__ rs, long long G, Split sp) {
+    rho_split_body<T>(blockIdx.x, rho, rs, G, sp);
 }
 
 // F = -grad U on the grid (include/physics.hpp:160-200)
@@ -729,21 +778,21 @@ struct SorTiles {
 };
 
 template <typename RT>
-__global__ void __launch_bounds__(256)
-k_sor_colour(double *__restrict__ phi, const RT *__restrict__ rho, Split sp, SorTiles tl, SorCoef cf,
+__device__ __forceinline__ void sor_colour_body(const int bid, double *__restrict__ phi, const RT *__restrict__ rho,
+                                                Split sp, SorTiles tl, SorCoef cf,
              int colour, int it, unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
     if (ctl->done) return;
     if (colour == 0 && it > 0) {
         const double prev = sor_row_max(maxd, it - 1);
         if (prev < cf.tol) {
-            if (threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x == 0) {
+            if (threadIdx.x == 0 && threadIdx.y == 0 && bid == 0) {
                 ctl->done = 1;
                 ctl->iters = it;
             }
             return;
         }
     }
-    const int b = blockIdx.x;
+    const int b = bid;
     const int t = (b % kXcds) * tl.per_xcd + b / kXcds;
     const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
     double diff = 0.0;
@@ -783,6 +832,13 @@ k_sor_colour(double *__restrict__ phi, const RT *__restrict__ rho, Split sp, Sor
     block_max_atomic(diff, &maxd[((long long)it * kMaxSlots + (b % kMaxSlots)) * kSlotStride]);
 }
 
+template <typename RT>
+__global__ void __launch_bounds__(256)
+k_sor_colour(double *__restrict__ phi, const RT *__restrict__ rho, Split sp, SorTiles tl, SorCoef cf,
+             int colour, int it, unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
+    sor_colour_body<RT>(blockIdx.x, phi, rho, sp, tl, cf, colour, it, maxd, ctl);
+}
+
 // ---- mixed precision SOR: defect correction --------------------------------
 // With the FP64 potential phi held fixed, the red-black iteration for
 // phi + delta, delta starting at 0, is
@@ -795,10 +851,10 @@ k_sor_colour(double *__restrict__ phi, const RT *__restrict__ rho, Split sp, Sor
 
 // R (FP32, split) for the interior nodes, zero elsewhere; delta set to zero.
 template <typename RT>
-__global__ void __launch_bounds__(kBlock)
-k_sor_defect(const double *__restrict__ phi, const RT *__restrict__ rho, float *__restrict__ R,
+__device__ __forceinline__ void sor_defect_body(const int bid, const double *__restrict__ phi,
+                                                const RT *__restrict__ rho, float *__restrict__ R,
              float *__restrict__ delta, Split sp, SorCoef cf) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)bid * blockDim.x + threadIdx.x;
     if (q >= 2 * sp.Gh) return;
     const int colour = q >= sp.Gh ? 1 : 0;
     const long long qq = q - (colour ? sp.Gh : 0);
@@ -821,21 +877,28 @@ k_sor_defect(const double *__restrict__ phi, const RT *__restrict__ rho, float *
 }
 
 template <typename RT>
-__global__ void __launch_bounds__(256)
-k_sor_colour_f32(float *__restrict__ delta, const float *__restrict__ R, Split sp, SorTiles tl, SorCoef cf,
+__global__ void __launch_bounds__(kBlock)
+k_sor_defect(const double *__restrict__ phi, const RT *__restrict__ rho, float *__restrict__ R,
+             float *__restrict__ delta, Split sp, SorCoef cf) {
+    sor_defect_body<RT>(blockIdx.x, phi, rho, R, delta, sp, cf);
+}
+
+template <typename RT>
+__device__ __forceinline__ void sor_colour_f32_body(const int bid, float *__restrict__ delta,
+                                                    const float *__restrict__ R, Split sp, SorTiles tl, SorCoef cf,
                  int colour, int it, unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
     if (ctl->done) return;
     if (colour == 0 && it > 0) {
         const double prev = sor_row_max(maxd, it - 1);
         if (prev < cf.tol) {
-            if (threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x == 0) {
+            if (threadIdx.x == 0 && threadIdx.y == 0 && bid == 0) {
                 ctl->done = 1;
                 ctl->iters = it;
             }
             return;
         }
     }
-    const int b = blockIdx.x;
+    const int b = bid;
     const int t = (b % kXcds) * tl.per_xcd + b / kXcds;
     const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
     float diff = 0.0f;
@@ -873,10 +936,22 @@ k_sor_colour_f32(float *__restrict__ delta, const float *__restrict__ R, Split s
     block_max_atomic((double)diff, &maxd[((long long)it * kMaxSlots + (b % kMaxSlots)) * kSlotStride]);
 }
 
+template <typename RT>
+__global__ void __launch_bounds__(256)
+k_sor_colour_f32(float *__restrict__ delta, const float *__restrict__ R, Split sp, SorTiles tl, SorCoef cf,
+                 int colour, int it, unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
+    sor_colour_f32_body<RT>(blockIdx.x, delta, R, sp, tl, cf, colour, it, maxd, ctl);
+}
+
 // phi += delta (both colours; delta is zero off the interior)
-__global__ void k_sor_fold(double *__restrict__ phi, const float *__restrict__ delta, long long n) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sor_fold_body(const int bid, double *__restrict__ phi,
+                                              const float *__restrict__ delta, long long n) {
+    const long long q = (long long)bid * blockDim.x + threadIdx.x;
     if (q < n) phi[q] += (double)delta[q];
+}
+
+__global__ void k_sor_fold(double *__restrict__ phi, const float *__restrict__ delta, long long n) {
+    sor_fold_body(blockIdx.x, phi, delta, n);
 }
 
 // ---- one launch per SOR iteration: red and black fused ------------------------
@@ -1003,12 +1078,18 @@ k_sor_fused(const VT *__restrict__ in, VT *__restrict__ out, const ST *__restric
 }
 
 // Folds the slot maxima of iterations [first, last) into out[0..last-first).
-__global__ void k_sor_collect(const unsigned long long *__restrict__ maxd, double *__restrict__ out, int first,
+__device__ __forceinline__ void sor_collect_body(const int bid, const unsigned long long *__restrict__ maxd,
+                                                 double *__restrict__ out, int first,
                               int last) {
-    const int it = first + blockIdx.x;
+    const int it = first + bid;
     if (it >= last) return;
     const double m = sor_row_max(maxd, it);
-    if (threadIdx.x == 0) out[blockIdx.x] = m;
+    if (threadIdx.x == 0) out[bid] = m;
+}
+
+__global__ void k_sor_collect(const unsigned long long *__restrict__ maxd, double *__restrict__ out, int first,
+                              int last) {
+    sor_collect_body(blockIdx.x, maxd, out, first, last);
 }
 
 }  // namespace hv

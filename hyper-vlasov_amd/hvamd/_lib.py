@@ -45,6 +45,7 @@ HV_OPT_SOR_RESIDENT = 1
 HV_OPT_SOR_FUSED = 2
 HV_OPT_SOR_RESIDENT_F32 = 3
 HV_OPT_SOR_F32_LAG = 4
+HV_OPT_SMEAR_TILE = 5
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1
@@ -52,7 +53,9 @@ HV_KCLASS_SMEAR = 2
 HV_KCLASS_POTENTIAL = 3
 HV_KCLASS_SOR_F32 = 4
 HV_KCLASS_SOR_RES_F32 = 5
-KCLASS_NAMES = ("sor_sweep", "particle_pass", "smear", "potential", "sor_sweep_f32", "sor_resident_f32")
+HV_KCLASS_SOR_STAGE = 6
+KCLASS_NAMES = ("sor_sweep", "particle_pass", "smear", "potential", "sor_sweep_f32", "sor_resident_f32",
+                "sor_stage")
 
 
 class HvWorld(ctypes.Structure):
@@ -134,6 +137,16 @@ SIGNATURES = {
     "hv_total_energy": (_I, [_P, ctypes.POINTER(HvSkyrme), _I, _DP]),
     "hv_sor_counts": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                            ctypes.POINTER(ctypes.c_longlong)]),
+    "hv_batch_create": (_I, [_I, ctypes.POINTER(HvWorld), _IP, _IP, _I, ctypes.POINTER(_P)]),
+    "hv_batch_destroy": (_I, [_P]),
+    "hv_batch_last_error": (ctypes.c_char_p, [_P]),
+    "hv_batch_size": (_I, [_P]),
+    "hv_batch_instance": (_P, [_P, _I]),
+    "hv_batch_set_option": (_I, [_P, _I, _I]),
+    "hv_batch_run_init": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _IP]),
+    "hv_batch_run_steps": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _I, _I, _P, _P]),
+    "hv_batch_profile_enable": (_I, [_P, _I]),
+    "hv_batch_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
     "hv_fermi_momentum": (_D, [_D]),
     "hv_sample_uniform_spheres": (_I, [ctypes.c_ulonglong, _I, _I, _D, _D] + [_P] * 6),
     "hv_mix64": (ctypes.c_ulonglong, [ctypes.c_ulonglong]),

@@ -100,4 +100,26 @@ def c4() -> Workload:
     return Workload("c4_A120_64cubed", 50, 70, 1000, 64, 0.4, 0.5, 1500, 15.0)
 
 
+def c3(n_instances: int = 256, seed: int = 128) -> list:
+    """Ensemble: n instances, A ~ U[16, 238], Z = round(0.4 A), test particles
+    per nucleon ~ U[200, 1000], grid 32^3, 64^3 or 96^3 over a box of
+    U[16, 24] fm, Skyrme B and C scaled by U[0.9, 1.1], dp ~ U[5, 30] MeV/c,
+    1,500 steps of dt = 0.5 fm/c.  Instance i draws from its own seeded
+    stream, so the first k instances do not depend on n."""
+    out = []
+    for i in range(n_instances):
+        rng = np.random.default_rng([seed, i])
+        A = int(rng.integers(16, 239))
+        Z = int(round(0.4 * A))
+        ppn = int(rng.integers(200, 1001))
+        n = int(rng.choice([32, 64, 96]))
+        box = float(rng.uniform(16.0, 24.0))
+        fb, fc = rng.uniform(0.9, 1.1, 2)
+        dp = float(rng.uniform(5.0, 30.0))
+        sk = Skyrme(-356.0, 303.0 * float(fb), 32.0 * float(fc), 7.0 / 6.0)
+        out.append(Workload(f"c3_{i}_A{A}_{n}cubed", Z, A - Z, ppn, n, box / n, 0.5, 1500, dp,
+                            seed=seed * 1000003 + i, skyrme=sk))
+    return out
+
+
 WORKLOADS = {"c0": c0, "c1": c1, "c2": c2, "c4": c4}

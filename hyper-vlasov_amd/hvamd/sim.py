@@ -130,12 +130,30 @@ class Simulation:
         if rc < 0:
             raise HvError("hv_ctx_create", rc, "bad world/particle counts or device allocation failure")
         self.ctx = h
+        self.owned = True
         self.last_warnings = 0
+
+    @classmethod
+    def view(cls, ctx, world: World, n_protons: int, n_neutrons: int, precision: int) -> "Simulation":
+        """A non-owning wrapper of a context that something else owns (an
+        instance of a Batch)."""
+        self = cls.__new__(cls)
+        self.lib = _lib.load()
+        self.world = world
+        self.np, self.nn = int(n_protons), int(n_neutrons)
+        self.total = self.np + self.nn
+        self.precision = precision
+        self._w = world.c()
+        self.ctx = ctypes.c_void_p(ctx)
+        self.owned = False
+        self.last_warnings = 0
+        return self
 
     # -- lifetime ---------------------------------------------------------------
     def close(self):
         if getattr(self, "ctx", None):
-            self.lib.hv_ctx_destroy(self.ctx)
+            if self.owned:
+                self.lib.hv_ctx_destroy(self.ctx)
             self.ctx = None
 
     def __del__(self):

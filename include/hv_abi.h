@@ -216,6 +216,10 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * range 1..64), so that its tiles never wait on a whole-grid reduction of the
  * current one. */
 #define HV_OPT_SOR_F32_LAG 4
+/* HV_OPT_SMEAR_TILE: widest column tile (8, 16, 32 or 64, default 64) the
+ * smearing's LDS tiles may use; narrower tiles trade halo re-reads for more
+ * blocks.  Same bits for every width. */
+#define HV_OPT_SMEAR_TILE 5
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
@@ -240,12 +244,45 @@ const char *hv_build_info(void);
 #define HV_KCLASS_POTENTIAL 3 /* Skyrme + Coulomb merge */
 #define HV_KCLASS_SOR_F32 4   /* one sweep of the mixed form's FP32 SOR correction */
 #define HV_KCLASS_SOR_RES_F32 5 /* one LDS-resident launch of the FP32 correction (a whole solve) */
-#define HV_KCLASS_COUNT 6
+#define HV_KCLASS_SOR_STAGE 6   /* ensembles: the whole SOR solve of a step (launches and read-backs) */
+#define HV_KCLASS_COUNT 7
 int hv_profile_enable(hv_ctx *ctx, int on); /* also resets the counters */
 int hv_profile_read(hv_ctx *ctx, int kclass, double *ms, long long *launches);
 /* SOR work since the last hv_profile_enable: solves, FP32 correction
  * iterations (mixed form) and FP64 iterations. */
 int hv_sor_counts(hv_ctx *ctx, long long *solves, long long *iters_f32, long long *iters_f64);
+
+/* ---- ensembles (no counterpart in the reference) ------------------------ */
+/* For callers that run many nuclei (Monte Carlo ensembles, parameter scans,
+ * the dipole response of a chart of nuclei): n instances, each with its own
+ * grid, particle counts, Skyrme set and loop parameters, stepped together with
+ * one launch per loop stage for the whole ensemble.  Instance i gives the bits
+ * of a solo context run of it with the same options (the LDS-resident and
+ * fused SOR forms off: a batch uses the per-sweep SOR launches).  All
+ * instances share the precision and sor_max_it. */
+typedef struct hv_batch hv_batch;
+int hv_batch_create(int n, const hv_world *worlds, const int *n_protons, const int *n_neutrons, int precision,
+                    hv_batch **out);
+int hv_batch_destroy(hv_batch *batch);
+const char *hv_batch_last_error(const hv_batch *batch);
+int hv_batch_size(const hv_batch *batch);
+/* instance i's context: particle/field transfer, diagnostics, total energy */
+hv_ctx *hv_batch_instance(hv_batch *batch, int i);
+/* HV_OPT_SORT_EVERY or HV_OPT_SMEAR_TILE, applied to every instance */
+int hv_batch_set_option(hv_batch *batch, int option, int value);
+/* skyrme[n], params[n]; sor_iters[n] (may be NULL): the pre-loop solves */
+int hv_batch_run_init(hv_batch *batch, const hv_skyrme *skyrme, const hv_params *params, int *sor_iters);
+/* steps [first_step, first_step + nsteps) of every instance; stats (may be
+ * NULL) receives n * nsteps * 4 doubles, instance-major, as hv_run_steps;
+ * sor_iters (may be NULL) n * nsteps ints, instance-major. */
+int hv_batch_run_steps(hv_batch *batch, const hv_skyrme *skyrme, const hv_params *params, int first_step,
+                       int nsteps, double *stats, int *sor_iters);
+/* time per stage of the batched loop (HV_KCLASS_PARTICLE, _SMEAR,
+ * _SOR_STAGE, _POTENTIAL; enabling synchronises after every stage) and of one
+ * SOR sweep launch in eight (HV_KCLASS_SOR, _SOR_F32: the sweep kernel's own
+ * dispatch time) */
+int hv_batch_profile_enable(hv_batch *batch, int on);
+int hv_batch_profile_read(hv_batch *batch, int kclass, double *ms, long long *launches);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,101 @@
+"""Ensembles (hv_batch_*): instance i of a batch equals a solo run of instance i,
+bit for bit, in the same precision mode (solo runs with the per-sweep SOR
+form, the one a batch uses)."""
+import numpy as np
+import pytest
+
+import hvamd
+from hvamd import configs
+from hvamd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances():
+    """Three uneven instances: different grids, particle
+    counts, Skyrme sets, boosts and boost conventions."""
+    ws = configs.c3(3, seed=7)
+    ws[0].grid, ws[0].dx = 20, 0.8
+    ws[1].grid, ws[1].dx = 24, 0.7
+    ws[2].grid, ws[2].dx = 16, 1.0
+    for w in ws:
+        w.ppn = 20 + (w.ppn % 17)
+    params = [w.params(8) for w in ws]
+    params[1].boost_mode = hvamd.HV_BOOST_REF_WEIGHTED
+    params[2].t_exc = 1.5  # boost at step 3
+    return ws, params
+
+
+def _solo(w, prm, prec, chunks, sort_every):
+    sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, prec)
+    sim.set_option(hvamd.HV_OPT_SOR_RESIDENT_F32, 0)
+    sim.set_option(hvamd.HV_OPT_SORT_EVERY, sort_every)
+    sim.upload_particles(*w.initial_state())
+    it0 = sim.run_init(w.sky(), prm)
+    stats, its, first = [], [], 0
+    for c in chunks:
+        s, it = sim.run_steps(w.sky(), prm, first, c)
+        stats.append(s)
+        its.append(it)
+        first += c
+    out = (it0, np.concatenate(stats), np.concatenate(its), sim.download_particles(),
+           sim.total_energy(w.sky(), w.ppn))
+    sim.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", [hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED])
+def test_batch_equals_solo_runs(prec):
+    print(f"batch vs solo prec={prec}", flush=True)
+    ws, params = _instances()
+    chunks = [5, 3]
+    b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws], prec)
+    b.set_option(hvamd.HV_OPT_SORT_EVERY, 2)
+    for i, w in enumerate(ws):
+        b.instance(i).upload_particles(*w.initial_state())
+    sks = [w.sky() for w in ws]
+    it0 = b.run_init(sks, params)
+    stats, its, first = [], [], 0
+    for c in chunks:
+        s, it = b.run_steps(sks, params, first, c)
+        stats.append(s)
+        its.append(it)
+        first += c
+    stats = np.concatenate(stats, axis=1)
+    its = np.concatenate(its, axis=1)
+    parts = [b.instance(i).download_particles() for i in range(len(ws))]
+    energies = [b.instance(i).total_energy(ws[i].sky(), ws[i].ppn) for i in range(len(ws))]
+    b.close()
+    for i, w in enumerate(ws):
+        s_it0, s_stats, s_its, s_part, s_e = _solo(w, params[i], prec, chunks, 2)
+        print(i, w.name, "iters", list(its[i]), "solo", list(s_its))
+        assert it0[i] == s_it0
+        assert np.array_equal(its[i], s_its)
+        assert np.array_equal(stats[i], s_stats), i
+        for k in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz"):
+            assert np.array_equal(parts[i][k], s_part[k]), (i, k)
+        assert np.array_equal(energies[i], s_e)
+
+
+def test_batch_options_do_not_change_bits():
+    """Sort cadence and smearing tile width (the autotuned knobs) leave every
+    bit of a batched run unchanged."""
+    print("batch options", flush=True)
+    ws, params = _instances()
+    outs = []
+    for sort_every, tile in ((0, 64), (3, 16)):
+        b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws],
+                  hvamd.HV_PREC_MIXED)
+        b.set_option(hvamd.HV_OPT_SORT_EVERY, sort_every)
+        b.set_option(hvamd.HV_OPT_SMEAR_TILE, tile)
+        for i, w in enumerate(ws):
+            b.instance(i).upload_particles(*w.initial_state())
+        sks = [w.sky() for w in ws]
+        b.run_init(sks, params)
+        s, it = b.run_steps(sks, params, 0, 6)
+        outs.append((s, it, [b.instance(i).download_particles()["kz"] for i in range(len(ws))]))
+        b.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    for a, c in zip(outs[0][2], outs[1][2]):
+        assert np.array_equal(a, c)
