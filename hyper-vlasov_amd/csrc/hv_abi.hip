@@ -704,16 +704,25 @@ SorCoef make_sor_coef(const hv_ctx *c, double omega, double tol) {
     return cf;
 }
 
-// tiles of the per-sweep SOR kernels (kSorPlanes planes x kSorRows rows x 64
+// tiles of the per-sweep SOR kernels (`planes` planes x kSorRows rows x 64
 // k-slots per block, XCD slabs)
-SorTiles make_sor_tiles(const hv_ctx *c) {
+SorTiles make_sor_tiles(const hv_ctx *c, int planes = kSorPlanes) {
     SorTiles tl;
-    tl.n_ic = (c->world.n[0] - 2 + kSorPlanes - 1) / kSorPlanes;
+    tl.n_ic = (c->world.n[0] - 2 + planes - 1) / planes;
     tl.n_jb = (c->world.n[1] - 2 + kSorRows - 1) / kSorRows;
     tl.n_hb = (c->sp.nzh + 63) / 64;
     const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
     tl.per_xcd = (ntiles + kXcds - 1) / kXcds;
     return tl;
+}
+
+// Iteration budget of the FP64 sweeps that end a mixed solve: what is left of
+// max_it after the FP32 correction iterations, and at least a few sweeps, so
+// that a solve that already spent its budget in FP32 stops (not converged)
+// about where the FP64 form would.
+inline int mixed_fp64_budget(int max_it, int iters32) {
+    const int left = max_it - iters32;
+    return left > 4 ? left : 4;
 }
 
 // Red-black SOR on c->phi with the proton density (first G values of `rho`).
@@ -789,7 +798,8 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         int it64 = 0;
         const int pred = iters;
         c->sor_pred = 2;
-        rc = sor_exact(c, rho, cf, max_it, fused, sgrd, sblk, tl, fgrd, fblk, ft, rows_lds, &it64, &md);
+        rc = sor_exact(c, rho, cf, mixed_fp64_budget(max_it, iters), fused, sgrd, sblk, tl, fgrd, fblk, ft, rows_lds,
+                       &it64, &md);
         if (rc) return rc;
         c->mixed_fp64_sweeps = it64;
         c->sor_pred = pred;

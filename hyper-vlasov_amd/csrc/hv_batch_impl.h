@@ -31,6 +31,9 @@ struct hv_batch {
     size_t smear_lds = 0;
     double *maxc = nullptr;               // collected SOR maxima, n x maxc_width
     int maxc_width = 0;
+    int *act = nullptr;                   // SOR launch subset: prefix table (n + 1) then ids (n)
+    int *hact = nullptr;                  // pinned staging of act
+    std::vector<int> sor_blocks;          // per-instance SOR sweep blocks
     double *pinned = nullptr;
     size_t pinned_cap = 0;
     std::vector<int> pred32, pred64;      // per-instance SOR iteration predictions
@@ -47,6 +50,7 @@ struct hv_batch {
         hipEvent_t a, e;
     };
     std::vector<Sample> samples;  // sampled sweep launches, folded at the next read-back
+    long long sweeps[2] = {0, 0};  // sweep launches since hv_batch_profile_enable: FP32 correction, FP64
     char err[512] = {0};
 };
 
@@ -134,7 +138,7 @@ void batch_fill_args(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
         A.rho_s = (T *)c->rho_s;
         A.R = c->sor_R;
         A.delta = c->sor_delta;
-        A.tl = make_sor_tiles(c);
+        A.tl = make_sor_tiles(c, kSorPlanesBatch);
         A.cf = make_sor_coef(c, p.sor_omega, p.sor_tol);
         A.maxd = c->maxd;
         A.ctl = c->ctl;
@@ -165,7 +169,7 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
         if (rc) return batch_fwd(b, c, rc);
         rc = ensure_slots(c, nslots);
         if (rc) return batch_fwd(b, c, rc);
-        const SorTiles tl = make_sor_tiles(c);
+        const SorTiles tl = make_sor_tiles(c, kSorPlanesBatch);
         const long long cnt[kBKinds] = {
             (long long)blocks_for(c->total),
             2LL * c->world.n[0] * c->stl.njt * c->stl.nkt,
@@ -182,6 +186,12 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
         lds = c->stl_lds > lds ? c->stl_lds : lds;
     }
     for (int k = 0; k < kBKinds; k++) b->blocks[k] = (unsigned)hf[(size_t)k * (b->n + 1) + b->n];
+    b->sor_blocks.resize(b->n);
+    for (int i = 0; i < b->n; i++)
+        b->sor_blocks[i] = hf[(size_t)kBSor * (b->n + 1) + i + 1] - hf[(size_t)kBSor * (b->n + 1) + i];
+    if (!b->act) HV_BHIP(b, hipMalloc((void **)&b->act, sizeof(int) * (2 * (size_t)b->n + 1)));
+    if (!b->hact)
+        HV_BHIP(b, hipHostMalloc((void **)&b->hact, sizeof(int) * (2 * (size_t)b->n + 1), hipHostMallocDefault));
     b->smear_lds = lds;
     if (!b->first) HV_BHIP(b, hipMalloc((void **)&b->first, sizeof(int) * hf.size()));
     HV_BHIP(b, hipMemcpyAsync(b->first, hf.data(), sizeof(int) * hf.size(), hipMemcpyHostToDevice, b->stream));
@@ -224,10 +234,13 @@ struct BStage {
 };
 
 // SOR iterations of every instance in chunks of batched sweeps; iters[i]
-// receives the reference stop rule's count of instance i.
+// receives instance i's count under the reference stop rule, capped at
+// budget[i].  Each chunk's launches cover only the instances still iterating
+// at its start (a converged instance's blocks would return at once, but
+// would still be dispatched).
 template <typename T>
 int batch_sor_iterate(hv_batch *b, const hv_params *prm, bool f32, const std::vector<int> &pred,
-                      std::vector<int> &iters, std::vector<double> &lastmax) {
+                      const std::vector<int> &budget, std::vector<int> &iters, std::vector<double> &lastmax) {
     const int n = b->n;
     const BInst<T> *a = (const BInst<T> *)b->args;
     const int max_it = prm[0].sor_max_it;
@@ -237,9 +250,27 @@ int batch_sor_iterate(hv_batch *b, const hv_params *prm, bool f32, const std::ve
     lastmax.assign(n, 0.0);
     int remaining = n, it = 0, chunk = 1;
     for (int i = 0; i < n; i++) chunk = pred[i] - 1 > chunk ? pred[i] - 1 : chunk;
-    const dim3 sblk(64, kSorRows, 1), sgrd(b->blocks[kBSor], 1, 1);
-    while (it < max_it && remaining > 0) {
-        const int start = it, end = it + chunk < max_it ? it + chunk : max_it;
+    const dim3 sblk(64, kSorRows, 1);
+    while (remaining > 0) {
+        // launch subset: instances not converged and under budget
+        int m = 0, blocks = 0, room = max_it;
+        b->hact[0] = 0;
+        for (int i = 0; i < n; i++) {
+            if (iters[i] >= 0) continue;
+            b->hact[n + 1 + m] = i;
+            blocks += b->sor_blocks[i];
+            b->hact[++m] = blocks;
+            room = budget[i] - it < room ? budget[i] - it : room;
+        }
+        // ids right after the m + 1 prefix entries
+        for (int k = 0; k < m; k++) b->hact[m + 1 + k] = b->hact[n + 1 + k];
+        HV_BHIP(b, hipMemcpyAsync(b->act, b->hact, sizeof(int) * (2 * (size_t)m + 1), hipMemcpyHostToDevice,
+                                  b->stream));
+        const int start = it;
+        const int len = chunk < room ? chunk : room;
+        const int end = it + len;
+        const dim3 sgrd((unsigned)blocks, 1, 1);
+        const int *afirst = b->act, *aids = b->act + m + 1;
         for (; it < end; it++)
             for (int colour = 0; colour < 2; colour++) {
                 // one launch in kSorSample is timed by start/stop events of its own dispatch
@@ -253,25 +284,25 @@ int batch_sor_iterate(hv_batch *b, const hv_params *prm, bool f32, const std::ve
                 }
                 if (f32)
                     hipExtLaunchKernelGGL(kb_sor_colour_f32<T>, sgrd, sblk, 0u, b->stream, ea, ee, 0u,
-                                          (const BInst<T> *)a, btab(b, kBSor), n, colour, it, max_it);
+                                          (const BInst<T> *)a, afirst, aids, m, colour, it, max_it);
                 else
                     hipExtLaunchKernelGGL(kb_sor_colour<T>, sgrd, sblk, 0u, b->stream, ea, ee, 0u,
-                                          (const BInst<T> *)a, btab(b, kBSor), n, colour, it, max_it);
+                                          (const BInst<T> *)a, afirst, aids, m, colour, it, max_it);
             }
         HV_BHIP(b, hipGetLastError());
-        const int width = end - start;
-        if (b->maxc_width < width) {
+        b->sweeps[f32 ? 0 : 1] += 2LL * len;
+        if (b->maxc_width < len) {
             if (b->maxc) (void)hipFree(b->maxc);
             b->maxc = nullptr;
             b->maxc_width = 0;
-            HV_BHIP(b, hipMalloc((void **)&b->maxc, sizeof(double) * (size_t)n * width));
-            b->maxc_width = width;
+            HV_BHIP(b, hipMalloc((void **)&b->maxc, sizeof(double) * (size_t)n * len));
+            b->maxc_width = len;
         }
-        int rc = batch_pinned(b, (size_t)n * width);
+        int rc = batch_pinned(b, (size_t)n * len);
         if (rc) return rc;
-        kb_sor_collect<T><<<(unsigned)(n * width), 64, 0, b->stream>>>(a, b->maxc, width, start, end);
+        kb_sor_collect<T><<<(unsigned)(n * len), 64, 0, b->stream>>>(a, b->maxc, len, start, end);
         HV_BHIP(b, hipGetLastError());
-        HV_BHIP(b, hipMemcpyAsync(b->pinned, b->maxc, sizeof(double) * (size_t)n * width, hipMemcpyDeviceToHost,
+        HV_BHIP(b, hipMemcpyAsync(b->pinned, b->maxc, sizeof(double) * (size_t)n * len, hipMemcpyDeviceToHost,
                                   b->stream));
         HV_BHIP(b, hipStreamSynchronize(b->stream));
         for (auto &smp : b->samples) {
@@ -286,21 +317,22 @@ int batch_sor_iterate(hv_batch *b, const hv_params *prm, bool f32, const std::ve
         b->samples.clear();
         for (int i = 0; i < n; i++) {
             if (iters[i] >= 0) continue;
-            const double *row = b->pinned + (size_t)i * width;
-            for (int m = 0; m < width; m++) {
-                if (row[m] < prm[i].sor_tol) {
-                    iters[i] = start + m + 1;
-                    lastmax[i] = row[m];
-                    remaining--;
+            const double *row = b->pinned + (size_t)i * len;
+            for (int q = 0; q < len; q++) {
+                if (row[q] < prm[i].sor_tol) {
+                    iters[i] = start + q + 1;
+                    lastmax[i] = row[q];
                     break;
                 }
             }
-            if (iters[i] < 0) lastmax[i] = row[width - 1];
+            if (iters[i] < 0) {
+                lastmax[i] = row[len - 1];
+                if (end >= budget[i]) iters[i] = budget[i];  // out of budget: not converged
+            }
+            if (iters[i] >= 0) remaining--;
         }
         chunk = 4;
     }
-    for (int i = 0; i < n; i++)
-        if (iters[i] < 0) iters[i] = max_it;
     return HV_OK;
 }
 
@@ -324,12 +356,14 @@ int batch_field_solve(hv_batch *b, const hv_params *prm, int *iters_out, int *wa
         if (b->precision == HV_PREC_MIXED) {
             kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
             HV_BHIP(b, hipGetLastError());
-            int rc = batch_sor_iterate<T>(b, prm, true, b->pred32, it1, md1);
+            const std::vector<int> cap(n, prm[0].sor_max_it);
+            int rc = batch_sor_iterate<T>(b, prm, true, b->pred32, cap, it1, md1);
             if (rc) return rc;
             kb_sor_fold<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
             HV_BHIP(b, hipGetLastError());
-            std::vector<int> two(n, 2);
-            rc = batch_sor_iterate<T>(b, prm, false, two, it2, md2);
+            std::vector<int> two(n, 2), budget(n);
+            for (int i = 0; i < n; i++) budget[i] = mixed_fp64_budget(prm[0].sor_max_it, it1[i]);
+            rc = batch_sor_iterate<T>(b, prm, false, two, budget, it2, md2);
             if (rc) return rc;
             for (int i = 0; i < n; i++) {
                 b->pred32[i] = it1[i];
@@ -340,7 +374,8 @@ int batch_field_solve(hv_batch *b, const hv_params *prm, int *iters_out, int *wa
                 if (md2[i] > prm[i].sor_tol) *warn |= HV_WARN_SOR_NOT_CONVERGED;
             }
         } else {
-            int rc = batch_sor_iterate<T>(b, prm, false, b->pred64, it1, md1);
+            const std::vector<int> cap(n, prm[0].sor_max_it);
+            int rc = batch_sor_iterate<T>(b, prm, false, b->pred64, cap, it1, md1);
             if (rc) return rc;
             for (int i = 0; i < n; i++) {
                 b->pred64[i] = it1[i];
@@ -524,6 +559,8 @@ int hv_batch_destroy(hv_batch *b) {
     if (b->args) (void)hipFree(b->args);
     if (b->first) (void)hipFree(b->first);
     if (b->maxc) (void)hipFree(b->maxc);
+    if (b->act) (void)hipFree(b->act);
+    if (b->hact) (void)hipHostFree(b->hact);
     if (b->pinned) (void)hipHostFree(b->pinned);
     for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -576,6 +613,23 @@ int hv_batch_profile_enable(hv_batch *b, int on) {
         b->prof_launches[k] = 0;
     }
     for (hv_ctx *c : b->inst) c->cnt_solves = c->cnt_it32 = c->cnt_it64 = 0;
+    b->sweeps[0] = b->sweeps[1] = 0;
+    return HV_OK;
+}
+
+int hv_batch_sor_counts(hv_batch *b, long long *sweeps_f32, long long *sweeps_f64, long long *node_iters_f32,
+                        long long *node_iters_f64) {
+    if (!b) return HV_ERR_ARG;
+    long long n32 = 0, n64 = 0;
+    for (hv_ctx *c : b->inst) {
+        const long long interior = (long long)(c->world.n[0] - 2) * (c->world.n[1] - 2) * (c->world.n[2] - 2);
+        n32 += c->cnt_it32 * interior;
+        n64 += c->cnt_it64 * interior;
+    }
+    if (sweeps_f32) *sweeps_f32 = b->sweeps[0];
+    if (sweeps_f64) *sweeps_f64 = b->sweeps[1];
+    if (node_iters_f32) *node_iters_f32 = n32;
+    if (node_iters_f64) *node_iters_f64 = n64;
     return HV_OK;
 }
 

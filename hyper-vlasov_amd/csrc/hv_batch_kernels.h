@@ -64,6 +64,13 @@ __device__ __forceinline__ int batch_instance(const int *__restrict__ first, int
     const int bid = (int)blockIdx.x - first[inst];            \
     const BInst<T> &A = a[inst]
 
+// launches over a subset of the instances (those still iterating): ids[k] is
+// the instance of the k-th entry of the prefix table
+#define HV_BATCH_LOCATE_IDS()                                 \
+    const int slot_ = batch_instance(first, n, blockIdx.x);   \
+    const int bid = (int)blockIdx.x - first[slot_];           \
+    const BInst<T> &A = a[ids[slot_]]
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_particle_step(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, int end_kick) {
@@ -150,23 +157,26 @@ __device__ __forceinline__ void zero_next_row(unsigned long long *maxd, int colo
         maxd[((long long)(it + 1) * kMaxSlots + threadIdx.x) * kSlotStride] = 0ULL;
 }
 
-// one colour of FP64 SOR (source: the colour-split proton density)
+// one colour of FP64 SOR (source: the colour-split proton density), over the
+// n instances listed in ids
 template <typename T>
 __global__ void __launch_bounds__(256)
-kb_sor_colour(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, int colour, int it, int max_it) {
-    HV_BATCH_LOCATE();
+kb_sor_colour(const BInst<T> *__restrict__ a, const int *__restrict__ first, const int *__restrict__ ids, int n,
+              int colour, int it, int max_it) {
+    HV_BATCH_LOCATE_IDS();
     zero_next_row(A.maxd, colour, bid, it, max_it);
-    sor_colour_body<T>(bid, A.phi, A.rho_s, A.sp, A.tl, A.cf, colour, it, A.maxd, A.ctl);
+    sor_colour_body<T, kSorPlanesBatch>(bid, A.phi, A.rho_s, A.sp, A.tl, A.cf, colour, it, A.maxd, A.ctl);
 }
 
-// one colour of the mixed form's FP32 correction iteration
+// one colour of the mixed form's FP32 correction iteration, over the n
+// instances listed in ids
 template <typename T>
 __global__ void __launch_bounds__(256)
-kb_sor_colour_f32(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, int colour, int it,
-                  int max_it) {
-    HV_BATCH_LOCATE();
+kb_sor_colour_f32(const BInst<T> *__restrict__ a, const int *__restrict__ first, const int *__restrict__ ids, int n,
+                  int colour, int it, int max_it) {
+    HV_BATCH_LOCATE_IDS();
     zero_next_row(A.maxd, colour, bid, it, max_it);
-    sor_colour_f32_body<T>(bid, A.delta, A.R, A.sp, A.tl, A.cf, colour, it, A.maxd, A.ctl);
+    sor_colour_f32_body<T, kSorPlanesBatch>(bid, A.delta, A.R, A.sp, A.tl, A.cf, colour, it, A.maxd, A.ctl);
 }
 
 // per-iteration maxima of iterations [it0, it1) of every instance into
@@ -188,5 +198,6 @@ kb_gather_kick(const BInst<T> *__restrict__ a, const int *__restrict__ first, in
 }
 
 #undef HV_BATCH_LOCATE
+#undef HV_BATCH_LOCATE_IDS
 
 }  // namespace hv

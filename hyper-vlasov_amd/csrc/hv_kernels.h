@@ -768,7 +768,8 @@ __device__ __forceinline__ double sor_row_max(const unsigned long long *__restri
 // sweep to the next.  Iteration `it` first checks iteration it-1's max |dphi|:
 // once it is below tol the solve is over, and this and every later launch
 // return at once (iteration count = the reference's stop rule).
-constexpr int kSorPlanes = 2;
+constexpr int kSorPlanes = 2;       // planes per block, single-instance launches
+constexpr int kSorPlanesBatch = 8;  // planes per block, ensemble launches
 constexpr int kSorRows = 4;
 constexpr int kXcds = 8;
 
@@ -777,10 +778,15 @@ struct SorTiles {
     int per_xcd;           // tiles per XCD (grid = kXcds * per_xcd blocks)
 };
 
-template <typename RT>
+// P planes per block (kSorPlanes for the single-instance launches, more for
+// the ensemble launches, whose grids are large enough to keep every CU busy
+// with fewer, longer blocks).  All P planes' operands are loaded before the
+// first update (the colour arrays do not alias), so the loads are in flight
+// together.
+template <typename RT, int P = kSorPlanes>
 __device__ __forceinline__ void sor_colour_body(const int bid, double *__restrict__ phi, const RT *__restrict__ rho,
-                                                Split sp, SorTiles tl, SorCoef cf,
-             int colour, int it, unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
+                                                Split sp, SorTiles tl, SorCoef cf, int colour, int it,
+                                                unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
     if (ctl->done) return;
     if (colour == 0 && it > 0) {
         const double prev = sor_row_max(maxd, it - 1);
@@ -803,30 +809,39 @@ __device__ __forceinline__ void sor_colour_body(const int bid, double *__restric
         const int j = 1 + jb * kSorRows + threadIdx.y;
         const int h = hb * 64 + threadIdx.x;
         const int ny = sp.ny, nz = sp.nz, nzh = sp.nzh;
-        double *dst = phi + (colour ? sp.Gh : 0);
-        const double *oth = phi + (colour ? 0 : sp.Gh);
-        const RT *src = rho + (colour ? sp.Gh : 0);
-        if (j < ny - 1) {
-            for (int di = 0; di < kSorPlanes; di++) {
-                const int i = 1 + ic * kSorPlanes + di;
-                if (i >= sp.nx - 1) break;
-                const int off = (i + j + colour) & 1;
-                const int k = 2 * h + off;
-                if (k < 1 || k > nz - 2) continue;
-                const long long row = ((long long)i * ny + j) * nzh;
-                const long long q = row + h;
-                const long long sx = (long long)ny * nzh;
-                double r = (double)src[q];
-                double phi_x = hv_div_fixed(oth[q + sx] + oth[q - sx], cf.dx2, cf.ydx2);
-                double phi_y = hv_div_fixed(oth[q + nzh] + oth[q - nzh], cf.dy2, cf.ydy2);
-                double phi_z = hv_div_fixed(oth[q + off] + oth[q + off - 1], cf.dz2, cf.ydz2);
-                double phi_star = (phi_x + phi_y + phi_z + cf.src * r) * cf.inv_dx2;
-                double phi_old = dst[q];
-                double nv = cf.one_minus_omega * phi_old + cf.omega * phi_star;
-                dst[q] = nv;
-                double d = fabs(nv - phi_old);
-                diff = d > diff ? d : diff;
+        double *__restrict__ dst = phi + (colour ? sp.Gh : 0);
+        const double *__restrict__ oth = phi + (colour ? 0 : sp.Gh);
+        const RT *__restrict__ src = rho + (colour ? sp.Gh : 0);
+        const long long sx = (long long)ny * nzh;
+        double star[P], old[P];
+        long long qq[P];
+        bool ok[P];
+#pragma unroll
+        for (int di = 0; di < P; di++) {
+            const int i = 1 + ic * P + di;
+            const int off = (i + j + colour) & 1;
+            const int k = 2 * h + off;
+            ok[di] = j < ny - 1 && i < sp.nx - 1 && k >= 1 && k <= nz - 2;
+            const long long q = ((long long)i * ny + j) * nzh + h;
+            qq[di] = q;
+            star[di] = 0.0;
+            old[di] = 0.0;
+            if (ok[di]) {
+                const double r = (double)src[q];
+                const double phi_x = hv_div_fixed(oth[q + sx] + oth[q - sx], cf.dx2, cf.ydx2);
+                const double phi_y = hv_div_fixed(oth[q + nzh] + oth[q - nzh], cf.dy2, cf.ydy2);
+                const double phi_z = hv_div_fixed(oth[q + off] + oth[q + off - 1], cf.dz2, cf.ydz2);
+                star[di] = (phi_x + phi_y + phi_z + cf.src * r) * cf.inv_dx2;
+                old[di] = dst[q];
             }
+        }
+#pragma unroll
+        for (int di = 0; di < P; di++) {
+            if (!ok[di]) continue;
+            const double nv = cf.one_minus_omega * old[di] + cf.omega * star[di];
+            dst[qq[di]] = nv;
+            const double d = fabs(nv - old[di]);
+            diff = d > diff ? d : diff;
         }
     }
     block_max_atomic(diff, &maxd[((long long)it * kMaxSlots + (b % kMaxSlots)) * kSlotStride]);
@@ -883,10 +898,11 @@ k_sor_defect(const double *__restrict__ phi, const RT *__restrict__ rho, float *
     sor_defect_body<RT>(blockIdx.x, phi, rho, R, delta, sp, cf);
 }
 
-template <typename RT>
+template <typename RT, int P = kSorPlanes>
 __device__ __forceinline__ void sor_colour_f32_body(const int bid, float *__restrict__ delta,
                                                     const float *__restrict__ R, Split sp, SorTiles tl, SorCoef cf,
-                 int colour, int it, unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
+                                                    int colour, int it, unsigned long long *__restrict__ maxd,
+                                                    SorCtl *__restrict__ ctl) {
     if (ctl->done) return;
     if (colour == 0 && it > 0) {
         const double prev = sor_row_max(maxd, it - 1);
@@ -909,28 +925,38 @@ __device__ __forceinline__ void sor_colour_f32_body(const int bid, float *__rest
         const int j = 1 + jb * kSorRows + threadIdx.y;
         const int h = hb * 64 + threadIdx.x;
         const int ny = sp.ny, nz = sp.nz, nzh = sp.nzh;
-        float *dst = delta + (colour ? sp.Gh : 0);
-        const float *oth = delta + (colour ? 0 : sp.Gh);
-        const float *src = R + (colour ? sp.Gh : 0);
+        float *__restrict__ dst = delta + (colour ? sp.Gh : 0);
+        const float *__restrict__ oth = delta + (colour ? 0 : sp.Gh);
+        const float *__restrict__ src = R + (colour ? sp.Gh : 0);
         const float yx = (float)cf.ydx2, yy = (float)cf.ydy2, yz = (float)cf.ydz2, inv = (float)cf.inv_dx2;
         const float om = (float)cf.omega, om1 = (float)cf.one_minus_omega;
-        if (j < ny - 1) {
-            for (int di = 0; di < kSorPlanes; di++) {
-                const int i = 1 + ic * kSorPlanes + di;
-                if (i >= sp.nx - 1) break;
-                const int off = (i + j + colour) & 1;
-                const int k = 2 * h + off;
-                if (k < 1 || k > nz - 2) continue;
-                const long long q = ((long long)i * ny + j) * nzh + h;
-                const long long sx = (long long)ny * nzh;
-                const float s = (oth[q + sx] + oth[q - sx]) * yx + (oth[q + nzh] + oth[q - nzh]) * yy +
-                                (oth[q + off] + oth[q + off - 1]) * yz + src[q];
-                const float old = dst[q];
-                const float nv = om1 * old + om * (s * inv);
-                dst[q] = nv;
-                const float d = fabsf(nv - old);
-                diff = d > diff ? d : diff;
+        const long long sx = (long long)ny * nzh;
+        float sum[P], old[P];
+        long long qq[P];
+        bool ok[P];
+#pragma unroll
+        for (int di = 0; di < P; di++) {
+            const int i = 1 + ic * P + di;
+            const int off = (i + j + colour) & 1;
+            const int k = 2 * h + off;
+            ok[di] = j < ny - 1 && i < sp.nx - 1 && k >= 1 && k <= nz - 2;
+            const long long q = ((long long)i * ny + j) * nzh + h;
+            qq[di] = q;
+            sum[di] = 0.0f;
+            old[di] = 0.0f;
+            if (ok[di]) {
+                sum[di] = (oth[q + sx] + oth[q - sx]) * yx + (oth[q + nzh] + oth[q - nzh]) * yy +
+                          (oth[q + off] + oth[q + off - 1]) * yz + src[q];
+                old[di] = dst[q];
             }
+        }
+#pragma unroll
+        for (int di = 0; di < P; di++) {
+            if (!ok[di]) continue;
+            const float nv = om1 * old[di] + om * (sum[di] * inv);
+            dst[qq[di]] = nv;
+            const float d = fabsf(nv - old[di]);
+            diff = d > diff ? d : diff;
         }
     }
     block_max_atomic((double)diff, &maxd[((long long)it * kMaxSlots + (b % kMaxSlots)) * kSlotStride]);

@@ -96,6 +96,14 @@ class Batch:
     def profile_enable(self, on: bool = True):
         self._rc("hv_batch_profile_enable", self.lib.hv_batch_profile_enable(self.b, 1 if on else 0))
 
+    def sor_counts(self) -> dict:
+        """Sweep launches and interior-node iterations (summed over instances)
+        of the FP32 correction and of FP64 since profile_enable."""
+        v = [ctypes.c_longlong() for _ in range(4)]
+        self._rc("hv_batch_sor_counts", self.lib.hv_batch_sor_counts(self.b, *[ctypes.byref(x) for x in v]))
+        return {"sweeps_f32": v[0].value, "sweeps_f64": v[1].value, "node_iters_f32": v[2].value,
+                "node_iters_f64": v[3].value}
+
     def profile_read(self) -> dict:
         out = {}
         for k, name in enumerate(_lib.KCLASS_NAMES):

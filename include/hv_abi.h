@@ -283,6 +283,12 @@ int hv_batch_run_steps(hv_batch *batch, const hv_skyrme *skyrme, const hv_params
  * dispatch time) */
 int hv_batch_profile_enable(hv_batch *batch, int on);
 int hv_batch_profile_read(hv_batch *batch, int kclass, double *ms, long long *launches);
+/* SOR work since hv_batch_profile_enable: sweep launches (each covers every
+ * instance still iterating) of the FP32 correction and of FP64, and the
+ * interior-node iterations summed over instances (sum_i iterations_i x
+ * (nx-2)(ny-2)(nz-2)_i) */
+int hv_batch_sor_counts(hv_batch *batch, long long *sweeps_f32, long long *sweeps_f64, long long *node_iters_f32,
+                        long long *node_iters_f64);
 
 #ifdef __cplusplus
 }
