@@ -115,6 +115,62 @@ def run_gpu(w, prec: str, steps: int, warmup: int):
     return dict(wall=wall, its=its, prof=prof, sor=sor, stats=stats, state=state)
 
 
+def run_batch(ws, prec: str, steps: int, tune: bool):
+    """The ensemble in one batch: init, autotuned warm-up (tile shape and sort
+    cadence), `steps` timed steps, then a few profiled steps for the sweep
+    kernel's dispatch time and the SOR work counters."""
+    import hvamd
+    from hvamd.autotune import tune_batch
+    from hvamd.batch import Batch
+    p = hvamd.HV_PREC_F64 if prec == "fp64" else hvamd.HV_PREC_MIXED
+    prof_steps = 4
+    total_steps = 16 + steps + prof_steps
+    b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws], p)
+    for i, w in enumerate(ws):
+        b.instance(i).upload_particles(*w.initial_state())
+    sks = [w.sky() for w in ws]
+    prm = [w.params(total_steps + 32) for w in ws]
+    b.run_init(sks, prm)
+    step = 0
+    best, timings = None, None
+    if tune:
+        best, timings, step = tune_batch(b, sks, prm, 0, steps_per_trial=1, sort_choices=(0, 8),
+                                         tile_choices=(16, 64))
+    b.run_steps(sks, prm, step, 2)
+    step += 2
+    t0 = time.perf_counter()
+    _, its = b.run_steps(sks, prm, step, steps)
+    wall = time.perf_counter() - t0
+    step += steps
+    b.profile_enable(True)
+    b.run_steps(sks, prm, step, prof_steps)
+    prof = b.profile_read()
+    cnt = b.sor_counts()
+    b.profile_enable(False)
+    b.close()
+    n_tot = sum(w.total for w in ws)
+    kcls = "sor_sweep_f32" if prec == "mixed" else "sor_sweep"
+    ms, nl = prof[kcls]
+    sweeps = cnt["sweeps_f32"] if prec == "mixed" else cnt["sweeps_f64"]
+    node_iters = cnt["node_iters_f32"] if prec == "mixed" else cnt["node_iters_f64"]
+    bpn = 12.0 if prec == "mixed" else 24.0
+    avg_ms = ms / max(nl, 1)
+    achieved = node_iters * bpn / (sweeps * avg_ms * 1e-3) / 1e9 if sweeps else 0.0
+    return {
+        "value": n_tot * steps / wall, "ms_per_step": 1e3 * wall / steps, "test_particles": n_tot,
+        "instances": len(ws), "grid_nodes": sum(w.grid ** 3 for w in ws),
+        "sor_iters_per_step": {"mean": float(its.mean()), "max": int(its.max())},
+        "tuned": best, "tuning_ms_per_step": {f"sort{k[0]}_tile{k[1]}": v for k, v in (timings or {}).items()},
+        "stage_ms_per_step": {k: v[0] / prof_steps for k, v in prof.items() if v[1] and k in
+                              ("particle_pass", "smear", "sor_stage", "potential")},
+        "roofline": {"bound": "hbm", "kernel": ("kb_sor_colour_f32" if prec == "mixed" else "kb_sor_colour") +
+                     " (one red or black sweep over every instance still iterating)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "avg_launch_us": 1e3 * avg_ms, "launches": sweeps,
+                     "bytes_per_launch": node_iters * bpn / max(sweeps, 1)},
+    }
+
+
 def cpu_baseline(w, state, budget_s: float):
     """The CPU restatement (oracle/liboracle.so, OpenMP) from the same warm
     mid-run state, on this host's cores; a bounded number of steps."""
@@ -162,6 +218,8 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--ensemble", type=int, default=256, help="instances of the c3 ensemble (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=8)
     ap.add_argument("--fp64", type=int, default=1, help="also time the plain FP64 form")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -214,6 +272,13 @@ def main():
         out["mixed_over_fp64"] = value / v64
         if args.cpu_budget > 0:
             out["cpu_baseline"] = cpu_baseline(w, fp64["state"], args.cpu_budget)
+    if args.ensemble > 0:
+        ens = configs.c3(args.ensemble)
+        bm = run_batch(ens, "mixed", args.ensemble_steps, tune=True)
+        b64 = run_batch(ens, "fp64", args.ensemble_steps, tune=False) if args.fp64 else None
+        out["ensemble_c3"] = {"mixed": bm, "fp64": b64,
+                              "mixed_over_fp64": bm["value"] / b64["value"] if b64 else None,
+                              "mixed_over_single_run_fp64": bm["value"] / out["fp64"]["value"] if args.fp64 else None}
     print(json.dumps(out), flush=True)
 
 
