@@ -1677,3 +1677,6 @@ const char *hv_build_info(void) {
 
 // ---- ensembles (hv_batch_*) -------------------------------------------------------
 #include "hv_batch_impl.h"
+
+// ---- adjoint of the loop (hv_dipole_loss_grad) -------------------------------------
+#include "hv_adjoint_impl.h"

@@ -148,6 +148,8 @@ SIGNATURES = {
     "hv_batch_profile_enable": (_I, [_P, _I]),
     "hv_batch_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
     "hv_batch_sor_counts": (_I, [_P] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
+    "hv_dipole_loss_grad": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _I, _DP, _I, _DP, _DP,
+                                 _DP]),
     "hv_fermi_momentum": (_D, [_D]),
     "hv_sample_uniform_spheres": (_I, [ctypes.c_ulonglong, _I, _I, _D, _D] + [_P] * 6),
     "hv_mix64": (ctypes.c_ulonglong, [ctypes.c_ulonglong]),

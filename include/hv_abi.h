@@ -290,6 +290,22 @@ int hv_batch_profile_read(hv_batch *batch, int kclass, double *ms, long long *la
 int hv_batch_sor_counts(hv_batch *batch, long long *sweeps_f32, long long *sweeps_f64, long long *node_iters_f32,
                         long long *node_iters_f64);
 
+/* ---- adjoint (no counterpart in the reference) ---------------------------- */
+/* For fitting the Skyrme coefficients to a dipole response (the use of the
+ * reference's fit scripts): starting from the particles the context holds
+ * (uploaded, caller's order), runs hv_run_init and nsteps steps (FP64 context
+ * only), and returns
+ *   loss = sum_{s < nsteps} (D(t_s) - d_target[s])^2,  D = N Z / A (<z>_p - <z>_n)
+ * of the positions at the start of step s (scripts/programs.py:99), the
+ * series D in d_series[nsteps] (may be NULL) and, when grad is not NULL, the
+ * gradient dloss/d(A, B, C, gamma) (hv_skyrme order) by the discrete adjoint of
+ * the loop, with trajectory checkpoints every checkpoint_every steps.  The
+ * adjoint treats each SOR solve as exact.  The particles are restored to the
+ * initial state on return (fields and phi are not), so calls can be repeated.
+ * Returns warning bits as hv_run_steps. */
+int hv_dipole_loss_grad(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, int nsteps,
+                        const double *d_target, int checkpoint_every, double *loss, double *d_series, double *grad);
+
 #ifdef __cplusplus
 }
 #endif
