@@ -40,6 +40,7 @@ struct hv_batch {
     std::vector<int> boost;               // per-instance boost flag of the current step
     bool ready = false;
     int sort_every = 8;
+    int group_kib = 0;  // SOR working-set budget (KiB) of one instance group (0: all instances together)
     // stage timing (hipEvent pairs per stage, when enabled)
     bool prof = false;
     double prof_ms[HV_KCLASS_COUNT] = {};
@@ -235,9 +236,14 @@ struct BStage {
 
 // SOR iterations of every instance in chunks of batched sweeps; iters[i]
 // receives instance i's count under the reference stop rule, capped at
-// budget[i].  Each chunk's launches cover only the instances still iterating
-// at its start (a converged instance's blocks would return at once, but
-// would still be dispatched).
+// budget[i].  Instances are iterated in groups whose sweep working set (the
+// iterate and its source) fits the budget group_kib KiB, one group after the
+// other (meant to keep a group's sweeps in the memory-side cache; on the
+// 256-nucleus ensemble one group of all instances measured fastest, hence the
+// default 0); each instance's iterates are the same whatever the grouping.
+// Each chunk's launches cover only the instances of the group still
+// iterating at its start (a converged instance's blocks would return at once,
+// but would still be dispatched).
 template <typename T>
 int batch_sor_iterate(hv_batch *b, const hv_params *prm, bool f32, const std::vector<int> &pred,
                       const std::vector<int> &budget, std::vector<int> &iters, std::vector<double> &lastmax) {
@@ -248,90 +254,112 @@ int batch_sor_iterate(hv_batch *b, const hv_params *prm, bool f32, const std::ve
     HV_BHIP(b, hipGetLastError());
     iters.assign(n, -1);
     lastmax.assign(n, 0.0);
-    int remaining = n, it = 0, chunk = 1;
-    for (int i = 0; i < n; i++) chunk = pred[i] - 1 > chunk ? pred[i] - 1 : chunk;
+    // groups of consecutive instances under the working-set budget
+    std::vector<int> grp(n, 0);
+    int ngroups = 1;
+    {
+        const double cap = b->group_kib > 0 ? b->group_kib * 1024.0 : 1e300;
+        double acc = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double bytes = (f32 ? 8.0 : (double)(sizeof(double) + sizeof(T))) * 2.0 * (double)b->inst[i]->sp.Gh;
+            if (acc > 0.0 && acc + bytes > cap) {
+                ngroups++;
+                acc = 0.0;
+            }
+            acc += bytes;
+            grp[i] = ngroups - 1;
+        }
+    }
     const dim3 sblk(64, kSorRows, 1);
-    while (remaining > 0) {
-        // launch subset: instances not converged and under budget
-        int m = 0, blocks = 0, room = max_it;
-        b->hact[0] = 0;
+    for (int g = 0; g < ngroups; g++) {
+        int remaining = 0, it = 0, chunk = 1;
         for (int i = 0; i < n; i++) {
-            if (iters[i] >= 0) continue;
-            b->hact[n + 1 + m] = i;
-            blocks += b->sor_blocks[i];
-            b->hact[++m] = blocks;
-            room = budget[i] - it < room ? budget[i] - it : room;
+            if (grp[i] != g) continue;
+            remaining++;
+            chunk = pred[i] - 1 > chunk ? pred[i] - 1 : chunk;
         }
-        // ids right after the m + 1 prefix entries
-        for (int k = 0; k < m; k++) b->hact[m + 1 + k] = b->hact[n + 1 + k];
-        HV_BHIP(b, hipMemcpyAsync(b->act, b->hact, sizeof(int) * (2 * (size_t)m + 1), hipMemcpyHostToDevice,
-                                  b->stream));
-        const int start = it;
-        const int len = chunk < room ? chunk : room;
-        const int end = it + len;
-        const dim3 sgrd((unsigned)blocks, 1, 1);
-        const int *afirst = b->act, *aids = b->act + m + 1;
-        for (; it < end; it++)
-            for (int colour = 0; colour < 2; colour++) {
-                // one launch in kSorSample is timed by start/stop events of its own dispatch
-                hipEvent_t ea = nullptr, ee = nullptr;
-                if (b->prof && ((it * 2 + colour) % kSorSample) == 3 &&
-                    hipEventCreate(&ea) == hipSuccess && hipEventCreate(&ee) == hipSuccess) {
-                    b->samples.push_back({f32 ? HV_KCLASS_SOR_F32 : HV_KCLASS_SOR, ea, ee});
-                } else if (ea) {
-                    (void)hipEventDestroy(ea);
-                    ea = nullptr;
+        while (remaining > 0) {
+            // launch subset: instances not converged and under budget
+            int m = 0, blocks = 0, room = max_it;
+            b->hact[0] = 0;
+            for (int i = 0; i < n; i++) {
+                if (iters[i] >= 0 || grp[i] != g) continue;
+                b->hact[n + 1 + m] = i;
+                blocks += b->sor_blocks[i];
+                b->hact[++m] = blocks;
+                room = budget[i] - it < room ? budget[i] - it : room;
+            }
+            // ids right after the m + 1 prefix entries
+            for (int k = 0; k < m; k++) b->hact[m + 1 + k] = b->hact[n + 1 + k];
+            HV_BHIP(b, hipMemcpyAsync(b->act, b->hact, sizeof(int) * (2 * (size_t)m + 1), hipMemcpyHostToDevice,
+                                      b->stream));
+            const int start = it;
+            const int len = chunk < room ? chunk : room;
+            const int end = it + len;
+            const dim3 sgrd((unsigned)blocks, 1, 1);
+            const int *afirst = b->act, *aids = b->act + m + 1;
+            for (; it < end; it++)
+                for (int colour = 0; colour < 2; colour++) {
+                    // one launch in kSorSample is timed by start/stop events of its own dispatch
+                    hipEvent_t ea = nullptr, ee = nullptr;
+                    if (b->prof && ((it * 2 + colour) % kSorSample) == 3 &&
+                        hipEventCreate(&ea) == hipSuccess && hipEventCreate(&ee) == hipSuccess) {
+                        b->samples.push_back({f32 ? HV_KCLASS_SOR_F32 : HV_KCLASS_SOR, ea, ee});
+                    } else if (ea) {
+                        (void)hipEventDestroy(ea);
+                        ea = nullptr;
+                    }
+                    if (f32)
+                        hipExtLaunchKernelGGL(kb_sor_colour_f32<T>, sgrd, sblk, 0u, b->stream, ea, ee, 0u,
+                                              (const BInst<T> *)a, afirst, aids, m, colour, it, max_it);
+                    else
+                        hipExtLaunchKernelGGL(kb_sor_colour<T>, sgrd, sblk, 0u, b->stream, ea, ee, 0u,
+                                              (const BInst<T> *)a, afirst, aids, m, colour, it, max_it);
                 }
-                if (f32)
-                    hipExtLaunchKernelGGL(kb_sor_colour_f32<T>, sgrd, sblk, 0u, b->stream, ea, ee, 0u,
-                                          (const BInst<T> *)a, afirst, aids, m, colour, it, max_it);
-                else
-                    hipExtLaunchKernelGGL(kb_sor_colour<T>, sgrd, sblk, 0u, b->stream, ea, ee, 0u,
-                                          (const BInst<T> *)a, afirst, aids, m, colour, it, max_it);
+            HV_BHIP(b, hipGetLastError());
+            b->sweeps[f32 ? 0 : 1] += 2LL * len;
+            if (b->maxc_width < len) {
+                if (b->maxc) (void)hipFree(b->maxc);
+                b->maxc = nullptr;
+                b->maxc_width = 0;
+                HV_BHIP(b, hipMalloc((void **)&b->maxc, sizeof(double) * (size_t)n * len));
+                b->maxc_width = len;
             }
-        HV_BHIP(b, hipGetLastError());
-        b->sweeps[f32 ? 0 : 1] += 2LL * len;
-        if (b->maxc_width < len) {
-            if (b->maxc) (void)hipFree(b->maxc);
-            b->maxc = nullptr;
-            b->maxc_width = 0;
-            HV_BHIP(b, hipMalloc((void **)&b->maxc, sizeof(double) * (size_t)n * len));
-            b->maxc_width = len;
-        }
-        int rc = batch_pinned(b, (size_t)n * len);
-        if (rc) return rc;
-        kb_sor_collect<T><<<(unsigned)(n * len), 64, 0, b->stream>>>(a, b->maxc, len, start, end);
-        HV_BHIP(b, hipGetLastError());
-        HV_BHIP(b, hipMemcpyAsync(b->pinned, b->maxc, sizeof(double) * (size_t)n * len, hipMemcpyDeviceToHost,
-                                  b->stream));
-        HV_BHIP(b, hipStreamSynchronize(b->stream));
-        for (auto &smp : b->samples) {
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, smp.a, smp.e) == hipSuccess) {
-                b->prof_ms[smp.cls] += ms;
-                b->prof_launches[smp.cls] += 1;
-            }
-            (void)hipEventDestroy(smp.a);
-            (void)hipEventDestroy(smp.e);
-        }
-        b->samples.clear();
-        for (int i = 0; i < n; i++) {
-            if (iters[i] >= 0) continue;
-            const double *row = b->pinned + (size_t)i * len;
-            for (int q = 0; q < len; q++) {
-                if (row[q] < prm[i].sor_tol) {
-                    iters[i] = start + q + 1;
-                    lastmax[i] = row[q];
-                    break;
+            int rc = batch_pinned(b, (size_t)n * len);
+            if (rc) return rc;
+            kb_sor_collect<T><<<(unsigned)(n * len), 64, 0, b->stream>>>(a, b->maxc, len, start, end);
+            HV_BHIP(b, hipGetLastError());
+            HV_BHIP(b, hipMemcpyAsync(b->pinned, b->maxc, sizeof(double) * (size_t)n * len, hipMemcpyDeviceToHost,
+                                      b->stream));
+            HV_BHIP(b, hipStreamSynchronize(b->stream));
+            for (auto &smp : b->samples) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, smp.a, smp.e) == hipSuccess) {
+                    b->prof_ms[smp.cls] += ms;
+                    b->prof_launches[smp.cls] += 1;
                 }
+                (void)hipEventDestroy(smp.a);
+                (void)hipEventDestroy(smp.e);
             }
-            if (iters[i] < 0) {
-                lastmax[i] = row[len - 1];
-                if (end >= budget[i]) iters[i] = budget[i];  // out of budget: not converged
+            b->samples.clear();
+            for (int i = 0; i < n; i++) {
+                if (iters[i] >= 0 || grp[i] != g) continue;
+                const double *row = b->pinned + (size_t)i * len;
+                for (int q = 0; q < len; q++) {
+                    if (row[q] < prm[i].sor_tol) {
+                        iters[i] = start + q + 1;
+                        lastmax[i] = row[q];
+                        break;
+                    }
+                }
+                if (iters[i] < 0) {
+                    lastmax[i] = row[len - 1];
+                    if (end >= budget[i]) iters[i] = budget[i];  // out of budget: not converged
+                }
+                if (iters[i] >= 0) remaining--;
             }
-            if (iters[i] >= 0) remaining--;
+            chunk = 4;
         }
-        chunk = 4;
     }
     return HV_OK;
 }
@@ -579,6 +607,11 @@ hv_ctx *hv_batch_instance(hv_batch *b, int i) {
 
 int hv_batch_set_option(hv_batch *b, int option, int value) {
     if (!b) return HV_ERR_ARG;
+    if (option == HV_OPT_BATCH_GROUP_KIB) {
+        if (value < 0) return batch_err(b, HV_ERR_ARG, "group budget must be >= 0");
+        b->group_kib = value;
+        return HV_OK;
+    }
     if (option != HV_OPT_SORT_EVERY && option != HV_OPT_SMEAR_TILE)
         return batch_err(b, HV_ERR_ARG, "option %d is not a batch option", option);
     for (hv_ctx *c : b->inst) {

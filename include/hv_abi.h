@@ -220,6 +220,10 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * smearing's LDS tiles may use; narrower tiles trade halo re-reads for more
  * blocks.  Same bits for every width. */
 #define HV_OPT_SMEAR_TILE 5
+/* HV_OPT_BATCH_GROUP_KIB (ensembles only): the SOR of a batch iterates its
+ * instances in groups whose sweep working set fits this many KiB (default 0:
+ * all instances together).  Same bits for every value. */
+#define HV_OPT_BATCH_GROUP_KIB 6
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
@@ -268,7 +272,8 @@ const char *hv_batch_last_error(const hv_batch *batch);
 int hv_batch_size(const hv_batch *batch);
 /* instance i's context: particle/field transfer, diagnostics, total energy */
 hv_ctx *hv_batch_instance(hv_batch *batch, int i);
-/* HV_OPT_SORT_EVERY or HV_OPT_SMEAR_TILE, applied to every instance */
+/* HV_OPT_SORT_EVERY or HV_OPT_SMEAR_TILE (applied to every instance), or
+ * HV_OPT_BATCH_GROUP_KIB */
 int hv_batch_set_option(hv_batch *batch, int option, int value);
 /* skyrme[n], params[n]; sor_iters[n] (may be NULL): the pre-loop solves */
 int hv_batch_run_init(hv_batch *batch, const hv_skyrme *skyrme, const hv_params *params, int *sor_iters);

@@ -78,16 +78,18 @@ def test_batch_equals_solo_runs(prec):
 
 
 def test_batch_options_do_not_change_bits():
-    """Sort cadence and smearing tile width (the autotuned knobs) leave every
+    """Sort cadence, smearing tile width (the autotuned knobs) and the SOR
+    instance grouping (1 KiB: every instance a group of its own) leave every
     bit of a batched run unchanged."""
     print("batch options", flush=True)
     ws, params = _instances()
     outs = []
-    for sort_every, tile in ((0, 64), (3, 16)):
+    for sort_every, tile, group_kib in ((0, 64, 0), (3, 16, 1)):
         b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws],
                   hvamd.HV_PREC_MIXED)
         b.set_option(hvamd.HV_OPT_SORT_EVERY, sort_every)
         b.set_option(hvamd.HV_OPT_SMEAR_TILE, tile)
+        b.set_option(hvamd.HV_OPT_BATCH_GROUP_KIB, group_kib)
         for i, w in enumerate(ws):
             b.instance(i).upload_particles(*w.initial_state())
         sks = [w.sky() for w in ws]
