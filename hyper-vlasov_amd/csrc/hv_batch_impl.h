@@ -11,11 +11,14 @@
 //
 // Per-instance results equal a solo hv_run_steps of the same instance with the
 // same options bit for bit: the same block bodies run on the same operands and
-// the SOR stop rule is evaluated per instance.  Batch instances use the
-// per-sweep SOR launches (the LDS-resident and fused SOR forms are
-// single-instance only).
+// the SOR stop rule is evaluated per instance.  The mixed form's FP32
+// correction runs LDS-resident for all instances in one persistent launch
+// (hv_batch_resident.h; the bits of a solo LDS-resident solve), or as
+// per-sweep launches when that is switched off or an instance does not tile;
+// the FP64 sweeps use per-sweep launches.
 
 #include "hv_batch_kernels.h"
+#include "hv_batch_resident.h"
 
 enum { kBPart, kBZy, kBN2, kBG, kBGh2, kBSor, kBKinds };
 
@@ -41,6 +44,28 @@ struct hv_batch {
     bool ready = false;
     int sort_every = 8;
     int group_kib = 0;  // SOR working-set budget (KiB) of one instance group (0: all instances together)
+    // LDS-resident FP32 correction of the whole batch (mixed precision)
+    int res32 = 1;                        // option: use it when every instance tiles
+    int res32_lag = 2;                    // option: lag of its convergence test
+    int res_state = 0;                    // 0: not set up, 1: ready, -1: unavailable
+    int res_max_it = 0;
+    int res_nblk = 0;                     // blocks of the persistent launch (one per CU)
+    size_t res_lds = 0;
+    std::vector<ResTiles> rtl;            // per-instance tiling
+    const void *rkernel = nullptr;        // ensemble kernel instance (its compile-time shape)
+    std::vector<size_t> rx_off, rslot_off;
+    size_t rx_words = 0, rslot_words = 0, rjobs_n = 0;
+    void *rinst = nullptr;                // BRes32[n]
+    unsigned long long *rx = nullptr;     // every instance's tagged exchange words
+    float *rslots = nullptr;              // every instance's slot rows
+    float *rres = nullptr;                // per instance: iterations, tested max
+    int *rerr = nullptr;
+    int *rjob = nullptr;                  // job_first (nblk + 1) then jobs (int2 per tile)
+    int *hrjob = nullptr;                 // pinned staging of rjob
+    float *hrres = nullptr;               // pinned read-back of rres and rerr
+#ifdef HV_BRES_TRACE
+    unsigned long long *jtrace = nullptr;
+#endif
     // stage timing (hipEvent pairs per stage, when enabled)
     bool prof = false;
     double prof_ms[HV_KCLASS_COUNT] = {};
@@ -364,6 +389,248 @@ int batch_sor_iterate(hv_batch *b, const hv_params *prm, bool f32, const std::ve
     return HV_OK;
 }
 
+// ---- LDS-resident FP32 correction of the whole batch ---------------------------
+// tile shapes with a compile-time body: (TI, TJ, nzh)
+constexpr int kRes32Shapes[][3] = {{16, 12, 48}, {16, 16, 32}, {32, 16, 16}};
+constexpr int kRes32NShapes = 3;
+
+const void *res32_batch_kernel(int shape) {
+    switch (shape) {
+        case 0: return (const void *)kb_sor_resident_f32<16, 12, 48>;
+        case 1: return (const void *)kb_sor_resident_f32<16, 16, 32>;
+        case 2: return (const void *)kb_sor_resident_f32<32, 16, 16>;
+        default: return (const void *)kb_sor_resident_f32<0, 0, 0>;
+    }
+}
+
+int res32_shape_of(const ResTiles &t) {
+    for (int k = 0; k < kRes32NShapes; k++)
+        if (t.TI == kRes32Shapes[k][0] && t.TJ == kRes32Shapes[k][1] && t.nzh == kRes32Shapes[k][2]) return k;
+    return -1;
+}
+
+// Tiling of one instance for the ensemble kernel: the fewest tiles (the
+// largest LDS-resident tiles) whose LDS and halo words fit, a shape with a
+// compile-time body counting as 30% fewer tiles.
+int batch_res_tiles(const hv_ctx *c, size_t maxlds, int maxblk, ResTiles *out) {
+    const int nx = c->world.n[0], ny = c->world.n[1];
+    long long best_cost = -1;
+    for (int PI = 1; PI <= nx; PI++) {
+        for (int PJ = 1; PJ <= ny; PJ++) {
+            ResTiles t;
+            t.TI = (nx + PI - 1) / PI;
+            t.TJ = (ny + PJ - 1) / PJ;
+            t.PI = (nx + t.TI - 1) / t.TI;
+            t.PJ = (ny + t.TJ - 1) / t.TJ;
+            if (t.PI != PI || t.PJ != PJ) continue;
+            const int nT = t.PI * t.PJ;
+            if (nT > maxblk || nT > kSlotPoll * kWave) continue;
+            t.nzh = c->sp.nzh;
+            t.side_i = t.TJ * t.nzh;
+            t.side_j = t.TI * t.nzh;
+            t.xstride = 2 * t.side_i + 2 * t.side_j;
+            if (res32_lds_bytes(t) > maxlds || t.xstride > kFetchMax * kRes32Threads) continue;
+            const bool fixed = res32_shape_of(t) >= 0;
+            const long long cost = (long long)nT * (fixed ? 100 : 130) * 4096 + t.TI + t.TJ;
+            if (best_cost < 0 || cost < best_cost) {
+                best_cost = cost;
+                *out = t;
+            }
+        }
+    }
+    return best_cost < 0 ? 1 : HV_OK;
+}
+
+// job table layout: job_first[nblk + 1], then the (instance, tile) pairs from
+// an 8-byte aligned offset
+inline size_t res32_jobs_at(int nblk) { return ((size_t)nblk + 2) & ~(size_t)1; }
+
+// Tilings and buffers for solves of up to max_it iterations; nonzero when the
+// device or an instance does not allow the resident form (the batch then
+// sweeps).
+int batch_res32_setup(hv_batch *b, int max_it) {
+    if (b->res_state < 0) return 1;
+    if (b->res_state > 0 && b->res_max_it >= max_it) return HV_OK;
+    b->res_state = -1;
+    int dev = 0, ncu = 0, maxlds = 0, coop = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return 1;
+    const int n = b->n;
+    b->rtl.assign(n, ResTiles{});
+    b->rx_off.assign(n + 1, 0);
+    b->rslot_off.assign(n + 1, 0);
+    size_t lds = 0;
+    double share[kRes32NShapes + 1] = {};  // nodes per compile-time shape
+    for (int i = 0; i < n; i++) {
+        if (batch_res_tiles(b->inst[i], (size_t)maxlds - 1024, ncu, &b->rtl[i]) != HV_OK) return 1;
+        const int sh = res32_shape_of(b->rtl[i]);
+        if (sh >= 0) share[sh] += (double)b->inst[i]->G;
+        const size_t nT = (size_t)b->rtl[i].PI * b->rtl[i].PJ;
+        b->rx_off[i + 1] = b->rx_off[i] + 2 * nT * (size_t)b->rtl[i].xstride;
+        b->rslot_off[i + 1] = b->rslot_off[i] + (size_t)max_it * nT;
+        const size_t l = res32_lds_bytes(b->rtl[i]);
+        lds = l > lds ? l : lds;
+    }
+    int best = -1;
+    for (int k = 0; k < kRes32NShapes; k++)
+        if (share[k] > 0.0 && (best < 0 || share[k] > share[best])) best = k;
+    const void *kern = res32_batch_kernel(best);
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return 1;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kRes32Threads, lds) != hipSuccess || per_cu < 1)
+        return 1;
+    b->res_nblk = ncu * per_cu;
+    size_t tiles = 0;
+    for (int i = 0; i < n; i++) tiles += (size_t)b->rtl[i].PI * b->rtl[i].PJ;
+    if (b->rx) (void)hipFree(b->rx);
+    if (b->rslots) (void)hipFree(b->rslots);
+    if (b->rjob) (void)hipFree(b->rjob);
+    if (b->hrjob) (void)hipHostFree(b->hrjob);
+    b->rx = nullptr;
+    b->rslots = nullptr;
+    b->rjob = nullptr;
+    b->hrjob = nullptr;
+    const size_t jw = res32_jobs_at(b->res_nblk) + 2 * tiles;
+    if (hipMalloc((void **)&b->rx, sizeof(unsigned long long) * b->rx_off[n]) != hipSuccess) return 1;
+    if (hipMalloc((void **)&b->rslots, sizeof(float) * b->rslot_off[n]) != hipSuccess) return 1;
+    if (hipMalloc((void **)&b->rjob, sizeof(int) * jw) != hipSuccess) return 1;
+    if (hipHostMalloc((void **)&b->hrjob, sizeof(int) * jw, hipHostMallocDefault) != hipSuccess) return 1;
+    if (!b->rinst && hipMalloc(&b->rinst, sizeof(BRes32) * (size_t)n) != hipSuccess) return 1;
+    if (!b->rres && hipMalloc((void **)&b->rres, sizeof(float) * (2 * (size_t)n + 4)) != hipSuccess) return 1;
+    if (!b->hrres && hipHostMalloc((void **)&b->hrres, sizeof(float) * (2 * (size_t)n + 4), hipHostMallocDefault) !=
+                         hipSuccess)
+        return 1;
+    b->rerr = (int *)(b->rres + 2 * (size_t)n);
+#ifdef HV_BRES_TRACE
+    if (b->jtrace) (void)hipFree(b->jtrace);
+    if (hipMalloc((void **)&b->jtrace, sizeof(unsigned long long) * 2 * tiles) != hipSuccess) return 1;
+#endif
+    b->rx_words = b->rx_off[n];
+    b->rslot_words = b->rslot_off[n];
+    b->rjobs_n = tiles;
+    b->res_lds = lds;
+    b->rkernel = kern;
+    b->res_max_it = max_it;
+    b->res_state = 1;
+    return HV_OK;
+}
+
+// Job queues of one solve.  Instance i's tiles all run for about pred[i]
+// iterations of a cost that grows with the tile size; instances are placed in
+// order of decreasing work on the blocks that are free first (list
+// scheduling), which keeps one global instance order on every queue.
+void batch_res32_plan(hv_batch *b, const std::vector<int> &pred) {
+    const int n = b->n, nb = b->res_nblk;
+    std::vector<double> w(n);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) {
+        const ResTiles &t = b->rtl[i];
+        w[i] = (double)(pred[i] > 1 ? pred[i] : 1) * ((double)t.TI * t.TJ * t.nzh + 512.0);
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return w[x] > w[y]; });
+    std::vector<double> freet(nb, 0.0);
+    std::vector<std::vector<std::pair<int, int>>> q(nb);
+    std::vector<int> blk(nb);
+    for (int i : order) {
+        const int nT = b->rtl[i].PI * b->rtl[i].PJ;
+        for (int k = 0; k < nb; k++) blk[k] = k;
+        std::partial_sort(blk.begin(), blk.begin() + nT, blk.end(), [&](int x, int y) {
+            return freet[x] < freet[y] || (freet[x] == freet[y] && x < y);
+        });
+        double start = 0.0;
+        for (int k = 0; k < nT; k++) start = freet[blk[k]] > start ? freet[blk[k]] : start;
+        for (int k = 0; k < nT; k++) {
+            q[blk[k]].push_back({i, k});
+            freet[blk[k]] = start + w[i];
+        }
+    }
+    int *jf = b->hrjob, *jobs = b->hrjob + res32_jobs_at(nb);
+    int pos = 0;
+    for (int k = 0; k < nb; k++) {
+        jf[k] = pos;
+        for (const auto &e : q[k]) {
+            jobs[2 * pos] = e.first;
+            jobs[2 * pos + 1] = e.second;
+            pos++;
+        }
+    }
+    jf[nb] = pos;
+}
+
+// The FP32 correction iteration of every instance (delta, R of its context)
+// in one cooperative launch; iters[i] receives instance i's iteration count.
+int batch_sor_resident(hv_batch *b, const hv_params *prm, std::vector<int> &iters) {
+    const int n = b->n, nb = b->res_nblk;
+    const int max_it = prm[0].sor_max_it;
+    std::vector<BRes32> h(n);
+    for (int i = 0; i < n; i++) {
+        hv_ctx *c = b->inst[i];
+        BRes32 &I = h[i];
+        I.delta = c->sor_delta;
+        I.R = c->sor_R;
+        I.sp = c->sp;
+        I.tl = b->rtl[i];
+        I.cf = make_sor_coef(c, prm[i].sor_omega, prm[i].sor_tol);
+        I.max_it = max_it;
+        const size_t nT = (size_t)I.tl.PI * I.tl.PJ;
+        I.sy.xr = b->rx + b->rx_off[i];
+        I.sy.xb = I.sy.xr + nT * (size_t)I.tl.xstride;
+        I.sy.slots = b->rslots + b->rslot_off[i];
+        I.sy.err = b->rerr;
+        I.sy.result = b->rres + 2 * (size_t)i;
+#ifdef HV_RES_TRACE
+        I.sy.trace = nullptr;
+#endif
+#ifdef HV_BRES_TRACE
+        I.jtrace = b->jtrace;
+#endif
+    }
+    batch_res32_plan(b, b->pred32);
+    const size_t jw = res32_jobs_at(nb) + 2 * b->rjobs_n;
+    HV_BHIP(b, hipMemcpyAsync(b->rinst, h.data(), sizeof(BRes32) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    HV_BHIP(b, hipMemcpyAsync(b->rjob, b->hrjob, sizeof(int) * jw, hipMemcpyHostToDevice, b->stream));
+    k_fill<float><<<4096, kThreads, 0, b->stream>>>(b->rslots, -1.0f, (long long)b->rslot_words);
+    k_fill<unsigned long long><<<4096, kThreads, 0, b->stream>>>(b->rx, 0ULL, (long long)b->rx_words);
+    k_fill<int><<<1, 64, 0, b->stream>>>(b->rerr, 0, 4);
+    HV_BHIP(b, hipGetLastError());
+    const BRes32 *ri = (const BRes32 *)b->rinst;
+    const int *jf = b->rjob;
+    const int2 *jobs = (const int2 *)(b->rjob + res32_jobs_at(nb));
+    int lag = b->res32_lag;
+    void *args[] = {(void *)&ri, (void *)&jf, (void *)&jobs, &lag};
+    // timed by events of its own (a stage timer inside the SOR stage would
+    // release the stage's events)
+    hipEvent_t ea = nullptr, ee = nullptr;
+    if (b->prof && (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&ee) != hipSuccess)) {
+        if (ea) (void)hipEventDestroy(ea);
+        ea = ee = nullptr;
+    }
+    if (ea) (void)hipEventRecord(ea, b->stream);
+    HV_BHIP(b, hipLaunchCooperativeKernel(b->rkernel, dim3((unsigned)nb), dim3(kRes32Threads), args,
+                                          (unsigned)b->res_lds, b->stream));
+    if (ee) (void)hipEventRecord(ee, b->stream);
+    HV_BHIP(b, hipMemcpyAsync(b->hrres, b->rres, sizeof(float) * (2 * (size_t)n + 4), hipMemcpyDeviceToHost,
+                              b->stream));
+    HV_BHIP(b, hipStreamSynchronize(b->stream));
+    if (ea) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ea, ee) == hipSuccess) {
+            b->prof_ms[HV_KCLASS_SOR_RES_F32] += ms;
+            b->prof_launches[HV_KCLASS_SOR_RES_F32] += 1;
+        }
+        (void)hipEventDestroy(ea);
+        (void)hipEventDestroy(ee);
+    }
+    if (*(const int *)(b->hrres + 2 * (size_t)n))
+        return batch_err(b, HV_ERR_HIP, "batched LDS-resident FP32 SOR: a tile wait exceeded its spin limit");
+    iters.assign(n, 0);
+    for (int i = 0; i < n; i++) iters[i] = (int)b->hrres[2 * (size_t)i];
+    return HV_OK;
+}
+
 template <typename T>
 int batch_field_solve(hv_batch *b, const hv_params *prm, int *iters_out, int *warn) {
     const int n = b->n;
@@ -384,8 +651,13 @@ int batch_field_solve(hv_batch *b, const hv_params *prm, int *iters_out, int *wa
         if (b->precision == HV_PREC_MIXED) {
             kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
             HV_BHIP(b, hipGetLastError());
-            const std::vector<int> cap(n, prm[0].sor_max_it);
-            int rc = batch_sor_iterate<T>(b, prm, true, b->pred32, cap, it1, md1);
+            int rc;
+            if (b->res32 && batch_res32_setup(b, prm[0].sor_max_it) == HV_OK) {
+                rc = batch_sor_resident(b, prm, it1);
+            } else {
+                const std::vector<int> cap(n, prm[0].sor_max_it);
+                rc = batch_sor_iterate<T>(b, prm, true, b->pred32, cap, it1, md1);
+            }
             if (rc) return rc;
             kb_sor_fold<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
             HV_BHIP(b, hipGetLastError());
@@ -569,7 +841,7 @@ int hv_batch_create(int n, const hv_world *worlds, const int *n_protons, const i
         (void)hipStreamDestroy(c->stream);
         c->stream = b->stream;
         c->own_stream = false;
-        // batched SOR is the per-sweep form
+        // solo solves on an instance (the pre-loop solve) use the per-sweep form
         c->sor_res32 = 0;
         c->sor_fused = 0;
         c->sor_resident = 0;
@@ -587,6 +859,13 @@ int hv_batch_destroy(hv_batch *b) {
     if (b->args) (void)hipFree(b->args);
     if (b->first) (void)hipFree(b->first);
     if (b->maxc) (void)hipFree(b->maxc);
+    if (b->rinst) (void)hipFree(b->rinst);
+    if (b->rx) (void)hipFree(b->rx);
+    if (b->rslots) (void)hipFree(b->rslots);
+    if (b->rres) (void)hipFree(b->rres);
+    if (b->rjob) (void)hipFree(b->rjob);
+    if (b->hrjob) (void)hipHostFree(b->hrjob);
+    if (b->hrres) (void)hipHostFree(b->hrres);
     if (b->act) (void)hipFree(b->act);
     if (b->hact) (void)hipHostFree(b->hact);
     if (b->pinned) (void)hipHostFree(b->pinned);
@@ -610,6 +889,15 @@ int hv_batch_set_option(hv_batch *b, int option, int value) {
     if (option == HV_OPT_BATCH_GROUP_KIB) {
         if (value < 0) return batch_err(b, HV_ERR_ARG, "group budget must be >= 0");
         b->group_kib = value;
+        return HV_OK;
+    }
+    if (option == HV_OPT_SOR_RESIDENT_F32) {
+        b->res32 = value ? 1 : 0;
+        return HV_OK;
+    }
+    if (option == HV_OPT_SOR_F32_LAG) {
+        if (value < 1 || value > 64) return batch_err(b, HV_ERR_ARG, "lag must be in 1..64");
+        b->res32_lag = value;
         return HV_OK;
     }
     if (option != HV_OPT_SORT_EVERY && option != HV_OPT_SMEAR_TILE)
@@ -649,6 +937,20 @@ int hv_batch_profile_enable(hv_batch *b, int on) {
     b->sweeps[0] = b->sweeps[1] = 0;
     return HV_OK;
 }
+
+#ifdef HV_BRES_TRACE
+// start/end clocks (100 MHz) of the last ensemble-resident launch's jobs, and
+// the job table (job_first[nblk + 1] then (instance, tile) pairs)
+int hv_batch_debug_trace(hv_batch *b, unsigned long long *times, int *jobs, int *nblk, int *njobs) {
+    if (!b || !b->jtrace) return HV_ERR_STATE;
+    HV_BHIP(b, hipStreamSynchronize(b->stream));
+    HV_BHIP(b, hipMemcpy(times, b->jtrace, sizeof(unsigned long long) * 2 * b->rjobs_n, hipMemcpyDeviceToHost));
+    memcpy(jobs, b->hrjob, sizeof(int) * (res32_jobs_at(b->res_nblk) + 2 * b->rjobs_n));
+    *nblk = b->res_nblk;
+    *njobs = (int)b->rjobs_n;
+    return HV_OK;
+}
+#endif
 
 int hv_batch_sor_counts(hv_batch *b, long long *sweeps_f32, long long *sweeps_f64, long long *node_iters_f32,
                         long long *node_iters_f64) {

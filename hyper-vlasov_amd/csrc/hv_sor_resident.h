@@ -392,16 +392,26 @@ __device__ __forceinline__ unsigned long long ld_tagged(const unsigned long long
 // time).  With a fixed shape every tile index folds into constants and LDS
 // immediate offsets; the host launches a fixed-shape instance when the chosen
 // tiling matches one.
+//
+// res32_solve_tile is the whole solve as seen by the block that owns tile t;
+// k_sor_resident_f32 runs it with one block per tile, the ensemble kernel
+// runs it once per queued (instance, tile) job.  A block that leaves it has
+// written its tile of delta back; the LDS is free for the next job after a
+// barrier.
 template <int CTI, int CTJ, int CNZH>
-__global__ void __launch_bounds__(kRes32Threads)
-k_sor_resident_f32(float *__restrict__ delta, const float *__restrict__ R, Split sp, ResTiles tl, SorCoef cf,
-                   int max_it, int lag, ResSync32 sy) {
+__device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, const float *__restrict__ R, Split sp,
+                                                 ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
+                                                 const int t) {
     extern __shared__ unsigned char smem[];
     const int TI = CTI ? CTI : tl.TI, TJ = CTJ ? CTJ : tl.TJ, nzh = CNZH ? CNZH : tl.nzh;
     const int side_i = TJ * nzh, side_j = TI * nzh, xstride = 2 * side_i + 2 * side_j;
     constexpr int nwaves = kRes32Threads / kWave;
     // rows a wave updates at once (fewer for the large fixed tile: register budget of 1024-thread blocks)
     constexpr int RR = (CTI * CTJ * CNZH >= 4096) ? 2 : kResRows;
+    // rows of fewer (or other than a multiple of 64) k slots: lanes take
+    // consecutive (row, slot) elements of the row set, so no lane idles
+    constexpr bool FLAT = CNZH == 0 || CNZH % kWave != 0;
+    constexpr int RF = 2;  // elements a lane updates at once in the FLAT sweep
     const int rows = TI * TJ, tsz = rows * nzh;
     float *pc[2], *rc[2];
     pc[0] = (float *)smem;
@@ -414,7 +424,6 @@ k_sor_resident_f32(float *__restrict__ delta, const float *__restrict__ R, Split
     float *red = halo + xstride;  // wave partials
 
     const int nx = sp.nx, ny = sp.ny, nz = sp.nz;
-    const int t = blockIdx.x;
     const int nT = tl.PI * tl.PJ;
     const int pi = t / tl.PJ, pj = t - pi * tl.PJ;
     const int i0 = pi * TI, j0 = pj * TJ;
@@ -509,7 +518,84 @@ k_sor_resident_f32(float *__restrict__ delta, const float *__restrict__ R, Split
     // operands (indexes clamped into the tile; values of nodes that are not
     // updated are discarded), so all rows' LDS reads are in flight together,
     // and only the store and the max are predicated.
+    // q / d for q < 2^22 with a precomputed 1 / d (exact: the fraction part of
+    // (q + 0.5) / d stays 0.5 / d away from an integer)
+    const float inv_nzh = 1.0f / (float)nzh, inv_tj = 1.0f / (float)TJ, inv_tj2 = 1.0f / (float)(TJ - 2);
+    auto fdiv = [](int q, int d, float inv, int cd) -> int {
+        if (cd) return q / cd;
+        (void)d;
+        return (int)(((float)q + 0.5f) * inv);
+    };
+    auto row_at_lane = [&](bool inner, int q, int &li, int &lj) {
+        if (inner) {
+            li = 1 + fdiv(q, TJ - 2, inv_tj2, CTJ ? CTJ - 2 : 0);
+            lj = 1 + q - (li - 1) * (TJ - 2);
+        } else if (!split_rows) {
+            li = fdiv(q, TJ, inv_tj, CTJ);
+            lj = q - li * TJ;
+        } else {
+            const int w = q - 2 * TJ;
+            li = q < TJ ? 0 : (q < 2 * TJ ? TI - 1 : 1 + (w >> 1));
+            lj = q < TJ ? q : (q < 2 * TJ ? q - TJ : ((w & 1) ? TJ - 1 : 0));
+        }
+    };
+    // the same update over (row, slot) elements: element f of the row set is
+    // slot f % nzh of its row f / nzh
+    auto sweep_flat = [&](int colour, bool inner, float m) -> float {
+        float *dst = pc[colour];
+        const float *oth = pc[colour ^ 1];
+        const float *src = rc[colour];
+        const int nel = (inner ? n_inner : n_ring) * nzh;
+        for (int f0 = wave * kWave; f0 < nel; f0 += nwaves * kWave * RF) {
+            float nv[RF], po[RF];
+            int ee[RF];
+            bool ok[RF];
+#pragma unroll
+            for (int u = 0; u < RF; u++) {
+                int f = f0 + u * nwaves * kWave + lane;
+                const bool fv = f < nel;
+                f = fv ? f : 0;
+                const int q = fdiv(f, nzh, inv_nzh, CNZH), h = f - q * nzh;
+                int li, lj;
+                row_at_lane(inner, q, li, lj);
+                const int r = li * TJ + lj;
+                const int i = i0 + li, j = j0 + lj;
+                const int off = (i + j + colour) & 1;
+                const int k = 2 * h + off;
+                ok[u] = fv && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2;
+                const int e = r * nzh + h;
+                const int ez = e + off - 1 > 0 ? e + off - 1 : 0;
+                float xm, xp, ym, yp;
+                if (inner) {
+                    xm = oth[e - TJ * nzh];
+                    xp = oth[e + TJ * nzh];
+                    ym = oth[e - nzh];
+                    yp = oth[e + nzh];
+                } else {
+                    xm = li > 0 ? oth[e - TJ * nzh] : h_im[lj * nzh + h];
+                    xp = li < TI - 1 ? oth[e + TJ * nzh] : h_ip[lj * nzh + h];
+                    ym = lj > 0 ? oth[e - nzh] : h_jm[li * nzh + h];
+                    yp = lj < TJ - 1 ? oth[e + nzh] : h_jp[li * nzh + h];
+                }
+                const float zm = oth[ez], zp = oth[e + off];
+                const float sv = (xp + xm) * yx + (yp + ym) * yy + (zp + zm) * yz + src[e];
+                po[u] = dst[e];
+                nv[u] = om1 * po[u] + om * (sv * inv);
+                ee[u] = e;
+            }
+#pragma unroll
+            for (int u = 0; u < RF; u++) {
+                if (!ok[u]) continue;
+                dst[ee[u]] = nv[u];
+                const float d = fabsf(nv[u] - po[u]);
+                m = d > m ? d : m;
+            }
+        }
+        return m;
+    };
+
     auto sweep = [&](int colour, bool inner, float m) -> float {
+        if constexpr (FLAT) return sweep_flat(colour, inner, m);
         float *dst = pc[colour];
         const float *oth = pc[colour ^ 1];
         const float *src = rc[colour];
@@ -736,9 +822,16 @@ k_sor_resident_f32(float *__restrict__ delta, const float *__restrict__ R, Split
         sy.result[1] = tested;
     }
 #ifdef HV_RES_TRACE
-    if (t == 0)
+    if (t == 0 && sy.trace)
         for (int q = tid; q < 256; q += kRes32Threads) sy.trace[q] = s_trace[q];
 #endif
+}
+
+template <int CTI, int CTJ, int CNZH>
+__global__ void __launch_bounds__(kRes32Threads)
+k_sor_resident_f32(float *__restrict__ delta, const float *__restrict__ R, Split sp, ResTiles tl, SorCoef cf,
+                   int max_it, int lag, ResSync32 sy) {
+    res32_solve_tile<CTI, CTJ, CNZH>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
 }
 
 }  // namespace hv

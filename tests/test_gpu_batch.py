@@ -26,9 +26,9 @@ def _instances():
     return ws, params
 
 
-def _solo(w, prm, prec, chunks, sort_every):
+def _solo(w, prm, prec, chunks, sort_every, resident=False):
     sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, prec)
-    sim.set_option(hvamd.HV_OPT_SOR_RESIDENT_F32, 0)
+    sim.set_option(hvamd.HV_OPT_SOR_RESIDENT_F32, 1 if resident else 0)
     sim.set_option(hvamd.HV_OPT_SORT_EVERY, sort_every)
     sim.upload_particles(*w.initial_state())
     it0 = sim.run_init(w.sky(), prm)
@@ -38,19 +38,36 @@ def _solo(w, prm, prec, chunks, sort_every):
         stats.append(s)
         its.append(it)
         first += c
+    if resident:
+        assert sim.get_option(hvamd.HV_OPT_SOR_RESIDENT_F32) == 1
     out = (it0, np.concatenate(stats), np.concatenate(its), sim.download_particles(),
            sim.total_energy(w.sky(), w.ppn))
     sim.close()
     return out
 
 
-@pytest.mark.parametrize("prec", [hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED])
-def test_batch_equals_solo_runs(prec):
-    print(f"batch vs solo prec={prec}", flush=True)
+def _instances_res():
+    """Grids with a compile-time tile body in the ensemble kernel (64^3) and
+    without one (32^3 and 20^3 run the run-time-shape body)."""
     ws, params = _instances()
+    ws[0].grid, ws[0].dx = 64, 0.35
+    ws[2].grid, ws[2].dx = 32, 0.55
+    return ws, params
+
+
+@pytest.mark.parametrize("form", ["f64", "mixed_sweeps", "mixed_resident"])
+def test_batch_equals_solo_runs(form):
+    """F64; mixed with per-sweep FP32 corrections; mixed with the ensemble
+    LDS-resident kernel against solo LDS-resident solves."""
+    print(f"batch vs solo form={form}", flush=True)
+    prec = hvamd.HV_PREC_F64 if form == "f64" else hvamd.HV_PREC_MIXED
+    resident = form == "mixed_resident"
+    ws, params = _instances_res() if resident else _instances()
     chunks = [5, 3]
     b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws], prec)
     b.set_option(hvamd.HV_OPT_SORT_EVERY, 2)
+    b.set_option(hvamd.HV_OPT_SOR_RESIDENT_F32, 1 if resident else 0)
+    b.profile_enable(True)
     for i, w in enumerate(ws):
         b.instance(i).upload_particles(*w.initial_state())
     sks = [w.sky() for w in ws]
@@ -65,9 +82,11 @@ def test_batch_equals_solo_runs(prec):
     its = np.concatenate(its, axis=1)
     parts = [b.instance(i).download_particles() for i in range(len(ws))]
     energies = [b.instance(i).total_energy(ws[i].sky(), ws[i].ppn) for i in range(len(ws))]
+    res_launches = b.profile_read()["sor_resident_f32"][1]
     b.close()
+    assert res_launches == (sum(chunks) if resident else 0)
     for i, w in enumerate(ws):
-        s_it0, s_stats, s_its, s_part, s_e = _solo(w, params[i], prec, chunks, 2)
+        s_it0, s_stats, s_its, s_part, s_e = _solo(w, params[i], prec, chunks, 2, resident)
         print(i, w.name, "iters", list(its[i]), "solo", list(s_its))
         assert it0[i] == s_it0
         assert np.array_equal(its[i], s_its)
