@@ -53,17 +53,20 @@ struct hv_ctx {
     double *phi_alt = nullptr;          // second potential buffer of the fused SOR
     float *sor_delta_alt = nullptr;
     Split sp{};
-    void *H = nullptr, *D = nullptr;    // smearing factor channels
-    size_t H_cap = 0, D_cap = 0;        // bytes
+    void *D = nullptr;                  // smearing: D channels of the two-pass form, (rx + 1) 2G T
+    size_t D_cap = 0;                   // bytes
     void *F[3] = {};                    // 2G T each (per-kernel API only)
     void *smear_tab = nullptr;          // SmearTab<T> on device
     double tab_sigma = -1.0;
     int tab_ppn = -1;
     int tab_warn = 0;
-    SmearTiles stl{};
+    SmearTiles stl{};                   // two-pass smearing tiles
     size_t stl_lds = 0;
-    unsigned char *sflags = nullptr;    // per-tile "has counts" flags of the smearing
+    unsigned char *sflags = nullptr;    // per-tile "has counts" flags of the two-pass smearing
     size_t sflags_cap = 0;
+    SmearTiles stf{};                   // fused (plane-walking) smearing tiles
+    size_t stf_lds = 0;
+    int smear_fused = 0;                // option: single-grid smearing in the fused form
     unsigned long long *maxd = nullptr; // per-iteration slot maxima of |dphi| (double bits)
     double *maxc = nullptr;             // per-iteration max |dphi|, collected for read-back
     int maxd_cap = 0;
@@ -330,25 +333,78 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
     HV_HIP(c, hipMemcpyAsync(c->smear_tab, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HV_HIP(c, hipStreamSynchronize(c->stream));
     size_t n2 = (size_t)(2 * c->G);
-    // LDS tiles: TJ = 8 rows, TK the widest of smear_tk_max, /2, ... columns that fits 64 KB
-    SmearTiles stl;
+    // two-pass form: the vector tile (8 rows x 8 VW columns) when it fits
+    // kSmear2LdsVec and smear_tk_max allows it, else TJ = 8 rows and TK the
+    // widest of smear_tk_max, /2, ... columns whose counts and H levels fit 64 KB
+    constexpr int kVecTK = 8 * smear_vw<T>();
+    SmearTiles stl{};
     stl.TJ = 8;
     size_t lds = 0;
-    for (int tk = c->smear_tk_max; tk >= 8; tk /= 2) {
-        stl.TK = tk;
-        const size_t R = stl.TJ + 2 * tb.ry, W = tk + 2 * tb.rz;
-        lds = sizeof(T) * (R * W + (size_t)(tb.rz + 1) * R * tk);
-        if (lds <= 65536) break;
+    stl.TK = kVecTK;
+    if (c->smear_tk_max >= kVecTK && (lds = smear2_lds_bytes(stl, tb.ry, tb.rz, sizeof(T))) <= kSmear2LdsVec) {
+    } else {
+        for (int tk = c->smear_tk_max; tk >= 8; tk /= 2) {
+            stl.TK = tk;
+            lds = smear2_lds_bytes(stl, tb.ry, tb.rz, sizeof(T));
+            if (lds <= 65536) break;
+        }
+        if (lds > 65536) return set_err(c, HV_ERR_ARG, "smearing radius too large for the LDS tiles");
     }
-    if (lds > 65536) return set_err(c, HV_ERR_ARG, "smearing radius too large for the LDS tiles");
+    if (hipFuncSetAttribute((const void *)k_smear_zy<T, unsigned long long>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmear2LdsVec) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_smear_zy<T, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kSmear2LdsVec) != hipSuccess)
+        return set_err(c, HV_ERR_HIP, "smearing: LDS size attribute refused");
     stl.njt = (c->world.n[1] + stl.TJ - 1) / stl.TJ;
     stl.nkt = (c->world.n[2] + stl.TK - 1) / stl.TK;
+    stl.IC = 1;
+    stl.nic = c->world.n[0];
+    // fused form: the same with the x-sum ring (vector tile within kSmearLdsVec,
+    // else the widest that fits kSmearLds)
+    SmearTiles stf{};
+    stf.TJ = 8;
+    size_t ldf = 0;
+    stf.TK = kVecTK;
+    if (c->smear_tk_max >= kVecTK &&
+        (ldf = smear_lds_bytes(stf, tb.rx, tb.ry, tb.rz, sizeof(T))) <= kSmearLdsVec) {
+    } else {
+        for (int tk = c->smear_tk_max; tk >= 8; tk /= 2) {
+            stf.TK = tk;
+            ldf = smear_lds_bytes(stf, tb.rx, tb.ry, tb.rz, sizeof(T));
+            if (ldf <= kSmearLds) break;
+        }
+        if (ldf > kSmearLds) return set_err(c, HV_ERR_ARG, "smearing radius too large for the LDS tiles");
+    }
+    stf.njt = (c->world.n[1] + stf.TJ - 1) / stf.TJ;
+    stf.nkt = (c->world.n[2] + stf.TK - 1) / stf.TK;
+    // chunks along i: enough blocks for two per CU, each chunk at least 2 rx
+    // planes long (the planes a chunk smears beyond its own cost at most as
+    // much again)
+    {
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 256;
+        const long long per_chunk = 2LL * stf.njt * stf.nkt;
+        const int nx = c->world.n[0];
+        int nic = (int)((2LL * ncu + per_chunk - 1) / per_chunk);
+        const int nic_max = nx / (2 * tb.rx > 1 ? 2 * tb.rx : 1);
+        if (nic > nic_max) nic = nic_max;
+        if (nic < 1) nic = 1;
+        stf.IC = (nx + nic - 1) / nic;
+        stf.nic = (nx + stf.IC - 1) / stf.IC;
+    }
+    if (hipFuncSetAttribute((const void *)k_smear_fused<T, unsigned long long>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmearLdsVec) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_smear_fused<T, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kSmearLdsVec) != hipSuccess)
+        return set_err(c, HV_ERR_HIP, "smearing: LDS size attribute refused");
     c->stl = stl;
     c->stl_lds = lds;
+    c->stf = stf;
+    c->stf_lds = ldf;
     const size_t ntiles = (size_t)2 * c->world.n[0] * stl.njt * stl.nkt;
     int rc = ensure_buf(c, (void **)&c->sflags, &c->sflags_cap, ntiles);
-    if (rc) return rc;
-    rc = ensure_buf(c, &c->H, &c->H_cap, sizeof(T));
     if (rc) return rc;
     rc = ensure_buf(c, &c->D, &c->D_cap, (size_t)(tb.rx + 1) * n2 * sizeof(T));
     if (rc) return rc;
@@ -366,19 +422,30 @@ int smear(hv_ctx *c, bool from_acc, double sigma_r, int ppn) {
     if (rc) return rc;
     const long long n2 = 2 * c->G;
     const SmearTab<T> *tb = (const SmearTab<T> *)c->smear_tab;
-    T *H = (T *)c->H, *D = (T *)c->D, *rho = (T *)c->density;
+    T *D = (T *)c->D, *rho = (T *)c->density;
     hipEvent_t pa = prof_begin(c);
-    (void)H;
     const int nx = c->world.n[0], ny = c->world.n[1], nz = c->world.n[2];
-    const unsigned ntiles = (unsigned)(2 * nx * c->stl.njt * c->stl.nkt);
-    if (from_acc)
-        k_smear_zy<T, unsigned long long><<<ntiles, 256, c->stl_lds, c->stream>>>(c->acc, D, c->sflags, tb, nx, ny,
-                                                                                 nz, c->stl);
-    else
-        k_smear_zy<T, T><<<ntiles, 256, c->stl_lds, c->stream>>>(rho, D, c->sflags, tb, nx, ny, nz, c->stl);
-    HV_CHECK_LAUNCH(c);
-    k_smear_xs<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, c->sflags, tb, nx, ny, nz, c->stl);
-    HV_CHECK_LAUNCH(c);
+    if (c->smear_fused) {
+        const unsigned nblk = (unsigned)smear_blocks(c->stf);
+        if (from_acc) {
+            k_smear_fused<T, unsigned long long><<<nblk, kSmearThreads, c->stf_lds, c->stream>>>(c->acc, rho, tb,
+                                                                                               nx, ny, nz, c->stf);
+        } else {  // not in place: the counts go to D first
+            HV_HIP(c, hipMemcpyAsync(D, rho, sizeof(T) * (size_t)n2, hipMemcpyDeviceToDevice, c->stream));
+            k_smear_fused<T, T><<<nblk, kSmearThreads, c->stf_lds, c->stream>>>(D, rho, tb, nx, ny, nz, c->stf);
+        }
+        HV_CHECK_LAUNCH(c);
+    } else {
+        const unsigned ntiles = (unsigned)(2 * nx * c->stl.njt * c->stl.nkt);
+        if (from_acc)
+            k_smear_zy<T, unsigned long long><<<ntiles, 256, c->stl_lds, c->stream>>>(c->acc, D, c->sflags, tb, nx,
+                                                                                     ny, nz, c->stl);
+        else
+            k_smear_zy<T, T><<<ntiles, 256, c->stl_lds, c->stream>>>(rho, D, c->sflags, tb, nx, ny, nz, c->stl);
+        HV_CHECK_LAUNCH(c);
+        k_smear_xs<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, c->sflags, tb, nx, ny, nz, c->stl);
+        HV_CHECK_LAUNCH(c);
+    }
     prof_end(c, HV_KCLASS_SMEAR, pa, 1);
     return c->tab_warn;
 }
@@ -1274,7 +1341,7 @@ int hv_ctx_destroy(hv_ctx *c) {
     for (void *b : sbufs)
         if (b) (void)hipFree(b);
     void *bufs[] = {c->acc, c->density, c->U, c->phi, c->phi_alt, c->rho_s, c->sor_R, c->sor_delta,
-                    c->sor_delta_alt, c->H, c->D, c->F[0], c->F[1], c->F[2],
+                    c->sor_delta_alt, c->D, c->F[0], c->F[1], c->F[2],
                     c->smear_tab, c->maxd, c->maxc, c->ctl, c->slots, c->status, c->partials, c->sflags,
                     c->en_partials, c->en_out};
     for (void *b : bufs)
@@ -1607,6 +1674,9 @@ int hv_set_option(hv_ctx *c, int option, int value) {
         c->smear_tk_max = value;
         c->tab_sigma = -1.0;  // rebuild the tiles at the next smearing
         return HV_OK;
+    case HV_OPT_SMEAR_FUSED:
+        c->smear_fused = value ? 1 : 0;
+        return HV_OK;
     case HV_OPT_SOR_F32_LAG:
         // the test of row it - lag runs before this iteration's black sweep,
         // so row it itself is never complete then: lag >= 1
@@ -1626,6 +1696,7 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
     case HV_OPT_SOR_RESIDENT_F32: *value = c->sor_res32 && c->res32_state >= 0; return HV_OK;
     case HV_OPT_SOR_F32_LAG: *value = c->res32_lag; return HV_OK;
     case HV_OPT_SMEAR_TILE: *value = c->smear_tk_max; return HV_OK;
+    case HV_OPT_SMEAR_FUSED: *value = c->smear_fused; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }

@@ -29,7 +29,7 @@ constexpr int kAdjBlocks = 1024;
 struct AdjBuf {
     double *X = nullptr, *Kb = nullptr, *cot = nullptr;  // 3N each
     double *fbar = nullptr;                              // 3 x 2G
-    double *Ubar = nullptr, *rhobar = nullptr, *cbar = nullptr, *D = nullptr;  // 2G (D: smear channels)
+    double *Ubar = nullptr, *rhobar = nullptr, *cbar = nullptr;  // 2G
     double *psi = nullptr, *psrc = nullptr;              // 2 Gh (split)
     double *partials = nullptr, *theta = nullptr, *amax = nullptr;
     double *seg_x = nullptr, *seg_U = nullptr, *seg_rho = nullptr;  // per replayed step
@@ -122,11 +122,8 @@ int adj_force_vjp(hv_ctx *c, AdjBuf &b, const double *x, const double *U, const 
     // smearing is symmetric: cbar = smear(rhobar)
     const SmearTab<double> *tb = (const SmearTab<double> *)c->smear_tab;
     const int nx = c->world.n[0], ny = c->world.n[1], nz = c->world.n[2];
-    const unsigned ntiles = (unsigned)(2 * nx * c->stl.njt * c->stl.nkt);
-    k_smear_zy<double, double><<<ntiles, 256, c->stl_lds, c->stream>>>(b.rhobar, b.D, c->sflags, tb, nx, ny, nz,
-                                                                       c->stl);
-    HV_CHECK_LAUNCH(c);
-    k_smear_xs<double><<<blocks_for(G2), kThreads, 0, c->stream>>>(b.D, b.cbar, c->sflags, tb, nx, ny, nz, c->stl);
+    k_smear_fused<double, double><<<(unsigned)smear_blocks(c->stf), kSmearThreads, c->stf_lds, c->stream>>>(
+        b.rhobar, b.cbar, tb, nx, ny, nz, c->stf);
     HV_CHECK_LAUNCH(c);
     k_adj_particle<<<blocks_for(N), kThreads, 0, c->stream>>>(pp, c->np, N, g, U, b.cot, b.cbar, b.X, gz_p, gz_n);
     HV_CHECK_LAUNCH(c);
@@ -143,7 +140,6 @@ int adj_setup(hv_ctx *c, AdjBuf &b, int seg_len, int nck) {
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.Ubar, sizeof(double) * G2);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.rhobar, sizeof(double) * G2);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.cbar, sizeof(double) * G2);
-    rc = rc ? rc : adj_alloc(c, b, (void **)&b.D, c->D_cap);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.psi, sizeof(double) * Gh2);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.psrc, sizeof(double) * Gh2);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.partials, sizeof(double) * 4 * blocks_for((long long)G));

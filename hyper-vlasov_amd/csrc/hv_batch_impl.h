@@ -20,7 +20,7 @@
 #include "hv_batch_kernels.h"
 #include "hv_batch_resident.h"
 
-enum { kBPart, kBZy, kBN2, kBG, kBGh2, kBSor, kBKinds };
+enum { kBPart, kBSmear, kBZy, kBN2, kBG, kBGh2, kBSor, kBKinds };
 
 struct hv_batch {
     int n = 0;
@@ -31,7 +31,8 @@ struct hv_batch {
     std::vector<unsigned char> hargs;     // host image of args
     int *first = nullptr;                 // kBKinds prefix tables of n + 1 ints
     unsigned blocks[kBKinds] = {};
-    size_t smear_lds = 0;
+    size_t smear_lds = 0, smear2_lds = 0;
+    int smear_fused = 1;                  // option: fused smearing (else two-pass)
     double *maxc = nullptr;               // collected SOR maxima, n x maxc_width
     int maxc_width = 0;
     int *act = nullptr;                   // SOR launch subset: prefix table (n + 1) then ids (n)
@@ -154,10 +155,11 @@ void batch_fill_args(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
         A.slots = c->slots;
         A.part_blocks = (int)blocks_for(c->total);
         A.tab = (const SmearTab<T> *)c->smear_tab;
-        A.D = (T *)c->D;
         A.rho = (T *)c->density;
+        A.st = c->stf;
+        A.st2 = c->stl;
+        A.D = (T *)c->D;
         A.flags = c->sflags;
-        A.st = c->stl;
         A.G = c->G;
         A.sp = c->sp;
         A.phi = c->phi;
@@ -186,11 +188,14 @@ int batch_upload_args(hv_batch *b) {
 template <typename T>
 int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslots) {
     std::vector<int> hf((size_t)kBKinds * (b->n + 1), 0);
-    size_t lds = 0;
+    size_t lds = 0, lds2 = 0;
     for (int i = 0; i < b->n; i++) {
         hv_ctx *c = b->inst[i];
         int rc = ensure_smear<T>(c, prm[i].sigma_r, prm[i].part_per_nucleon);
         if (rc) return batch_fwd(b, c, rc);
+        // one chunk of planes per tile: the ensemble has blocks enough
+        c->stf.IC = c->world.n[0];
+        c->stf.nic = 1;
         rc = ensure_maxd(c, prm[i].sor_max_it);
         if (rc) return batch_fwd(b, c, rc);
         rc = ensure_slots(c, nslots);
@@ -198,6 +203,7 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
         const SorTiles tl = make_sor_tiles(c, kSorPlanesBatch);
         const long long cnt[kBKinds] = {
             (long long)blocks_for(c->total),
+            smear_blocks(c->stf),
             2LL * c->world.n[0] * c->stl.njt * c->stl.nkt,
             (long long)blocks_for(2 * c->G),
             (long long)blocks_for(c->G),
@@ -209,7 +215,8 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
             if (nx > 0x7fffffffLL) return batch_err(b, HV_ERR_ARG, "batch too large for one launch");
             hf[(size_t)k * (b->n + 1) + i + 1] = (int)nx;
         }
-        lds = c->stl_lds > lds ? c->stl_lds : lds;
+        lds = c->stf_lds > lds ? c->stf_lds : lds;
+        lds2 = c->stl_lds > lds2 ? c->stl_lds : lds2;
     }
     for (int k = 0; k < kBKinds; k++) b->blocks[k] = (unsigned)hf[(size_t)k * (b->n + 1) + b->n];
     b->sor_blocks.resize(b->n);
@@ -219,6 +226,11 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
     if (!b->hact)
         HV_BHIP(b, hipHostMalloc((void **)&b->hact, sizeof(int) * (2 * (size_t)b->n + 1), hipHostMallocDefault));
     b->smear_lds = lds;
+    b->smear2_lds = lds2;
+    HV_BHIP(b, hipFuncSetAttribute((const void *)kb_smear_fused<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kSmearLdsVec));
+    HV_BHIP(b, hipFuncSetAttribute((const void *)kb_smear_zy<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kSmear2LdsVec));
     if (!b->first) HV_BHIP(b, hipMalloc((void **)&b->first, sizeof(int) * hf.size()));
     HV_BHIP(b, hipMemcpyAsync(b->first, hf.data(), sizeof(int) * hf.size(), hipMemcpyHostToDevice, b->stream));
     if (!b->args) HV_BHIP(b, hipMalloc(&b->args, sizeof(BInst<T>) * (size_t)b->n));
@@ -637,8 +649,13 @@ int batch_field_solve(hv_batch *b, const hv_params *prm, int *iters_out, int *wa
     const BInst<T> *a = (const BInst<T> *)b->args;
     {
         BStage st(b, HV_KCLASS_SMEAR);
-        kb_smear_zy<T><<<b->blocks[kBZy], 256, b->smear_lds, b->stream>>>(a, btab(b, kBZy), n);
-        kb_smear_xs<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
+        if (b->smear_fused) {
+            kb_smear_fused<T><<<b->blocks[kBSmear], kSmearThreads, b->smear_lds, b->stream>>>(a, btab(b, kBSmear),
+                                                                                              n);
+        } else {
+            kb_smear_zy<T><<<b->blocks[kBZy], 256, b->smear2_lds, b->stream>>>(a, btab(b, kBZy), n);
+            kb_smear_xs<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
+        }
         kb_zero_acc<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
         HV_BHIP(b, hipGetLastError());
     }
@@ -893,6 +910,10 @@ int hv_batch_set_option(hv_batch *b, int option, int value) {
     }
     if (option == HV_OPT_SOR_RESIDENT_F32) {
         b->res32 = value ? 1 : 0;
+        return HV_OK;
+    }
+    if (option == HV_OPT_SMEAR_FUSED) {
+        b->smear_fused = value ? 1 : 0;
         return HV_OK;
     }
     if (option == HV_OPT_SOR_F32_LAG) {

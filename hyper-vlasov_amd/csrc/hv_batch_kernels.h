@@ -31,10 +31,11 @@ struct BInst {
     int part_blocks;
     // smearing
     const SmearTab<T> *tab;
-    T *D;
     T *rho;
-    unsigned char *flags;
-    SmearTiles st;
+    SmearTiles st;        // fused form
+    SmearTiles st2;       // two-pass form
+    T *D;                 // two-pass D channels
+    unsigned char *flags; // two-pass tile flags
     // grid, SOR, potentials
     long long G;
     Split sp;
@@ -90,14 +91,21 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 kb_smear_zy(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
-    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.flags, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st);
+    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.flags, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st2);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_smear_xs(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
-    smear_xs_body<T>(bid, A.D, A.rho, A.flags, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st);
+    smear_xs_body<T>(bid, A.D, A.rho, A.flags, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kSmearThreads)
+kb_smear_fused(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    smear_fused_body<T, unsigned long long>(bid, A.acc, A.rho, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st);
 }
 
 // acc = 0 (2G words), blocks of the 2G table

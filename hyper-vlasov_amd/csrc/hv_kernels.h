@@ -366,7 +366,7 @@ __global__ void k_fill(V *__restrict__ p, V v, long long n) {
     for (; q < n; q += stride) p[q] = v;
 }
 
-// ---- Gaussian smearing (include/physics.hpp:203-234), three factor passes ----
+// ---- Gaussian smearing (include/physics.hpp:203-234) ----------------------------
 // Device copy of the tap table, in T.
 template <typename T>
 struct SmearTab {
@@ -377,7 +377,6 @@ struct SmearTab {
     T term;
 };
 
-// z pass: H_K(q) = (H_{K-1} + gz(K) c(q-K)) + gz(K) c(q+K), H_0 = gz(0) c(q).
 // CT is the count type: unsigned long long (fixed point) or T.
 template <typename T, typename CT>
 __device__ __forceinline__ T count_value(const CT *c, long long q) {
@@ -387,68 +386,218 @@ __device__ __forceinline__ T count_value(const CT *c, long long q) {
         return (T)c[q];
 }
 
+// The y factor of channel a for VW consecutive columns of one row, with
+// 16-byte LDS reads: acc[v] = sum over b = -kb(a)..kb(a) of
+// gy(|b|) H_{kc(a,|b|)}(row + b, column + v), in ascending b.  h0 is the Hs
+// offset of (level 0, row + ry, column); Hs and h0 are VW-aligned.
+template <typename T>
+constexpr int smear_vw() { return 16 / (int)sizeof(T); }
+
+template <typename T>
+__device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, const SmearTab<T> *__restrict__ tb, int a,
+                                                  int h0, int R, int TK, T (&acc)[smear_vw<T>()]) {
+    constexpr int VW = smear_vw<T>();
+    typedef T V __attribute__((ext_vector_type(VW)));
+    const int kb = tb->kb[a];
+#pragma unroll
+    for (int v = 0; v < VW; v++) acc[v] = T(0.0);
+    for (int b = -kb; b <= kb; b++) {
+        const int ab = b < 0 ? -b : b;
+        const T g = tb->gy[ab];
+        const V h = *(const V *)(Hs + (tb->kc[a][ab] * R + b) * TK + h0);
+#pragma unroll
+        for (int v = 0; v < VW; v++) acc[v] = acc[v] + g * h[v];
+    }
+}
+
+// The vector D stage needs TJ = 8 rows of TK = 8 VW columns: one wave covers
+// the tile's 64 VW nodes of one channel.
+template <typename T>
+__host__ __device__ constexpr bool smear_vec_tile(int TJ, int TK) { return TJ == 8 && TK == 8 * smear_vw<T>(); }
+
+// counts staged in LDS take R x W values rounded up to a multiple of VW, so
+// that the H levels after them are 16-byte aligned
+__host__ __device__ inline int smear_c_words(int R, int W, int vw) { return (R * W + vw - 1) / vw * vw; }
+
+// The spherical-cut Gaussian factorises (hv_smear_table_build), and the sum
+// is taken as three nested factor sums, in this order of operations:
+//   z:  H_K(c) = (H_{K-1} + gz(K) c(k-K)) + gz(K) c(k+K),  H_0 = gz(0) c(k)
+//   y:  D_a = sum over b = -kb(a)..kb(a) of gy(|b|) H_{kc(a,|b|)}(j+b)
+//   x:  rho = (sum over a = -rx..rx, 0 <= i+a < nx of gx(|a|) D_|a|(i+a)) term
+// each sum from 0 in ascending offset order.  One block owns the TJ x TK
+// (row, column) tile of one species over a chunk of IC planes: it walks the
+// planes p that reach the chunk in ascending order, builds H and the D_a
+// channels of plane p in LDS from the counts of the tile plus its (ry, rz)
+// halo, and adds gx(|a|) D_|a|(p) to the running x sums of the chunk planes
+// p -+ a, kept in an LDS ring of 2 rx + 1 planes; the sums of plane p - rx
+// are complete after plane p and written out.  The D channels never leave
+// the CU.  A plane whose tile and halo hold no count contributes exact zeros
+// and is skipped.  Every x sum receives its terms in ascending a, so the
+// result does not depend on the tiling or the chunking.
+struct SmearTiles {
+    int TJ, TK;     // tile rows, tile columns (TJ * TK a multiple of 64, at most 512)
+    int njt, nkt;   // tiles per plane along j and k
+    int IC, nic;    // planes per chunk, chunks along i
+};
+
+constexpr int kSmearThreads = 256;
+constexpr int kSmearNpl = 8;            // nodes per lane at most (TJ * TK <= 512)
+constexpr size_t kSmearLds = 81920;     // LDS per block: two blocks per CU
+constexpr size_t kSmearLdsVec = 114688;  // fused form, vector tile: one block per CU
+constexpr size_t kSmear2LdsVec = 98304;  // two-pass form, vector tile
+
+inline size_t smear_lds_bytes(const SmearTiles &st, int rx, int ry, int rz, size_t tsize) {
+    const int R = st.TJ + 2 * ry, W = st.TK + 2 * rz;
+    return tsize * ((size_t)smear_c_words(R, W, (int)(16 / tsize)) + (size_t)(rz + 1) * R * st.TK +
+                    (size_t)(2 * rx + 1) * st.TJ * st.TK);
+}
+
+// the two-pass form's zy kernel: counts and H levels
+inline size_t smear2_lds_bytes(const SmearTiles &st, int ry, int rz, size_t tsize) {
+    const int R = st.TJ + 2 * ry, W = st.TK + 2 * rz;
+    return tsize * ((size_t)smear_c_words(R, W, (int)(16 / tsize)) + (size_t)(rz + 1) * R * st.TK);
+}
+
+inline long long smear_blocks(const SmearTiles &st) { return 2LL * st.njt * st.nkt * st.nic; }
+
 template <typename T, typename CT>
-__global__ void __launch_bounds__(kBlock)
-k_smear_z(const CT *__restrict__ cnt, T *__restrict__ H, const SmearTab<T> *__restrict__ tb, int nz,
-          long long n2) {
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n2) return;
-    const int k = (int)(q % nz);
-    const int rz = tb->rz;
-    T h = tb->gz[0] * count_value<T, CT>(cnt, q);
-    H[q] = h;
-    for (int K = 1; K <= rz; K++) {
-        T lo = (k - K >= 0) ? count_value<T, CT>(cnt, q - K) : T(0.0);
-        T hi = (k + K < nz) ? count_value<T, CT>(cnt, q + K) : T(0.0);
-        h = (h + tb->gz[K] * lo) + tb->gz[K] * hi;
-        H[(long long)K * n2 + q] = h;
-    }
-}
-
-// y pass: D_a(q) = sum_{b=-kb(a)..kb(a)} gy(|b|) H_{kc(a,|b|)}(q + b nz)
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_smear_y(const T *__restrict__ H, T *__restrict__ D, const SmearTab<T> *__restrict__ tb, int ny, int nz,
-          long long n2) {
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n2) return;
-    const int j = (int)((q / nz) % ny);
-    for (int a = 0; a <= tb->rx; a++) {
-        const int kb = tb->kb[a];
-        T acc = T(0.0);
-        for (int b = -kb; b <= kb; b++) {
-            const int ab = b < 0 ? -b : b;
-            const int jb = j + b;
-            T hv = (jb >= 0 && jb < ny) ? H[(long long)tb->kc[a][ab] * n2 + q + (long long)b * nz] : T(0.0);
-            acc = acc + tb->gy[ab] * hv;
+__device__ __forceinline__ void smear_fused_body(const int bid, const CT *__restrict__ cnt, T *__restrict__ rho,
+                                                 const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz,
+                                                 SmearTiles st) {
+    extern __shared__ unsigned char smem[];
+    const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
+    const int TJ = st.TJ, TK = st.TK;
+    const int R = TJ + 2 * ry, W = TK + 2 * rz, NS = 2 * rx + 1;
+    const int nnode = TJ * TK, npl = nnode / kWave;
+    constexpr int VW = smear_vw<T>();
+    const bool vec = smear_vec_tile<T>(TJ, TK);
+    T *C = (T *)smem;                          // R x W counts of plane p
+    T *Hs = C + smear_c_words(R, W, VW);       // (rz + 1) x R x TK
+    T *ring = Hs + (rz + 1) * R * TK;  // NS x nnode running x sums, plane o in slot o % NS
+    int t = bid;
+    const int kt = t % st.nkt;
+    t /= st.nkt;
+    const int jt = t % st.njt;
+    t /= st.njt;
+    const int ic = t % st.nic;
+    const int sp = t / st.nic;
+    const long long G = (long long)nx * ny * nz, plane = (long long)ny * nz;
+    const long long sbase = sp * G;
+    const int j0 = jt * TJ, k0 = kt * TK;
+    const int ia = ic * st.IC, ib = ia + st.IC < nx ? ia + st.IC : nx;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), nwaves = kSmearThreads / kWave;
+    for (int e = threadIdx.x; e < NS * nnode; e += kSmearThreads) ring[e] = T(0.0);
+    const T term = tb->term;
+    auto flush = [&](int o) {  // write plane o of the chunk and clear its slot
+        T *rs = ring + (o % NS) * nnode;
+        for (int e = threadIdx.x; e < nnode; e += kSmearThreads) {
+            const int r = e / TK, cc = e - r * TK;
+            const int j = j0 + r, k = k0 + cc;
+            const T v = rs[e];
+            rs[e] = T(0.0);
+            if (j < ny && k < nz) rho[sbase + o * plane + (long long)j * nz + k] = v * term;
         }
-        D[(long long)a * n2 + q] = acc;
+    };
+    // this lane's nodes: Hs offset of (row + ry, column) and ring offset
+    int hoff[kSmearNpl];
+#pragma unroll
+    for (int u = 0; u < kSmearNpl; u++) {
+        const int node = lane + u * kWave;
+        const int r = node / TK, cc = node - r * TK;
+        hoff[u] = (r + ry) * TK + cc;
     }
+    const int p_lo = ia - rx > 0 ? ia - rx : 0, p_hi = ib - 1 + rx < nx - 1 ? ib - 1 + rx : nx - 1;
+    __syncthreads();
+    for (int p = p_lo; p <= p_hi; p++) {
+        const long long base = sbase + (long long)p * plane;
+        int nonzero = 0;
+        for (int e = threadIdx.x; e < R * W; e += kSmearThreads) {
+            const int r = e / W, cc = e - r * W;
+            const int j = j0 - ry + r, k = k0 - rz + cc;
+            T v = T(0.0);
+            if (j >= 0 && j < ny && k >= 0 && k < nz) v = count_value<T, CT>(cnt, base + (long long)j * nz + k);
+            C[e] = v;
+            nonzero |= (v != T(0.0));
+        }
+        if (__syncthreads_or(nonzero)) {
+            for (int e = threadIdx.x; e < R * TK; e += kSmearThreads) {
+                const int r = e / TK, cc = e - r * TK;
+                const T *cp = C + r * W + cc + rz;
+                T h = tb->gz[0] * cp[0];
+                Hs[r * TK + cc] = h;
+                for (int K = 1; K <= rz; K++) {
+                    h = (h + tb->gz[K] * cp[-K]) + tb->gz[K] * cp[K];
+                    Hs[(K * R + r) * TK + cc] = h;
+                }
+            }
+            __syncthreads();
+            if (vec) {
+                // channel a per wave, VW consecutive columns per lane
+                const int r = lane >> 3, c0 = (lane & 7) * VW;
+                const int h0 = (r + ry) * TK + c0, n0 = r * TK + c0;
+                for (int a = wave; a <= rx; a += nwaves) {
+                    T acc[VW];
+                    smear_channel_vec<T>(Hs, tb, a, h0, R, TK, acc);
+                    const T gxa = tb->gx[a];
+                    const int o1 = p - a, o2 = p + a;
+                    const bool w1 = o1 >= ia && o1 < ib, w2 = a > 0 && o2 >= ia && o2 < ib;
+                    T *r1 = ring + (w1 ? o1 % NS : 0) * nnode + n0, *r2 = ring + (w2 ? o2 % NS : 0) * nnode + n0;
+#pragma unroll
+                    for (int v = 0; v < VW; v++) {
+                        const T cval = gxa * acc[v];
+                        if (w1) r1[v] = r1[v] + cval;
+                        if (w2) r2[v] = r2[v] + cval;
+                    }
+                }
+            } else {
+            // channel a per wave (uniform table reads), npl nodes per lane
+            for (int a = wave; a <= rx; a += nwaves) {
+                const int kb = tb->kb[a];
+                T acc[kSmearNpl];
+#pragma unroll
+                for (int u = 0; u < kSmearNpl; u++) acc[u] = T(0.0);
+                for (int b = -kb; b <= kb; b++) {
+                    const int ab = b < 0 ? -b : b;
+                    const T g = tb->gy[ab];
+                    const T *hb = Hs + (tb->kc[a][ab] * R + b) * TK;
+#pragma unroll
+                    for (int u = 0; u < kSmearNpl; u++)
+                        if (u < npl) acc[u] = acc[u] + g * hb[hoff[u]];
+                }
+                const T gxa = tb->gx[a];
+                const int o1 = p - a, o2 = p + a;
+                const bool w1 = o1 >= ia && o1 < ib, w2 = a > 0 && o2 >= ia && o2 < ib;
+                T *r1 = ring + (w1 ? o1 % NS : 0) * nnode, *r2 = ring + (w2 ? o2 % NS : 0) * nnode;
+#pragma unroll
+                for (int u = 0; u < kSmearNpl; u++) {
+                    if (u >= npl) continue;
+                    const int node = lane + u * kWave;
+                    const T cval = gxa * acc[u];
+                    if (w1) r1[node] = r1[node] + cval;
+                    if (w2) r2[node] = r2[node] + cval;
+                }
+            }
+            }
+        }
+        __syncthreads();
+        if (p - rx >= ia && p - rx < ib) flush(p - rx);
+    }
+    // planes whose last contributing plane is nx - 1
+    for (int o = (p_hi - rx + 1 > ia ? p_hi - rx + 1 : ia); o < ib; o++) flush(o);
 }
 
-// x pass: rho(q) = (sum_a gx(|a|) D_{|a|}(q + a ny nz)) * term
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_smear_x(const T *__restrict__ D, T *__restrict__ rho, const SmearTab<T> *__restrict__ tb, int nx, int ny,
-          int nz, long long n2) {
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n2) return;
-    const long long plane = (long long)ny * nz;
-    const int i = (int)((q / plane) % nx);
-    T acc = T(0.0);
-    for (int a = -tb->rx; a <= tb->rx; a++) {
-        const int aa = a < 0 ? -a : a;
-        if (tb->kb[aa] < 0) continue;
-        const int ia = i + a;
-        T dv = (ia >= 0 && ia < nx) ? D[(long long)aa * n2 + q + (long long)a * plane] : T(0.0);
-        acc = acc + tb->gx[aa] * dv;
-    }
-    rho[q] = acc * tb->term;
+template <typename T, typename CT>
+__global__ void __launch_bounds__(kSmearThreads)
+k_smear_fused(const CT *__restrict__ cnt, T *__restrict__ rho, const SmearTab<T> *__restrict__ tb, int nx, int ny,
+              int nz, SmearTiles st) {
+    smear_fused_body<T, CT>(blockIdx.x, cnt, rho, tb, nx, ny, nz, st);
 }
 
-// ---- tiled smearing: z and y factors fused in LDS, sparse x pass --------------
-// Same factor passes and the same operation order as k_smear_z/y/x above
-// (bit-identical results), organised for the hardware:
+// ---- two-pass smearing: z and y factors in LDS, then a sparse x pass ----------
+// The same factor sums in the same order (bit-identical to k_smear_fused),
+// with the D channels stored between two launches; no plane walk, so every
+// (plane, tile) is a block of its own (the faster form for a single grid):
 //   k_smear_zy: one block per (species, plane i, row tile of TJ rows, column
 //     tile of TK columns).  It stages the counts of the tile plus its
 //     (ry rows, rz columns) halo in LDS, builds every H_K of the (TJ + 2 ry) x
@@ -457,10 +606,6 @@ k_smear_x(const T *__restrict__ D, T *__restrict__ rho, const SmearTab<T> *__res
 //     its H and D are exactly zero.
 //   k_smear_xs: the x factor, skipping planes whose tile is flagged empty
 //     (their terms are exact zeros, so skipping them changes no bit).
-struct SmearTiles {
-    int TJ, TK;     // tile rows, tile columns
-    int njt, nkt;   // tiles per plane along j and k
-};
 
 template <typename T, typename CT>
 __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restrict__ cnt, T *__restrict__ D,
@@ -470,8 +615,9 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
     const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
     const int TJ = st.TJ, TK = st.TK;
     const int R = TJ + 2 * ry, W = TK + 2 * rz;
-    T *C = (T *)smem;          // R x W counts
-    T *Hs = C + R * W;         // (rz + 1) x R x TK
+    constexpr int VW = smear_vw<T>();
+    T *C = (T *)smem;                       // R x W counts
+    T *Hs = C + smear_c_words(R, W, VW);    // (rz + 1) x R x TK
     int t = bid;
     const int kt = t % st.nkt;
     t /= st.nkt;
@@ -507,6 +653,25 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
         }
     }
     __syncthreads();
+    if (smear_vec_tile<T>(TJ, TK)) {
+        // channel a per wave, VW consecutive columns per lane
+        const int lane = threadIdx.x & (kWave - 1);
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), nwaves = blockDim.x / kWave;
+        const int r = lane >> 3, c0 = (lane & 7) * VW;
+        const int j = j0 + r;
+        const long long q = base + (long long)j * nz + k0 + c0;
+        for (int a = wave; a <= rx; a += nwaves) {
+            T acc[VW];
+            smear_channel_vec<T>(Hs, tb, a, (r + ry) * TK + c0, R, TK, acc);
+            if (j < ny) {
+#pragma unroll
+                for (int v = 0; v < VW; v++)
+                    if (k0 + c0 + v < nz) D[(long long)a * n2 + q + v] = acc[v];
+            }
+        }
+        if (threadIdx.x == 0) flags[tile] = 1;
+        return;
+    }
     for (int e = threadIdx.x; e < TJ * TK; e += blockDim.x) {
         const int r = e / TK, cc = e - r * TK;
         const int j = j0 + r, k = k0 + cc;

@@ -47,6 +47,7 @@ HV_OPT_SOR_RESIDENT_F32 = 3
 HV_OPT_SOR_F32_LAG = 4
 HV_OPT_SMEAR_TILE = 5
 HV_OPT_BATCH_GROUP_KIB = 6
+HV_OPT_SMEAR_FUSED = 7
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1

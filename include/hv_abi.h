@@ -224,6 +224,11 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * instances in groups whose sweep working set fits this many KiB (default 0:
  * all instances together).  Same bits for every value. */
 #define HV_OPT_BATCH_GROUP_KIB 6
+/* HV_OPT_SMEAR_FUSED: smear a single grid with the fused plane-walking kernel
+ * (the D channels stay in LDS) instead of the two-pass form (default 0: the
+ * two-pass form, faster on one grid; ensembles always use the fused kernel).
+ * Same bits either way. */
+#define HV_OPT_SMEAR_FUSED 7
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 

@@ -97,18 +97,19 @@ def test_batch_equals_solo_runs(form):
 
 
 def test_batch_options_do_not_change_bits():
-    """Sort cadence, smearing tile width (the autotuned knobs) and the SOR
-    instance grouping (1 KiB: every instance a group of its own) leave every
-    bit of a batched run unchanged."""
+    """Sort cadence, smearing tile width (the autotuned knobs), the smearing
+    form and the SOR instance grouping (1 KiB: every instance a group of its
+    own) leave every bit of a batched run unchanged."""
     print("batch options", flush=True)
     ws, params = _instances()
     outs = []
-    for sort_every, tile, group_kib in ((0, 64, 0), (3, 16, 1)):
+    for sort_every, tile, group_kib, fused in ((0, 64, 0, 1), (3, 16, 1, 0)):
         b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws],
                   hvamd.HV_PREC_MIXED)
         b.set_option(hvamd.HV_OPT_SORT_EVERY, sort_every)
         b.set_option(hvamd.HV_OPT_SMEAR_TILE, tile)
         b.set_option(hvamd.HV_OPT_BATCH_GROUP_KIB, group_kib)
+        b.set_option(hvamd.HV_OPT_SMEAR_FUSED, fused)
         for i, w in enumerate(ws):
             b.instance(i).upload_particles(*w.initial_state())
         sks = [w.sky() for w in ws]

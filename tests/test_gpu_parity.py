@@ -49,12 +49,15 @@ def test_deposit_bitwise(oracle, n, d):
     assert skipped > 0 and (warn & 2)
 
 
-@pytest.mark.parametrize("n,d", GRIDS)
-def test_smear_bitwise(oracle, n, d):
-    print(f"smear grid={n}", flush=True)
+@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("n,d", GRIDS + [((64, 60, 64), (16.0, 15.0, 16.0))])  # last: chunks along i
+def test_smear_bitwise(oracle, n, d, fused):
+    """Both smearing forms (two-pass; fused plane walk) against the oracle."""
+    print(f"smear grid={n} fused={fused}", flush=True)
     st = particles(12, 500, 600, 0.5 * min(d))
     counts, _, _ = oracle.deposit(st["x"], st["y"], st["z"], 500, 600, n, d)
     sim = _sim(n, d, 500, 600)
+    sim.set_option(hvamd.HV_OPT_SMEAR_FUSED, fused)
     sim.upload_field(hvamd.HV_FIELD_DENSITY, counts)
     sim.compute_volumetric_densities(SIGMA_R, 100)
     got = sim.download_field(hvamd.HV_FIELD_DENSITY)
