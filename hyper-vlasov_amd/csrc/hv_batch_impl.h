@@ -64,6 +64,12 @@ struct hv_batch {
     int *rjob = nullptr;                  // job_first (nblk + 1) then jobs (int2 per tile)
     int *hrjob = nullptr;                 // pinned staging of rjob
     float *hrres = nullptr;               // pinned read-back of rres and rerr
+    void *hrinst = nullptr;               // pinned staging of rinst
+    // step graphs
+    int use_graph = 1;                    // option: replay a captured graph of each step's device part
+    int *dstep = nullptr;                 // device step counter (diagnostics slot of the step - 1)
+    hipGraphExec_t gexec = nullptr;
+    long long graph_steps = 0, captures = 0;
 #ifdef HV_BRES_TRACE
     unsigned long long *jtrace = nullptr;
 #endif
@@ -514,6 +520,8 @@ int batch_res32_setup(hv_batch *b, int max_it) {
     if (!b->hrres && hipHostMalloc((void **)&b->hrres, sizeof(float) * (2 * (size_t)n + 4), hipHostMallocDefault) !=
                          hipSuccess)
         return 1;
+    if (!b->hrinst && hipHostMalloc(&b->hrinst, sizeof(BRes32) * (size_t)n, hipHostMallocDefault) != hipSuccess)
+        return 1;
     b->rerr = (int *)(b->rres + 2 * (size_t)n);
 #ifdef HV_BRES_TRACE
     if (b->jtrace) (void)hipFree(b->jtrace);
@@ -573,11 +581,14 @@ void batch_res32_plan(hv_batch *b, const std::vector<int> &pred) {
 }
 
 // The FP32 correction iteration of every instance (delta, R of its context)
-// in one cooperative launch; iters[i] receives instance i's iteration count.
-int batch_sor_resident(hv_batch *b, const hv_params *prm, std::vector<int> &iters) {
-    const int n = b->n, nb = b->res_nblk;
+// in one cooperative launch, in three parts: the host image of the instance
+// arguments and the job plan (pinned staging), the device work (copies from
+// the staging, clears, the launch, the results to pinned memory; it takes no
+// host decision and may be captured into a graph), and the read-back.
+int batch_res32_prepare(hv_batch *b, const hv_params *prm) {
+    const int n = b->n;
     const int max_it = prm[0].sor_max_it;
-    std::vector<BRes32> h(n);
+    BRes32 *h = (BRes32 *)b->hrinst;
     for (int i = 0; i < n; i++) {
         hv_ctx *c = b->inst[i];
         BRes32 &I = h[i];
@@ -601,8 +612,13 @@ int batch_sor_resident(hv_batch *b, const hv_params *prm, std::vector<int> &iter
 #endif
     }
     batch_res32_plan(b, b->pred32);
+    return HV_OK;
+}
+
+int batch_res32_enqueue(hv_batch *b) {
+    const int n = b->n, nb = b->res_nblk;
     const size_t jw = res32_jobs_at(nb) + 2 * b->rjobs_n;
-    HV_BHIP(b, hipMemcpyAsync(b->rinst, h.data(), sizeof(BRes32) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    HV_BHIP(b, hipMemcpyAsync(b->rinst, b->hrinst, sizeof(BRes32) * (size_t)n, hipMemcpyHostToDevice, b->stream));
     HV_BHIP(b, hipMemcpyAsync(b->rjob, b->hrjob, sizeof(int) * jw, hipMemcpyHostToDevice, b->stream));
     k_fill<float><<<4096, kThreads, 0, b->stream>>>(b->rslots, -1.0f, (long long)b->rslot_words);
     k_fill<unsigned long long><<<4096, kThreads, 0, b->stream>>>(b->rx, 0ULL, (long long)b->rx_words);
@@ -614,7 +630,7 @@ int batch_sor_resident(hv_batch *b, const hv_params *prm, std::vector<int> &iter
     int lag = b->res32_lag;
     void *args[] = {(void *)&ri, (void *)&jf, (void *)&jobs, &lag};
     // timed by events of its own (a stage timer inside the SOR stage would
-    // release the stage's events)
+    // release the stage's events); never while a graph is captured
     hipEvent_t ea = nullptr, ee = nullptr;
     if (b->prof && (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&ee) != hipSuccess)) {
         if (ea) (void)hipEventDestroy(ea);
@@ -624,18 +640,25 @@ int batch_sor_resident(hv_batch *b, const hv_params *prm, std::vector<int> &iter
     HV_BHIP(b, hipLaunchCooperativeKernel(b->rkernel, dim3((unsigned)nb), dim3(kRes32Threads), args,
                                           (unsigned)b->res_lds, b->stream));
     if (ee) (void)hipEventRecord(ee, b->stream);
+    if (ea) b->samples.push_back({HV_KCLASS_SOR_RES_F32, ea, ee});
     HV_BHIP(b, hipMemcpyAsync(b->hrres, b->rres, sizeof(float) * (2 * (size_t)n + 4), hipMemcpyDeviceToHost,
                               b->stream));
+    return HV_OK;
+}
+
+int batch_res32_collect(hv_batch *b, std::vector<int> &iters) {
+    const int n = b->n;
     HV_BHIP(b, hipStreamSynchronize(b->stream));
-    if (ea) {
+    for (auto &smp : b->samples) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, ea, ee) == hipSuccess) {
-            b->prof_ms[HV_KCLASS_SOR_RES_F32] += ms;
-            b->prof_launches[HV_KCLASS_SOR_RES_F32] += 1;
+        if (hipEventElapsedTime(&ms, smp.a, smp.e) == hipSuccess) {
+            b->prof_ms[smp.cls] += ms;
+            b->prof_launches[smp.cls] += 1;
         }
-        (void)hipEventDestroy(ea);
-        (void)hipEventDestroy(ee);
+        (void)hipEventDestroy(smp.a);
+        (void)hipEventDestroy(smp.e);
     }
+    b->samples.clear();
     if (*(const int *)(b->hrres + 2 * (size_t)n))
         return batch_err(b, HV_ERR_HIP, "batched LDS-resident FP32 SOR: a tile wait exceeded its spin limit");
     iters.assign(n, 0);
@@ -643,10 +666,22 @@ int batch_sor_resident(hv_batch *b, const hv_params *prm, std::vector<int> &iter
     return HV_OK;
 }
 
+// The device part of one step (no host decision, no host wait: the part a
+// graph captures): particle pass with the start-of-step kick (end_kick) and
+// the diagnostics of its positions, smearing, the colour-split source and,
+// with the resident FP32 correction (res), the defect, that solve and the
+// fold of its correction.
 template <typename T>
-int batch_field_solve(hv_batch *b, const hv_params *prm, int *iters_out, int *warn) {
+int batch_step_device(hv_batch *b, int end_kick, bool res) {
     const int n = b->n;
     const BInst<T> *a = (const BInst<T> *)b->args;
+    {
+        BStage st(b, HV_KCLASS_PARTICLE);
+        kb_particle_step<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n, end_kick);
+        kb_stats_finalize<T><<<n, 256, 0, b->stream>>>(a, b->dstep);
+        kb_step_advance<<<1, 64, 0, b->stream>>>(b->dstep);
+        HV_BHIP(b, hipGetLastError());
+    }
     {
         BStage st(b, HV_KCLASS_SMEAR);
         if (b->smear_fused) {
@@ -659,25 +694,46 @@ int batch_field_solve(hv_batch *b, const hv_params *prm, int *iters_out, int *wa
         kb_zero_acc<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
         HV_BHIP(b, hipGetLastError());
     }
+    kb_rho_split<T><<<b->blocks[kBG], kThreads, 0, b->stream>>>(a, btab(b, kBG), n);
+    HV_BHIP(b, hipGetLastError());
+    if (res) {
+        kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
+        HV_BHIP(b, hipGetLastError());
+        int rc = batch_res32_enqueue(b);
+        if (rc) return rc;
+        kb_sor_fold<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
+        HV_BHIP(b, hipGetLastError());
+    }
+    return HV_OK;
+}
+
+// The rest of the step, after batch_step_device: the SOR iterations the host
+// steers (the FP32 correction by sweeps when it is not resident, the FP64
+// finishing sweeps of the mixed form, the FP64 solve), then the Skyrme +
+// Coulomb potentials.
+template <typename T>
+int batch_step_finish(hv_batch *b, const hv_params *prm, bool res, int *iters_out, int *warn) {
+    const int n = b->n;
+    const BInst<T> *a = (const BInst<T> *)b->args;
     std::vector<int> it1, it2;
     std::vector<double> md1, md2;
     {
         BStage st(b, HV_KCLASS_SOR_STAGE);
-        kb_rho_split<T><<<b->blocks[kBG], kThreads, 0, b->stream>>>(a, btab(b, kBG), n);
-        HV_BHIP(b, hipGetLastError());
         if (b->precision == HV_PREC_MIXED) {
-            kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
-            HV_BHIP(b, hipGetLastError());
             int rc;
-            if (b->res32 && batch_res32_setup(b, prm[0].sor_max_it) == HV_OK) {
-                rc = batch_sor_resident(b, prm, it1);
+            if (res) {
+                rc = batch_res32_collect(b, it1);
             } else {
+                kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
+                HV_BHIP(b, hipGetLastError());
                 const std::vector<int> cap(n, prm[0].sor_max_it);
                 rc = batch_sor_iterate<T>(b, prm, true, b->pred32, cap, it1, md1);
+                if (!rc) {
+                    kb_sor_fold<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
+                    if (hipGetLastError() != hipSuccess) rc = batch_err(b, HV_ERR_HIP, "fold launch failed");
+                }
             }
             if (rc) return rc;
-            kb_sor_fold<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
-            HV_BHIP(b, hipGetLastError());
             std::vector<int> two(n, 2), budget(n);
             for (int i = 0; i < n; i++) budget[i] = mixed_fp64_budget(prm[0].sor_max_it, it1[i]);
             rc = batch_sor_iterate<T>(b, prm, false, two, budget, it2, md2);
@@ -709,6 +765,40 @@ int batch_field_solve(hv_batch *b, const hv_params *prm, int *iters_out, int *wa
         HV_BHIP(b, hipGetLastError());
     }
     return HV_OK;
+}
+
+// Captures batch_step_device into b->gexec; nonzero (and no graph) when the
+// runtime refuses any part of it, the stream then back in normal mode.
+template <typename T>
+int batch_capture_step(hv_batch *b, bool res) {
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    const int rc = batch_step_device<T>(b, 1, res);
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    (void)hipGetLastError();
+    if (rc || ec != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        b->err[0] = 0;
+        return 1;
+    }
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess || !ge) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    b->gexec = ge;
+    b->captures++;
+    return HV_OK;
+}
+
+void batch_drop_graph(hv_batch *b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
 }
 
 int batch_check(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
@@ -760,6 +850,13 @@ int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int 
     rc = batch_build<T>(b, sk, prm, nsteps + 1);
     if (rc) return rc;
     const BInst<T> *a = (const BInst<T> *)b->args;
+    const bool res = b->precision == HV_PREC_MIXED && b->res32 && batch_res32_setup(b, prm[0].sor_max_it) == HV_OK;
+    k_fill<int><<<1, 64, 0, b->stream>>>(b->dstep, 0, 1);  // diagnostics slot of the first step: 1
+    HV_BHIP(b, hipGetLastError());
+    // steps after the first of this call (start-of-step kick) replay one
+    // captured graph of the device part, unless stages are being timed
+    bool graphs = b->use_graph && !b->prof;
+    batch_drop_graph(b);
     std::vector<int> its(n);
     std::vector<std::vector<int>> iters_all;
     int warn = 0;
@@ -785,17 +882,24 @@ int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int 
             rc = batch_upload_args<T>(b);
             if (rc) return rc;
         }
-        {
-            BStage st(b, HV_KCLASS_PARTICLE);
-            kb_particle_step<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n, s > 0 ? 1 : 0);
-            kb_stats_finalize<T><<<n, 256, 0, b->stream>>>(a, s + 1);
-            HV_BHIP(b, hipGetLastError());
+        if (res) {
+            rc = batch_res32_prepare(b, prm);
+            if (rc) return rc;
         }
-        rc = batch_field_solve<T>(b, prm, its.data(), &warn);
+        if (s > 0 && graphs && !b->gexec && batch_capture_step<T>(b, res) != HV_OK) graphs = false;
+        if (s > 0 && graphs) {
+            HV_BHIP(b, hipGraphLaunch(b->gexec, b->stream));
+            b->graph_steps++;
+        } else {
+            rc = batch_step_device<T>(b, s > 0 ? 1 : 0, res);
+            if (rc) return rc;
+        }
+        rc = batch_step_finish<T>(b, prm, res, its.data(), &warn);
         if (rc) return rc;
         if (sor_iters)
             for (int i = 0; i < n; i++) sor_iters[(size_t)i * nsteps + s] = its[i];
     }
+    batch_drop_graph(b);
     // closing half kick of the last step, with the force stored (src/hyper_vlasov.cpp:101-103)
     kb_gather_kick<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n);
     HV_BHIP(b, hipGetLastError());
@@ -843,7 +947,8 @@ int hv_batch_create(int n, const hv_world *worlds, const int *n_protons, const i
     b->pred32.assign(n, 16);
     b->pred64.assign(n, 16);
     b->boost.assign(n, 0);
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&b->dstep, sizeof(int)) != hipSuccess) {
         hv_batch_destroy(b);
         return HV_ERR_HIP;
     }
@@ -883,6 +988,9 @@ int hv_batch_destroy(hv_batch *b) {
     if (b->rjob) (void)hipFree(b->rjob);
     if (b->hrjob) (void)hipHostFree(b->hrjob);
     if (b->hrres) (void)hipHostFree(b->hrres);
+    if (b->hrinst) (void)hipHostFree(b->hrinst);
+    if (b->dstep) (void)hipFree(b->dstep);
+    batch_drop_graph(b);
     if (b->act) (void)hipFree(b->act);
     if (b->hact) (void)hipHostFree(b->hact);
     if (b->pinned) (void)hipHostFree(b->pinned);
@@ -914,6 +1022,10 @@ int hv_batch_set_option(hv_batch *b, int option, int value) {
     }
     if (option == HV_OPT_SMEAR_FUSED) {
         b->smear_fused = value ? 1 : 0;
+        return HV_OK;
+    }
+    if (option == HV_OPT_BATCH_GRAPH) {
+        b->use_graph = value ? 1 : 0;
         return HV_OK;
     }
     if (option == HV_OPT_SOR_F32_LAG) {
@@ -972,6 +1084,13 @@ int hv_batch_debug_trace(hv_batch *b, unsigned long long *times, int *jobs, int 
     return HV_OK;
 }
 #endif
+
+int hv_batch_graph_info(hv_batch *b, long long *graph_steps, long long *captures) {
+    if (!b) return HV_ERR_ARG;
+    if (graph_steps) *graph_steps = b->graph_steps;
+    if (captures) *captures = b->captures;
+    return HV_OK;
+}
 
 int hv_batch_sor_counts(hv_batch *b, long long *sweeps_f32, long long *sweeps_f64, long long *node_iters_f32,
                         long long *node_iters_f64) {

@@ -80,11 +80,16 @@ kb_particle_step(const BInst<T> *__restrict__ a, const int *__restrict__ first, 
                              A.boost_p, A.boost_n, A.partials, A.status);
 }
 
-// one block per instance
+// one block per instance; the diagnostics slot is the step counter + 1 (the
+// counter lives on the device, so a captured step graph needs no update)
 template <typename T>
-__global__ void __launch_bounds__(256) kb_stats_finalize(const BInst<T> *__restrict__ a, int slot) {
+__global__ void __launch_bounds__(256) kb_stats_finalize(const BInst<T> *__restrict__ a, const int *__restrict__ step) {
     const BInst<T> &A = a[blockIdx.x];
-    stats_finalize_block(A.partials, A.part_blocks, A.slots + slot);
+    stats_finalize_block(A.partials, A.part_blocks, A.slots + *step + 1);
+}
+
+__global__ void kb_step_advance(int *step) {
+    if (threadIdx.x == 0) *step += 1;
 }
 
 template <typename T>

@@ -48,6 +48,7 @@ HV_OPT_SOR_F32_LAG = 4
 HV_OPT_SMEAR_TILE = 5
 HV_OPT_BATCH_GROUP_KIB = 6
 HV_OPT_SMEAR_FUSED = 7
+HV_OPT_BATCH_GRAPH = 8
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1
@@ -150,6 +151,7 @@ SIGNATURES = {
     "hv_batch_profile_enable": (_I, [_P, _I]),
     "hv_batch_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
     "hv_batch_sor_counts": (_I, [_P] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
+    "hv_batch_graph_info": (_I, [_P] + [ctypes.POINTER(ctypes.c_longlong)] * 2),
     "hv_dipole_loss_grad": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _I, _DP, _I, _DP, _DP,
                                  _DP]),
     "hv_fermi_momentum": (_D, [_D]),

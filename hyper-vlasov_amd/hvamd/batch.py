@@ -104,6 +104,12 @@ class Batch:
         return {"sweeps_f32": v[0].value, "sweeps_f64": v[1].value, "node_iters_f32": v[2].value,
                 "node_iters_f64": v[3].value}
 
+    def graph_info(self) -> dict:
+        """Steps replayed from a captured step graph, and graphs captured."""
+        v = [ctypes.c_longlong() for _ in range(2)]
+        self._rc("hv_batch_graph_info", self.lib.hv_batch_graph_info(self.b, *[ctypes.byref(x) for x in v]))
+        return {"graph_steps": v[0].value, "captures": v[1].value}
+
     def profile_read(self) -> dict:
         out = {}
         for k, name in enumerate(_lib.KCLASS_NAMES):

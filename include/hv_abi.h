@@ -229,6 +229,13 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * two-pass form, faster on one grid; ensembles always use the fused kernel).
  * Same bits either way. */
 #define HV_OPT_SMEAR_FUSED 7
+/* HV_OPT_BATCH_GRAPH (ensembles only): from the second step of a
+ * hv_batch_run_steps call on, replay one captured hipGraph of the step's
+ * device part (particle pass, diagnostics, smearing, source split and, in the
+ * mixed form, the defect, the persistent LDS-resident FP32 SOR launch and the
+ * fold) instead of launching its kernels one by one (default 1; off while
+ * stages are timed).  Same bits either way. */
+#define HV_OPT_BATCH_GRAPH 8
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
@@ -278,7 +285,8 @@ int hv_batch_size(const hv_batch *batch);
 /* instance i's context: particle/field transfer, diagnostics, total energy */
 hv_ctx *hv_batch_instance(hv_batch *batch, int i);
 /* HV_OPT_SORT_EVERY or HV_OPT_SMEAR_TILE (applied to every instance), or
- * HV_OPT_BATCH_GROUP_KIB */
+ * HV_OPT_BATCH_GROUP_KIB, HV_OPT_SOR_RESIDENT_F32, HV_OPT_SOR_F32_LAG,
+ * HV_OPT_SMEAR_FUSED (default 1 for a batch), HV_OPT_BATCH_GRAPH */
 int hv_batch_set_option(hv_batch *batch, int option, int value);
 /* skyrme[n], params[n]; sor_iters[n] (may be NULL): the pre-loop solves */
 int hv_batch_run_init(hv_batch *batch, const hv_skyrme *skyrme, const hv_params *params, int *sor_iters);
@@ -299,6 +307,9 @@ int hv_batch_profile_read(hv_batch *batch, int kclass, double *ms, long long *la
  * (nx-2)(ny-2)(nz-2)_i) */
 int hv_batch_sor_counts(hv_batch *batch, long long *sweeps_f32, long long *sweeps_f64, long long *node_iters_f32,
                         long long *node_iters_f64);
+/* steps run by replaying a captured step graph, and graphs captured, since
+ * the batch was created (HV_OPT_BATCH_GRAPH) */
+int hv_batch_graph_info(hv_batch *batch, long long *graph_steps, long long *captures);
 
 /* ---- adjoint (no counterpart in the reference) ---------------------------- */
 /* For fitting the Skyrme coefficients to a dipole response (the use of the

@@ -121,3 +121,36 @@ def test_batch_options_do_not_change_bits():
     assert np.array_equal(outs[0][1], outs[1][1])
     for a, c in zip(outs[0][2], outs[1][2]):
         assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("prec", [hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED])
+def test_batch_step_graph_replays_equal_eager_steps(prec):
+    """Steps replayed from the captured step graph (with sorting, a boost and
+    the host-steered SOR between replays) give the bits of eagerly launched
+    steps, step by step."""
+    print(f"batch graph vs eager prec={prec}", flush=True)
+    ws, params = _instances_res()
+    outs = []
+    for graph in (0, 1):
+        b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws], prec)
+        b.set_option(hvamd.HV_OPT_SORT_EVERY, 2)
+        b.set_option(hvamd.HV_OPT_BATCH_GRAPH, graph)
+        for i, w in enumerate(ws):
+            b.instance(i).upload_particles(*w.initial_state())
+        sks = [w.sky() for w in ws]
+        b.run_init(sks, params)
+        s1, i1 = b.run_steps(sks, params, 0, 6)
+        s2, i2 = b.run_steps(sks, params, 6, 3)
+        gi = b.graph_info()
+        parts = [b.instance(i).download_particles() for i in range(len(ws))]
+        b.close()
+        print("graph", graph, gi, flush=True)
+        assert gi["graph_steps"] == (5 + 2 if graph else 0)
+        assert gi["captures"] == (2 if graph else 0)
+        outs.append((s1, i1, s2, i2, parts))
+    e, g = outs
+    for k in range(4):
+        assert np.array_equal(e[k], g[k]), k
+    for pe, pg in zip(e[4], g[4]):
+        for key in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz"):
+            assert np.array_equal(pe[key], pg[key]), key
