@@ -697,6 +697,7 @@ int batch_step_device(hv_batch *b, int end_kick, bool res) {
     kb_rho_split<T><<<b->blocks[kBG], kThreads, 0, b->stream>>>(a, btab(b, kBG), n);
     HV_BHIP(b, hipGetLastError());
     if (res) {
+        BStage st(b, HV_KCLASS_SOR_STAGE);
         kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
         HV_BHIP(b, hipGetLastError());
         int rc = batch_res32_enqueue(b);

@@ -386,6 +386,43 @@ __device__ __forceinline__ T count_value(const CT *c, long long q) {
         return (T)c[q];
 }
 
+// The z factor: H_0..H_rz of the R x TK columns of a tile from its staged
+// counts C (R x W, the tile's columns plus rz on each side).  The chain of
+// each column runs in its order; the counts and weights of four levels are
+// loaded ahead of their four steps.
+template <typename T>
+__device__ __forceinline__ void smear_h_levels(const T *__restrict__ C, T *__restrict__ Hs,
+                                               const SmearTab<T> *__restrict__ tb, int R, int W, int TK, int rz,
+                                               int nthr) {
+    const int lvl = R * TK;
+    for (int e = threadIdx.x; e < lvl; e += nthr) {
+        const int r = e / TK, cc = e - r * TK;
+        const T *cp = C + r * W + cc + rz;
+        T *hp = Hs + e;
+        T h = tb->gz[0] * cp[0];
+        hp[0] = h;
+        int K = 1;
+        for (; K + 3 <= rz; K += 4) {
+            T lo[4], hi[4], g[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                lo[u] = cp[-(K + u)];
+                hi[u] = cp[K + u];
+                g[u] = tb->gz[K + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                h = (h + g[u] * lo[u]) + g[u] * hi[u];
+                hp[(K + u) * lvl] = h;
+            }
+        }
+        for (; K <= rz; K++) {
+            h = (h + tb->gz[K] * cp[-K]) + tb->gz[K] * cp[K];
+            hp[K * lvl] = h;
+        }
+    }
+}
+
 // The y factor of channel a for VW consecutive columns of one row, with
 // 16-byte LDS reads: acc[v] = sum over b = -kb(a)..kb(a) of
 // gy(|b|) H_{kc(a,|b|)}(row + b, column + v), in ascending b.  h0 is the Hs
@@ -401,7 +438,18 @@ __device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, cons
     const int kb = tb->kb[a];
 #pragma unroll
     for (int v = 0; v < VW; v++) acc[v] = T(0.0);
-    for (int b = -kb; b <= kb; b++) {
+    int b = -kb;
+    for (; b + 1 <= kb; b += 2) {  // two taps' loads ahead of their sums
+        const int ab0 = b < 0 ? -b : b, ab1 = b + 1 < 0 ? -(b + 1) : b + 1;
+        const T g0 = tb->gy[ab0], g1 = tb->gy[ab1];
+        const V h0v = *(const V *)(Hs + (tb->kc[a][ab0] * R + b) * TK + h0);
+        const V h1v = *(const V *)(Hs + (tb->kc[a][ab1] * R + b + 1) * TK + h0);
+#pragma unroll
+        for (int v = 0; v < VW; v++) acc[v] = acc[v] + g0 * h0v[v];
+#pragma unroll
+        for (int v = 0; v < VW; v++) acc[v] = acc[v] + g1 * h1v[v];
+    }
+    if (b <= kb) {
         const int ab = b < 0 ? -b : b;
         const T g = tb->gy[ab];
         const V h = *(const V *)(Hs + (tb->kc[a][ab] * R + b) * TK + h0);
@@ -521,16 +569,7 @@ __device__ __forceinline__ void smear_fused_body(const int bid, const CT *__rest
             nonzero |= (v != T(0.0));
         }
         if (__syncthreads_or(nonzero)) {
-            for (int e = threadIdx.x; e < R * TK; e += kSmearThreads) {
-                const int r = e / TK, cc = e - r * TK;
-                const T *cp = C + r * W + cc + rz;
-                T h = tb->gz[0] * cp[0];
-                Hs[r * TK + cc] = h;
-                for (int K = 1; K <= rz; K++) {
-                    h = (h + tb->gz[K] * cp[-K]) + tb->gz[K] * cp[K];
-                    Hs[(K * R + r) * TK + cc] = h;
-                }
-            }
+            smear_h_levels<T>(C, Hs, tb, R, W, TK, rz, kSmearThreads);
             __syncthreads();
             if (vec) {
                 // channel a per wave, VW consecutive columns per lane
@@ -642,16 +681,7 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
         if (threadIdx.x == 0) flags[tile] = 0;
         return;
     }
-    for (int e = threadIdx.x; e < R * TK; e += blockDim.x) {
-        const int r = e / TK, cc = e - r * TK;
-        const T *cp = C + r * W + cc + rz;
-        T h = tb->gz[0] * cp[0];
-        Hs[r * TK + cc] = h;
-        for (int K = 1; K <= rz; K++) {
-            h = (h + tb->gz[K] * cp[-K]) + tb->gz[K] * cp[K];
-            Hs[(K * R + r) * TK + cc] = h;
-        }
-    }
+    smear_h_levels<T>(C, Hs, tb, R, W, TK, rz, (int)blockDim.x);
     __syncthreads();
     if (smear_vec_tile<T>(TJ, TK)) {
         // channel a per wave, VW consecutive columns per lane

@@ -690,6 +690,9 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
             }
             if (++spin > kSpinLimit) bad = 1;
             else if ((spin & 255) == 0 && ld_dev_i(sy.err)) bad = 1;
+            // a wave still waiting yields its issue slots (about 64 clocks)
+            // to the waves of its SIMD that have work
+            if (pending) __builtin_amdgcn_s_sleep(1);
         }
         if (bad) {
             st_dev_i(sy.err, 1);
@@ -731,6 +734,7 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
             for (int q = 0; q < kSlotPoll; q++)
                 if (!(got & (1 << q)) && v[q] >= 0.0f) got |= 1 << q;
             if (++spin > kSpinLimit || ((spin & 255) == 0 && ld_dev_i(sy.err))) bad = 1;
+            if (got != (1 << kSlotPoll) - 1) __builtin_amdgcn_s_sleep(1);
         }
         float g = 0.0f;
 #pragma unroll
