@@ -27,6 +27,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "hyper-vlasov_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (counts packed FP32, which this build does not use)
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_c2.json"
 
 
@@ -85,6 +86,47 @@ def copy_calibration(mib: int):
         b.copy_(a)
     torch.cuda.synchronize()
     del a, b
+
+
+def measured_copy_gbs(mib: int = 1024, reps: int = 5) -> float:
+    """HBM bandwidth of a plain device copy on this GPU (read + write bytes
+    per second, best of `reps`), for context beside the 8 TB/s spec."""
+    import torch
+    a = torch.ones(mib << 20, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    best = 0.0
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = max(best, 2.0 * (mib << 20) / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del a, b
+    torch.cuda.synchronize()
+    return best
+
+
+def smear_flops_per_call(w) -> float:
+    """Algorithmic floating-point operations of one smearing of the
+    workload's grid: per node and species, 4 per z level (two products, two
+    sums), 2 per (a, b) tap of the y factor, 2 per x term and the final
+    product, over the spherical tap table of hv_smear_table_build (radius
+    4 sigma_r; sigma_r from the reference's r_fwhm = 1.444 fm)."""
+    sigma = 1.444 / (2.0 * np.sqrt(2.0 * np.log(2.0)))
+    cut2 = 16.0 * sigma * sigma
+    n = w.grid
+    h = w.world().d_max[0] * 2.0 / n  # node spacing (2 d_max / n)
+    r = 0
+    while r + 1 < n and ((r + 1) * h) ** 2 <= cut2:
+        r += 1
+    taps_y = 0
+    for a in range(r + 1):
+        kb = max(b for b in range(r + 1) if (a * h) ** 2 + (b * h) ** 2 <= cut2)
+        taps_y += 2 * kb + 1
+    per_node = 4 * r + 2 * taps_y + 2 * (2 * r + 1) + 1
+    return 2.0 * n ** 3 * per_node
 
 
 def run_gpu(w, prec: str, steps: int, warmup: int):
@@ -182,6 +224,16 @@ def run_batch(ws, prec: str, steps: int, tune: bool):
     }
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(w, state, budget_s: float):
     """The CPU restatement (oracle/liboracle.so, OpenMP) from the same warm
     mid-run state, on this host's cores; a bounded number of steps."""
@@ -219,7 +271,8 @@ def cpu_baseline(w, state, budget_s: float):
         "kind": "port",
         "sample": (f"{nsteps} steps of the {w.name} loop from the GPU's warm mid-run state "
                    f"(mean {its[1:].mean():.0f} SOR iterations/step), oracle/liboracle.so, "
-                   f"OpenMP {cores} threads, -O3 -march=x86-64-v3"),
+                   f"OpenMP {cores} threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                   f"OMP_PLACES={os.environ.get('OMP_PLACES')}), -O3 -march=x86-64-v3, {cpu_model()}"),
     }
 
 
@@ -241,6 +294,9 @@ def main():
               "figure of record for this loop", file=sys.stderr)
         sys.exit(2)
 
+    # the CPU baseline's OpenMP placement (read when libgomp starts)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from hvamd import configs
     w = configs.WORKLOADS[args.workload]()
     if args.copy_calibration > 0:
@@ -272,6 +328,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(rl, args.workload), **rl},
     }
+    sm_ms = mixed["prof"]["smear"][0] / max(mixed["prof"]["smear"][1], 1)
+    sflop = smear_flops_per_call(w)
+    out["smear_flops"] = {"flop_per_call": sflop, "avg_call_us": 1e3 * sm_ms,
+                          "achieved_TFLOPs": sflop / (sm_ms * 1e-3) / 1e12, "peak_TFLOPs_fp32": FP32_PEAK_TFLOPS,
+                          "frac": sflop / (sm_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS}
     if fp64 is not None:
         v64 = w.total * args.steps / fp64["wall"]
         r64 = sor_roofline(w, fp64)
@@ -290,6 +351,7 @@ def main():
         out["ensemble_c3"] = {"mixed": bm, "fp64": b64,
                               "mixed_over_fp64": bm["value"] / b64["value"] if b64 else None,
                               "mixed_over_single_run_fp64": bm["value"] / out["fp64"]["value"] if args.fp64 else None}
+    out["roofline"]["measured_copy_GBs"] = measured_copy_gbs()
     print(json.dumps(out), flush=True)
 
 

@@ -1523,6 +1523,25 @@ int hv_forces_fdm(hv_ctx *c) {
     return HV_OK;
 }
 
+int hv_write_vtk(hv_ctx *c, const char *path) {
+    if (!c || !path) return HV_ERR_ARG;
+    int rc = hv_forces_fdm(c);
+    if (rc) return rc;
+    const size_t n2 = 2 * (size_t)c->G;
+    std::vector<double> d(n2), u(n2), fx(n2), fy(n2), fz(n2);
+    const int which[5] = {HV_FIELD_DENSITY, HV_FIELD_POTENTIALS, HV_FIELD_FORCES_X, HV_FIELD_FORCES_Y,
+                          HV_FIELD_FORCES_Z};
+    double *dst[5] = {d.data(), u.data(), fx.data(), fy.data(), fz.data()};
+    for (int q = 0; q < 5; q++) {
+        rc = hv_download_field(c, which[q], dst[q]);
+        if (rc) return rc;
+    }
+    rc = hv_vtk_write_fields(path, &c->world, d.data(), u.data(), fx.data(), fy.data(), fz.data(),
+                             c->precision == HV_PREC_MIXED);
+    if (rc) return set_err(c, rc, "cannot write the snapshot %s", path);
+    return HV_OK;
+}
+
 int hv_cic_gather(hv_ctx *c) {
     if (!c) return HV_ERR_ARG;
     void *p;

@@ -11,13 +11,14 @@
 // nucleus is the seeded uniform-sphere sample of include/hv_init.h (radius
 // r_init A^(1/3), momenta uniform in the Fermi ball of rho0): the reference's
 // Woods-Saxon / Fermi-level fit needs GSL, so V0, a, epsilon_p, epsilon_n and
-// k_fwhm are read (and required, as there) but not used.  VTK snapshots are
-// not written.
+// k_fwhm are read (and required, as there) but not used.  Every `substeps`
+// steps it writes the stats line and, as the reference does, the snapshot
+// output_dir/out-%04d.vtk (hv_write_vtk: forces, density, potentials).
 //
 // Optional keys beyond the reference's 21: sor_omega, sor_tol, sor_max_it
 // (default: the reference's 1.5, 1e-4, 512), boost (0 = the reference's
 // weighted split, 1 = protons +eta, neutrons -eta), seed (128), r_init (1.2),
-// sort_every.
+// sort_every, vtk (1; 0 writes no snapshots).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -38,7 +39,7 @@ struct Run {
     double t_f = 0, t_exc = 0, eta_exc = 0, d_max_scale = 0;
     int nx = 0, num_test_part = 0, steps = 0, n = 0, z = 0, substeps = 0, use_gpu = 0;
     double sor_omega = 1.5, sor_tol = 1e-4;
-    int sor_max_it = 512, boost = HV_BOOST_REF_WEIGHTED, sort_every = 8;
+    int sor_max_it = 512, boost = HV_BOOST_REF_WEIGHTED, sort_every = 8, vtk = 1;
     unsigned long long seed = 128;
     double r_init = 1.2;
 };
@@ -62,7 +63,7 @@ bool read_input(const char *path, Run &r, std::string &why) {
     while (in >> key) {
         bool known = false;
         for (const char *k : kRequired) known = known || key == k;
-        const char *opt[] = {"sor_omega", "sor_tol", "sor_max_it", "boost", "seed", "r_init", "sort_every"};
+        const char *opt[] = {"sor_omega", "sor_tol", "sor_max_it", "boost", "seed", "r_init", "sort_every", "vtk"};
         for (const char *k : opt) known = known || key == k;
         if (!known) continue;
         std::string v;
@@ -97,7 +98,7 @@ bool read_input(const char *path, Run &r, std::string &why) {
               i("steps", r.steps) && i("n", r.n) && i("z", r.z) && i("substeps", r.substeps) &&
               i("use_gpu", r.use_gpu) && d("sor_omega", r.sor_omega) && d("sor_tol", r.sor_tol) &&
               i("sor_max_it", r.sor_max_it) && i("boost", r.boost) && d("seed", seed) && d("r_init", r.r_init) &&
-              i("sort_every", r.sort_every);
+              i("sort_every", r.sort_every) && i("vtk", r.vtk);
     r.seed = (unsigned long long)seed;
     if (!ok) {
         why = "a value does not parse";
@@ -173,9 +174,10 @@ int main(int argc, char **argv) {
         std::printf("{\"n\": %d, \"d_max\": %.17g, \"n_protons\": %d, \"n_neutrons\": %d, \"sigma_r\": %.17g, "
                     "\"dt\": %.17g, \"steps\": %d, \"substeps\": %d, \"t_exc\": %.17g, \"eta_exc\": %.17g, "
                     "\"skyrme\": [%.17g, %.17g, %.17g, %.17g], \"sor\": [%.17g, %.17g, %d], \"boost\": %d, "
-                    "\"seed\": %llu, \"r_init\": %.17g, \"k_fermi\": %.17g}\n",
+                    "\"seed\": %llu, \"r_init\": %.17g, \"k_fermi\": %.17g, \"vtk\": %d}\n",
                     w.n[0], w.d_max[0], np, nn, p.sigma_r, p.t_f / p.steps, p.steps, p.substeps, p.t_exc, p.eta_exc,
-                    s.A, s.B, s.C, s.gamma, p.sor_omega, p.sor_tol, p.sor_max_it, p.boost_mode, r.seed, rmax, kmax);
+                    s.A, s.B, s.C, s.gamma, p.sor_omega, p.sor_tol, p.sor_max_it, p.boost_mode, r.seed, rmax, kmax,
+                    r.vtk);
         return 0;
     }
     const char *outdir = mixed ? argv[3] : argv[2];
@@ -205,10 +207,19 @@ int main(int argc, char **argv) {
     int warn = hv_run_init(c, &s, &p, nullptr);
     if (warn < 0) return fail(c, "hv_run_init", warn);
     const double dt = p.t_f / p.steps;
-    const int chunk = 256;
+    // with snapshots, one chunk of steps per stats interval: the snapshot of
+    // step s (s % substeps == 0) holds the fields at the top of step s
+    const int chunk = r.vtk ? p.substeps : 256;
     std::vector<double> st(4 * (size_t)chunk);
     for (int first = 0; first < p.steps; first += chunk) {
         const int cnt = p.steps - first < chunk ? p.steps - first : chunk;
+        if (r.vtk) {
+            char name[64];
+            std::snprintf(name, sizeof name, "/out-%04d.vtk", first / p.substeps);  // set_output_filename
+            rc = hv_write_vtk(c, (std::string(outdir) + name).c_str());
+            if (rc) return fail(c, "hv_write_vtk", rc);
+            std::printf("Processed step: %i/%i.\n", first, p.steps);
+        }
         rc = hv_run_steps(c, &s, &p, first, cnt, st.data(), nullptr);
         if (rc < 0) return fail(c, "hv_run_steps", rc);
         warn |= rc;

@@ -20,6 +20,7 @@ HV_ERR_ARG = -1
 HV_ERR_HIP = -2
 HV_ERR_ALLOC = -3
 HV_ERR_STATE = -4
+HV_ERR_IO = -5
 HV_WARN_SOR_NOT_CONVERGED = 1
 HV_WARN_PARTICLE_OUTSIDE = 2
 HV_WARN_SMEAR_MARGIN = 4
@@ -152,6 +153,8 @@ SIGNATURES = {
     "hv_batch_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
     "hv_batch_sor_counts": (_I, [_P] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
     "hv_batch_graph_info": (_I, [_P] + [ctypes.POINTER(ctypes.c_longlong)] * 2),
+    "hv_write_vtk": (_I, [_P, ctypes.c_char_p]),
+    "hv_vtk_write_fields": (_I, [ctypes.c_char_p, ctypes.POINTER(HvWorld)] + [_P] * 5 + [_I]),
     "hv_dipole_loss_grad": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _I, _DP, _I, _DP, _DP,
                                  _DP]),
     "hv_fermi_momentum": (_D, [_D]),

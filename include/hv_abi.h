@@ -46,6 +46,7 @@ extern "C" {
 #define HV_ERR_HIP (-2)     /* HIP runtime error */
 #define HV_ERR_ALLOC (-3)   /* device allocation failed */
 #define HV_ERR_STATE (-4)   /* call not valid in the current context state */
+#define HV_ERR_IO (-5)      /* a file could not be opened or written */
 #define HV_WARN_SOR_NOT_CONVERGED 1  /* SOR hit max_it with max|dphi| > tol */
 #define HV_WARN_PARTICLE_OUTSIDE 2   /* some particles were outside the grid or non-finite */
 #define HV_WARN_SMEAR_MARGIN 4       /* a smearing tap lies within 1e-9 of the cutoff sphere */
@@ -174,6 +175,21 @@ int hv_mean_squared_radius(hv_ctx *ctx, int type, double *out);
  * the closing half kick).  Each |k_i|^2 and dV*e is quantised to 2^-30 and
  * summed exactly (order-free).  out = {E_kin + E_sky + E_coul, E_kin, E_sky, E_coul}. */
 int hv_total_energy(hv_ctx *ctx, const hv_skyrme *skyrme, int part_per_nucleon, double out[4]);
+
+/* ---- VTK snapshots (include/init.hpp:160-254) ------------------------------ */
+/* The reference's snapshot of the current fields, written where
+ * src/hyper_vlasov.cpp:72-85 writes out-%04d.vtk every `substeps` steps:
+ * legacy binary STRUCTURED_POINTS, VECTORS forces_p / forces_n (the FDM force
+ * of the current potentials, output_vector_field), SCALARS density_{p,n,t}
+ * and potentials_{p,n,t} (output_scalar_field), big-endian float32, x fastest.
+ * Replaces the output_vtk_header_start / output_vector_field /
+ * output_scalar_field sequence of cpu_simulate. */
+int hv_write_vtk(hv_ctx *ctx, const char *path);
+/* The same file from host fields in the reference layout ((i ny + j) nz + k,
+ * protons then neutrons; 2G values each); single != 0 writes what the float
+ * instantiation writes (d_max and the p + n sum in float).  No GPU needed. */
+int hv_vtk_write_fields(const char *path, const hv_world *world, const double *density, const double *potentials,
+                        const double *fx, const double *fy, const double *fz, int single);
 
 /* ---- fused step loop (the body of cpu_simulate, src/hyper_vlasov.cpp:53-104) */
 /* hv_run_init: Coulomb faces + pre-loop field solve + gather (:53-62).

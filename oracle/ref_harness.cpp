@@ -16,7 +16,9 @@
 // Test infrastructure only; the product never loads this library.
 
 #define FIT_ALGORITHM_H
+#include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "init.hpp"
 #include "tools.hpp"
@@ -223,6 +225,50 @@ void ref_simulate(double *x, double *y, double *z, double *kx, double *ky, doubl
     free_scalar_field(&coulomb);
     free_scalar_field(&potentials);
     free_scalar_field(&temp_density);
+}
+
+// The snapshot cpu_simulate writes every `substeps` steps
+// (src/hyper_vlasov.cpp:78-82, include/init.hpp:160-254), from given fields
+// (forces 3 x 2G, density 2G, potentials 2G); single != 0 runs the float
+// instantiation on the fields rounded to float.  Returns 0, or 1 if the file
+// cannot be opened.
+int ref_write_vtk(const char *path, const int n[3], const double d_max[3], const double *fx, const double *fy,
+                  const double *fz, const double *density, const double *potentials, int single) {
+    FILE *out = std::fopen(path, "wb");
+    if (!out) return 1;
+    const int G2 = 2 * n[0] * n[1] * n[2];
+    if (single) {
+        World<float> w;
+        for (int i = 0; i < 3; i++) {
+            w.n[i] = n[i];
+            w.d_max[i] = (float)d_max[i];
+        }
+        std::vector<float> f[5];
+        const double *src[5] = {fx, fy, fz, density, potentials};
+        for (int q = 0; q < 5; q++) {
+            f[q].resize(G2);
+            for (int e = 0; e < G2; e++) f[q][e] = (float)src[q][e];
+        }
+        output_vtk_header_start(out, w);
+        VectorField<float> fv{f[0].data(), f[1].data(), f[2].data()};
+        output_vector_field(out, fv, w, "forces");
+        ScalarField<float> d{f[3].data()}, u{f[4].data()};
+        output_scalar_field(out, d, w, "density");
+        output_scalar_field(out, u, w, "potentials");
+    } else {
+        World<double> w = make_world(n, d_max);
+        std::vector<double> f[5];
+        const double *src[5] = {fx, fy, fz, density, potentials};
+        for (int q = 0; q < 5; q++) f[q].assign(src[q], src[q] + G2);
+        output_vtk_header_start(out, w);
+        VectorField<double> fv{f[0].data(), f[1].data(), f[2].data()};
+        output_vector_field(out, fv, w, "forces");
+        ScalarField<double> d{f[3].data()}, u{f[4].data()};
+        output_scalar_field(out, d, w, "density");
+        output_scalar_field(out, u, w, "potentials");
+    }
+    std::fclose(out);
+    return 0;
 }
 
 }  // extern "C"

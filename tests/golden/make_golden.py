@@ -7,7 +7,8 @@ OpenMP reductions run in index order.
 The fixtures are data only (small arrays); they let the oracle be pinned
 against the reference where the reference checkout is absent.
 
-    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py        (kernels, loop)
+    python tests/golden/make_golden.py vtk                      (snapshot files)
 """
 import os
 import sys
@@ -71,5 +72,25 @@ def main():
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
 
 
+def snapshots():
+    """The reference's VTK snapshot (output_vtk_header_start,
+    output_vector_field, output_scalar_field) of seeded fields on a small
+    uneven grid, from the double and the float instantiation."""
+    R = Reference()
+    n, d = (5, 4, 3), (1.5, 2.0, 2.7)
+    G2 = 2 * int(np.prod(n))
+    rng = np.random.default_rng(31)
+    fields = {key: rng.normal(0.0, 3.0, G2) for key in ("fx", "fy", "fz", "potentials")}
+    fields["density"] = rng.uniform(0.0, 0.2, G2)
+    np.savez_compressed(HERE / "snapshot_5x4x3.npz", n=np.array(n), d_max=np.array(d), **fields)
+    for single, tag in ((False, "f64"), (True, "f32")):
+        R.write_vtk(HERE / f"snapshot_5x4x3_{tag}.vtk", n, d, fields["fx"], fields["fy"], fields["fz"],
+                    fields["density"], fields["potentials"], single=single)
+    print("wrote", sorted(p.name for p in HERE.glob("snapshot_*")))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["vtk"]:
+        snapshots()
+    else:
+        main()

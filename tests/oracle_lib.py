@@ -186,6 +186,8 @@ class Reference:
         L.ref_drift.argtypes = [_P] * 6 + [_I, _I, _D]
         L.ref_excite.argtypes = [_P, _I, _I, _D, _I, _I]
         L.ref_coulomb_bc.argtypes = [_P, _P, _P, _I]
+        L.ref_write_vtk.restype = _I
+        L.ref_write_vtk.argtypes = [ctypes.c_char_p, _P, _P] + [_P] * 5 + [_I]
         L.ref_simulate.argtypes = ([_P] * 9 + [_I, _I, _P, _P] + [_D] * 4 + [_D, _I, _D, _D, _I, _I, _D, _I]
                                    + [_P, _P])
         self.L = L
@@ -193,6 +195,13 @@ class Reference:
     @staticmethod
     def _nd(n, d_max):
         return (ctypes.c_int * 3)(*n), (ctypes.c_double * 3)(*d_max)
+
+    def write_vtk(self, path, n, d_max, fx, fy, fz, density, potentials, single=False):
+        nn_, dd = self._nd(n, d_max)
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (fx, fy, fz, density, potentials)]
+        rc = self.L.ref_write_vtk(str(path).encode(), nn_, dd, *[p(v) for v in a], 1 if single else 0)
+        if rc:
+            raise OSError(f"cannot write {path}")
 
     def deposit(self, x, y, z, np_, nn, n, d_max):
         counts = np.zeros(2 * int(np.prod(n)))

@@ -53,9 +53,9 @@ def test_check_derives_the_reference_run(driver, tmp_path):
 
 
 def test_optional_keys(driver, tmp_path):
-    _write(tmp_path / "input.txt", INPUT, "sor_omega 1.8\nsor_tol 1e-6\nboost 1\nseed 7\nsome_unknown_token\n")
+    _write(tmp_path / "input.txt", INPUT, "sor_omega 1.8\nsor_tol 1e-6\nboost 1\nseed 7\nvtk 0\nsome_unknown_token\n")
     d = json.loads(_check(tmp_path / "input.txt").stdout)
-    assert d["sor"] == [1.8, 1e-6, 512] and d["boost"] == 1 and d["seed"] == 7
+    assert d["sor"] == [1.8, 1e-6, 512] and d["boost"] == 1 and d["seed"] == 7 and d["vtk"] == 0
 
 
 @pytest.mark.parametrize("edit", ["missing", "duplicate", "garbage"])
@@ -103,8 +103,16 @@ def test_run_writes_reference_stats(driver, tmp_path):
                        boost_mode=hvamd.HV_BOOST_REF_WEIGHTED, z=20, n=20, sigma_r=d["sigma_r"],
                        part_per_nucleon=50, sor_omega=1.5, sor_tol=1e-4, sor_max_it=512)
     sim.run_init(sk, prm)
-    stats, _ = sim.run_steps(sk, prm, 0, 20)
+    s1, _ = sim.run_steps(sk, prm, 0, 10)
+    snap = tmp_path / "snap.vtk"
+    assert sim.lib.hv_write_vtk(sim.ctx, str(snap).encode()) == 0
+    s2, _ = sim.run_steps(sk, prm, 10, 10)
     sim.close()
+    stats = np.concatenate([s1, s2])
+    # snapshots at the top of steps 0, 5, 10, 15 (src/hyper_vlasov.cpp:64-85)
+    assert sorted(p.name for p in out.glob("*.vtk")) == [f"out-{i:04d}.vtk" for i in range(4)]
+    assert (out / "out-0002.vtk").read_bytes() == snap.read_bytes()
+    assert r.stdout.count("Processed step:") == 4
     rows = stats[::5]
     want = np.column_stack([0.5 * np.arange(0, 20, 5), np.sqrt(rows[:, 2]), np.sqrt(rows[:, 3]), rows[:, 0],
                             rows[:, 1]])

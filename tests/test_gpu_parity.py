@@ -7,6 +7,8 @@ deterministic hv_det_pow and the smearing weights / Coulomb faces come from the
 same host-side tables.  The mixed-precision path is held to the tolerances
 stated in each test.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -447,3 +449,45 @@ def test_mixed_sor_matches_fp64_iterations(res32):
     assert f[0] == m[0]
     assert abs(f[1] - m[1]) <= 12 + 2 * res32 and m[2] < 1e-6
     assert np.abs(f[3] - m[3]).max() < 2e-5
+
+
+@pytest.mark.parametrize("prec", [hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED])
+def test_write_vtk_snapshot_of_loop_state(oracle, tmp_path, prec):
+    """hv_write_vtk after a few loop steps writes the reference's snapshot of
+    the context's density, merged potentials and their FDM forces.  FP64: the
+    forces are the oracle's FDM (bitwise the reference's) and the file equals
+    the host writer's (byte-exact to the reference's writer on its own) and,
+    where the reference harness is built, the reference writer's.  Mixed form:
+    the same with the device's FP32 forces and the float instantiation's
+    header and p + n sums."""
+    print(f"vtk snapshot prec={prec}", flush=True)
+    from oracle_lib import Reference
+    w = configs.c0()
+    sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, prec)
+    sim.upload_particles(*w.initial_state())
+    prm = w.params(4)
+    sim.run_init(w.sky(), prm)
+    sim.run_steps(w.sky(), prm, 0, 4)
+    got = tmp_path / "got.vtk"
+    assert sim.lib.hv_write_vtk(sim.ctx, str(got).encode()) == 0
+    rho = sim.download_field(hvamd.HV_FIELD_DENSITY)
+    U = sim.download_field(hvamd.HV_FIELD_POTENTIALS)
+    n, d = w.world().n, w.world().d_max
+    single = prec == hvamd.HV_PREC_MIXED
+    if single:
+        Fx, Fy, Fz = (sim.download_field(f) for f in (hvamd.HV_FIELD_FORCES_X, hvamd.HV_FIELD_FORCES_Y,
+                                                        hvamd.HV_FIELD_FORCES_Z))
+    else:
+        Fx, Fy, Fz = oracle.fdm(U, n, d)
+    sim.close()
+    want = tmp_path / "want.vtk"
+    wd = hvamd._lib.HvWorld((ctypes.c_int * 3)(*n), (ctypes.c_double * 3)(*d))
+    arr = [np.ascontiguousarray(v, dtype=np.float64) for v in (rho, U, Fx, Fy, Fz)]
+    assert hvamd.load().hv_vtk_write_fields(str(want).encode(), ctypes.byref(wd),
+                                            *[a.ctypes.data_as(ctypes.c_void_p) for a in arr],
+                                            1 if single else 0) == 0
+    assert got.read_bytes() == want.read_bytes()
+    if Reference.available and not single:
+        ref = tmp_path / "ref.vtk"
+        Reference().write_vtk(ref, n, d, Fx, Fy, Fz, rho, U)
+        assert got.read_bytes() == ref.read_bytes()
