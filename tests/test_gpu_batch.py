@@ -154,3 +154,34 @@ def test_batch_step_graph_replays_equal_eager_steps(prec):
     for pe, pg in zip(e[4], g[4]):
         for key in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz"):
             assert np.array_equal(pe[key], pg[key]), key
+
+
+@pytest.mark.parametrize("big", [96, 32])
+def test_batch_resident_shape_bodies_equal_solo(big):
+    """The ensemble kernel runs its compile-time tile body on the grid shape
+    with the most nodes (16 x 12 rows x 48 slots on 96^3, 32 x 16 x 16 on
+    32^3) and the run-time-shape body on the others: both against solo
+    LDS-resident solves, bit for bit."""
+    print(f"batch resident shapes big={big}", flush=True)
+    ws, params = _instances()
+    ws = ws[:2]
+    params = params[:2]
+    ws[0].grid, ws[0].dx = big, 16.0 / big
+    ws[1].grid, ws[1].dx = 20, 0.8
+    b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws], hvamd.HV_PREC_MIXED)
+    b.set_option(hvamd.HV_OPT_SORT_EVERY, 2)
+    for i, w in enumerate(ws):
+        b.instance(i).upload_particles(*w.initial_state())
+    sks = [w.sky() for w in ws]
+    it0 = b.run_init(sks, params)
+    s1, i1 = b.run_steps(sks, params, 0, 4)
+    parts = [b.instance(i).download_particles() for i in range(len(ws))]
+    b.close()
+    for i, w in enumerate(ws):
+        s_it0, s_stats, s_its, s_part, _ = _solo(w, params[i], hvamd.HV_PREC_MIXED, [4], 2, resident=True)
+        print(i, w.grid, "iters", list(i1[i]), "solo", list(s_its), flush=True)
+        assert it0[i] == s_it0
+        assert np.array_equal(i1[i], s_its)
+        assert np.array_equal(s1[i], s_stats)
+        for k in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz"):
+            assert np.array_equal(parts[i][k], s_part[k]), (i, k)
