@@ -118,6 +118,7 @@ struct hv_ctx {
     int res32_slots_cap = 0;
     int *res32_err = nullptr;
     float *res32_result = nullptr;
+    float *res32_hres = nullptr;        // pinned: iterations, tested max, error word
     int since_sort = 1 << 30;
     bool permuted = false;
 
@@ -569,6 +570,7 @@ int res32_setup(hv_ctx *c) {
     if (hipMalloc((void **)&c->res32_x, sizeof(unsigned long long) * 2 * nT * best.xstride) != hipSuccess) return 1;
     if (hipMalloc((void **)&c->res32_err, sizeof(int) * 4) != hipSuccess) return 1;
     if (hipMalloc((void **)&c->res32_result, sizeof(float) * 4) != hipSuccess) return 1;
+    if (hipHostMalloc((void **)&c->res32_hres, sizeof(float) * 4, hipHostMallocDefault) != hipSuccess) return 1;
     c->rtl32 = best;
     c->res32_lds = lds;
     c->res32_state = 1;
@@ -576,8 +578,10 @@ int res32_setup(hv_ctx *c) {
 }
 
 // The mixed form's FP32 correction iteration on c->sor_delta with source
-// c->sor_R, in one cooperative launch; returns the iterations done.
-int sor_resident_f32(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out) {
+// c->sor_R, in one cooperative launch, enqueued with the copy of its
+// iteration count and error word to pinned memory; sor_resident_f32_collect
+// reads them after a wait on the stream.
+int sor_resident_f32_enqueue(hv_ctx *c, const SorCoef &cf, int max_it) {
     const int nT = c->rtl32.PI * c->rtl32.PJ;
     if (c->res32_slots_cap < max_it) {
         if (c->res32_slots) (void)hipFree(c->res32_slots);
@@ -619,16 +623,19 @@ int sor_resident_f32(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out) {
         pb = prof_event(c);
         if (pb) (void)hipEventRecord(pb, c->stream);
     }
-    float res[2];
-    int err = 0;
-    HV_HIP(c, hipMemcpyAsync(res, c->res32_result, sizeof res, hipMemcpyDeviceToHost, c->stream));
-    HV_HIP(c, hipMemcpyAsync(&err, c->res32_err, sizeof err, hipMemcpyDeviceToHost, c->stream));
-    HV_HIP(c, hipStreamSynchronize(c->stream));
-    if (err) return set_err(c, HV_ERR_HIP, "LDS-resident FP32 SOR: a tile wait exceeded its spin limit");
-    const int iters = (int)res[0];
     if (pa && pb) c->pending.push_back({HV_KCLASS_SOR_RES_F32, pa, pb, 1});
+    HV_HIP(c, hipMemcpyAsync(c->res32_hres, c->res32_result, 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(c, hipMemcpyAsync(c->res32_hres + 2, c->res32_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    return HV_OK;
+}
+
+int sor_resident_f32_collect(hv_ctx *c, int *iters_out) {
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    int err = 0;
+    std::memcpy(&err, c->res32_hres + 2, sizeof err);
+    if (err) return set_err(c, HV_ERR_HIP, "LDS-resident FP32 SOR: a tile wait exceeded its spin limit");
+    *iters_out = (int)c->res32_hres[0];
     prof_flush(c);
-    *iters_out = iters;
     return HV_OK;
 }
 
@@ -693,8 +700,17 @@ int sor_resident(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, int *i
 // max |update| is below tol (or max_it), launching them in chunks and reading
 // back each chunk's per-iteration maxima.  launch(it) enqueues iteration it;
 // launches past the converged iteration return at once.
-template <typename Launch>
-int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double *md_out, Launch launch) {
+// after_first_wait(max_it) runs once, after the first chunk's read-back, and
+// may lower max_it (the budget of the mixed form's FP64 sweeps, known only
+// when the FP32 stage's count is read back in the same wait); the first chunk
+// must then not exceed any budget it can set.
+struct NoHook {
+    int operator()(int &) const { return HV_OK; }
+};
+
+template <typename Launch, typename Hook = NoHook>
+int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double *md_out, Launch launch,
+                Hook after_first_wait = Hook()) {
     const double tol = cf.tol;
     int rc = fill<unsigned long long>(c, c->maxd, 0ULL, (long long)maxd_words(max_it));
     if (rc) return rc;
@@ -702,6 +718,7 @@ int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double
     if (rc) return rc;
     int it = 0, iters = -1, start = 0;
     int chunk = c->sor_pred > 2 ? c->sor_pred - 1 : 1;
+    bool first = true;
     while (it < max_it) {
         start = it;
         int end = it + chunk < max_it ? it + chunk : max_it;
@@ -712,6 +729,11 @@ int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double
         HV_HIP(c, hipMemcpyAsync(c->pinned, c->maxc, sizeof(double) * (size_t)(end - start),
                                  hipMemcpyDeviceToHost, c->stream));
         HV_HIP(c, hipStreamSynchronize(c->stream));
+        if (first) {
+            first = false;
+            rc = after_first_wait(max_it);
+            if (rc) return rc;
+        }
         for (int m = 0; m < end - start; m++) {
             if (c->pinned[m] < tol) {
                 iters = start + m + 1;
@@ -731,25 +753,32 @@ int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double
 // The reference's SOR on c->phi (FP64), one launch per iteration (fused) or
 // per sweep.  The fused form alternates phi and phi_alt; the result is
 // left in c->phi.
-template <typename RT>
+template <typename RT, typename Hook = NoHook>
 int sor_exact(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, bool fused, dim3 sgrd, dim3 sblk,
               const SorTiles &tl, dim3 fgrd, dim3 fblk, const FusedTiles &ft, size_t rows_lds, int *iters,
-              double *md) {
+              double *md, Hook after_first_wait = Hook()) {
     if (fused) {
         double *pbuf[2] = {c->phi, c->phi_alt};
-        int rc = sor_iterate(c, cf, max_it, iters, md, [&](int it) {
-            sor_launch(c, it, 1, HV_KCLASS_SOR, k_sor_fused<true, double, RT>, fgrd, fblk, rows_lds * sizeof(double),
-                       pbuf[it & 1], pbuf[(it + 1) & 1], (const RT *)rho, c->sp, ft, cf, it, c->maxd, c->ctl);
-        });
+        int rc = sor_iterate(
+            c, cf, max_it, iters, md,
+            [&](int it) {
+                sor_launch(c, it, 1, HV_KCLASS_SOR, k_sor_fused<true, double, RT>, fgrd, fblk,
+                           rows_lds * sizeof(double), pbuf[it & 1], pbuf[(it + 1) & 1], (const RT *)rho, c->sp, ft, cf,
+                           it, c->maxd, c->ctl);
+            },
+            after_first_wait);
         if (rc) return rc;
         if (*iters & 1) std::swap(c->phi, c->phi_alt);
         return HV_OK;
     }
-    return sor_iterate(c, cf, max_it, iters, md, [&](int it) {
-        for (int colour = 0; colour < 2; colour++)
-            sor_launch(c, it, colour, HV_KCLASS_SOR, k_sor_colour<RT>, sgrd, sblk, 0, c->phi, (const RT *)rho, c->sp,
-                       tl, cf, colour, it, c->maxd, c->ctl);
-    });
+    return sor_iterate(
+        c, cf, max_it, iters, md,
+        [&](int it) {
+            for (int colour = 0; colour < 2; colour++)
+                sor_launch(c, it, colour, HV_KCLASS_SOR, k_sor_colour<RT>, sgrd, sblk, 0, c->phi, (const RT *)rho,
+                           c->sp, tl, cf, colour, it, c->maxd, c->ctl);
+        },
+        after_first_wait);
 }
 
 // SOR coefficients of the context's grid (include/physics.hpp:101-103)
@@ -843,7 +872,8 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         float *dbuf[2] = {c->sor_delta, c->sor_delta_alt};
         bool res32 = false;
         if (c->sor_res32 && res32_setup(c) == HV_OK) {
-            rc = sor_resident_f32(c, cf, max_it, &iters);
+            // its iteration count is read in the FP64 sweeps' first wait
+            rc = sor_resident_f32_enqueue(c, cf, max_it);
             res32 = true;
         } else if (fused) {
             rc = sor_iterate(c, cf, max_it, &iters, &md, [&](int it) {
@@ -863,13 +893,22 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         k_sor_fold<<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, dres, 2 * c->sp.Gh);
         HV_CHECK_LAUNCH(c);
         int it64 = 0;
-        const int pred = iters;
+        // FP64 sweeps: the first chunk is one iteration (prediction 2), within
+        // every budget (at least 4), so it is queued before the resident
+        // FP32 stage's count is known; the budget is set in the first wait
         c->sor_pred = 2;
-        rc = sor_exact(c, rho, cf, mixed_fp64_budget(max_it, iters), fused, sgrd, sblk, tl, fgrd, fblk, ft, rows_lds,
-                       &it64, &md);
+        auto hook = [&](int &budget) -> int {
+            if (!res32) return HV_OK;
+            int rc2 = sor_resident_f32_collect(c, &iters);
+            if (rc2) return rc2;
+            budget = mixed_fp64_budget(max_it, iters);
+            return HV_OK;
+        };
+        rc = sor_exact(c, rho, cf, res32 ? max_it : mixed_fp64_budget(max_it, iters), fused, sgrd, sblk, tl, fgrd,
+                       fblk, ft, rows_lds, &it64, &md, hook);
         if (rc) return rc;
         c->mixed_fp64_sweeps = it64;
-        c->sor_pred = pred;
+        c->sor_pred = iters;
         c->cnt_it32 += iters;
         c->cnt_it64 += it64;
         iters += it64;
@@ -1347,6 +1386,7 @@ int hv_ctx_destroy(hv_ctx *c) {
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->res32_hres) (void)hipHostFree(c->res32_hres);
     prof_flush(c);
     for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
