@@ -109,6 +109,8 @@ struct hv_ctx {
     // LDS-resident FP32 correction iteration of the mixed form
     int sor_res32 = 1;        // option: use it when the grid fits
     int res32_lag = 2;        // option: iterations between a max and its convergence test
+    int res32_per_cu = 1;     // option: LDS-resident FP32 tiles per CU (1 or 2)
+    int res32_threads = 1024;
     int res32_state = 0;
     ResTiles rtl32{};
     size_t res32_lds = 0;
@@ -484,7 +486,7 @@ size_t res32_lds_bytes(const ResTiles &t) {
 // every tile block can be resident at once.
 template <typename LdsFn>
 int pick_res_tiles(hv_ctx *c, const void *kernel, int threads, LdsFn lds_bytes, ResTiles *out, size_t *lds_out,
-                   int max_xstride = 1 << 30) {
+                   int max_xstride = 1 << 30, int per_cu = 1, int max_tiles = kSlotPoll * kWave) {
     int dev = 0, ncu = 0, maxlds = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
@@ -493,10 +495,10 @@ int pick_res_tiles(hv_ctx *c, const void *kernel, int threads, LdsFn lds_bytes, 
     if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return 1;
     const int nx = c->world.n[0], ny = c->world.n[1];
     const long long rows = (long long)nx * ny;
-    long long target = rows / 32;
+    long long target = rows / (32 / per_cu);
     if (target < 1) target = 1;
-    if (target > ncu) target = ncu;
-    if (target > kSlotPoll * kWave) target = kSlotPoll * kWave;
+    if (target > (long long)per_cu * ncu) target = (long long)per_cu * ncu;
+    if (target > max_tiles) target = max_tiles;
     ResTiles best{};
     long long best_cost = -1;
     for (int PI = 1; PI <= nx && PI <= target; PI++) {
@@ -522,9 +524,9 @@ int pick_res_tiles(hv_ctx *c, const void *kernel, int threads, LdsFn lds_bytes, 
     if (best_cost < 0) return 1;
     const size_t lds = lds_bytes(best);
     if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return 1;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu < 1 ||
-        (long long)best.PI * best.PJ > (long long)per_cu * ncu)
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, lds) != hipSuccess || occ < 1 ||
+        (long long)best.PI * best.PJ > (long long)occ * ncu)
         return 1;
     *out = best;
     *lds_out = lds;
@@ -555,17 +557,39 @@ int res32_setup(hv_ctx *c) {
     c->res32_state = -1;
     ResTiles best{};
     size_t lds = 0;
-    if (pick_res_tiles(c, (const void *)k_sor_resident_f32<0, 0, 0>, kRes32Threads, res32_lds_bytes, &best, &lds,
-                       kFetchMax * kRes32Threads) != HV_OK)
+    // one tile per CU (1024-thread blocks) or two (512-thread blocks)
+    const int per = c->res32_per_cu;
+    const int nt = per == 2 ? 512 : kRes32Threads;
+    const void *generic = per == 2 ? (const void *)k_sor_resident_f32<0, 0, 0, 512>
+                                   : (const void *)k_sor_resident_f32<0, 0, 0>;
+    if (pick_res_tiles(c, generic, nt, res32_lds_bytes, &best, &lds, kFetchMax * nt, per,
+                       (per == 2 ? slot_poll32<512>() : slot_poll32<kRes32Threads>()) * kWave) !=
+        HV_OK)
         return 1;
     // a fixed-shape instance for the common tilings (same LDS size and block)
-    c->res32_kernel = (const void *)k_sor_resident_f32<0, 0, 0>;
-    if (best.TI == 8 && best.TJ == 8 && best.nzh == 64) c->res32_kernel = (const void *)k_sor_resident_f32<8, 8, 64>;
-    if (best.TI == 8 && best.TJ == 4 && best.nzh == 32) c->res32_kernel = (const void *)k_sor_resident_f32<8, 4, 32>;
-    if (best.TI == 4 && best.TJ == 8 && best.nzh == 32) c->res32_kernel = (const void *)k_sor_resident_f32<4, 8, 32>;
-    if (c->res32_kernel != (const void *)k_sor_resident_f32<0, 0, 0> &&
-        hipFuncSetAttribute(c->res32_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        c->res32_kernel = (const void *)k_sor_resident_f32<0, 0, 0>;
+    c->res32_kernel = generic;
+    if (per == 2) {
+        if (best.TI == 8 && best.TJ == 4 && best.nzh == 64)
+            c->res32_kernel = (const void *)k_sor_resident_f32<8, 4, 64, 512>;
+    } else {
+        if (best.TI == 8 && best.TJ == 8 && best.nzh == 64)
+            c->res32_kernel = (const void *)k_sor_resident_f32<8, 8, 64>;
+        if (best.TI == 8 && best.TJ == 4 && best.nzh == 32)
+            c->res32_kernel = (const void *)k_sor_resident_f32<8, 4, 32>;
+        if (best.TI == 4 && best.TJ == 8 && best.nzh == 32)
+            c->res32_kernel = (const void *)k_sor_resident_f32<4, 8, 32>;
+    }
+    if (c->res32_kernel != generic) {
+        int occ = 0, dev = 0, ncu = 0;
+        const bool ok = hipFuncSetAttribute(c->res32_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
+                            hipSuccess &&
+                        hipGetDevice(&dev) == hipSuccess &&
+                        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+                        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, c->res32_kernel, nt, lds) == hipSuccess &&
+                        (long long)best.PI * best.PJ <= (long long)occ * ncu;
+        if (!ok) c->res32_kernel = generic;
+    }
+    c->res32_threads = nt;
     const size_t nT = (size_t)best.PI * best.PJ;
     if (hipMalloc((void **)&c->res32_x, sizeof(unsigned long long) * 2 * nT * best.xstride) != hipSuccess) return 1;
     if (hipMalloc((void **)&c->res32_err, sizeof(int) * 4) != hipSuccess) return 1;
@@ -616,7 +640,7 @@ int sor_resident_f32_enqueue(hv_ctx *c, const SorCoef &cf, int max_it) {
     int mi = max_it, lag = c->res32_lag;
     void *args[] = {&delta, (void *)&R, &sp, &tl, &cfv, &mi, &lag, &sy};
     hipEvent_t pa = prof_begin(c);
-    HV_HIP(c, hipLaunchCooperativeKernel(c->res32_kernel, dim3((unsigned)nT), dim3(kRes32Threads), args,
+    HV_HIP(c, hipLaunchCooperativeKernel(c->res32_kernel, dim3((unsigned)nT), dim3((unsigned)c->res32_threads), args,
                                          (unsigned)c->res32_lds, c->stream));
     hipEvent_t pb = nullptr;
     if (pa) {
@@ -1736,6 +1760,24 @@ int hv_set_option(hv_ctx *c, int option, int value) {
     case HV_OPT_SMEAR_FUSED:
         c->smear_fused = value ? 1 : 0;
         return HV_OK;
+    case HV_OPT_SOR_F32_TILES_PER_CU:
+        if (value != 1 && value != 2) return set_err(c, HV_ERR_ARG, "FP32 resident tiles per CU must be 1 or 2");
+        if (value != c->res32_per_cu) {
+            HV_HIP(c, hipStreamSynchronize(c->stream));
+            void *bufs[] = {c->res32_x, c->res32_err, c->res32_result, c->res32_slots};
+            for (void *b : bufs)
+                if (b) (void)hipFree(b);
+            if (c->res32_hres) (void)hipHostFree(c->res32_hres);
+            c->res32_x = nullptr;
+            c->res32_err = nullptr;
+            c->res32_result = nullptr;
+            c->res32_slots = nullptr;
+            c->res32_slots_cap = 0;
+            c->res32_hres = nullptr;
+            c->res32_state = 0;  // tiles chosen again at the next solve
+            c->res32_per_cu = value;
+        }
+        return HV_OK;
     case HV_OPT_SOR_F32_LAG:
         // the test of row it - lag runs before this iteration's black sweep,
         // so row it itself is never complete then: lag >= 1
@@ -1754,6 +1796,7 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
     case HV_OPT_SOR_FUSED: *value = c->sor_fused; return HV_OK;
     case HV_OPT_SOR_RESIDENT_F32: *value = c->sor_res32 && c->res32_state >= 0; return HV_OK;
     case HV_OPT_SOR_F32_LAG: *value = c->res32_lag; return HV_OK;
+    case HV_OPT_SOR_F32_TILES_PER_CU: *value = c->res32_per_cu; return HV_OK;
     case HV_OPT_SMEAR_TILE: *value = c->smear_tk_max; return HV_OK;
     case HV_OPT_SMEAR_FUSED: *value = c->smear_fused; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);

@@ -359,6 +359,10 @@ k_sor_resident(double *__restrict__ phi, const RT *__restrict__ rho, Split sp, R
 // published after that read), so one slice per colour suffices and a word's
 // tag equals the awaited iteration when it is accepted.
 constexpr int kRes32Threads = 1024;
+// slots per lane of the FP32 kernel's convergence poll: 4 (at most 256 tiles)
+// for 1024-thread blocks, 8 (512 tiles) for the two-per-CU 512-thread blocks
+template <int NT>
+constexpr int slot_poll32() { return NT == 512 ? 8 : 4; }
 constexpr int kFetchMax = 4;  // halo words per thread and colour
 
 struct ResSync32 {
@@ -398,14 +402,14 @@ __device__ __forceinline__ unsigned long long ld_tagged(const unsigned long long
 // runs it once per queued (instance, tile) job.  A block that leaves it has
 // written its tile of delta back; the LDS is free for the next job after a
 // barrier.
-template <int CTI, int CTJ, int CNZH>
+template <int CTI, int CTJ, int CNZH, int NT = kRes32Threads>
 __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, const float *__restrict__ R, Split sp,
                                                  ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
                                                  const int t) {
     extern __shared__ unsigned char smem[];
     const int TI = CTI ? CTI : tl.TI, TJ = CTJ ? CTJ : tl.TJ, nzh = CNZH ? CNZH : tl.nzh;
     const int side_i = TJ * nzh, side_j = TI * nzh, xstride = 2 * side_i + 2 * side_j;
-    constexpr int nwaves = kRes32Threads / kWave;
+    constexpr int nwaves = NT / kWave;
     // rows a wave updates at once (fewer for the large fixed tile: register budget of 1024-thread blocks)
     constexpr int RR = (CTI * CTJ * CNZH >= 4096) ? 2 : kResRows;
     // rows of fewer (or other than a multiple of 64) k slots: lanes take
@@ -436,7 +440,7 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
     __shared__ float s_max[2];  // by iteration parity: written in iteration it, read at its end
     if (tid == 0) s_bad = 0;
 
-    for (int e = tid; e < tsz; e += kRes32Threads) {
+    for (int e = tid; e < tsz; e += NT) {
         const int r = e / nzh, h = e - r * nzh;
         const int li = r / TJ, lj = r - li * TJ;
         const int i = i0 + li, j = j0 + lj;
@@ -455,13 +459,13 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
     }
     {
         const float *src = delta + sp.Gh;  // black values around the tile for the first red sweep
-        for (int e = tid; e < side_i; e += kRes32Threads) {
+        for (int e = tid; e < side_i; e += NT) {
             const int lj = e / nzh, h = e - lj * nzh, j = j0 + lj;
             if (j >= ny) continue;
             if (has_im) h_im[e] = src[((long long)(i0 - 1) * ny + j) * nzh + h];
             if (has_ip) h_ip[e] = src[((long long)(i0 + TI) * ny + j) * nzh + h];
         }
-        for (int e = tid; e < side_j; e += kRes32Threads) {
+        for (int e = tid; e < side_j; e += NT) {
             const int li = e / nzh, h = e - li * nzh, i = i0 + li;
             if (i >= nx) continue;
             if (has_jm) h_jm[e] = src[((long long)i * ny + (j0 - 1)) * nzh + h];
@@ -470,10 +474,10 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
     }
     // this thread's halo words: exchange offset in the neighbour's slice and
     // halo index (at most kFetchMax; the host checks xstride fits)
-    // (offsets fit in int: nT * xstride <= 256 * kFetchMax * kRes32Threads)
+    // (offsets fit in int: nT * xstride <= 256 * kFetchMax * NT)
     int src_off[kFetchMax];
     int nfetch = 0;
-    for (int x = tid; x < xstride && nfetch < kFetchMax; x += kRes32Threads) {
+    for (int x = tid; x < xstride && nfetch < kFetchMax; x += NT) {
         int off = -1;
         if (x < side_i) {
             if (has_im) off = (t - tl.PJ) * xstride + side_i + x;  // i-high face of (pi-1, pj)
@@ -654,12 +658,12 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
     auto publish = [&](int colour, unsigned tag) {
         unsigned long long *x = (colour ? sy.xb : sy.xr) + (long long)t * xstride;
         const float *v = pc[colour];
-        for (int e = tid; e < side_i; e += kRes32Threads) {
+        for (int e = tid; e < side_i; e += NT) {
             const int lj = e / nzh, h = e - lj * nzh;
             st_tagged(x + e, v[(0 * TJ + lj) * nzh + h], tag);
             st_tagged(x + side_i + e, v[((TI - 1) * TJ + lj) * nzh + h], tag);
         }
-        for (int e = tid; e < side_j; e += kRes32Threads) {
+        for (int e = tid; e < side_j; e += NT) {
             const int li = e / nzh, h = e - li * nzh;
             st_tagged(x + 2 * side_i + e, v[(li * TJ + 0) * nzh + h], tag);
             st_tagged(x + 2 * side_i + side_j + e, v[(li * TJ + TJ - 1) * nzh + h], tag);
@@ -684,7 +688,7 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
 #pragma unroll
             for (int q = 0; q < kFetchMax; q++) {
                 if ((pending & (1u << q)) && (unsigned)(w[q] >> 32) >= tag) {
-                    halo[tid + q * kRes32Threads] = __uint_as_float((unsigned)w[q]);
+                    halo[tid + q * NT] = __uint_as_float((unsigned)w[q]);
                     pending &= ~(1u << q);
                 }
             }
@@ -717,28 +721,29 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
 #endif
     // wave 0: max over all tiles of the write-once slot row `row_it`
     auto poll_slots = [&](int row_it, int parity) {
-        float v[kSlotPoll];
+        constexpr int kSlotPoll32 = slot_poll32<NT>();
+        float v[kSlotPoll32];
         int got = 0, bad = 0;
         const float *row = sy.slots + (long long)row_it * nT;
 #pragma unroll
-        for (int q = 0; q < kSlotPoll; q++) {
+        for (int q = 0; q < kSlotPoll32; q++) {
             v[q] = 0.0f;
             if (lane + q * kWave >= nT) got |= 1 << q;
         }
         long long spin = 0;
-        while (!__all(got == (1 << kSlotPoll) - 1) && !bad) {
+        while (!__all(got == (1 << kSlotPoll32) - 1) && !bad) {
 #pragma unroll
-            for (int q = 0; q < kSlotPoll; q++)
+            for (int q = 0; q < kSlotPoll32; q++)
                 if (!(got & (1 << q))) v[q] = ld_dev_f(row + lane + q * kWave);
 #pragma unroll
-            for (int q = 0; q < kSlotPoll; q++)
+            for (int q = 0; q < kSlotPoll32; q++)
                 if (!(got & (1 << q)) && v[q] >= 0.0f) got |= 1 << q;
             if (++spin > kSpinLimit || ((spin & 255) == 0 && ld_dev_i(sy.err))) bad = 1;
-            if (got != (1 << kSlotPoll) - 1) __builtin_amdgcn_s_sleep(1);
+            if (got != (1 << kSlotPoll32) - 1) __builtin_amdgcn_s_sleep(1);
         }
         float g = 0.0f;
 #pragma unroll
-        for (int q = 0; q < kSlotPoll; q++) g = v[q] > g ? v[q] : g;
+        for (int q = 0; q < kSlotPoll32; q++) g = v[q] > g ? v[q] : g;
         for (int o = kWave / 2; o > 0; o >>= 1) {
             const float w = __shfl_xor(g, o);
             g = w > g ? w : g;
@@ -811,7 +816,7 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
         }
     }
     __syncthreads();
-    for (int e = tid; e < tsz; e += kRes32Threads) {
+    for (int e = tid; e < tsz; e += NT) {
         const int r = e / nzh, h = e - r * nzh;
         const int li = r / TJ, lj = r - li * TJ;
         const int i = i0 + li, j = j0 + lj;
@@ -827,15 +832,17 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
     }
 #ifdef HV_RES_TRACE
     if (t == 0 && sy.trace)
-        for (int q = tid; q < 256; q += kRes32Threads) sy.trace[q] = s_trace[q];
+        for (int q = tid; q < 256; q += NT) sy.trace[q] = s_trace[q];
 #endif
 }
 
-template <int CTI, int CTJ, int CNZH>
-__global__ void __launch_bounds__(kRes32Threads)
+// NT threads per block: 1024 with one tile per CU, 512 with two (a tile's
+// halo wait then overlaps the other tile's sweeps)
+template <int CTI, int CTJ, int CNZH, int NT = kRes32Threads>
+__global__ void __launch_bounds__(NT, 4)  // 4 waves per SIMD: two 512-thread blocks per CU
 k_sor_resident_f32(float *__restrict__ delta, const float *__restrict__ R, Split sp, ResTiles tl, SorCoef cf,
                    int max_it, int lag, ResSync32 sy) {
-    res32_solve_tile<CTI, CTJ, CNZH>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
+    res32_solve_tile<CTI, CTJ, CNZH, NT>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
 }
 
 }  // namespace hv

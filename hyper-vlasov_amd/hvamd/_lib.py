@@ -50,6 +50,7 @@ HV_OPT_SMEAR_TILE = 5
 HV_OPT_BATCH_GROUP_KIB = 6
 HV_OPT_SMEAR_FUSED = 7
 HV_OPT_BATCH_GRAPH = 8
+HV_OPT_SOR_F32_TILES_PER_CU = 9
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1

@@ -252,6 +252,11 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * fold) instead of launching its kernels one by one (default 1; off while
  * stages are timed).  Same bits either way. */
 #define HV_OPT_BATCH_GRAPH 8
+/* HV_OPT_SOR_F32_TILES_PER_CU: LDS-resident FP32 correction tiles per CU of a
+ * single grid: 1 (1024-thread blocks) or 2 (512-thread blocks, half-size
+ * tiles; a tile's wait for its neighbours' faces overlaps the other tile's
+ * sweeps).  Same bits either way. */
+#define HV_OPT_SOR_F32_TILES_PER_CU 9
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 

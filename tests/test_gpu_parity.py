@@ -394,13 +394,14 @@ def test_sor_fused_equals_per_sweep(oracle, n, d):
     _eq(outs[0][2], outs[1][2])
 
 
-@pytest.mark.parametrize("n", [(48, 48, 48), (40, 36, 50), (64, 64, 64), (96, 96, 96)])
-def test_resident_f32_equals_sweeps_at_fixed_iterations(n):
+@pytest.mark.parametrize("per_cu", [1, 2])
+@pytest.mark.parametrize("n", [(48, 48, 48), (40, 36, 50), (64, 64, 64), (96, 96, 96), (128, 128, 128)])
+def test_resident_f32_equals_sweeps_at_fixed_iterations(n, per_cu):
     """With a tolerance no iteration reaches, the FP32 correction runs exactly
     max_it iterations in both forms, so the LDS-resident kernel (its tiles,
     row sets and lane mappings) and the per-sweep launches must give the same
     potential bit for bit."""
-    print("resident FP32 vs sweeps, fixed iterations", n, flush=True)
+    print("resident FP32 vs sweeps, fixed iterations", n, "tiles per CU", per_cu, flush=True)
     d = tuple(0.2 * x for x in n)
     G = int(np.prod(n))
     rho_p = smooth_field(16, n, 0.09)
@@ -408,6 +409,7 @@ def test_resident_f32_equals_sweeps_at_fixed_iterations(n):
     for res32 in (0, 1):
         sim = _sim(n, d, 10, 10, hvamd.HV_PREC_MIXED)
         sim.set_option(hvamd.HV_OPT_SOR_RESIDENT_F32, res32)
+        sim.set_option(hvamd.HV_OPT_SOR_F32_TILES_PER_CU, per_cu)
         sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p, np.zeros(G)]))
         sim.set_initial_coulomb_boundaries(82)
         sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 512)  # cold: FP64
