@@ -864,12 +864,19 @@ int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int 
     for (int s = 0; s < nsteps; s++) {
         const int step = first + s;
         bool dirty = false;
+        {
+            // sorting counts in the particle stage's time
+            BStage st(b, HV_KCLASS_PARTICLE);
+            for (int i = 0; i < n; i++) {
+                hv_ctx *c = b->inst[i];
+                void *before = c->part[0];
+                rc = maybe_sort<T>(c);
+                if (rc) return batch_fwd(b, c, rc);
+                if (c->part[0] != before) dirty = true;
+            }
+        }
         for (int i = 0; i < n; i++) {
             hv_ctx *c = b->inst[i];
-            void *before = c->part[0];
-            rc = maybe_sort<T>(c);
-            if (rc) return batch_fwd(b, c, rc);
-            if (c->part[0] != before) dirty = true;
             int bo = 0;
             if (step * step_dt(prm[i]) >= prm[i].t_exc && !c->excited) {
                 c->excited = 1;

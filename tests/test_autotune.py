@@ -1,7 +1,7 @@
-"""The ensemble autotuner's search over a stand-in batch whose step time is a
-known function of the knobs (CPU only)."""
+"""The ensemble autotuner's search over a stand-in batch whose stage times
+are known functions of the knobs, with a SOR stage that grows along the run
+(which must not sway the choice).  CPU only."""
 import sys
-import time
 from pathlib import Path
 
 sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "hyper-vlasov_amd"))
@@ -11,31 +11,55 @@ from hvamd.autotune import tune_batch  # noqa: E402
 
 
 class StandInBatch:
-    """Records options and steps; a step sleeps for a time that is smallest at
-    sort 8, tile 32, fused 0."""
+    """Smearing is fastest at tile 32, two-pass; the particle stage is
+    fastest with sorting every 8 steps (a sort costs 5, an unsorted pass 3, a
+    sorted one 1); the SOR stage grows with the step number."""
 
     def __init__(self):
         self.opt = {}
         self.steps = []
+        self.prof = None
+        self.since_sort = 1 << 30
 
     def set_option(self, option, value):
         self.opt[option] = value
+        if option == HV_OPT_SORT_EVERY:
+            self.since_sort = 1 << 30
+
+    def profile_enable(self, on):
+        self.prof = {"smear": [0.0, 0], "particle_pass": [0.0, 0], "sor_stage": [0.0, 0]} if on else None
+
+    def profile_read(self):
+        return {k: tuple(v) for k, v in self.prof.items()}
 
     def run_steps(self, skyrmes, params, first, n):
         self.steps.append((first, n))
-        cost = (abs(self.opt[HV_OPT_SORT_EVERY] - 8) + abs(self.opt[HV_OPT_SMEAR_TILE] - 32) / 8 +
-                3 * self.opt[HV_OPT_SMEAR_FUSED])
-        time.sleep(n * (0.002 + 0.004 * cost))
+        for s in range(first, first + n):
+            sort_every = self.opt.get(HV_OPT_SORT_EVERY, 0)
+            part = 3.0
+            if sort_every > 0:
+                part = 1.0
+                if self.since_sort >= sort_every:
+                    part += 5.0
+                    self.since_sort = 0
+                self.since_sort += 1
+            smear = abs(self.opt[HV_OPT_SMEAR_TILE] - 32) / 8 + 2 * self.opt[HV_OPT_SMEAR_FUSED] + 1
+            if self.prof is not None:
+                for k, v in (("smear", smear), ("particle_pass", part), ("sor_stage", 100.0 + 10 * s)):
+                    self.prof[k][0] += v
+                    self.prof[k][1] += 1
 
 
-def test_tune_batch_picks_fastest_and_leaves_it_set():
+def test_tune_batch_picks_fastest_stages_and_leaves_them_set():
     b = StandInBatch()
-    best, timings, nxt = tune_batch(b, [], [], 5, steps_per_trial=1, sort_choices=(0, 8), tile_choices=(16, 32),
+    best, timings, nxt = tune_batch(b, [], [], 5, steps_per_trial=2, sort_choices=(0, 8), tile_choices=(16, 32),
                                     form_choices=(0, 1))
     assert best == {"sort_every": 8, "smear_tile": 32, "smear_fused": 0}
     assert b.opt == {HV_OPT_SORT_EVERY: 8, HV_OPT_SMEAR_TILE: 32, HV_OPT_SMEAR_FUSED: 0}
     assert set(timings) == {(s, t, f) for s in (0, 8) for t in (16, 32) for f in (0, 1)}
-    # each trial: one untimed step then one timed step, consecutive step numbers
-    assert nxt == 5 + 8 * 2
+    # smear trials 4 x (1 + 2) steps, then sort trials (1 + 2) and (1 + 8)
+    assert nxt == 5 + 4 * 3 + 3 + 9
     firsts = [f for f, _ in b.steps]
-    assert firsts == list(range(5, 5 + 16))
+    assert firsts == sorted(firsts) and firsts[0] == 5
+    # the sort-8 trial's cost includes one sort over its 8 steps
+    assert abs(timings[(8, 32, 0)] - (1.0 + 5.0 / 8 + 1.0)) < 1e-12
