@@ -21,6 +21,7 @@
 #include "hv_abi.h"
 #include "hv_kernels.h"
 #include "hv_sor_resident.h"
+#include "hv_sor_owned.h"
 
 #define HV_XSTR(x) HV_STR(x)
 #define HV_STR(x) #x
@@ -110,6 +111,8 @@ struct hv_ctx {
     int sor_res32 = 1;        // option: use it when the grid fits
     int res32_lag = 2;        // option: iterations between a max and its convergence test
     int res32_per_cu = 1;     // option: LDS-resident FP32 tiles per CU (1 or 2)
+    int res32_owned = 1;      // option: the lane-owned body where the tiling fits it
+    int res32_is_owned = 0;   // the chosen kernel is the lane-owned body
     int res32_threads = 1024;
     int res32_state = 0;
     ResTiles rtl32{};
@@ -589,6 +592,22 @@ int res32_setup(hv_ctx *c) {
                         (long long)best.PI * best.PJ <= (long long)occ * ncu;
         if (!ok) c->res32_kernel = generic;
     }
+    // the lane-owned body where the tiling fits it (one tile per CU)
+    if (per == 1 && c->res32_owned && owned_fits<HV_OWNED_SOLO, kRes32Threads>(best)) {
+        const void *k = (const void *)k_sor_resident_f32o<HV_OWNED_SOLO>;
+        const size_t l = OwnedLayout<kOwnedSoloP, kOwnedSoloQJ, kOwnedSoloIM>::bytes(kRes32Threads / kWave);
+        int occ = 0, dev = 0, ncu = 0;
+        const bool ok = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l) == hipSuccess &&
+                        hipGetDevice(&dev) == hipSuccess &&
+                        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+                        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, nt, l) == hipSuccess &&
+                        (long long)best.PI * best.PJ <= (long long)occ * ncu;
+        if (ok) {
+            c->res32_kernel = k;
+            lds = l;
+        }
+    }
+    c->res32_is_owned = c->res32_kernel == (const void *)k_sor_resident_f32o<HV_OWNED_SOLO>;
     c->res32_threads = nt;
     const size_t nT = (size_t)best.PI * best.PJ;
     if (hipMalloc((void **)&c->res32_x, sizeof(unsigned long long) * 2 * nT * best.xstride) != hipSuccess) return 1;
@@ -1734,6 +1753,24 @@ int hv_timer_stop(hv_ctx *c, double *ms) {
     return HV_OK;
 }
 
+// drops the FP32 resident form's tiling and buffers: chosen again at the
+// next solve
+int res32_reset(hv_ctx *c) {
+    HV_HIP(c, hipStreamSynchronize(c->stream));
+    void *bufs[] = {c->res32_x, c->res32_err, c->res32_result, c->res32_slots};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->res32_hres) (void)hipHostFree(c->res32_hres);
+    c->res32_x = nullptr;
+    c->res32_err = nullptr;
+    c->res32_result = nullptr;
+    c->res32_slots = nullptr;
+    c->res32_slots_cap = 0;
+    c->res32_hres = nullptr;
+    c->res32_state = 0;
+    return HV_OK;
+}
+
 int hv_set_option(hv_ctx *c, int option, int value) {
     if (!c) return HV_ERR_ARG;
     switch (option) {
@@ -1763,19 +1800,16 @@ int hv_set_option(hv_ctx *c, int option, int value) {
     case HV_OPT_SOR_F32_TILES_PER_CU:
         if (value != 1 && value != 2) return set_err(c, HV_ERR_ARG, "FP32 resident tiles per CU must be 1 or 2");
         if (value != c->res32_per_cu) {
-            HV_HIP(c, hipStreamSynchronize(c->stream));
-            void *bufs[] = {c->res32_x, c->res32_err, c->res32_result, c->res32_slots};
-            for (void *b : bufs)
-                if (b) (void)hipFree(b);
-            if (c->res32_hres) (void)hipHostFree(c->res32_hres);
-            c->res32_x = nullptr;
-            c->res32_err = nullptr;
-            c->res32_result = nullptr;
-            c->res32_slots = nullptr;
-            c->res32_slots_cap = 0;
-            c->res32_hres = nullptr;
-            c->res32_state = 0;  // tiles chosen again at the next solve
+            int rc = res32_reset(c);
+            if (rc) return rc;
             c->res32_per_cu = value;
+        }
+        return HV_OK;
+    case HV_OPT_SOR_F32_OWNED:
+        if ((value ? 1 : 0) != c->res32_owned) {
+            int rc = res32_reset(c);
+            if (rc) return rc;
+            c->res32_owned = value ? 1 : 0;
         }
         return HV_OK;
     case HV_OPT_SOR_F32_LAG:
@@ -1797,6 +1831,9 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
     case HV_OPT_SOR_RESIDENT_F32: *value = c->sor_res32 && c->res32_state >= 0; return HV_OK;
     case HV_OPT_SOR_F32_LAG: *value = c->res32_lag; return HV_OK;
     case HV_OPT_SOR_F32_TILES_PER_CU: *value = c->res32_per_cu; return HV_OK;
+    case HV_OPT_SOR_F32_OWNED:  // once tiled: whether the lane-owned body runs
+        *value = c->res32_state > 0 ? c->res32_is_owned : c->res32_owned;
+        return HV_OK;
     case HV_OPT_SMEAR_TILE: *value = c->smear_tk_max; return HV_OK;
     case HV_OPT_SMEAR_FUSED: *value = c->smear_fused; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);

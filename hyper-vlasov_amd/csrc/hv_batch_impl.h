@@ -47,6 +47,8 @@ struct hv_batch {
     int group_kib = 0;  // SOR working-set budget (KiB) of one instance group (0: all instances together)
     // LDS-resident FP32 correction of the whole batch (mixed precision)
     int res32 = 1;                        // option: use it when every instance tiles
+    int res32_owned = 1;                  // option: the lane-owned body when every instance fits it
+    int res32_is_owned = 0;               // the chosen kernel is the lane-owned body
     int res32_lag = 2;                    // option: lag of its convergence test
     int res_state = 0;                    // 0: not set up, 1: ready, -1: unavailable
     int res_max_it = 0;
@@ -459,6 +461,36 @@ int batch_res_tiles(const hv_ctx *c, size_t maxlds, int maxblk, ResTiles *out) {
     return best_cost < 0 ? 1 : HV_OK;
 }
 
+// Tiling of one instance for the lane-owned ensemble body: the fewest tiles
+// that fit it.
+int batch_owned_tiles(const hv_ctx *c, int maxblk, ResTiles *out) {
+    const int nx = c->world.n[0], ny = c->world.n[1];
+    long long best_cost = -1;
+    for (int PI = 1; PI <= nx; PI++) {
+        for (int PJ = 1; PJ <= ny; PJ++) {
+            ResTiles t;
+            t.TI = (nx + PI - 1) / PI;
+            t.TJ = (ny + PJ - 1) / PJ;
+            t.PI = (nx + t.TI - 1) / t.TI;
+            t.PJ = (ny + t.TJ - 1) / t.TJ;
+            if (t.PI != PI || t.PJ != PJ) continue;
+            const int nT = t.PI * t.PJ;
+            if (nT > maxblk || nT > slot_poll32<kRes32Threads>() * kWave) continue;
+            t.nzh = c->sp.nzh;
+            t.side_i = t.TJ * t.nzh;
+            t.side_j = t.TI * t.nzh;
+            t.xstride = 2 * t.side_i + 2 * t.side_j;
+            if (!owned_fits<HV_OWNED_BATCH, kRes32Threads>(t)) continue;
+            const long long cost = (long long)nT * 4096 + t.TI + t.TJ;
+            if (best_cost < 0 || cost < best_cost) {
+                best_cost = cost;
+                *out = t;
+            }
+        }
+    }
+    return best_cost < 0 ? 1 : HV_OK;
+}
+
 // job table layout: job_first[nblk + 1], then the (instance, tile) pairs from
 // an 8-byte aligned offset
 inline size_t res32_jobs_at(int nblk) { return ((size_t)nblk + 2) & ~(size_t)1; }
@@ -480,21 +512,33 @@ int batch_res32_setup(hv_batch *b, int max_it) {
     b->rx_off.assign(n + 1, 0);
     b->rslot_off.assign(n + 1, 0);
     size_t lds = 0;
-    double share[kRes32NShapes + 1] = {};  // nodes per compile-time shape
+    const void *kern = nullptr;
+    // the lane-owned body when every instance fits it
+    bool owned = b->res32_owned != 0;
+    for (int i = 0; i < n && owned; i++) owned = batch_owned_tiles(b->inst[i], ncu, &b->rtl[i]) == HV_OK;
+    b->res32_is_owned = owned ? 1 : 0;
+    if (owned) {
+        kern = (const void *)kb_sor_resident_f32o<HV_OWNED_BATCH>;
+        lds = OwnedLayout<kOwnedBatchP, kOwnedBatchQJ, kOwnedBatchIM>::bytes(kRes32Threads / kWave);
+    } else {
+        double share[kRes32NShapes + 1] = {};  // nodes per compile-time shape
+        for (int i = 0; i < n; i++) {
+            if (batch_res_tiles(b->inst[i], (size_t)maxlds - 1024, ncu, &b->rtl[i]) != HV_OK) return 1;
+            const int sh = res32_shape_of(b->rtl[i]);
+            if (sh >= 0) share[sh] += (double)b->inst[i]->G;
+            const size_t l = res32_lds_bytes(b->rtl[i]);
+            lds = l > lds ? l : lds;
+        }
+        int best = -1;
+        for (int k = 0; k < kRes32NShapes; k++)
+            if (share[k] > 0.0 && (best < 0 || share[k] > share[best])) best = k;
+        kern = res32_batch_kernel(best);
+    }
     for (int i = 0; i < n; i++) {
-        if (batch_res_tiles(b->inst[i], (size_t)maxlds - 1024, ncu, &b->rtl[i]) != HV_OK) return 1;
-        const int sh = res32_shape_of(b->rtl[i]);
-        if (sh >= 0) share[sh] += (double)b->inst[i]->G;
         const size_t nT = (size_t)b->rtl[i].PI * b->rtl[i].PJ;
         b->rx_off[i + 1] = b->rx_off[i] + 2 * nT * (size_t)b->rtl[i].xstride;
         b->rslot_off[i + 1] = b->rslot_off[i] + (size_t)max_it * nT;
-        const size_t l = res32_lds_bytes(b->rtl[i]);
-        lds = l > lds ? l : lds;
     }
-    int best = -1;
-    for (int k = 0; k < kRes32NShapes; k++)
-        if (share[k] > 0.0 && (best < 0 || share[k] > share[best])) best = k;
-    const void *kern = res32_batch_kernel(best);
     if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return 1;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kRes32Threads, lds) != hipSuccess || per_cu < 1)
@@ -1017,6 +1061,22 @@ hv_ctx *hv_batch_instance(hv_batch *b, int i) {
     return b->inst[i];
 }
 
+int hv_batch_get_option(hv_batch *b, int option, int *value) {
+    if (!b || !value) return HV_ERR_ARG;
+    switch (option) {
+    case HV_OPT_SORT_EVERY: *value = b->sort_every; return HV_OK;
+    case HV_OPT_BATCH_GROUP_KIB: *value = b->group_kib; return HV_OK;
+    case HV_OPT_SOR_RESIDENT_F32: *value = b->res32 && b->res_state >= 0; return HV_OK;
+    case HV_OPT_SOR_F32_LAG: *value = b->res32_lag; return HV_OK;
+    case HV_OPT_SOR_F32_OWNED:  // once tiled: whether the lane-owned body runs
+        *value = b->res_state > 0 ? b->res32_is_owned : b->res32_owned;
+        return HV_OK;
+    case HV_OPT_SMEAR_FUSED: *value = b->smear_fused; return HV_OK;
+    case HV_OPT_BATCH_GRAPH: *value = b->use_graph; return HV_OK;
+    default: return batch_err(b, HV_ERR_ARG, "option %d is not a batch option", option);
+    }
+}
+
 int hv_batch_set_option(hv_batch *b, int option, int value) {
     if (!b) return HV_ERR_ARG;
     if (option == HV_OPT_BATCH_GROUP_KIB) {
@@ -1030,6 +1090,16 @@ int hv_batch_set_option(hv_batch *b, int option, int value) {
     }
     if (option == HV_OPT_SMEAR_FUSED) {
         b->smear_fused = value ? 1 : 0;
+        return HV_OK;
+    }
+    if (option == HV_OPT_SOR_F32_OWNED) {
+        if ((value ? 1 : 0) != b->res32_owned) {
+            // tilings and kernel chosen again at the next solve
+            if (b->stream) (void)hipStreamSynchronize(b->stream);
+            batch_drop_graph(b);
+            b->res_state = 0;
+            b->res32_owned = value ? 1 : 0;
+        }
         return HV_OK;
     }
     if (option == HV_OPT_BATCH_GRAPH) {

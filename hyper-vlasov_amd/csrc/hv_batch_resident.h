@@ -18,6 +18,7 @@
 // gets the bits of a single-instance LDS-resident solve with the same lag.
 #pragma once
 
+#include "hv_sor_owned.h"
 #include "hv_sor_resident.h"
 
 namespace hv {
@@ -54,6 +55,32 @@ kb_sor_resident_f32(const BRes32 *__restrict__ inst, const int *__restrict__ job
             res32_solve_tile<CTI, CTJ, CNZH>(I.delta, I.R, I.sp, I.tl, I.cf, I.max_it, lag, I.sy, jb.y);
         else
             res32_solve_tile<0, 0, 0>(I.delta, I.R, I.sp, I.tl, I.cf, I.max_it, lag, I.sy, jb.y);
+        __syncthreads();  // the tile's write-back has read the LDS the next job overwrites
+#ifdef HV_BRES_TRACE
+        if (threadIdx.x == 0) I.jtrace[2 * j + 1] = wall_clock64();
+#endif
+    }
+}
+
+// The same job loop with the lane-owned body <SI, SR, P, QJ, IM>; the host
+// launches it only when every instance's tiling fits that body (a tile that
+// does not is skipped with the error word set: its instance's other tiles
+// skip too, so nothing waits on it).
+template <int SI, int SR, int P, int QJ, int IM>
+__global__ void __launch_bounds__(kRes32Threads)
+kb_sor_resident_f32o(const BRes32 *__restrict__ inst, const int *__restrict__ job_first,
+                     const int2 *__restrict__ jobs, int lag) {
+    const int j1 = job_first[blockIdx.x + 1];
+    for (int j = job_first[blockIdx.x]; j < j1; j++) {
+        const int2 jb = jobs[j];
+        const BRes32 &I = inst[jb.x];
+#ifdef HV_BRES_TRACE
+        if (threadIdx.x == 0) I.jtrace[2 * j] = wall_clock64();
+#endif
+        if (owned_fits<SI, SR, P, QJ, IM, kRes32Threads>(I.tl))
+            res32o_solve_tile<SI, SR, P, QJ, IM>(I.delta, I.R, I.sp, I.tl, I.cf, I.max_it, lag, I.sy, jb.y);
+        else if (threadIdx.x == 0)
+            st_dev_i(I.sy.err, 1);
         __syncthreads();  // the tile's write-back has read the LDS the next job overwrites
 #ifdef HV_BRES_TRACE
         if (threadIdx.x == 0) I.jtrace[2 * j + 1] = wall_clock64();

@@ -1,0 +1,416 @@
+// hv_sor_owned.h -- the mixed form's FP32 correction iteration with every
+// node of a tile owned by a fixed lane.
+//
+// res32o_solve_tile runs the solve of res32_solve_tile (the same red-black
+// updates on the same operand values, the same half-sweep order, exchange
+// words and lagged stop test, so a tile solved by either body gives the same
+// bits), arranged so that a node update costs little index arithmetic:
+//  * LDS holds each colour's values of the tile with a one-row border on all
+//    four sides, where the neighbours' face values land: tile row (li, lj) is
+//    LDS row (li + 1) * QJ + lj + 1 of P slots.  A node then reads its six
+//    neighbours at fixed offsets, on the tile's faces as inside it.
+//  * The (row, slot) elements of each row set (interior rows, ring rows) are
+//    cut into 64-element chunks; chunk c is slot c / nwaves of wave
+//    c % nwaves.  Per slot a lane keeps one descriptor word (LDS offset, the
+//    k parity and update flag of each colour) and both colours' right-hand
+//    sides in registers, set up once per solve.
+#pragma once
+
+#include "hv_sor_resident.h"
+
+namespace hv {
+
+// LDS of the lane-owned body, in floats: the colour-0 array of (IM + 2) * QJ
+// rows of P slots, 64 sink words, the colour-1 array, 64 sink words and a
+// guard of QJ * P + 2 words, then the wave partials, the error flag and the
+// two tested maxima (no static LDS: the arrays start at address 0, which
+// the descriptors' offsets then are).  Lanes without a node (the tail of a
+// partial chunk) read around and store into the sinks.
+template <int P, int QJ, int IM>
+struct OwnedLayout {
+    static constexpr int E = (IM + 2) * QJ * P;  // one colour's array
+    static constexpr int ext1 = E + kWave;       // start of the colour-1 array
+    static constexpr int red = ext1 + E + kWave + QJ * P + 2;
+    static constexpr size_t bytes(int nwaves) { return sizeof(float) * ((size_t)red + (size_t)nwaves + 3); }
+};
+
+// does tiling t fit the lane-owned body <SI, SR, P, QJ, IM> of NT threads
+// (slots per lane: SI interior, SR ring)
+template <int SI, int SR, int P, int QJ, int IM, int NT>
+__host__ __device__ inline bool owned_fits(const ResTiles &t) {
+    if (t.TI > IM || t.TJ + 2 > QJ || t.nzh > P || t.TI < 1 || t.TJ < 1) return false;
+    if (t.xstride > kFetchMax * NT) return false;
+    const bool split_rows = t.TI >= 3 && t.TJ >= 3;
+    const int n_inner = split_rows ? (t.TI - 2) * (t.TJ - 2) : 0;
+    const int n_ring = t.TI * t.TJ - n_inner;
+    const int nw = NT / kWave;
+    const int ch_in = (n_inner * t.nzh + kWave - 1) / kWave, ch_rg = (n_ring * t.nzh + kWave - 1) / kWave;
+    // 16-bit-free descriptor offsets: the arrays stay below 2^18 bytes
+    return ch_in <= SI * nw && ch_rg <= SR * nw && OwnedLayout<P, QJ, IM>::red < (1 << 18);
+}
+
+typedef __attribute__((address_space(3))) float lds_float;
+
+template <int SI, int SR, int P, int QJ, int IM, int NT = kRes32Threads>
+__device__ __forceinline__ void res32o_solve_tile(float *__restrict__ delta, const float *__restrict__ R, Split sp,
+                                                  ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
+                                                  const int t) {
+    using L = OwnedLayout<P, QJ, IM>;
+    extern __shared__ unsigned char smem[];
+    float *const x0 = (float *)smem;  // colour-0 values (bordered rows)
+    float *const x1 = x0 + L::ext1;   // colour-1 values
+    float *const red = x0 + L::red;   // wave partials
+    constexpr int nwaves = NT / kWave;
+    constexpr int XS = QJ * P;  // distance of the i-neighbours
+    constexpr int S = SI + SR;
+    const int TI = tl.TI, TJ = tl.TJ, nzh = tl.nzh;
+    const int side_i = TJ * nzh, side_j = TI * nzh, xstride = 2 * side_i + 2 * side_j;
+    const int nx = sp.nx, ny = sp.ny, nz = sp.nz;
+    const int nT = tl.PI * tl.PJ;
+    const int pi = t / tl.PJ, pj = t - pi * tl.PJ;
+    const int i0 = pi * TI, j0 = pj * TJ;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const bool has_im = pi > 0, has_ip = pi < tl.PI - 1, has_jm = pj > 0, has_jp = pj < tl.PJ - 1;
+    const float yx = (float)cf.ydx2, yy = (float)cf.ydy2, yz = (float)cf.ydz2, inv = (float)cf.inv_dx2;
+    const float om = (float)cf.omega, om1 = (float)cf.one_minus_omega, tol = (float)cf.tol;
+    int &s_bad = *(int *)(red + nwaves);
+    float *const s_max = red + nwaves + 1;  // by iteration parity: written in iteration it, read at its end
+    if (tid == 0) s_bad = 0;
+
+    // the tile's values of both colours (rows outside the grid hold 0)
+    for (int e = tid; e < TI * TJ * nzh; e += NT) {
+        const int r = e / nzh, h = e - r * nzh;
+        const int li = r / TJ, lj = r - li * TJ;
+        const int i = i0 + li, j = j0 + lj;
+        float v0 = 0.0f, v1 = 0.0f;
+        if (i < nx && j < ny) {
+            const long long q = ((long long)i * ny + j) * nzh + h;
+            v0 = delta[q];
+            v1 = delta[sp.Gh + q];
+        }
+        const int w = ((li + 1) * QJ + lj + 1) * P + h;
+        x0[w] = v0;
+        x1[w] = v1;
+    }
+    // black values around the tile for the first red sweep
+    {
+        const float *src = delta + sp.Gh;
+        for (int e = tid; e < side_i; e += NT) {
+            const int lj = e / nzh, h = e - lj * nzh, j = j0 + lj;
+            if (j >= ny) continue;
+            if (has_im) x1[(lj + 1) * P + h] = src[((long long)(i0 - 1) * ny + j) * nzh + h];
+            if (has_ip) x1[((TI + 1) * QJ + lj + 1) * P + h] = src[((long long)(i0 + TI) * ny + j) * nzh + h];
+        }
+        for (int e = tid; e < side_j; e += NT) {
+            const int li = e / nzh, h = e - li * nzh, i = i0 + li;
+            if (i >= nx) continue;
+            if (has_jm) x1[(li + 1) * QJ * P + h] = src[((long long)i * ny + (j0 - 1)) * nzh + h];
+            if (has_jp) x1[((li + 1) * QJ + TJ + 1) * P + h] = src[((long long)i * ny + (j0 + TJ)) * nzh + h];
+        }
+    }
+
+    // Row sets: the ring of rows on the tile's faces (the only rows that read
+    // border values) and the interior rows.  Tiles thinner than 3 rows in i
+    // or j have no interior; their ring is every row.
+    const bool split_rows = TI >= 3 && TJ >= 3;
+    const int n_inner = split_rows ? (TI - 2) * (TJ - 2) : 0;
+    const int nel_in = n_inner * nzh, nel_rg = (TI * TJ - n_inner) * nzh;
+    const int nch_in = (nel_in + kWave - 1) / kWave, nch_rg = (nel_rg + kWave - 1) / kWave;
+    // slots this wave holds in each set (wave-uniform)
+    const int ns_in = nch_in > wave ? (nch_in - wave + nwaves - 1) / nwaves : 0;
+    const int ns_rg = nch_rg > wave ? (nch_rg - wave + nwaves - 1) / nwaves : 0;
+
+    // Descriptor of a slot: bits 0-19 the LDS address of the node's
+    // i-minus neighbour in the colour-0 array, bit 20 + c the k parity of colour c (its
+    // k neighbours are slots h + parity - 1 and h + parity of the other
+    // colour), bit 22 + c set when colour c's node is updated (not on the
+    // grid's faces, not outside the grid, not a sink).
+    const unsigned lds0 = (unsigned)(size_t)(lds_float *)x0;
+    unsigned D[S];
+    float R0[S], R1[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const bool inner = s < SI;
+        const int ss = inner ? s : s - SI;
+        const int f = (ss * nwaves + wave) * kWave + lane;
+        unsigned d = lds0 + 4u * (unsigned)(L::E + lane - XS);  // sink
+        float r0 = 0.0f, r1 = 0.0f;
+        if (f < (inner ? nel_in : nel_rg)) {
+            const int q = f / nzh, h = f - q * nzh;
+            int li, lj;
+            if (inner) {
+                li = 1 + q / (TJ - 2);
+                lj = 1 + q - (li - 1) * (TJ - 2);
+            } else if (!split_rows) {
+                li = q / TJ;
+                lj = q - li * TJ;
+            } else if (q < TJ) {
+                li = 0;
+                lj = q;
+            } else if (q < 2 * TJ) {
+                li = TI - 1;
+                lj = q - TJ;
+            } else {
+                li = 1 + ((q - 2 * TJ) >> 1);
+                lj = ((q - 2 * TJ) & 1) ? TJ - 1 : 0;
+            }
+            const int i = i0 + li, j = j0 + lj;
+            const unsigned o0 = (unsigned)(i + j) & 1u, o1 = o0 ^ 1u;
+            const int k0 = 2 * h + (int)o0, k1 = 2 * h + (int)o1;
+            const bool okij = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+            const unsigned ok0 = okij && k0 >= 1 && k0 <= nz - 2, ok1 = okij && k1 >= 1 && k1 <= nz - 2;
+            const int w = ((li + 1) * QJ + lj + 1) * P + h;
+            d = (lds0 + 4u * (unsigned)(w - XS)) | o0 << 20 | o1 << 21 | ok0 << 22 | ok1 << 23;
+            if (i < nx && j < ny) {
+                const long long g = ((long long)i * ny + j) * nzh + h;
+                r0 = R[g];
+                r1 = R[sp.Gh + g];
+            }
+        }
+        D[s] = d;
+        R0[s] = r0;
+        R1[s] = r1;
+    }
+
+    // this thread's exchange words (at most kFetchMax; the host checks that
+    // xstride fits): word x = tid + q * NT of the tile's slice, the byte
+    // offset of its counterpart in the exchange array (bit q of fmask: there
+    // is a neighbour), the border slot it fills and the face slot it
+    // publishes.  32-bit offsets from the array bases keep them one register
+    // each.
+    unsigned src_b[kFetchMax], fmask = 0;
+    int fdst[kFetchMax], psrc[kFetchMax];
+    int nfetch = 0;
+    for (int x = tid; x < xstride && nfetch < kFetchMax; x += NT) {
+        int off = -1, fd, ps;
+        if (x < side_i) {
+            const int lj = x / nzh, h = x - lj * nzh;
+            if (has_im) off = (t - tl.PJ) * xstride + side_i + x;  // i-high face of (pi-1, pj)
+            fd = (lj + 1) * P + h;
+            ps = (QJ + lj + 1) * P + h;
+        } else if (x < 2 * side_i) {
+            const int y = x - side_i, lj = y / nzh, h = y - lj * nzh;
+            if (has_ip) off = (t + tl.PJ) * xstride + y;  // i-low face of (pi+1, pj)
+            fd = ((TI + 1) * QJ + lj + 1) * P + h;
+            ps = (TI * QJ + lj + 1) * P + h;
+        } else if (x < 2 * side_i + side_j) {
+            const int y = x - 2 * side_i, li = y / nzh, h = y - li * nzh;
+            if (has_jm) off = (t - 1) * xstride + 2 * side_i + side_j + y;
+            fd = (li + 1) * QJ * P + h;
+            ps = ((li + 1) * QJ + 1) * P + h;
+        } else {
+            const int y = x - 2 * side_i - side_j, li = y / nzh, h = y - li * nzh;
+            if (has_jp) off = (t + 1) * xstride + 2 * side_i + y;
+            fd = ((li + 1) * QJ + TJ + 1) * P + h;
+            ps = ((li + 1) * QJ + TJ) * P + h;
+        }
+        src_b[nfetch] = off >= 0 ? 8u * (unsigned)off : 0u;
+        if (off >= 0) fmask |= 1u << nfetch;
+        fdst[nfetch] = fd;
+        psrc[nfetch] = ps;
+        nfetch++;
+    }
+    __syncthreads();
+
+    // the update of k_sor_colour_f32 over this lane's slots of one row set:
+    // colour C reads the other colour's array and writes its own
+    auto sweep = [&](auto cc, auto inn, float m) -> float {
+        constexpr int C = decltype(cc)::value;
+        constexpr bool INNER = decltype(inn)::value;
+        constexpr int B = INNER ? 0 : SI, NS = INNER ? SI : SR;
+        const int ns = INNER ? ns_in : ns_rg;
+        constexpr unsigned OB = C ? 0 : sizeof(float) * L::ext1, DB = C ? sizeof(float) * L::ext1 : 0;
+        // addresses and flags are derived from the descriptors in every sweep
+        // rather than held across sweeps (registers for the slots)
+#pragma unroll
+        for (int u = 0; u < NS; u++) asm volatile("" : "+v"(D[B + u]));
+#pragma unroll
+        for (int u = 0; u < NS; u++) {
+            if (u >= ns) break;
+            const int s = B + u;
+            const unsigned d = D[s];
+            const unsigned a = d & 0xfffffu;
+            const lds_float *o = (const lds_float *)(size_t)(a + OB);
+            const float xm = o[0], ym = o[XS - P], yp = o[XS + P], xp = o[2 * XS];
+            const lds_float *oz = o + ((d >> (20 + C)) & 1u);
+            const float zm = oz[XS - 1], zp = oz[XS];
+            const float sv = (xp + xm) * yx + (yp + ym) * yy + (zp + zm) * yz + (C ? R1[s] : R0[s]);
+            lds_float *dv = (lds_float *)(size_t)(a + DB + sizeof(float) * XS);
+            const float po = *dv;
+            float nv = om1 * po + om * (sv * inv);
+            nv = ((d >> (22 + C)) & 1u) ? nv : po;
+            *dv = nv;
+            m = fmaxf(m, fabsf(nv - po));
+        }
+        return m;
+    };
+    using C0 = std::integral_constant<int, 0>;
+    using C1 = std::integral_constant<int, 1>;
+    using In = std::integral_constant<bool, true>;
+    using Rg = std::integral_constant<bool, false>;
+
+    // store the tile's colour faces as tagged words (no completion wait: each
+    // word carries its own tag)
+    // (the opaque copies keep the compiler from holding 64-bit addresses of
+    // every exchange word across the iteration loop)
+    auto publish = [&](const float *xc, unsigned long long *xw, unsigned tag) {
+        unsigned b = 8u * (unsigned)(t * xstride + tid);
+        asm volatile("" : "+v"(b));
+#pragma unroll
+        for (int q = 0; q < kFetchMax; q++)
+            if (q < nfetch) st_tagged((unsigned long long *)((char *)xw + (b + 8u * q * NT)), xc[psrc[q]], tag);
+    };
+
+    // poll this thread's neighbour words until they carry `tag` and put them
+    // in the border of array xc; all loads of a round are in flight together.
+    // Returns nonzero (block-uniform) on error.
+    auto fetch = [&](float *xc, const unsigned long long *xs, unsigned tag) -> int {
+        unsigned pending = fmask;
+        unsigned sb[kFetchMax];
+#pragma unroll
+        for (int q = 0; q < kFetchMax; q++) {
+            sb[q] = src_b[q];
+            asm volatile("" : "+v"(sb[q]));
+        }
+        long long spin = 0;
+        int bad = 0;
+        while (pending && !bad) {
+            unsigned long long w[kFetchMax];
+#pragma unroll
+            for (int q = 0; q < kFetchMax; q++)
+                if (pending & (1u << q)) w[q] = ld_tagged((const unsigned long long *)((const char *)xs + sb[q]));
+#pragma unroll
+            for (int q = 0; q < kFetchMax; q++) {
+                if ((pending & (1u << q)) && (unsigned)(w[q] >> 32) >= tag) {
+                    xc[fdst[q]] = __uint_as_float((unsigned)w[q]);
+                    pending &= ~(1u << q);
+                }
+            }
+            if (++spin > kSpinLimit) bad = 1;
+            else if ((spin & 255) == 0 && ld_dev_i(sy.err)) bad = 1;
+            // a wave still waiting yields its issue slots to the waves of
+            // its SIMD that have work
+            if (pending) __builtin_amdgcn_s_sleep(1);
+        }
+        if (bad) {
+            st_dev_i(sy.err, 1);
+            s_bad = 1;
+        }
+        __syncthreads();
+        return s_bad;
+    };
+
+    // wave 0: max over all tiles of the write-once slot row `row_it`
+    auto poll_slots = [&](int row_it, int parity) {
+        constexpr int kSlotPoll32 = slot_poll32<NT>();
+        float v[kSlotPoll32];
+        int got = 0, bad = 0;
+        const float *row = sy.slots + (long long)row_it * nT;
+#pragma unroll
+        for (int q = 0; q < kSlotPoll32; q++) {
+            v[q] = 0.0f;
+            if (lane + q * kWave >= nT) got |= 1 << q;
+        }
+        long long spin = 0;
+        while (!__all(got == (1 << kSlotPoll32) - 1) && !bad) {
+#pragma unroll
+            for (int q = 0; q < kSlotPoll32; q++)
+                if (!(got & (1 << q))) v[q] = ld_dev_f(row + lane + q * kWave);
+#pragma unroll
+            for (int q = 0; q < kSlotPoll32; q++)
+                if (!(got & (1 << q)) && v[q] >= 0.0f) got |= 1 << q;
+            if (++spin > kSpinLimit || ((spin & 255) == 0 && ld_dev_i(sy.err))) bad = 1;
+            if (got != (1 << kSlotPoll32) - 1) __builtin_amdgcn_s_sleep(1);
+        }
+        float g = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kSlotPoll32; q++) g = v[q] > g ? v[q] : g;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const float w = __shfl_xor(g, o);
+            g = w > g ? w : g;
+        }
+        bad = __any(bad);
+        if (lane == 0) {
+            if (bad) {
+                st_dev_i(sy.err, 1);
+                s_bad = 1;
+            }
+            s_max[parity] = g;
+        }
+    };
+
+    // Iteration it (as res32_solve_tile):
+    //   red interior -> [black border of it-1] -> red ring -> publish red
+    //   -> black interior -> [red border of it] -> black ring -> publish black
+    int iters = max_it;
+    float tested = -1.0f;
+    for (int it = 0; it < max_it; it++) {
+        const unsigned tag = (unsigned)it + 1u;
+        float m = sweep(C0{}, In{}, 0.0f);
+        if (it >= lag && wave == 0) poll_slots(it - lag, it & 1);
+        if (it > 0) {
+            if (fetch(x1, sy.xb, tag - 1u)) break;
+        } else {
+            __syncthreads();
+        }
+        m = sweep(C0{}, Rg{}, m);
+        __syncthreads();
+        publish(x0, sy.xr, tag);
+        m = sweep(C1{}, In{}, m);
+        if (fetch(x0, sy.xr, tag)) break;
+        m = sweep(C1{}, Rg{}, m);
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const float w = __shfl_xor(m, o);
+            m = w > m ? w : m;
+        }
+        if (lane == 0) red[wave] = m;
+        __syncthreads();
+        publish(x1, sy.xb, tag);
+        if (tid == 0) {
+            float bm = 0.0f;
+            for (int w = 0; w < nwaves; w++) bm = red[w] > bm ? red[w] : bm;
+            st_dev_f(sy.slots + (long long)it * nT + t, bm);
+        }
+        if (it >= lag) {
+            tested = s_max[it & 1];
+            if (tested < tol) {
+                iters = it + 1;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < TI * TJ * nzh; e += NT) {
+        const int r = e / nzh, h = e - r * nzh;
+        const int li = r / TJ, lj = r - li * TJ;
+        const int i = i0 + li, j = j0 + lj;
+        if (i < nx && j < ny) {
+            const long long q = ((long long)i * ny + j) * nzh + h;
+            const int w = ((li + 1) * QJ + lj + 1) * P + h;
+            delta[q] = x0[w];
+            delta[sp.Gh + q] = x1[w];
+        }
+    }
+    if (t == 0 && tid == 0) {
+        sy.result[0] = (float)iters;
+        sy.result[1] = tested;
+    }
+}
+
+// lane-owned shapes: the ensemble kernel's (tiles up to 16 x 12 rows of 48
+// slots: 96^3 grids and smaller) and the single-grid kernel's (8 x 8 rows of
+// 64 slots: 128^3 on 256 CUs)
+#define HV_OWNED_BATCH 7, 3, 48, 14, 16
+#define HV_OWNED_SOLO 3, 2, 64, 10, 8
+constexpr int kOwnedBatchP = 48, kOwnedBatchQJ = 14, kOwnedBatchIM = 16;
+constexpr int kOwnedSoloP = 64, kOwnedSoloQJ = 10, kOwnedSoloIM = 8;
+
+template <int SI, int SR, int P, int QJ, int IM>
+__global__ void __launch_bounds__(kRes32Threads)
+k_sor_resident_f32o(float *__restrict__ delta, const float *__restrict__ R, Split sp, ResTiles tl, SorCoef cf,
+                    int max_it, int lag, ResSync32 sy) {
+    res32o_solve_tile<SI, SR, P, QJ, IM>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
+}
+
+}  // namespace hv

@@ -51,6 +51,7 @@ HV_OPT_BATCH_GROUP_KIB = 6
 HV_OPT_SMEAR_FUSED = 7
 HV_OPT_BATCH_GRAPH = 8
 HV_OPT_SOR_F32_TILES_PER_CU = 9
+HV_OPT_SOR_F32_OWNED = 10
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1
@@ -148,6 +149,7 @@ SIGNATURES = {
     "hv_batch_size": (_I, [_P]),
     "hv_batch_instance": (_P, [_P, _I]),
     "hv_batch_set_option": (_I, [_P, _I, _I]),
+    "hv_batch_get_option": (_I, [_P, _I, _IP]),
     "hv_batch_run_init": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _IP]),
     "hv_batch_run_steps": (_I, [_P, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvParams), _I, _I, _P, _P]),
     "hv_batch_profile_enable": (_I, [_P, _I]),

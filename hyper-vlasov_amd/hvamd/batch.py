@@ -70,6 +70,11 @@ class Batch:
     def set_option(self, option: int, value: int) -> int:
         return self._rc("hv_batch_set_option", self.lib.hv_batch_set_option(self.b, option, value))
 
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int()
+        self._rc("hv_batch_get_option", self.lib.hv_batch_get_option(self.b, option, ctypes.byref(v)))
+        return v.value
+
     def _arrays(self, skyrmes: Sequence[Skyrme], params: Sequence[Params]):
         if len(skyrmes) != self.n or len(params) != self.n:
             raise ValueError("one Skyrme set and one Params per instance")

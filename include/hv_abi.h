@@ -257,6 +257,12 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * tiles; a tile's wait for its neighbours' faces overlaps the other tile's
  * sweeps).  Same bits either way. */
 #define HV_OPT_SOR_F32_TILES_PER_CU 9
+/* HV_OPT_SOR_F32_OWNED: run the LDS-resident FP32 correction with the
+ * lane-owned body where the tiling fits it (every node of a tile updated by a
+ * fixed lane from precomputed descriptors; tiles with a one-row border in
+ * LDS) instead of the body that derives each node's indexes in every sweep
+ * (default 1; for a batch: when every instance fits).  Same bits either way. */
+#define HV_OPT_SOR_F32_OWNED 10
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
@@ -307,8 +313,12 @@ int hv_batch_size(const hv_batch *batch);
 hv_ctx *hv_batch_instance(hv_batch *batch, int i);
 /* HV_OPT_SORT_EVERY or HV_OPT_SMEAR_TILE (applied to every instance), or
  * HV_OPT_BATCH_GROUP_KIB, HV_OPT_SOR_RESIDENT_F32, HV_OPT_SOR_F32_LAG,
- * HV_OPT_SMEAR_FUSED (default 1 for a batch), HV_OPT_BATCH_GRAPH */
+ * HV_OPT_SOR_F32_OWNED, HV_OPT_SMEAR_FUSED (default 1 for a batch),
+ * HV_OPT_BATCH_GRAPH */
 int hv_batch_set_option(hv_batch *batch, int option, int value);
+/* the value of a batch option (HV_OPT_SOR_RESIDENT_F32 and
+ * HV_OPT_SOR_F32_OWNED: whether that form runs, once the batch has solved) */
+int hv_batch_get_option(hv_batch *batch, int option, int *value);
 /* skyrme[n], params[n]; sor_iters[n] (may be NULL): the pre-loop solves */
 int hv_batch_run_init(hv_batch *batch, const hv_skyrme *skyrme, const hv_params *params, int *sor_iters);
 /* steps [first_step, first_step + nsteps) of every instance; stats (may be

@@ -26,9 +26,10 @@ def _instances():
     return ws, params
 
 
-def _solo(w, prm, prec, chunks, sort_every, resident=False):
+def _solo(w, prm, prec, chunks, sort_every, resident=False, owned=0):
     sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, prec)
     sim.set_option(hvamd.HV_OPT_SOR_RESIDENT_F32, 1 if resident else 0)
+    sim.set_option(hvamd.HV_OPT_SOR_F32_OWNED, owned)
     sim.set_option(hvamd.HV_OPT_SORT_EVERY, sort_every)
     sim.upload_particles(*w.initial_state())
     it0 = sim.run_init(w.sky(), prm)
@@ -40,6 +41,7 @@ def _solo(w, prm, prec, chunks, sort_every, resident=False):
         first += c
     if resident:
         assert sim.get_option(hvamd.HV_OPT_SOR_RESIDENT_F32) == 1
+        assert sim.get_option(hvamd.HV_OPT_SOR_F32_OWNED) == owned
     out = (it0, np.concatenate(stats), np.concatenate(its), sim.download_particles(),
            sim.total_energy(w.sky(), w.ppn))
     sim.close()
@@ -58,7 +60,8 @@ def _instances_res():
 @pytest.mark.parametrize("form", ["f64", "mixed_sweeps", "mixed_resident"])
 def test_batch_equals_solo_runs(form):
     """F64; mixed with per-sweep FP32 corrections; mixed with the ensemble
-    LDS-resident kernel against solo LDS-resident solves."""
+    LDS-resident kernel (lane-owned body) against solo LDS-resident solves
+    (the body that derives node indexes per sweep)."""
     print(f"batch vs solo form={form}", flush=True)
     prec = hvamd.HV_PREC_F64 if form == "f64" else hvamd.HV_PREC_MIXED
     resident = form == "mixed_resident"
@@ -83,8 +86,11 @@ def test_batch_equals_solo_runs(form):
     parts = [b.instance(i).download_particles() for i in range(len(ws))]
     energies = [b.instance(i).total_energy(ws[i].sky(), ws[i].ppn) for i in range(len(ws))]
     res_launches = b.profile_read()["sor_resident_f32"][1]
+    owned = b.get_option(hvamd.HV_OPT_SOR_F32_OWNED)
     b.close()
     assert res_launches == (sum(chunks) if resident else 0)
+    if resident:
+        assert owned == 1
     for i, w in enumerate(ws):
         s_it0, s_stats, s_its, s_part, s_e = _solo(w, params[i], prec, chunks, 2, resident)
         print(i, w.name, "iters", list(its[i]), "solo", list(s_its))
@@ -156,13 +162,15 @@ def test_batch_step_graph_replays_equal_eager_steps(prec):
             assert np.array_equal(pe[key], pg[key]), key
 
 
+@pytest.mark.parametrize("owned", [0, 1])
 @pytest.mark.parametrize("big", [96, 32])
-def test_batch_resident_shape_bodies_equal_solo(big):
-    """The ensemble kernel runs its compile-time tile body on the grid shape
-    with the most nodes (16 x 12 rows x 48 slots on 96^3, 32 x 16 x 16 on
-    32^3) and the run-time-shape body on the others: both against solo
-    LDS-resident solves, bit for bit."""
-    print(f"batch resident shapes big={big}", flush=True)
+def test_batch_resident_shape_bodies_equal_solo(big, owned):
+    """Without the lane-owned body the ensemble kernel runs its compile-time
+    tile body on the grid shape with the most nodes (16 x 12 rows x 48 slots
+    on 96^3, 32 x 16 x 16 on 32^3) and the run-time-shape body on the others;
+    with it, the lane-owned body on every tile: each against solo LDS-resident
+    solves with the other body, bit for bit."""
+    print(f"batch resident shapes big={big} owned={owned}", flush=True)
     ws, params = _instances()
     ws = ws[:2]
     params = params[:2]
@@ -170,15 +178,18 @@ def test_batch_resident_shape_bodies_equal_solo(big):
     ws[1].grid, ws[1].dx = 20, 0.8
     b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws], hvamd.HV_PREC_MIXED)
     b.set_option(hvamd.HV_OPT_SORT_EVERY, 2)
+    b.set_option(hvamd.HV_OPT_SOR_F32_OWNED, owned)
     for i, w in enumerate(ws):
         b.instance(i).upload_particles(*w.initial_state())
     sks = [w.sky() for w in ws]
     it0 = b.run_init(sks, params)
     s1, i1 = b.run_steps(sks, params, 0, 4)
     parts = [b.instance(i).download_particles() for i in range(len(ws))]
+    assert b.get_option(hvamd.HV_OPT_SOR_F32_OWNED) == owned
     b.close()
     for i, w in enumerate(ws):
-        s_it0, s_stats, s_its, s_part, _ = _solo(w, params[i], hvamd.HV_PREC_MIXED, [4], 2, resident=True)
+        s_it0, s_stats, s_its, s_part, _ = _solo(w, params[i], hvamd.HV_PREC_MIXED, [4], 2, resident=True,
+                                                 owned=1 - owned)
         print(i, w.grid, "iters", list(i1[i]), "solo", list(s_its), flush=True)
         assert it0[i] == s_it0
         assert np.array_equal(i1[i], s_its)

@@ -394,14 +394,17 @@ def test_sor_fused_equals_per_sweep(oracle, n, d):
     _eq(outs[0][2], outs[1][2])
 
 
-@pytest.mark.parametrize("per_cu", [1, 2])
+@pytest.mark.parametrize("form", ["rows", "rows_two_per_cu", "owned"])
 @pytest.mark.parametrize("n", [(48, 48, 48), (40, 36, 50), (64, 64, 64), (96, 96, 96), (128, 128, 128)])
-def test_resident_f32_equals_sweeps_at_fixed_iterations(n, per_cu):
+def test_resident_f32_equals_sweeps_at_fixed_iterations(n, form):
     """With a tolerance no iteration reaches, the FP32 correction runs exactly
     max_it iterations in both forms, so the LDS-resident kernel (its tiles,
-    row sets and lane mappings) and the per-sweep launches must give the same
-    potential bit for bit."""
-    print("resident FP32 vs sweeps, fixed iterations", n, "tiles per CU", per_cu, flush=True)
+    row sets and lane mappings; the body that derives node indexes per sweep,
+    one or two tiles per CU, or the lane-owned body) and the per-sweep
+    launches must give the same potential bit for bit."""
+    print("resident FP32 vs sweeps, fixed iterations", n, form, flush=True)
+    per_cu = 2 if form == "rows_two_per_cu" else 1
+    owned = 1 if form == "owned" else 0
     d = tuple(0.2 * x for x in n)
     G = int(np.prod(n))
     rho_p = smooth_field(16, n, 0.09)
@@ -410,12 +413,15 @@ def test_resident_f32_equals_sweeps_at_fixed_iterations(n, per_cu):
         sim = _sim(n, d, 10, 10, hvamd.HV_PREC_MIXED)
         sim.set_option(hvamd.HV_OPT_SOR_RESIDENT_F32, res32)
         sim.set_option(hvamd.HV_OPT_SOR_F32_TILES_PER_CU, per_cu)
+        sim.set_option(hvamd.HV_OPT_SOR_F32_OWNED, owned)
         sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p, np.zeros(G)]))
         sim.set_initial_coulomb_boundaries(82)
         sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 512)  # cold: FP64
         sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p * 1.01, np.zeros(G)]))
         it, _ = sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-30, 40)
         assert sim.get_option(hvamd.HV_OPT_SOR_RESIDENT_F32) == res32
+        if res32:
+            assert sim.get_option(hvamd.HV_OPT_SOR_F32_OWNED) == owned
         out[res32] = (it, sim.download_field(hvamd.HV_FIELD_COULOMB))
         sim.close()
     assert out[0][0] == out[1][0] == 40 + 4
