@@ -335,6 +335,12 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
         h.gz[a] = (T)tb.gz[a];
     }
     h.term = (T)tb.term;
+    for (int a = 0; a <= tb.rx; a++)
+        for (int t = 0; t <= 2 * tb.kb[a]; t++) {
+            const int ab = t > tb.kb[a] ? t - tb.kb[a] : tb.kb[a] - t;
+            h.tl[a][t] = tb.kc[a][ab];
+            h.tg[a][t] = (T)tb.gy[ab];
+        }
     if (!c->smear_tab) HV_HIP(c, hipMalloc(&c->smear_tab, sizeof(SmearTab<T>)));
     HV_HIP(c, hipMemcpyAsync(c->smear_tab, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HV_HIP(c, hipStreamSynchronize(c->stream));
@@ -444,10 +450,10 @@ int smear(hv_ctx *c, bool from_acc, double sigma_r, int ppn) {
     } else {
         const unsigned ntiles = (unsigned)(2 * nx * c->stl.njt * c->stl.nkt);
         if (from_acc)
-            k_smear_zy<T, unsigned long long><<<ntiles, 256, c->stl_lds, c->stream>>>(c->acc, D, c->sflags, tb, nx,
+            k_smear_zy<T, unsigned long long><<<ntiles, kSmearThreads, c->stl_lds, c->stream>>>(c->acc, D, c->sflags, tb, nx,
                                                                                      ny, nz, c->stl);
         else
-            k_smear_zy<T, T><<<ntiles, 256, c->stl_lds, c->stream>>>(rho, D, c->sflags, tb, nx, ny, nz, c->stl);
+            k_smear_zy<T, T><<<ntiles, kSmearThreads, c->stl_lds, c->stream>>>(rho, D, c->sflags, tb, nx, ny, nz, c->stl);
         HV_CHECK_LAUNCH(c);
         k_smear_xs<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, c->sflags, tb, nx, ny, nz, c->stl);
         HV_CHECK_LAUNCH(c);

@@ -732,7 +732,7 @@ int batch_step_device(hv_batch *b, int end_kick, bool res) {
             kb_smear_fused<T><<<b->blocks[kBSmear], kSmearThreads, b->smear_lds, b->stream>>>(a, btab(b, kBSmear),
                                                                                               n);
         } else {
-            kb_smear_zy<T><<<b->blocks[kBZy], 256, b->smear2_lds, b->stream>>>(a, btab(b, kBZy), n);
+            kb_smear_zy<T><<<b->blocks[kBZy], kSmearThreads, b->smear2_lds, b->stream>>>(a, btab(b, kBZy), n);
             kb_smear_xs<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
         }
         kb_zero_acc<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);

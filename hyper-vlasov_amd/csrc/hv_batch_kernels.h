@@ -60,17 +60,20 @@ __device__ __forceinline__ int batch_instance(const int *__restrict__ first, int
     return lo;
 }
 
-#define HV_BATCH_LOCATE()                                     \
-    const int inst = batch_instance(first, n, blockIdx.x);    \
-    const int bid = (int)blockIdx.x - first[inst];            \
+// (the instance index is read back as a wave-uniform value, so that the
+// instance's arguments and the tables they point to load through the scalar
+// cache, not per lane)
+#define HV_BATCH_LOCATE()                                                           \
+    const int inst = __builtin_amdgcn_readfirstlane(batch_instance(first, n, blockIdx.x)); \
+    const int bid = (int)blockIdx.x - first[inst];                                  \
     const BInst<T> &A = a[inst]
 
 // launches over a subset of the instances (those still iterating): ids[k] is
 // the instance of the k-th entry of the prefix table
-#define HV_BATCH_LOCATE_IDS()                                 \
-    const int slot_ = batch_instance(first, n, blockIdx.x);   \
-    const int bid = (int)blockIdx.x - first[slot_];           \
-    const BInst<T> &A = a[ids[slot_]]
+#define HV_BATCH_LOCATE_IDS()                                                        \
+    const int slot_ = __builtin_amdgcn_readfirstlane(batch_instance(first, n, blockIdx.x)); \
+    const int bid = (int)blockIdx.x - first[slot_];                                  \
+    const BInst<T> &A = a[__builtin_amdgcn_readfirstlane(ids[slot_])]
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -93,24 +96,24 @@ __global__ void kb_step_advance(int *step) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSmearThreads)
 kb_smear_zy(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
-    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.flags, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st2);
+    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.flags, smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st2);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_smear_xs(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
-    smear_xs_body<T>(bid, A.D, A.rho, A.flags, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st2);
+    smear_xs_body<T>(bid, A.D, A.rho, A.flags, smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st2);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kSmearThreads)
 kb_smear_fused(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
-    smear_fused_body<T, unsigned long long>(bid, A.acc, A.rho, A.tab, A.g.nx, A.g.ny, A.g.nz, A.st);
+    smear_fused_body<T, unsigned long long>(bid, A.acc, A.rho, smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st);
 }
 
 // acc = 0 (2G words), blocks of the 2G table

@@ -375,7 +375,36 @@ struct SmearTab {
     int kc[HV_SMEAR_MAXR + 1][HV_SMEAR_MAXR + 1];
     T gx[HV_SMEAR_MAXR + 1], gy[HV_SMEAR_MAXR + 1], gz[HV_SMEAR_MAXR + 1];
     T term;
+    // channel a's taps in summation order, t = 0..2 kb(a) for b = t - kb(a):
+    // the H level kc(a, |b|) and the weight gy(|b|) (consecutive, so that a
+    // run of taps loads with a few wide scalar loads)
+    int tl[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
+    T tg[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
 };
+
+// One tap of a smearing sum, acc + g h: FP64 rounds the product and the sum
+// (the oracle's and the reference's arithmetic); the FP32 form of the mixed
+// precision path rounds once (a fused multiply-add), which halves the work per
+// tap.  Every smearing kernel takes its taps through here, so both forms and
+// every tiling give the same bits in each precision.
+template <typename T>
+__device__ __forceinline__ T smear_madd(T acc, T g, T h) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_fmaf(g, h, acc);
+    else
+        return acc + g * h;
+}
+
+// The tap table as the smearing bodies read it: through the constant address
+// space, so that wave-uniform reads go through the scalar cache even where the
+// table pointer was itself loaded from memory (ensembles).
+template <typename T>
+using SmearTabC = const __attribute__((address_space(4))) SmearTab<T>;
+
+template <typename T>
+__device__ __forceinline__ SmearTabC<T> *smear_tab_c(const SmearTab<T> *p) {
+    return (SmearTabC<T> *)p;
+}
 
 // CT is the count type: unsigned long long (fixed point) or T.
 template <typename T, typename CT>
@@ -387,12 +416,13 @@ __device__ __forceinline__ T count_value(const CT *c, long long q) {
 }
 
 // The z factor: H_0..H_rz of the R x TK columns of a tile from its staged
-// counts C (R x W, the tile's columns plus rz on each side).  The chain of
+// counts C (R x W, the tile's columns plus rz on each side), taps through
+// smear_madd.  The chain of
 // each column runs in its order; the counts and weights of four levels are
 // loaded ahead of their four steps.
 template <typename T>
 __device__ __forceinline__ void smear_h_levels(const T *__restrict__ C, T *__restrict__ Hs,
-                                               const SmearTab<T> *__restrict__ tb, int R, int W, int TK, int rz,
+                                               SmearTabC<T> *__restrict__ tb, int R, int W, int TK, int rz,
                                                int nthr) {
     const int lvl = R * TK;
     for (int e = threadIdx.x; e < lvl; e += nthr) {
@@ -412,12 +442,12 @@ __device__ __forceinline__ void smear_h_levels(const T *__restrict__ C, T *__res
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                h = (h + g[u] * lo[u]) + g[u] * hi[u];
+                h = smear_madd(smear_madd(h, g[u], lo[u]), g[u], hi[u]);
                 hp[(K + u) * lvl] = h;
             }
         }
         for (; K <= rz; K++) {
-            h = (h + tb->gz[K] * cp[-K]) + tb->gz[K] * cp[K];
+            h = smear_madd(smear_madd(h, tb->gz[K], cp[-K]), tb->gz[K], cp[K]);
             hp[K * lvl] = h;
         }
     }
@@ -431,31 +461,40 @@ template <typename T>
 constexpr int smear_vw() { return 16 / (int)sizeof(T); }
 
 template <typename T>
-__device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, const SmearTab<T> *__restrict__ tb, int a,
+__device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, SmearTabC<T> *__restrict__ tb, int a,
                                                   int h0, int R, int TK, T (&acc)[smear_vw<T>()]) {
     constexpr int VW = smear_vw<T>();
     typedef T V __attribute__((ext_vector_type(VW)));
-    const int kb = tb->kb[a];
+    const int kb = tb->kb[a], nt = 2 * kb + 1;
 #pragma unroll
     for (int v = 0; v < VW; v++) acc[v] = T(0.0);
-    int b = -kb;
-    for (; b + 1 <= kb; b += 2) {  // two taps' loads ahead of their sums
-        const int ab0 = b < 0 ? -b : b, ab1 = b + 1 < 0 ? -(b + 1) : b + 1;
-        const T g0 = tb->gy[ab0], g1 = tb->gy[ab1];
-        const V h0v = *(const V *)(Hs + (tb->kc[a][ab0] * R + b) * TK + h0);
-        const V h1v = *(const V *)(Hs + (tb->kc[a][ab1] * R + b + 1) * TK + h0);
+    const T *hb = Hs + h0 - kb * TK;  // tap t reads row kc * R + t of this base
+    // runs of U taps: their table entries, then their H vectors, load
+    // together ahead of the sums (one load latency per run, not per tap)
+    auto run = [&](auto uu, int t) {
+        constexpr int U = decltype(uu)::value;
+        int L[U];
+        T g[U];
+        V h[U];
 #pragma unroll
-        for (int v = 0; v < VW; v++) acc[v] = acc[v] + g0 * h0v[v];
+        for (int u = 0; u < U; u++) {
+            L[u] = tb->tl[a][t + u];
+            g[u] = tb->tg[a][t + u];
+        }
 #pragma unroll
-        for (int v = 0; v < VW; v++) acc[v] = acc[v] + g1 * h1v[v];
+        for (int u = 0; u < U; u++) h[u] = *(const V *)(hb + (L[u] * R + t + u) * TK);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int v = 0; v < VW; v++) acc[v] = smear_madd(acc[v], g[u], h[u][v]);
+    };
+    int t = 0;
+    for (; t + 8 <= nt; t += 8) run(std::integral_constant<int, 8>{}, t);
+    if (t + 4 <= nt) {
+        run(std::integral_constant<int, 4>{}, t);
+        t += 4;
     }
-    if (b <= kb) {
-        const int ab = b < 0 ? -b : b;
-        const T g = tb->gy[ab];
-        const V h = *(const V *)(Hs + (tb->kc[a][ab] * R + b) * TK + h0);
-#pragma unroll
-        for (int v = 0; v < VW; v++) acc[v] = acc[v] + g * h[v];
-    }
+    for (; t < nt; t++) run(std::integral_constant<int, 1>{}, t);
 }
 
 // The vector D stage needs TJ = 8 rows of TK = 8 VW columns: one wave covers
@@ -472,7 +511,8 @@ __host__ __device__ inline int smear_c_words(int R, int W, int vw) { return (R *
 //   z:  H_K(c) = (H_{K-1} + gz(K) c(k-K)) + gz(K) c(k+K),  H_0 = gz(0) c(k)
 //   y:  D_a = sum over b = -kb(a)..kb(a) of gy(|b|) H_{kc(a,|b|)}(j+b)
 //   x:  rho = (sum over a = -rx..rx, 0 <= i+a < nx of gx(|a|) D_|a|(i+a)) term
-// each sum from 0 in ascending offset order.  One block owns the TJ x TK
+// each sum from 0 in ascending offset order, every tap acc + g h through
+// smear_madd.  One block owns the TJ x TK
 // (row, column) tile of one species over a chunk of IC planes: it walks the
 // planes p that reach the chunk in ascending order, builds H and the D_a
 // channels of plane p in LDS from the counts of the tile plus its (ry, rz)
@@ -510,7 +550,7 @@ inline long long smear_blocks(const SmearTiles &st) { return 2LL * st.njt * st.n
 
 template <typename T, typename CT>
 __device__ __forceinline__ void smear_fused_body(const int bid, const CT *__restrict__ cnt, T *__restrict__ rho,
-                                                 const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz,
+                                                 SmearTabC<T> *__restrict__ tb, int nx, int ny, int nz,
                                                  SmearTiles st) {
     extern __shared__ unsigned char smem[];
     const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
@@ -568,7 +608,9 @@ __device__ __forceinline__ void smear_fused_body(const int bid, const CT *__rest
             C[e] = v;
             nonzero |= (v != T(0.0));
         }
-        if (__syncthreads_or(nonzero)) {
+        // (read back as a wave-uniform value: the channel loops below then stay
+        // scalar, with the tap tables in scalar loads)
+        if (__builtin_amdgcn_readfirstlane(__syncthreads_or(nonzero))) {
             smear_h_levels<T>(C, Hs, tb, R, W, TK, rz, kSmearThreads);
             __syncthreads();
             if (vec) {
@@ -584,9 +626,8 @@ __device__ __forceinline__ void smear_fused_body(const int bid, const CT *__rest
                     T *r1 = ring + (w1 ? o1 % NS : 0) * nnode + n0, *r2 = ring + (w2 ? o2 % NS : 0) * nnode + n0;
 #pragma unroll
                     for (int v = 0; v < VW; v++) {
-                        const T cval = gxa * acc[v];
-                        if (w1) r1[v] = r1[v] + cval;
-                        if (w2) r2[v] = r2[v] + cval;
+                        if (w1) r1[v] = smear_madd(r1[v], gxa, acc[v]);
+                        if (w2) r2[v] = smear_madd(r2[v], gxa, acc[v]);
                     }
                 }
             } else {
@@ -602,7 +643,7 @@ __device__ __forceinline__ void smear_fused_body(const int bid, const CT *__rest
                     const T *hb = Hs + (tb->kc[a][ab] * R + b) * TK;
 #pragma unroll
                     for (int u = 0; u < kSmearNpl; u++)
-                        if (u < npl) acc[u] = acc[u] + g * hb[hoff[u]];
+                        if (u < npl) acc[u] = smear_madd(acc[u], g, hb[hoff[u]]);
                 }
                 const T gxa = tb->gx[a];
                 const int o1 = p - a, o2 = p + a;
@@ -612,9 +653,8 @@ __device__ __forceinline__ void smear_fused_body(const int bid, const CT *__rest
                 for (int u = 0; u < kSmearNpl; u++) {
                     if (u >= npl) continue;
                     const int node = lane + u * kWave;
-                    const T cval = gxa * acc[u];
-                    if (w1) r1[node] = r1[node] + cval;
-                    if (w2) r2[node] = r2[node] + cval;
+                    if (w1) r1[node] = smear_madd(r1[node], gxa, acc[u]);
+                    if (w2) r2[node] = smear_madd(r2[node], gxa, acc[u]);
                 }
             }
             }
@@ -630,7 +670,7 @@ template <typename T, typename CT>
 __global__ void __launch_bounds__(kSmearThreads)
 k_smear_fused(const CT *__restrict__ cnt, T *__restrict__ rho, const SmearTab<T> *__restrict__ tb, int nx, int ny,
               int nz, SmearTiles st) {
-    smear_fused_body<T, CT>(blockIdx.x, cnt, rho, tb, nx, ny, nz, st);
+    smear_fused_body<T, CT>(blockIdx.x, cnt, rho, smear_tab_c(tb), nx, ny, nz, st);
 }
 
 // ---- two-pass smearing: z and y factors in LDS, then a sparse x pass ----------
@@ -649,7 +689,7 @@ k_smear_fused(const CT *__restrict__ cnt, T *__restrict__ rho, const SmearTab<T>
 template <typename T, typename CT>
 __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restrict__ cnt, T *__restrict__ D,
                                               unsigned char *__restrict__ flags,
-           const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
+           SmearTabC<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
     extern __shared__ unsigned char smem[];
     const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
     const int TJ = st.TJ, TK = st.TK;
@@ -668,7 +708,7 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
     const long long base = sp * G + (long long)i * ny * nz;
     const int j0 = jt * TJ, k0 = kt * TK;
     int nonzero = 0;
-    for (int e = threadIdx.x; e < R * W; e += blockDim.x) {
+    for (int e = threadIdx.x; e < R * W; e += kSmearThreads) {
         const int r = e / W, cc = e - r * W;
         const int j = j0 - ry + r, k = k0 - rz + cc;
         T v = T(0.0);
@@ -677,16 +717,18 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
         nonzero |= (v != T(0.0));
     }
     const long long tile = bid;
-    if (!__syncthreads_or(nonzero)) {
+    // (read back as a wave-uniform value: the channel loop below then stays
+    // scalar, with the tap tables in scalar loads)
+    if (!__builtin_amdgcn_readfirstlane(__syncthreads_or(nonzero))) {
         if (threadIdx.x == 0) flags[tile] = 0;
         return;
     }
-    smear_h_levels<T>(C, Hs, tb, R, W, TK, rz, (int)blockDim.x);
+    smear_h_levels<T>(C, Hs, tb, R, W, TK, rz, kSmearThreads);
     __syncthreads();
     if (smear_vec_tile<T>(TJ, TK)) {
         // channel a per wave, VW consecutive columns per lane
         const int lane = threadIdx.x & (kWave - 1);
-        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), nwaves = blockDim.x / kWave;
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), nwaves = kSmearThreads / kWave;
         const int r = lane >> 3, c0 = (lane & 7) * VW;
         const int j = j0 + r;
         const long long q = base + (long long)j * nz + k0 + c0;
@@ -702,7 +744,7 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
         if (threadIdx.x == 0) flags[tile] = 1;
         return;
     }
-    for (int e = threadIdx.x; e < TJ * TK; e += blockDim.x) {
+    for (int e = threadIdx.x; e < TJ * TK; e += kSmearThreads) {
         const int r = e / TK, cc = e - r * TK;
         const int j = j0 + r, k = k0 + cc;
         if (j >= ny || k >= nz) continue;
@@ -712,7 +754,7 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
             T acc = T(0.0);
             for (int b = -kb; b <= kb; b++) {
                 const int ab = b < 0 ? -b : b;
-                acc = acc + tb->gy[ab] * Hs[(tb->kc[a][ab] * R + (r + ry + b)) * TK + cc];
+                acc = smear_madd(acc, tb->gy[ab], Hs[(tb->kc[a][ab] * R + (r + ry + b)) * TK + cc]);
             }
             D[(long long)a * n2 + q] = acc;
         }
@@ -721,16 +763,16 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
 }
 
 template <typename T, typename CT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSmearThreads)
 k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned char *__restrict__ flags,
            const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
-    smear_zy_body<T, CT>(blockIdx.x, cnt, D, flags, tb, nx, ny, nz, st);
+    smear_zy_body<T, CT>(blockIdx.x, cnt, D, flags, smear_tab_c(tb), nx, ny, nz, st);
 }
 
 template <typename T>
 __device__ __forceinline__ void smear_xs_body(const int bid, const T *__restrict__ D, T *__restrict__ rho,
                                               const unsigned char *__restrict__ flags,
-           const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
+           SmearTabC<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
     const long long G = (long long)nx * ny * nz, n2 = 2 * G;
     const long long q = (long long)bid * blockDim.x + threadIdx.x;
     if (q >= n2) return;
@@ -747,7 +789,7 @@ __device__ __forceinline__ void smear_xs_body(const int bid, const T *__restrict
         const int ia = i + a;
         if (ia < 0 || ia >= nx) continue;
         if (!flags[trow + ia * tstride]) continue;
-        acc = acc + tb->gx[aa] * D[(long long)aa * n2 + q + (long long)a * plane];
+        acc = smear_madd(acc, tb->gx[aa], D[(long long)aa * n2 + q + (long long)a * plane]);
     }
     rho[q] = acc * tb->term;
 }
@@ -756,7 +798,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_smear_xs(const T *__restrict__ D, T *__restrict__ rho, const unsigned char *__restrict__ flags,
            const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
-    smear_xs_body<T>(blockIdx.x, D, rho, flags, tb, nx, ny, nz, st);
+    smear_xs_body<T>(blockIdx.x, D, rho, flags, smear_tab_c(tb), nx, ny, nz, st);
 }
 
 // ---- Skyrme potentials (include/physics.hpp:273-281), optionally + phi ------

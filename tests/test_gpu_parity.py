@@ -66,6 +66,30 @@ def test_smear_bitwise(oracle, n, d, fused):
     _eq(got, oracle.smear(counts, n, d, SIGMA_R, 100))
 
 
+@pytest.mark.parametrize("n,d", GRIDS + [((64, 60, 64), (16.0, 15.0, 16.0)), ((96, 96, 96), (19.2, 19.2, 19.2))])
+def test_smear_mixed_near_oracle(oracle, n, d):
+    """The mixed form smears in FP32 with one rounding per tap (a fused
+    multiply-add): within 2e-5 of the largest value of the oracle's FP64
+    smearing (a few hundred taps per node at FP32's 6e-8 unit roundoff), and
+    the two forms give the same bits."""
+    print(f"smear mixed grid={n}", flush=True)
+    st = particles(12, 500, 600, 0.5 * min(d))
+    counts, _, _ = oracle.deposit(st["x"], st["y"], st["z"], 500, 600, n, d)
+    want = oracle.smear(counts, n, d, SIGMA_R, 100)
+    got = []
+    for fused in (0, 1):
+        sim = _sim(n, d, 500, 600, hvamd.HV_PREC_MIXED)
+        sim.set_option(hvamd.HV_OPT_SMEAR_FUSED, fused)
+        sim.upload_field(hvamd.HV_FIELD_DENSITY, counts)
+        sim.compute_volumetric_densities(SIGMA_R, 100)
+        got.append(sim.download_field(hvamd.HV_FIELD_DENSITY))
+        sim.close()
+    _eq(got[0], got[1])
+    err = np.abs(got[0] - want).max() / np.abs(want).max()
+    print("max error / max", err)
+    assert err < 2e-5
+
+
 @pytest.mark.parametrize("n,d", GRIDS)
 def test_skyrme_merge_fdm_bitwise(oracle, n, d):
     print(f"skyrme/merge/fdm grid={n}", flush=True)
