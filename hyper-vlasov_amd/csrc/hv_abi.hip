@@ -111,8 +111,8 @@ struct hv_ctx {
     int sor_res32 = 1;        // option: use it when the grid fits
     int res32_lag = 2;        // option: iterations between a max and its convergence test
     int res32_per_cu = 1;     // option: LDS-resident FP32 tiles per CU (1 or 2)
-    int res32_owned = 1;      // option: the lane-owned body where the tiling fits it
-    int res32_is_owned = 0;   // the chosen kernel is the lane-owned body
+    int res32_owned = 2;      // option: 2 deep-halo, 1 lane-owned, 0 per-row body (the best that fits)
+    int res32_is_owned = 0;   // the body the chosen kernel runs (same values)
     int res32_threads = 1024;
     int res32_state = 0;
     ResTiles rtl32{};
@@ -598,8 +598,30 @@ int res32_setup(hv_ctx *c) {
                         (long long)best.PI * best.PJ <= (long long)occ * ncu;
         if (!ok) c->res32_kernel = generic;
     }
-    // the lane-owned body where the tiling fits it (one tile per CU)
-    if (per == 1 && c->res32_owned && owned_fits<HV_OWNED_SOLO, kRes32Threads>(best)) {
+    // the deep-halo lane-owned body, else the lane-owned body, where the
+    // tiling fits it (one tile per CU)
+    auto try_kernel = [&](const void *k, size_t l) {
+        int occ = 0, dev = 0, ncu = 0;
+        return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l) == hipSuccess &&
+               hipGetDevice(&dev) == hipSuccess &&
+               hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+               hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, nt, l) == hipSuccess &&
+               (long long)best.PI * best.PJ <= (long long)occ * ncu;
+    };
+    bool deep = false;
+    if (per == 1 && c->res32_owned >= 2) {
+        ResTiles d = best;
+        d.xstride = deep_xstride(d);
+        const void *k = (const void *)k_sor_resident_f32d<HV_DEEP_SOLO>;
+        const size_t l = DeepLayout<kDeepSoloP, kDeepSoloQJ, kDeepSoloIM>::bytes(kRes32Threads / kWave);
+        if (deep_fits<HV_DEEP_SOLO, kRes32Threads>(d) && try_kernel(k, l)) {
+            best = d;
+            c->res32_kernel = k;
+            lds = l;
+            deep = true;
+        }
+    }
+    if (!deep && per == 1 && c->res32_owned >= 1 && owned_fits<HV_OWNED_SOLO, kRes32Threads>(best)) {
         const void *k = (const void *)k_sor_resident_f32o<HV_OWNED_SOLO>;
         const size_t l = OwnedLayout<kOwnedSoloP, kOwnedSoloQJ, kOwnedSoloIM>::bytes(kRes32Threads / kWave);
         int occ = 0, dev = 0, ncu = 0;
@@ -613,7 +635,7 @@ int res32_setup(hv_ctx *c) {
             lds = l;
         }
     }
-    c->res32_is_owned = c->res32_kernel == (const void *)k_sor_resident_f32o<HV_OWNED_SOLO>;
+    c->res32_is_owned = deep ? 2 : (c->res32_kernel == (const void *)k_sor_resident_f32o<HV_OWNED_SOLO> ? 1 : 0);
     c->res32_threads = nt;
     const size_t nT = (size_t)best.PI * best.PJ;
     if (hipMalloc((void **)&c->res32_x, sizeof(unsigned long long) * 2 * nT * best.xstride) != hipSuccess) return 1;
@@ -1812,10 +1834,11 @@ int hv_set_option(hv_ctx *c, int option, int value) {
         }
         return HV_OK;
     case HV_OPT_SOR_F32_OWNED:
-        if ((value ? 1 : 0) != c->res32_owned) {
+        if (value < 0 || value > 2) return set_err(c, HV_ERR_ARG, "the FP32 SOR body must be 0, 1 or 2");
+        if (value != c->res32_owned) {
             int rc = res32_reset(c);
             if (rc) return rc;
-            c->res32_owned = value ? 1 : 0;
+            c->res32_owned = value;
         }
         return HV_OK;
     case HV_OPT_SOR_F32_LAG:

@@ -47,8 +47,8 @@ struct hv_batch {
     int group_kib = 0;  // SOR working-set budget (KiB) of one instance group (0: all instances together)
     // LDS-resident FP32 correction of the whole batch (mixed precision)
     int res32 = 1;                        // option: use it when every instance tiles
-    int res32_owned = 1;                  // option: the lane-owned body when every instance fits it
-    int res32_is_owned = 0;               // the chosen kernel is the lane-owned body
+    int res32_owned = 2;                  // option: 2 deep-halo, 1 lane-owned, 0 per-row (when every instance fits)
+    int res32_is_owned = 0;               // the body the chosen kernel runs (same values)
     int res32_lag = 2;                    // option: lag of its convergence test
     int res_state = 0;                    // 0: not set up, 1: ready, -1: unavailable
     int res_max_it = 0;
@@ -461,9 +461,9 @@ int batch_res_tiles(const hv_ctx *c, size_t maxlds, int maxblk, ResTiles *out) {
     return best_cost < 0 ? 1 : HV_OK;
 }
 
-// Tiling of one instance for the lane-owned ensemble body: the fewest tiles
-// that fit it.
-int batch_owned_tiles(const hv_ctx *c, int maxblk, ResTiles *out) {
+// Tiling of one instance for the lane-owned ensemble body (deep: the
+// deep-halo body, with its exchange stride): the fewest tiles that fit it.
+int batch_owned_tiles(const hv_ctx *c, int maxblk, ResTiles *out, bool deep = false) {
     const int nx = c->world.n[0], ny = c->world.n[1];
     long long best_cost = -1;
     for (int PI = 1; PI <= nx; PI++) {
@@ -480,7 +480,12 @@ int batch_owned_tiles(const hv_ctx *c, int maxblk, ResTiles *out) {
             t.side_i = t.TJ * t.nzh;
             t.side_j = t.TI * t.nzh;
             t.xstride = 2 * t.side_i + 2 * t.side_j;
-            if (!owned_fits<HV_OWNED_BATCH, kRes32Threads>(t)) continue;
+            if (deep) {
+                t.xstride = deep_xstride(t);
+                if (!deep_fits<HV_DEEP_BATCH, kRes32Threads>(t)) continue;
+            } else if (!owned_fits<HV_OWNED_BATCH, kRes32Threads>(t)) {
+                continue;
+            }
             const long long cost = (long long)nT * 4096 + t.TI + t.TJ;
             if (best_cost < 0 || cost < best_cost) {
                 best_cost = cost;
@@ -513,11 +518,16 @@ int batch_res32_setup(hv_batch *b, int max_it) {
     b->rslot_off.assign(n + 1, 0);
     size_t lds = 0;
     const void *kern = nullptr;
-    // the lane-owned body when every instance fits it
-    bool owned = b->res32_owned != 0;
+    // the deep-halo body, else the lane-owned body, when every instance fits it
+    bool deep = b->res32_owned >= 2;
+    for (int i = 0; i < n && deep; i++) deep = batch_owned_tiles(b->inst[i], ncu, &b->rtl[i], true) == HV_OK;
+    bool owned = !deep && b->res32_owned >= 1;
     for (int i = 0; i < n && owned; i++) owned = batch_owned_tiles(b->inst[i], ncu, &b->rtl[i]) == HV_OK;
-    b->res32_is_owned = owned ? 1 : 0;
-    if (owned) {
+    b->res32_is_owned = deep ? 2 : (owned ? 1 : 0);
+    if (deep) {
+        kern = (const void *)kb_sor_resident_f32d<HV_DEEP_BATCH>;
+        lds = DeepLayout<kDeepBatchP, kDeepBatchQJ, kDeepBatchIM>::bytes(kRes32Threads / kWave);
+    } else if (owned) {
         kern = (const void *)kb_sor_resident_f32o<HV_OWNED_BATCH>;
         lds = OwnedLayout<kOwnedBatchP, kOwnedBatchQJ, kOwnedBatchIM>::bytes(kRes32Threads / kWave);
     } else {
@@ -1093,12 +1103,13 @@ int hv_batch_set_option(hv_batch *b, int option, int value) {
         return HV_OK;
     }
     if (option == HV_OPT_SOR_F32_OWNED) {
-        if ((value ? 1 : 0) != b->res32_owned) {
+        if (value < 0 || value > 2) return batch_err(b, HV_ERR_ARG, "the FP32 SOR body must be 0, 1 or 2");
+        if (value != b->res32_owned) {
             // tilings and kernel chosen again at the next solve
             if (b->stream) (void)hipStreamSynchronize(b->stream);
             batch_drop_graph(b);
             b->res_state = 0;
-            b->res32_owned = value ? 1 : 0;
+            b->res32_owned = value;
         }
         return HV_OK;
     }

@@ -88,4 +88,29 @@ kb_sor_resident_f32o(const BRes32 *__restrict__ inst, const int *__restrict__ jo
     }
 }
 
+// The same job loop with the deep-halo body <SI, SR, SH, P, QJ, IM> (tilings
+// with the deep exchange stride; the host launches it only when every
+// instance fits).
+template <int SI, int SR, int SH, int P, int QJ, int IM>
+__global__ void __launch_bounds__(kRes32Threads)
+kb_sor_resident_f32d(const BRes32 *__restrict__ inst, const int *__restrict__ job_first,
+                     const int2 *__restrict__ jobs, int lag) {
+    const int j1 = job_first[blockIdx.x + 1];
+    for (int j = job_first[blockIdx.x]; j < j1; j++) {
+        const int2 jb = jobs[j];
+        const BRes32 &I = inst[jb.x];
+#ifdef HV_BRES_TRACE
+        if (threadIdx.x == 0) I.jtrace[2 * j] = wall_clock64();
+#endif
+        if (deep_fits<SI, SR, SH, P, QJ, IM, kRes32Threads>(I.tl) && I.tl.xstride == deep_xstride(I.tl))
+            res32d_solve_tile<SI, SR, SH, P, QJ, IM>(I.delta, I.R, I.sp, I.tl, I.cf, I.max_it, lag, I.sy, jb.y);
+        else if (threadIdx.x == 0)
+            st_dev_i(I.sy.err, 1);
+        __syncthreads();
+#ifdef HV_BRES_TRACE
+        if (threadIdx.x == 0) I.jtrace[2 * j + 1] = wall_clock64();
+#endif
+    }
+}
+
 }  // namespace hv

@@ -398,6 +398,402 @@ __device__ __forceinline__ void res32o_solve_tile(float *__restrict__ delta, con
     }
 }
 
+// ---- one exchange per iteration: the deep-halo lane-owned body ----------------
+// res32d_solve_tile runs the same solve as res32o_solve_tile (same updates on
+// the same operands, the same lagged stop test, so the same bits) with one
+// neighbour exchange per iteration instead of two.  After its black sweep a
+// tile publishes its black values of the two rows along each face and of its
+// four corner rows; a neighbour then has every black value within two rows
+// of its tile (and its diagonal corners) and recomputes, redundantly, the red
+// values of the one-row ring around its tile from them (the update its owner
+// makes, on the same operands in the same order, so the same bits).  Its own
+// black sweep then reads red values it holds, without a wait:
+//   red interior -> [black border of it-1] -> red ring and border ring
+//   -> black everything -> publish black faces (buffer it & 1)
+// The exchange words are double-buffered by iteration parity: a tile
+// overwrites a buffer two iterations after filling it, by which time each
+// neighbour has read it (the neighbour's faces that this tile waited for in
+// between were published after that read).
+//
+// LDS: each colour's array holds (IM + 4) x QJ rows of P slots, tile row
+// (li, lj) at (li + 2) * QJ + lj + 2 (two border rows on each side), then
+// the sinks, guard and partials as in OwnedLayout.
+template <int P, int QJ, int IM>
+struct DeepLayout {
+    static constexpr int E = (IM + 4) * QJ * P;
+    static constexpr int ext1 = E + kWave;
+    static constexpr int red = ext1 + E + kWave + QJ * P + 2;
+    static constexpr size_t bytes(int nwaves) { return sizeof(float) * ((size_t)red + (size_t)nwaves + 3); }
+};
+
+constexpr int kFetchMaxD = 6;  // exchange words per thread of the deep-halo body
+
+// exchange words per tile and buffer: two rows along each face, four corners
+__host__ __device__ inline int deep_xstride(const ResTiles &t) { return 4 * (t.TI + t.TJ) * t.nzh + 4 * t.nzh; }
+
+// does tiling t fit the deep-halo body <SI, SR, SH, P, QJ, IM> of NT threads
+// (slots per lane: SI interior, SR ring, SH border ring)
+template <int SI, int SR, int SH, int P, int QJ, int IM, int NT>
+__host__ __device__ inline bool deep_fits(const ResTiles &t) {
+    if (t.TI > IM || t.TJ + 4 > QJ || t.nzh > P || t.TI < 3 || t.TJ < 3) return false;
+    if (deep_xstride(t) > kFetchMaxD * NT) return false;
+    const int n_inner = (t.TI - 2) * (t.TJ - 2), n_ring = t.TI * t.TJ - n_inner, n_halo = 2 * (t.TI + t.TJ);
+    const int nw = NT / kWave;
+    auto ch = [&](int rows) { return (rows * t.nzh + kWave - 1) / kWave; };
+    return ch(n_inner) <= SI * nw && ch(n_ring) <= SR * nw && ch(n_halo) <= SH * nw &&
+           DeepLayout<P, QJ, IM>::red < (1 << 18);
+}
+
+template <int SI, int SR, int SH, int P, int QJ, int IM, int NT = kRes32Threads>
+__device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, const float *__restrict__ R, Split sp,
+                                                  ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
+                                                  const int t) {
+    using L = DeepLayout<P, QJ, IM>;
+    extern __shared__ unsigned char smem[];
+    float *const x0 = (float *)smem;  // red values (bordered rows)
+    float *const x1 = x0 + L::ext1;   // black values
+    float *const red = x0 + L::red;   // wave partials
+    constexpr int nwaves = NT / kWave;
+    constexpr int XS = QJ * P;
+    constexpr int S = SI + SR + SH;
+    const int TI = tl.TI, TJ = tl.TJ, nzh = tl.nzh;
+    const int side_i = TJ * nzh, side_j = TI * nzh, xstride = tl.xstride;
+    const int nx = sp.nx, ny = sp.ny, nz = sp.nz;
+    const int nT = tl.PI * tl.PJ, PJ = tl.PJ;
+    const int pi = t / PJ, pj = t - pi * PJ;
+    const int i0 = pi * TI, j0 = pj * TJ;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const bool has_im = pi > 0, has_ip = pi < tl.PI - 1, has_jm = pj > 0, has_jp = pj < PJ - 1;
+    const float yx = (float)cf.ydx2, yy = (float)cf.ydy2, yz = (float)cf.ydz2, inv = (float)cf.inv_dx2;
+    const float om = (float)cf.omega, om1 = (float)cf.one_minus_omega, tol = (float)cf.tol;
+    int &s_bad = *(int *)(red + nwaves);
+    float *const s_max = red + nwaves + 1;
+    if (tid == 0) s_bad = 0;
+    auto lrow = [&](int li, int lj) { return ((li + 2) * QJ + lj + 2) * P; };
+
+    // the tile's values of both colours, the red values of the ring around it
+    // and the black values of the two rows around it and of its corners
+    // (values outside the grid are never read by an update)
+    for (int e = tid; e < (TI + 4) * (TJ + 4) * nzh; e += NT) {
+        const int r = e / nzh, h = e - r * nzh;
+        const int li = r / (TJ + 4) - 2, lj = r - (li + 2) * (TJ + 4) - 2;
+        const bool in_i = li >= 0 && li < TI, in_j = lj >= 0 && lj < TJ;
+        const bool own = in_i && in_j;
+        const bool ring1 = (in_j && (li == -1 || li == TI)) || (in_i && (lj == -1 || lj == TJ));
+        const bool black = own || (in_j && !in_i) || (in_i && !in_j) ||
+                           ((li == -1 || li == TI) && (lj == -1 || lj == TJ));
+        const int i = i0 + li, j = j0 + lj;
+        if (!(i >= 0 && i < nx && j >= 0 && j < ny)) continue;
+        const long long q = ((long long)i * ny + j) * nzh + h;
+        const int w = lrow(li, lj) + h;
+        if (own || ring1) x0[w] = delta[q];
+        if (black) x1[w] = delta[sp.Gh + q];
+    }
+
+    // Row sets: interior rows, the ring of the tile's face rows, and the ring
+    // of rows just outside the tile (no corners), whose red values the black
+    // sweep reads.
+    const int n_inner = (TI - 2) * (TJ - 2);
+    const int nel_in = n_inner * nzh, nel_rg = (TI * TJ - n_inner) * nzh, nel_hr = 2 * (TI + TJ) * nzh;
+    auto nslots = [&](int nel) {
+        const int nch = (nel + kWave - 1) / kWave;
+        return nch > wave ? (nch - wave + nwaves - 1) / nwaves : 0;
+    };
+    const int ns_in = nslots(nel_in), ns_rg = nslots(nel_rg), ns_hr = nslots(nel_hr);
+
+    // slot descriptors as in res32o_solve_tile; R1 for tile rows only (the
+    // border ring is updated in red only)
+    const unsigned lds0 = (unsigned)(size_t)(lds_float *)x0;
+    unsigned D[S];
+    float R0[S], R1[SI + SR];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int set = s < SI ? 0 : (s < SI + SR ? 1 : 2);
+        const int ss = set == 0 ? s : (set == 1 ? s - SI : s - SI - SR);
+        const int f = (ss * nwaves + wave) * kWave + lane;
+        unsigned d = lds0 + 4u * (unsigned)(L::E + lane - XS);  // sink
+        float r0 = 0.0f, r1 = 0.0f;
+        if (f < (set == 0 ? nel_in : (set == 1 ? nel_rg : nel_hr))) {
+            const int q = f / nzh, h = f - q * nzh;
+            int li, lj;
+            if (set == 0) {
+                li = 1 + q / (TJ - 2);
+                lj = 1 + q - (li - 1) * (TJ - 2);
+            } else if (set == 1) {
+                if (q < TJ) {
+                    li = 0;
+                    lj = q;
+                } else if (q < 2 * TJ) {
+                    li = TI - 1;
+                    lj = q - TJ;
+                } else {
+                    li = 1 + ((q - 2 * TJ) >> 1);
+                    lj = ((q - 2 * TJ) & 1) ? TJ - 1 : 0;
+                }
+            } else {
+                if (q < TJ) {
+                    li = -1;
+                    lj = q;
+                } else if (q < 2 * TJ) {
+                    li = TI;
+                    lj = q - TJ;
+                } else {
+                    li = (q - 2 * TJ) >> 1;
+                    lj = ((q - 2 * TJ) & 1) ? TJ : -1;
+                }
+            }
+            const int i = i0 + li, j = j0 + lj;
+            const unsigned o0 = (unsigned)(i + j) & 1u, o1 = o0 ^ 1u;
+            const int k0 = 2 * h + (int)o0, k1 = 2 * h + (int)o1;
+            const bool okij = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+            const unsigned ok0 = okij && k0 >= 1 && k0 <= nz - 2;
+            const unsigned ok1 = set < 2 && okij && k1 >= 1 && k1 <= nz - 2;
+            const int w = lrow(li, lj) + h;
+            d = (lds0 + 4u * (unsigned)(w - XS)) | o0 << 20 | o1 << 21 | ok0 << 22 | ok1 << 23;
+            if (i >= 0 && i < nx && j >= 0 && j < ny) {
+                const long long g = ((long long)i * ny + j) * nzh + h;
+                r0 = R[g];
+                if (set < 2) r1 = R[sp.Gh + g];
+            }
+        }
+        D[s] = d;
+        R0[s] = r0;
+        if (s < SI + SR) R1[s] = r1;
+    }
+
+    // exchange words of this thread: word x = tid + q * NT of a tile's slice,
+    // laid out as [im rows d = 0, 1][ip rows][jm rows][jp rows][corners
+    // (0,0), (0,TJ-1), (TI-1,0), (TI-1,TJ-1)]: the face row it publishes
+    // (its LDS slot, low 16 bits of fx) and the border row it receives from
+    // the neighbour's matching word (LDS slot in the high bits; byte offset
+    // of that word in src_b; bit q of fmask: the neighbour exists)
+    const int oIM = 0, oIP = 2 * side_i, oJM = 4 * side_i, oJP = 4 * side_i + 2 * side_j, oC = oJP + 2 * side_j;
+    unsigned src_b[kFetchMaxD], fx[kFetchMaxD], fmask = 0;
+    int nfetch = 0;
+    for (int x = tid; x < xstride && nfetch < kFetchMaxD; x += NT) {
+        int off = -1, ps, fd;
+        if (x < 4 * side_i) {
+            const int sec = x / side_i, y = x - sec * side_i, lj = y / nzh, h = y - lj * nzh, d = sec & 1;
+            if (sec < 2) {  // my rows li = d; I receive li = -1 - d from (pi-1)'s ip rows
+                ps = lrow(d, lj) + h;
+                fd = lrow(-1 - d, lj) + h;
+                if (has_im) off = (t - PJ) * xstride + oIP + d * side_i + y;
+            } else {  // my rows li = TI-1-d; I receive li = TI + d from (pi+1)'s im rows
+                ps = lrow(TI - 1 - d, lj) + h;
+                fd = lrow(TI + d, lj) + h;
+                if (has_ip) off = (t + PJ) * xstride + oIM + d * side_i + y;
+            }
+        } else if (x < oC) {
+            const int x2 = x - oJM, sec = x2 / side_j, y = x2 - sec * side_j, li = y / nzh, h = y - li * nzh,
+                      d = sec & 1;
+            if (sec < 2) {  // my rows lj = d; I receive lj = -1 - d from (pj-1)'s jp rows
+                ps = lrow(li, d) + h;
+                fd = lrow(li, -1 - d) + h;
+                if (has_jm) off = (t - 1) * xstride + oJP + d * side_j + y;
+            } else {  // my rows lj = TJ-1-d; I receive lj = TJ + d from (pj+1)'s jm rows
+                ps = lrow(li, TJ - 1 - d) + h;
+                fd = lrow(li, TJ + d) + h;
+                if (has_jp) off = (t + 1) * xstride + oJM + d * side_j + y;
+            }
+        } else {
+            const int c = (x - oC) / nzh, h = x - oC - c * nzh;
+            // corner c of mine; corner c of the diagonal tile that it touches
+            // lands on my opposite corner
+            const int cli = (c & 2) ? TI - 1 : 0, clj = (c & 1) ? TJ - 1 : 0;
+            ps = lrow(cli, clj) + h;
+            const int di = (c & 2) ? -1 : 1, dj = (c & 1) ? -1 : 1;  // the tile whose corner c faces me
+            fd = lrow(di > 0 ? TI : -1, dj > 0 ? TJ : -1) + h;
+            const bool ex = (di > 0 ? has_ip : has_im) && (dj > 0 ? has_jp : has_jm);
+            if (ex) off = (t + di * PJ + dj) * xstride + oC + c * nzh + h;
+        }
+        src_b[nfetch] = off >= 0 ? 8u * (unsigned)off : 0u;
+        if (off >= 0) fmask |= 1u << nfetch;
+        fx[nfetch] = (unsigned)ps | (unsigned)fd << 16;
+        nfetch++;
+    }
+    __syncthreads();
+
+    // the update over this lane's slots of one row set; colour C reads the
+    // other colour's array and writes its own
+    auto sweep = [&](auto cc, auto set_, float m) -> float {
+        constexpr int C = decltype(cc)::value;
+        constexpr int SET = decltype(set_)::value;
+        constexpr int B = SET == 0 ? 0 : (SET == 1 ? SI : SI + SR), NS = SET == 0 ? SI : (SET == 1 ? SR : SH);
+        const int ns = SET == 0 ? ns_in : (SET == 1 ? ns_rg : ns_hr);
+        constexpr unsigned OB = C ? 0 : sizeof(float) * L::ext1, DB = C ? sizeof(float) * L::ext1 : 0;
+#pragma unroll
+        for (int u = 0; u < NS; u++) asm volatile("" : "+v"(D[B + u]));
+#pragma unroll
+        for (int u = 0; u < NS; u++) {
+            if (u >= ns) break;
+            const int s = B + u;
+            const unsigned d = D[s];
+            const unsigned a = d & 0xfffffu;
+            const lds_float *o = (const lds_float *)(size_t)(a + OB);
+            const float xm = o[0], ym = o[XS - P], yp = o[XS + P], xp = o[2 * XS];
+            const lds_float *oz = o + ((d >> (20 + C)) & 1u);
+            const float zm = oz[XS - 1], zp = oz[XS];
+            float rs;
+            if constexpr (C == 0)
+                rs = R0[s];
+            else
+                rs = R1[s < SI + SR ? s : 0];
+            const float sv = (xp + xm) * yx + (yp + ym) * yy + (zp + zm) * yz + rs;
+            lds_float *dv = (lds_float *)(size_t)(a + DB + sizeof(float) * XS);
+            const float po = *dv;
+            float nv = om1 * po + om * (sv * inv);
+            nv = ((d >> (22 + C)) & 1u) ? nv : po;
+            *dv = nv;
+            m = fmaxf(m, fabsf(nv - po));
+        }
+        return m;
+    };
+    using C0 = std::integral_constant<int, 0>;
+    using C1 = std::integral_constant<int, 1>;
+    using In = std::integral_constant<int, 0>;
+    using Rg = std::integral_constant<int, 1>;
+    using Hr = std::integral_constant<int, 2>;
+
+    auto publish = [&](unsigned long long *xw, unsigned tag) {
+        unsigned b = 8u * (unsigned)(t * xstride + tid);
+        asm volatile("" : "+v"(b));
+#pragma unroll
+        for (int q = 0; q < kFetchMaxD; q++)
+            if (q < nfetch)
+                st_tagged((unsigned long long *)((char *)xw + (b + 8u * q * NT)), x1[fx[q] & 0xffffu], tag);
+    };
+
+    // poll this thread's neighbour words until they carry `tag`, into the
+    // black border; all loads of a round are in flight together.  Returns
+    // nonzero (block-uniform) on error.
+    auto fetch = [&](const unsigned long long *xs, unsigned tag) -> int {
+        unsigned pending = fmask;
+        unsigned sb[kFetchMaxD];
+#pragma unroll
+        for (int q = 0; q < kFetchMaxD; q++) {
+            sb[q] = src_b[q];
+            asm volatile("" : "+v"(sb[q]));
+        }
+        long long spin = 0;
+        int bad = 0;
+        while (pending && !bad) {
+            unsigned long long w[kFetchMaxD];
+#pragma unroll
+            for (int q = 0; q < kFetchMaxD; q++)
+                if (pending & (1u << q)) w[q] = ld_tagged((const unsigned long long *)((const char *)xs + sb[q]));
+#pragma unroll
+            for (int q = 0; q < kFetchMaxD; q++) {
+                if ((pending & (1u << q)) && (unsigned)(w[q] >> 32) >= tag) {
+                    x1[fx[q] >> 16] = __uint_as_float((unsigned)w[q]);
+                    pending &= ~(1u << q);
+                }
+            }
+            if (++spin > kSpinLimit) bad = 1;
+            else if ((spin & 255) == 0 && ld_dev_i(sy.err)) bad = 1;
+            if (pending) __builtin_amdgcn_s_sleep(1);
+        }
+        if (bad) {
+            st_dev_i(sy.err, 1);
+            s_bad = 1;
+        }
+        __syncthreads();
+        return s_bad;
+    };
+
+    // wave 0: max over all tiles of the write-once slot row `row_it`
+    auto poll_slots = [&](int row_it, int parity) {
+        constexpr int kSlotPoll32 = slot_poll32<NT>();
+        float v[kSlotPoll32];
+        int got = 0, bad = 0;
+        const float *row = sy.slots + (long long)row_it * nT;
+#pragma unroll
+        for (int q = 0; q < kSlotPoll32; q++) {
+            v[q] = 0.0f;
+            if (lane + q * kWave >= nT) got |= 1 << q;
+        }
+        long long spin = 0;
+        while (!__all(got == (1 << kSlotPoll32) - 1) && !bad) {
+#pragma unroll
+            for (int q = 0; q < kSlotPoll32; q++)
+                if (!(got & (1 << q))) v[q] = ld_dev_f(row + lane + q * kWave);
+#pragma unroll
+            for (int q = 0; q < kSlotPoll32; q++)
+                if (!(got & (1 << q)) && v[q] >= 0.0f) got |= 1 << q;
+            if (++spin > kSpinLimit || ((spin & 255) == 0 && ld_dev_i(sy.err))) bad = 1;
+            if (got != (1 << kSlotPoll32) - 1) __builtin_amdgcn_s_sleep(1);
+        }
+        float g = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kSlotPoll32; q++) g = v[q] > g ? v[q] : g;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const float w = __shfl_xor(g, o);
+            g = w > g ? w : g;
+        }
+        bad = __any(bad);
+        if (lane == 0) {
+            if (bad) {
+                st_dev_i(sy.err, 1);
+                s_bad = 1;
+            }
+            s_max[parity] = g;
+        }
+    };
+
+    int iters = max_it;
+    float tested = -1.0f;
+    for (int it = 0; it < max_it; it++) {
+        const unsigned tag = (unsigned)it + 1u;
+        float m = sweep(C0{}, In{}, 0.0f);
+        if (it >= lag && wave == 0) poll_slots(it - lag, it & 1);
+        if (it > 0) {
+            if (fetch((it - 1) & 1 ? sy.xb : sy.xr, tag - 1u)) break;
+        } else {
+            __syncthreads();
+        }
+        m = sweep(C0{}, Rg{}, m);
+        (void)sweep(C0{}, Hr{}, 0.0f);  // the border ring: not this tile's nodes
+        __syncthreads();
+        m = sweep(C1{}, In{}, m);
+        m = sweep(C1{}, Rg{}, m);
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const float w = __shfl_xor(m, o);
+            m = w > m ? w : m;
+        }
+        if (lane == 0) red[wave] = m;
+        __syncthreads();
+        publish(it & 1 ? sy.xb : sy.xr, tag);
+        if (tid == 0) {
+            float bm = 0.0f;
+            for (int w = 0; w < nwaves; w++) bm = red[w] > bm ? red[w] : bm;
+            st_dev_f(sy.slots + (long long)it * nT + t, bm);
+        }
+        if (it >= lag) {
+            tested = s_max[it & 1];
+            if (tested < tol) {
+                iters = it + 1;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < TI * TJ * nzh; e += NT) {
+        const int r = e / nzh, h = e - r * nzh;
+        const int li = r / TJ, lj = r - li * TJ;
+        const int i = i0 + li, j = j0 + lj;
+        if (i < nx && j < ny) {
+            const long long q = ((long long)i * ny + j) * nzh + h;
+            const int w = lrow(li, lj) + h;
+            delta[q] = x0[w];
+            delta[sp.Gh + q] = x1[w];
+        }
+    }
+    if (t == 0 && tid == 0) {
+        sy.result[0] = (float)iters;
+        sy.result[1] = tested;
+    }
+}
+
 // lane-owned shapes: the ensemble kernel's (tiles up to 16 x 12 rows of 48
 // slots: 96^3 grids and smaller) and the single-grid kernel's (8 x 8 rows of
 // 64 slots: 128^3 on 256 CUs)
@@ -405,12 +801,24 @@ __device__ __forceinline__ void res32o_solve_tile(float *__restrict__ delta, con
 #define HV_OWNED_SOLO 3, 2, 64, 10, 8
 constexpr int kOwnedBatchP = 48, kOwnedBatchQJ = 14, kOwnedBatchIM = 16;
 constexpr int kOwnedSoloP = 64, kOwnedSoloQJ = 10, kOwnedSoloIM = 8;
+// deep-halo shapes: <SI, SR, SH, P, QJ, IM>
+#define HV_DEEP_BATCH 7, 3, 3, 48, 16, 16
+#define HV_DEEP_SOLO 3, 2, 2, 64, 12, 8
+constexpr int kDeepBatchP = 48, kDeepBatchQJ = 16, kDeepBatchIM = 16;
+constexpr int kDeepSoloP = 64, kDeepSoloQJ = 12, kDeepSoloIM = 8;
 
 template <int SI, int SR, int P, int QJ, int IM>
 __global__ void __launch_bounds__(kRes32Threads)
 k_sor_resident_f32o(float *__restrict__ delta, const float *__restrict__ R, Split sp, ResTiles tl, SorCoef cf,
                     int max_it, int lag, ResSync32 sy) {
     res32o_solve_tile<SI, SR, P, QJ, IM>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
+}
+
+template <int SI, int SR, int SH, int P, int QJ, int IM>
+__global__ void __launch_bounds__(kRes32Threads)
+k_sor_resident_f32d(float *__restrict__ delta, const float *__restrict__ R, Split sp, ResTiles tl, SorCoef cf,
+                    int max_it, int lag, ResSync32 sy) {
+    res32d_solve_tile<SI, SR, SH, P, QJ, IM>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
 }
 
 }  // namespace hv

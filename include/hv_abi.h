@@ -257,11 +257,17 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * tiles; a tile's wait for its neighbours' faces overlaps the other tile's
  * sweeps).  Same bits either way. */
 #define HV_OPT_SOR_F32_TILES_PER_CU 9
-/* HV_OPT_SOR_F32_OWNED: run the LDS-resident FP32 correction with the
- * lane-owned body where the tiling fits it (every node of a tile updated by a
- * fixed lane from precomputed descriptors; tiles with a one-row border in
- * LDS) instead of the body that derives each node's indexes in every sweep
- * (default 1; for a batch: when every instance fits).  Same bits either way. */
+/* HV_OPT_SOR_F32_OWNED: the body of the LDS-resident FP32 correction, the
+ * highest that fits the tiling (for a batch: every instance's):
+ *   2 (default) deep-halo lane-owned: one neighbour exchange per iteration
+ *     (two face rows and the corners; the red ring around the tile is
+ *     recomputed from them),
+ *   1 lane-owned: every node of a tile updated by a fixed lane from
+ *     precomputed descriptors, tiles with a one-row border in LDS, two
+ *     exchanges per iteration,
+ *   0 the body that derives each node's indexes in every sweep.
+ * Same bits for every value.  hv_get_option / hv_batch_get_option report the
+ * body in use once the form has solved. */
 #define HV_OPT_SOR_F32_OWNED 10
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);

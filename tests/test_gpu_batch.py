@@ -60,8 +60,8 @@ def _instances_res():
 @pytest.mark.parametrize("form", ["f64", "mixed_sweeps", "mixed_resident"])
 def test_batch_equals_solo_runs(form):
     """F64; mixed with per-sweep FP32 corrections; mixed with the ensemble
-    LDS-resident kernel (lane-owned body) against solo LDS-resident solves
-    (the body that derives node indexes per sweep)."""
+    LDS-resident kernel (deep-halo lane-owned body) against solo LDS-resident
+    solves (the body that derives node indexes per sweep)."""
     print(f"batch vs solo form={form}", flush=True)
     prec = hvamd.HV_PREC_F64 if form == "f64" else hvamd.HV_PREC_MIXED
     resident = form == "mixed_resident"
@@ -90,7 +90,7 @@ def test_batch_equals_solo_runs(form):
     b.close()
     assert res_launches == (sum(chunks) if resident else 0)
     if resident:
-        assert owned == 1
+        assert owned == 2
     for i, w in enumerate(ws):
         s_it0, s_stats, s_its, s_part, s_e = _solo(w, params[i], prec, chunks, 2, resident)
         print(i, w.name, "iters", list(its[i]), "solo", list(s_its))
@@ -162,14 +162,14 @@ def test_batch_step_graph_replays_equal_eager_steps(prec):
             assert np.array_equal(pe[key], pg[key]), key
 
 
-@pytest.mark.parametrize("owned", [0, 1])
+@pytest.mark.parametrize("owned", [0, 1, 2])
 @pytest.mark.parametrize("big", [96, 32])
 def test_batch_resident_shape_bodies_equal_solo(big, owned):
-    """Without the lane-owned body the ensemble kernel runs its compile-time
-    tile body on the grid shape with the most nodes (16 x 12 rows x 48 slots
-    on 96^3, 32 x 16 x 16 on 32^3) and the run-time-shape body on the others;
-    with it, the lane-owned body on every tile: each against solo LDS-resident
-    solves with the other body, bit for bit."""
+    """With the per-row body the ensemble kernel runs its compile-time tile
+    body on the grid shape with the most nodes (16 x 12 rows x 48 slots on
+    96^3, 32 x 16 x 16 on 32^3) and the run-time-shape body on the others;
+    with the lane-owned (1) or deep-halo (2) body, that body on every tile:
+    each against solo LDS-resident solves with another body, bit for bit."""
     print(f"batch resident shapes big={big} owned={owned}", flush=True)
     ws, params = _instances()
     ws = ws[:2]
@@ -189,7 +189,7 @@ def test_batch_resident_shape_bodies_equal_solo(big, owned):
     b.close()
     for i, w in enumerate(ws):
         s_it0, s_stats, s_its, s_part, _ = _solo(w, params[i], hvamd.HV_PREC_MIXED, [4], 2, resident=True,
-                                                 owned=1 - owned)
+                                                 owned=2 if owned == 0 else 0)
         print(i, w.grid, "iters", list(i1[i]), "solo", list(s_its), flush=True)
         assert it0[i] == s_it0
         assert np.array_equal(i1[i], s_its)
