@@ -444,7 +444,10 @@ __host__ __device__ inline bool deep_fits(const ResTiles &t) {
            DeepLayout<P, QJ, IM>::red < (1 << 18);
 }
 
-template <int SI, int SR, int SH, int P, int QJ, int IM, int NT = kRes32Threads>
+// MP: the convergence poll rides in the rounds of the neighbour fetch (one
+// round trip fewer per iteration; more registers, so the ensemble kernel,
+// whose job loop leaves fewer, polls on its own)
+template <int SI, int SR, int SH, int P, int QJ, int IM, bool MP = true, int NT = kRes32Threads>
 __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, const float *__restrict__ R, Split sp,
                                                   ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
                                                   const int t) {
@@ -664,10 +667,14 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
                 st_tagged((unsigned long long *)((char *)xw + (b + 8u * q * NT)), x1[fx[q] & 0xffffu], tag);
     };
 
-    // poll this thread's neighbour words until they carry `tag`, into the
-    // black border; all loads of a round are in flight together.  Returns
-    // nonzero (block-uniform) on error.
-    auto fetch = [&](const unsigned long long *xs, unsigned tag) -> int {
+    // Poll this thread's neighbour words until they carry `tag`, into the
+    // black border, and (poll_row >= 0) in wave 0 the write-once slot row
+    // poll_row of every tile's max: both in the same rounds of loads, so the
+    // convergence test costs no round trip of its own.  Wave 0 then leaves
+    // the max over all tiles in s_max[parity].  Returns nonzero
+    // (block-uniform) on error.
+    auto fetch = [&](const unsigned long long *xs, unsigned tag, int poll_row, int parity) -> int {
+        constexpr int kSlotPoll32 = slot_poll32<NT>();
         unsigned pending = fmask;
         unsigned sb[kFetchMaxD];
 #pragma unroll
@@ -675,13 +682,28 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
             sb[q] = src_b[q];
             asm volatile("" : "+v"(sb[q]));
         }
+        const bool slots = MP && poll_row >= 0 && wave == 0;
+        const float *row = sy.slots + (long long)(poll_row > 0 ? poll_row : 0) * nT;
+        float v[kSlotPoll32];
+        unsigned spend = 0;
+#pragma unroll
+        for (int q = 0; q < kSlotPoll32; q++) {
+            v[q] = 0.0f;
+            if (slots && lane + q * kWave < nT) spend |= 1u << q;
+        }
         long long spin = 0;
         int bad = 0;
-        while (pending && !bad) {
+        while ((pending || spend) && !bad) {
             unsigned long long w[kFetchMaxD];
 #pragma unroll
             for (int q = 0; q < kFetchMaxD; q++)
                 if (pending & (1u << q)) w[q] = ld_tagged((const unsigned long long *)((const char *)xs + sb[q]));
+            float sv[kSlotPoll32];
+            if constexpr (MP) {
+#pragma unroll
+                for (int q = 0; q < kSlotPoll32; q++)
+                    if (spend & (1u << q)) sv[q] = ld_dev_f(row + lane + q * kWave);
+            }
 #pragma unroll
             for (int q = 0; q < kFetchMaxD; q++) {
                 if ((pending & (1u << q)) && (unsigned)(w[q] >> 32) >= tag) {
@@ -689,9 +711,28 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
                     pending &= ~(1u << q);
                 }
             }
+            if constexpr (MP) {
+#pragma unroll
+                for (int q = 0; q < kSlotPoll32; q++) {
+                    if ((spend & (1u << q)) && sv[q] >= 0.0f) {
+                        v[q] = sv[q];
+                        spend &= ~(1u << q);
+                    }
+                }
+            }
             if (++spin > kSpinLimit) bad = 1;
             else if ((spin & 255) == 0 && ld_dev_i(sy.err)) bad = 1;
-            if (pending) __builtin_amdgcn_s_sleep(1);
+            if (pending || spend) __builtin_amdgcn_s_sleep(1);
+        }
+        if (MP && slots) {
+            float g = 0.0f;
+#pragma unroll
+            for (int q = 0; q < kSlotPoll32; q++) g = v[q] > g ? v[q] : g;
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                const float u = __shfl_xor(g, o);
+                g = u > g ? u : g;
+            }
+            if (lane == 0) s_max[parity] = g;
         }
         if (bad) {
             st_dev_i(sy.err, 1);
@@ -701,7 +742,20 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
         return s_bad;
     };
 
-    // wave 0: max over all tiles of the write-once slot row `row_it`
+#ifdef HV_RES_TRACE
+    // phase clocks of tile 0, thread 0, iterations 64-79 (trace builds only;
+    // kept in LDS during the loop so that no global store joins a barrier)
+    __shared__ unsigned long long s_trace[256];
+#define HV_TRACE_D(slot)                                                                                   \
+    do {                                                                                                   \
+        if (t == 0 && tid == 0 && it >= 64 && it < 80) s_trace[(it - 64) * 16 + (slot)] = wall_clock64(); \
+    } while (0)
+#else
+#define HV_TRACE_D(slot) \
+    do {                  \
+    } while (0)
+#endif
+    // wave 0, without MP: max over all tiles of the write-once slot row
     auto poll_slots = [&](int row_it, int parity) {
         constexpr int kSlotPoll32 = slot_poll32<NT>();
         float v[kSlotPoll32];
@@ -744,25 +798,35 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
     float tested = -1.0f;
     for (int it = 0; it < max_it; it++) {
         const unsigned tag = (unsigned)it + 1u;
+        HV_TRACE_D(0);
         float m = sweep(C0{}, In{}, 0.0f);
-        if (it >= lag && wave == 0) poll_slots(it - lag, it & 1);
+        HV_TRACE_D(1);
+        if (!MP && it >= lag && wave == 0) poll_slots(it - lag, it & 1);
+        HV_TRACE_D(2);
+        // (lag >= 1: with MP, every iteration that tests has a fetch)
         if (it > 0) {
-            if (fetch((it - 1) & 1 ? sy.xb : sy.xr, tag - 1u)) break;
+            if (fetch((it - 1) & 1 ? sy.xb : sy.xr, tag - 1u, MP && it >= lag ? it - lag : -1, it & 1)) break;
         } else {
             __syncthreads();
         }
+        HV_TRACE_D(3);
         m = sweep(C0{}, Rg{}, m);
         (void)sweep(C0{}, Hr{}, 0.0f);  // the border ring: not this tile's nodes
+        HV_TRACE_D(4);
         __syncthreads();
+        HV_TRACE_D(5);
         m = sweep(C1{}, In{}, m);
         m = sweep(C1{}, Rg{}, m);
+        HV_TRACE_D(6);
         for (int o = kWave / 2; o > 0; o >>= 1) {
             const float w = __shfl_xor(m, o);
             m = w > m ? w : m;
         }
         if (lane == 0) red[wave] = m;
         __syncthreads();
+        HV_TRACE_D(7);
         publish(it & 1 ? sy.xb : sy.xr, tag);
+        HV_TRACE_D(8);
         if (tid == 0) {
             float bm = 0.0f;
             for (int w = 0; w < nwaves; w++) bm = red[w] > bm ? red[w] : bm;
@@ -792,6 +856,11 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
         sy.result[0] = (float)iters;
         sy.result[1] = tested;
     }
+#ifdef HV_RES_TRACE
+    if (t == 0 && sy.trace)
+        for (int q = tid; q < 256; q += NT) sy.trace[q] = s_trace[q];
+#endif
+#undef HV_TRACE_D
 }
 
 // lane-owned shapes: the ensemble kernel's (tiles up to 16 x 12 rows of 48
