@@ -989,6 +989,20 @@ struct SorCtl {
     int iters;
 };
 
+// The FP32 correction sweep of the mixed form (L delta = R): the FP64 rule's
+// stencil sum and relaxation with each product fused into its sum (one
+// rounding each; the FP32 stage only brings delta near the solution, the
+// FP64 finishing sweeps apply the exact rule).  Every FP32 sweep form goes
+// through these two functions, so all of them give the same bits.
+__device__ __forceinline__ float sor_f32_sum(float xm, float xp, float ym, float yp, float zm, float zp, float src,
+                                             float yx, float yy, float yz) {
+    return __builtin_fmaf(zp + zm, yz, __builtin_fmaf(yp + ym, yy, (xp + xm) * yx)) + src;
+}
+
+__device__ __forceinline__ float sor_f32_relax(float po, float s, float inv, float om, float om1) {
+    return __builtin_fmaf(om1, po, om * (s * inv));
+}
+
 // Max |dphi| of iteration it: max over the kMaxSlots slots of row it.
 // Called by whole waves: lane l reads slot l (kMaxSlots == wave width).
 __device__ __forceinline__ double sor_row_max(const unsigned long long *__restrict__ maxd, int it) {
@@ -1182,15 +1196,15 @@ __device__ __forceinline__ void sor_colour_f32_body(const int bid, float *__rest
             sum[di] = 0.0f;
             old[di] = 0.0f;
             if (ok[di]) {
-                sum[di] = (oth[q + sx] + oth[q - sx]) * yx + (oth[q + nzh] + oth[q - nzh]) * yy +
-                          (oth[q + off] + oth[q + off - 1]) * yz + src[q];
+                sum[di] = sor_f32_sum(oth[q - sx], oth[q + sx], oth[q - nzh], oth[q + nzh], oth[q + off - 1],
+                                      oth[q + off], src[q], yx, yy, yz);
                 old[di] = dst[q];
             }
         }
 #pragma unroll
         for (int di = 0; di < P; di++) {
             if (!ok[di]) continue;
-            const float nv = om1 * old[di] + om * (sum[di] * inv);
+            const float nv = sor_f32_relax(old[di], sum[di], inv, om, om1);
             dst[qq[di]] = nv;
             const float d = fabsf(nv - old[di]);
             diff = d > diff ? d : diff;
@@ -1248,9 +1262,9 @@ __device__ __forceinline__ VT sor_update(VT xm, VT xp, VT ym, VT yp, VT zm, VT z
         const double phi_star = (phi_x + phi_y + phi_z + cf.src * (double)srcv) * cf.inv_dx2;
         return cf.one_minus_omega * old + cf.omega * phi_star;
     } else {
-        const float s = (xp + xm) * (float)cf.ydx2 + (yp + ym) * (float)cf.ydy2 + (zp + zm) * (float)cf.ydz2 +
-                        (float)srcv;
-        return (float)cf.one_minus_omega * old + (float)cf.omega * (s * (float)cf.inv_dx2);
+        const float s = sor_f32_sum(xm, xp, ym, yp, zm, zp, (float)srcv, (float)cf.ydx2, (float)cf.ydy2,
+                                    (float)cf.ydz2);
+        return sor_f32_relax(old, s, (float)cf.inv_dx2, (float)cf.omega, (float)cf.one_minus_omega);
     }
 }
 

@@ -51,6 +51,19 @@ __host__ __device__ inline bool owned_fits(const ResTiles &t) {
 
 typedef __attribute__((address_space(3))) float lds_float;
 
+// max over a wave of non-negative, non-NaN floats (their bit patterns order
+// them), in every lane: DPP row shifts and row broadcasts into lane 63
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+    unsigned u = __float_as_uint(v);
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x111, 0xf, 0xf, false));  // row_shr:1
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x112, 0xf, 0xf, false));  // row_shr:2
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x114, 0xf, 0xf, false));  // row_shr:4
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x118, 0xf, 0xf, false));  // row_shr:8
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return __uint_as_float((unsigned)__builtin_amdgcn_readlane((int)u, 63));
+}
+
 template <int SI, int SR, int P, int QJ, int IM, int NT = kRes32Threads>
 __device__ __forceinline__ void res32o_solve_tile(float *__restrict__ delta, const float *__restrict__ R, Split sp,
                                                   ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
@@ -235,10 +248,10 @@ __device__ __forceinline__ void res32o_solve_tile(float *__restrict__ delta, con
             const float xm = o[0], ym = o[XS - P], yp = o[XS + P], xp = o[2 * XS];
             const lds_float *oz = o + ((d >> (20 + C)) & 1u);
             const float zm = oz[XS - 1], zp = oz[XS];
-            const float sv = (xp + xm) * yx + (yp + ym) * yy + (zp + zm) * yz + (C ? R1[s] : R0[s]);
+            const float sv = sor_f32_sum(xm, xp, ym, yp, zm, zp, C ? R1[s] : R0[s], yx, yy, yz);
             lds_float *dv = (lds_float *)(size_t)(a + DB + sizeof(float) * XS);
             const float po = *dv;
-            float nv = om1 * po + om * (sv * inv);
+            float nv = sor_f32_relax(po, sv, inv, om, om1);
             nv = ((d >> (22 + C)) & 1u) ? nv : po;
             *dv = nv;
             m = fmaxf(m, fabsf(nv - po));
@@ -360,10 +373,7 @@ __device__ __forceinline__ void res32o_solve_tile(float *__restrict__ delta, con
         m = sweep(C1{}, In{}, m);
         if (fetch(x0, sy.xr, tag)) break;
         m = sweep(C1{}, Rg{}, m);
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            const float w = __shfl_xor(m, o);
-            m = w > m ? w : m;
-        }
+        m = wave_max_nonneg(m);  // the lanes' maxima are never NaN (fmaxf)
         if (lane == 0) red[wave] = m;
         __syncthreads();
         publish(x1, sy.xb, tag);
@@ -642,10 +652,10 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
                 rs = R0[s];
             else
                 rs = R1[s < SI + SR ? s : 0];
-            const float sv = (xp + xm) * yx + (yp + ym) * yy + (zp + zm) * yz + rs;
+            const float sv = sor_f32_sum(xm, xp, ym, yp, zm, zp, rs, yx, yy, yz);
             lds_float *dv = (lds_float *)(size_t)(a + DB + sizeof(float) * XS);
             const float po = *dv;
-            float nv = om1 * po + om * (sv * inv);
+            float nv = sor_f32_relax(po, sv, inv, om, om1);
             nv = ((d >> (22 + C)) & 1u) ? nv : po;
             *dv = nv;
             m = fmaxf(m, fabsf(nv - po));
@@ -818,10 +828,7 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
         m = sweep(C1{}, In{}, m);
         m = sweep(C1{}, Rg{}, m);
         HV_TRACE_D(6);
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            const float w = __shfl_xor(m, o);
-            m = w > m ? w : m;
-        }
+        m = wave_max_nonneg(m);  // the lanes' maxima are never NaN (fmaxf)
         if (lane == 0) red[wave] = m;
         __syncthreads();
         HV_TRACE_D(7);

@@ -582,9 +582,9 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
                     yp = lj < TJ - 1 ? oth[e + nzh] : h_jp[li * nzh + h];
                 }
                 const float zm = oth[ez], zp = oth[e + off];
-                const float sv = (xp + xm) * yx + (yp + ym) * yy + (zp + zm) * yz + src[e];
+                const float sv = sor_f32_sum(xm, xp, ym, yp, zm, zp, src[e], yx, yy, yz);
                 po[u] = dst[e];
-                nv[u] = om1 * po[u] + om * (sv * inv);
+                nv[u] = sor_f32_relax(po[u], sv, inv, om, om1);
                 ee[u] = e;
             }
 #pragma unroll
@@ -636,9 +636,9 @@ __device__ __forceinline__ void res32_solve_tile(float *__restrict__ delta, cons
                         yp = lj < TJ - 1 ? oth[e + nzh] : h_jp[li * nzh + h];
                     }
                     const float zm = oth[ez], zp = oth[e + off];
-                    const float s = (xp + xm) * yx + (yp + ym) * yy + (zp + zm) * yz + src[e];
+                    const float s = sor_f32_sum(xm, xp, ym, yp, zm, zp, src[e], yx, yy, yz);
                     po[u] = dst[e];
-                    nv[u] = om1 * po[u] + om * (s * inv);
+                    nv[u] = sor_f32_relax(po[u], s, inv, om, om1);
                     ee[u] = e;
                 }
 #pragma unroll
