@@ -241,7 +241,8 @@ def cpu_baseline(w, state, budget_s: float):
     from oracle_lib import OParams, OSkyrme, Oracle, oworld, p
     o = Oracle()
     L = o.L
-    cores = len(os.sched_getaffinity(0))
+    L.hvo_threads.restype = ctypes.c_int
+    cores = L.hvo_threads()  # the OpenMP team the oracle's loop runs on
     n, d = w.world().n, w.world().d_max
     prm = w.params()
     sk = w.sky()

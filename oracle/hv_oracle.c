@@ -15,6 +15,18 @@
 
 #include "hv_detmath.h"
 
+#include <omp.h>
+
+int hvo_threads(void) {
+    int n = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+        n = omp_get_num_threads();
+    }
+    return n;
+}
+
 #define IDX3(i, j, k, ny, nz) ((((long long)(i)) * (ny) + (j)) * (nz) + (k))
 
 static long long world_size(const hvo_world *w) {

@@ -115,6 +115,10 @@ int hvo_simulate_loop(double *x, double *y, double *z, double *kx, double *ky, d
                       const hvo_skyrme *s, const hvo_params *p, double *phi_inout, double *stats,
                       int *sor_iters);
 
+/* Threads of the oracle's OpenMP parallel regions in this process (what a
+ * timing of the oracle ran on; no reference counterpart). */
+int hvo_threads(void);
+
 /* Total energy of a state (no reference counterpart; hv_energy_density in
  * hv_detmath.h): out = {E_kin + E_sky + E_coul, E_kin, E_sky, E_coul} [MeV],
  * E_kin = (hbar c)^2/(2 m c^2) sum_i |k_i|^2 / ppn, the grid parts
