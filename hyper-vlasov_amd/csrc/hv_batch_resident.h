@@ -103,8 +103,7 @@ kb_sor_resident_f32d(const BRes32 *__restrict__ inst, const int *__restrict__ jo
         if (threadIdx.x == 0) I.jtrace[2 * j] = wall_clock64();
 #endif
         if (deep_fits<SI, SR, SH, P, QJ, IM, kRes32Threads>(I.tl) && I.tl.xstride == deep_xstride(I.tl))
-            res32d_solve_tile<SI, SR, SH, P, QJ, IM, false>(I.delta, I.R, I.sp, I.tl, I.cf, I.max_it, lag, I.sy,
-                                                           jb.y);
+            res32d_solve_tile<SI, SR, SH, P, QJ, IM>(I.delta, I.R, I.sp, I.tl, I.cf, I.max_it, lag, I.sy, jb.y);
         else if (threadIdx.x == 0)
             st_dev_i(I.sy.err, 1);
         __syncthreads();

@@ -455,8 +455,8 @@ __host__ __device__ inline bool deep_fits(const ResTiles &t) {
 }
 
 // MP: the convergence poll rides in the rounds of the neighbour fetch (one
-// round trip fewer per iteration; more registers, so the ensemble kernel,
-// whose job loop leaves fewer, polls on its own)
+// round trip fewer per iteration); without it wave 0 polls the slot row on
+// its own before the fetch
 template <int SI, int SR, int SH, int P, int QJ, int IM, bool MP = true, int NT = kRes32Threads>
 __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, const float *__restrict__ R, Split sp,
                                                   ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
@@ -583,6 +583,8 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
     // of that word in src_b; bit q of fmask: the neighbour exists)
     const int oIM = 0, oIP = 2 * side_i, oJM = 4 * side_i, oJP = 4 * side_i + 2 * side_j, oC = oJP + 2 * side_j;
     unsigned src_b[kFetchMaxD], fx[kFetchMaxD], fmask = 0;
+#pragma unroll
+    for (int q = 0; q < kFetchMaxD; q++) src_b[q] = fx[q] = 0u;
     int nfetch = 0;
     for (int x = tid; x < xstride && nfetch < kFetchMaxD; x += NT) {
         int off = -1, ps, fd;
@@ -703,25 +705,23 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
         }
         long long spin = 0;
         int bad = 0;
-        while ((pending || spend) && !bad) {
-            unsigned long long w[kFetchMaxD];
+        if constexpr (MP) {
+            while ((pending || spend) && !bad) {
+                unsigned long long w[kFetchMaxD];
 #pragma unroll
-            for (int q = 0; q < kFetchMaxD; q++)
-                if (pending & (1u << q)) w[q] = ld_tagged((const unsigned long long *)((const char *)xs + sb[q]));
-            float sv[kSlotPoll32];
-            if constexpr (MP) {
+                for (int q = 0; q < kFetchMaxD; q++)
+                    if (pending & (1u << q)) w[q] = ld_tagged((const unsigned long long *)((const char *)xs + sb[q]));
+                float sv[kSlotPoll32];
 #pragma unroll
                 for (int q = 0; q < kSlotPoll32; q++)
                     if (spend & (1u << q)) sv[q] = ld_dev_f(row + lane + q * kWave);
-            }
 #pragma unroll
-            for (int q = 0; q < kFetchMaxD; q++) {
-                if ((pending & (1u << q)) && (unsigned)(w[q] >> 32) >= tag) {
-                    x1[fx[q] >> 16] = __uint_as_float((unsigned)w[q]);
-                    pending &= ~(1u << q);
+                for (int q = 0; q < kFetchMaxD; q++) {
+                    if ((pending & (1u << q)) && (unsigned)(w[q] >> 32) >= tag) {
+                        x1[fx[q] >> 16] = __uint_as_float((unsigned)w[q]);
+                        pending &= ~(1u << q);
+                    }
                 }
-            }
-            if constexpr (MP) {
 #pragma unroll
                 for (int q = 0; q < kSlotPoll32; q++) {
                     if ((spend & (1u << q)) && sv[q] >= 0.0f) {
@@ -729,10 +729,27 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
                         spend &= ~(1u << q);
                     }
                 }
+                if (++spin > kSpinLimit) bad = 1;
+                else if ((spin & 255) == 0 && ld_dev_i(sy.err)) bad = 1;
+                if (pending || spend) __builtin_amdgcn_s_sleep(1);
             }
-            if (++spin > kSpinLimit) bad = 1;
-            else if ((spin & 255) == 0 && ld_dev_i(sy.err)) bad = 1;
-            if (pending || spend) __builtin_amdgcn_s_sleep(1);
+        } else {
+            while ((pending || spend) && !bad) {
+                unsigned long long w[kFetchMaxD];
+#pragma unroll
+                for (int q = 0; q < kFetchMaxD; q++)
+                    if (pending & (1u << q)) w[q] = ld_tagged((const unsigned long long *)((const char *)xs + sb[q]));
+#pragma unroll
+                for (int q = 0; q < kFetchMaxD; q++) {
+                    if ((pending & (1u << q)) && (unsigned)(w[q] >> 32) >= tag) {
+                        x1[fx[q] >> 16] = __uint_as_float((unsigned)w[q]);
+                        pending &= ~(1u << q);
+                    }
+                }
+                if (++spin > kSpinLimit) bad = 1;
+                else if ((spin & 255) == 0 && ld_dev_i(sy.err)) bad = 1;
+                if (pending) __builtin_amdgcn_s_sleep(1);
+            }
         }
         if (MP && slots) {
             float g = 0.0f;
