@@ -609,12 +609,16 @@ int res32_setup(hv_ctx *c) {
                (long long)best.PI * best.PJ <= (long long)occ * ncu;
     };
     bool deep = false;
-    if (per == 1 && c->res32_owned >= 2) {
+    if (c->res32_owned >= 2) {
         ResTiles d = best;
         d.xstride = deep_xstride(d);
-        const void *k = (const void *)k_sor_resident_f32d<HV_DEEP_SOLO>;
-        const size_t l = DeepLayout<kDeepSoloP, kDeepSoloQJ, kDeepSoloIM>::bytes(kRes32Threads / kWave);
-        if (deep_fits<HV_DEEP_SOLO, kRes32Threads>(d) && try_kernel(k, l)) {
+        const void *k = per == 2 ? (const void *)k_sor_resident_f32d<HV_DEEP_SOLO2, 512>
+                                 : (const void *)k_sor_resident_f32d<HV_DEEP_SOLO>;
+        const size_t l = per == 2 ? DeepLayout<kDeepSolo2P, kDeepSolo2QJ, kDeepSolo2IM>::bytes(512 / kWave)
+                                  : DeepLayout<kDeepSoloP, kDeepSoloQJ, kDeepSoloIM>::bytes(kRes32Threads / kWave);
+        const bool fits =
+            per == 2 ? deep_fits<HV_DEEP_SOLO2, 512>(d) : deep_fits<HV_DEEP_SOLO, kRes32Threads>(d);
+        if (fits && try_kernel(k, l)) {
             best = d;
             c->res32_kernel = k;
             lds = l;

@@ -436,7 +436,10 @@ struct DeepLayout {
     static constexpr size_t bytes(int nwaves) { return sizeof(float) * ((size_t)red + (size_t)nwaves + 3); }
 };
 
-constexpr int kFetchMaxD = 6;  // exchange words per thread of the deep-halo body
+// exchange words per thread of the deep-halo body: 6 with 1024-thread
+// blocks (one tile per CU), 8 with 512-thread blocks (two per CU)
+template <int NT>
+constexpr int fetch_max_d() { return NT == 512 ? 8 : 6; }
 
 // exchange words per tile and buffer: two rows along each face, four corners
 __host__ __device__ inline int deep_xstride(const ResTiles &t) { return 4 * (t.TI + t.TJ) * t.nzh + 4 * t.nzh; }
@@ -446,7 +449,7 @@ __host__ __device__ inline int deep_xstride(const ResTiles &t) { return 4 * (t.T
 template <int SI, int SR, int SH, int P, int QJ, int IM, int NT>
 __host__ __device__ inline bool deep_fits(const ResTiles &t) {
     if (t.TI > IM || t.TJ + 4 > QJ || t.nzh > P || t.TI < 3 || t.TJ < 3) return false;
-    if (deep_xstride(t) > kFetchMaxD * NT) return false;
+    if (deep_xstride(t) > fetch_max_d<NT>() * NT) return false;
     const int n_inner = (t.TI - 2) * (t.TJ - 2), n_ring = t.TI * t.TJ - n_inner, n_halo = 2 * (t.TI + t.TJ);
     const int nw = NT / kWave;
     auto ch = [&](int rows) { return (rows * t.nzh + kWave - 1) / kWave; };
@@ -462,6 +465,7 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
                                                   ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
                                                   const int t) {
     using L = DeepLayout<P, QJ, IM>;
+    constexpr int kFetchMaxD = fetch_max_d<NT>();
     extern __shared__ unsigned char smem[];
     float *const x0 = (float *)smem;  // red values (bordered rows)
     float *const x1 = x0 + L::ext1;   // black values
@@ -897,8 +901,10 @@ constexpr int kOwnedSoloP = 64, kOwnedSoloQJ = 10, kOwnedSoloIM = 8;
 // deep-halo shapes: <SI, SR, SH, P, QJ, IM>
 #define HV_DEEP_BATCH 7, 3, 3, 48, 16, 16
 #define HV_DEEP_SOLO 3, 2, 2, 64, 12, 8
+#define HV_DEEP_SOLO2 2, 3, 3, 64, 8, 8  // two tiles per CU (512 threads): up to 8 x 4 rows
 constexpr int kDeepBatchP = 48, kDeepBatchQJ = 16, kDeepBatchIM = 16;
 constexpr int kDeepSoloP = 64, kDeepSoloQJ = 12, kDeepSoloIM = 8;
+constexpr int kDeepSolo2P = 64, kDeepSolo2QJ = 8, kDeepSolo2IM = 8;
 
 template <int SI, int SR, int P, int QJ, int IM>
 __global__ void __launch_bounds__(kRes32Threads)
@@ -907,11 +913,13 @@ k_sor_resident_f32o(float *__restrict__ delta, const float *__restrict__ R, Spli
     res32o_solve_tile<SI, SR, P, QJ, IM>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
 }
 
-template <int SI, int SR, int SH, int P, int QJ, int IM>
-__global__ void __launch_bounds__(kRes32Threads)
+// NT threads per block: 1024 (one tile per CU) or 512 (two per CU: one
+// tile's wait for its neighbours' faces overlaps the other's sweeps)
+template <int SI, int SR, int SH, int P, int QJ, int IM, int NT = kRes32Threads>
+__global__ void __launch_bounds__(NT, kRes32Threads / NT)
 k_sor_resident_f32d(float *__restrict__ delta, const float *__restrict__ R, Split sp, ResTiles tl, SorCoef cf,
                     int max_it, int lag, ResSync32 sy) {
-    res32d_solve_tile<SI, SR, SH, P, QJ, IM>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
+    res32d_solve_tile<SI, SR, SH, P, QJ, IM, true, NT>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
 }
 
 }  // namespace hv

@@ -418,18 +418,18 @@ def test_sor_fused_equals_per_sweep(oracle, n, d):
     _eq(outs[0][2], outs[1][2])
 
 
-@pytest.mark.parametrize("form", ["rows", "rows_two_per_cu", "owned", "deep"])
+@pytest.mark.parametrize("form", ["rows", "rows_two_per_cu", "owned", "deep", "deep_two_per_cu"])
 @pytest.mark.parametrize("n", [(48, 48, 48), (40, 36, 50), (64, 64, 64), (96, 96, 96), (128, 128, 128)])
 def test_resident_f32_equals_sweeps_at_fixed_iterations(n, form):
     """With a tolerance no iteration reaches, the FP32 correction runs exactly
     max_it iterations in both forms, so the LDS-resident kernel (its tiles,
     row sets and lane mappings; the body that derives node indexes per sweep,
     one or two tiles per CU, the lane-owned body, or the deep-halo body with
-    one exchange per iteration) and the per-sweep launches must give the same
-    potential bit for bit."""
+    one exchange per iteration, one or two tiles per CU) and the per-sweep
+    launches must give the same potential bit for bit."""
     print("resident FP32 vs sweeps, fixed iterations", n, form, flush=True)
-    per_cu = 2 if form == "rows_two_per_cu" else 1
-    owned = {"owned": 1, "deep": 2}.get(form, 0)
+    per_cu = 2 if form.endswith("two_per_cu") else 1
+    owned = {"owned": 1, "deep": 2, "deep_two_per_cu": 2}.get(form, 0)
     d = tuple(0.2 * x for x in n)
     G = int(np.prod(n))
     rho_p = smooth_field(16, n, 0.09)
