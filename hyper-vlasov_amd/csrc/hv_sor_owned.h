@@ -11,7 +11,7 @@
 //    neighbours at fixed offsets, on the tile's faces as inside it.
 //  * The (row, slot) elements of each row set (interior rows, ring rows) are
 //    cut into 64-element chunks; chunk c is slot c / nwaves of wave
-//    c % nwaves.  Per slot a lane keeps one descriptor word (LDS offset, the
+//    c % nwaves.  Per slot a lane keeps one descriptor word (LDS address, the
 //    k parity and update flag of each colour) and both colours' right-hand
 //    sides in registers, set up once per solve.
 #pragma once
@@ -23,9 +23,9 @@ namespace hv {
 // LDS of the lane-owned body, in floats: the colour-0 array of (IM + 2) * QJ
 // rows of P slots, 64 sink words, the colour-1 array, 64 sink words and a
 // guard of QJ * P + 2 words, then the wave partials, the error flag and the
-// two tested maxima (no static LDS: the arrays start at address 0, which
-// the descriptors' offsets then are).  Lanes without a node (the tail of a
-// partial chunk) read around and store into the sinks.
+// two tested maxima (all in the dynamic LDS; the body declares none of its
+// own).  Lanes without a node (the tail of a partial chunk) read around and
+// store into the sinks.
 template <int P, int QJ, int IM>
 struct OwnedLayout {
     static constexpr int E = (IM + 2) * QJ * P;  // one colour's array
@@ -45,7 +45,7 @@ __host__ __device__ inline bool owned_fits(const ResTiles &t) {
     const int n_ring = t.TI * t.TJ - n_inner;
     const int nw = NT / kWave;
     const int ch_in = (n_inner * t.nzh + kWave - 1) / kWave, ch_rg = (n_ring * t.nzh + kWave - 1) / kWave;
-    // 16-bit-free descriptor offsets: the arrays stay below 2^18 bytes
+    // the descriptors' LDS address field has 20 bits: the arrays stay below 2^18 words
     return ch_in <= SI * nw && ch_rg <= SR * nw && OwnedLayout<P, QJ, IM>::red < (1 << 18);
 }
 
@@ -465,6 +465,7 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
                                                   ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
                                                   const int t) {
     using L = DeepLayout<P, QJ, IM>;
+    static_assert(L::E < (1 << 16), "the exchange words' LDS slots are 16-bit fields");
     constexpr int kFetchMaxD = fetch_max_d<NT>();
     extern __shared__ unsigned char smem[];
     float *const x0 = (float *)smem;  // red values (bordered rows)
