@@ -413,7 +413,8 @@ __device__ __forceinline__ void res32o_solve_tile(float *__restrict__ delta, con
 // the same operands, the same lagged stop test, so the same bits) with one
 // neighbour exchange per iteration instead of two.  After its black sweep a
 // tile publishes its black values of the two rows along each face and of its
-// four corner rows; a neighbour then has every black value within two rows
+// four corner rows (three values to two 64-bit words, each word with the low
+// 16 bits of the iteration tag); a neighbour then has every black value within two rows
 // of its tile (and its diagonal corners) and recomputes, redundantly, the red
 // values of the one-row ring around its tile from them (the update its owner
 // makes, on the same operands in the same order, so the same bits).  Its own
@@ -423,7 +424,9 @@ __device__ __forceinline__ void res32o_solve_tile(float *__restrict__ delta, con
 // The exchange words are double-buffered by iteration parity: a tile
 // overwrites a buffer two iterations after filling it, by which time each
 // neighbour has read it (the neighbour's faces that this tile waited for in
-// between were published after that read).
+// between were published after that read).  A word a reader polls therefore
+// holds the tag it waits for or the one two iterations older (or the clear
+// value 0 before its first write), so 16 bits of tag tell them apart.
 //
 // LDS: each colour's array holds (IM + 4) x QJ rows of P slots, tile row
 // (li, lj) at (li + 2) * QJ + lj + 2 (two border rows on each side), then
@@ -436,13 +439,21 @@ struct DeepLayout {
     static constexpr size_t bytes(int nwaves) { return sizeof(float) * ((size_t)red + (size_t)nwaves + 3); }
 };
 
-// exchange words per thread of the deep-halo body: 6 with 1024-thread
-// blocks (one tile per CU), 8 with 512-thread blocks (two per CU)
+// exchange words per thread of the deep-halo body: 4 with 1024-thread
+// blocks (one tile per CU), 6 with 512-thread blocks (two per CU)
 template <int NT>
-constexpr int fetch_max_d() { return NT == 512 ? 8 : 6; }
+constexpr int fetch_max_d() { return NT == 512 ? 6 : 4; }
+
+// Exchange words of a section of n values: three values ride in two 64-bit
+// words.  Word 2g holds value 3g (bits 0-31) and the low half of value 3g + 1
+// (bits 32-47), word 2g + 1 value 3g + 2 and the high half of 3g + 1; bits
+// 48-63 of each carry the low 16 bits of the iteration tag.
+__host__ __device__ inline int deep_words(int n) { return 2 * ((n + 2) / 3); }
 
 // exchange words per tile and buffer: two rows along each face, four corners
-__host__ __device__ inline int deep_xstride(const ResTiles &t) { return 4 * (t.TI + t.TJ) * t.nzh + 4 * t.nzh; }
+__host__ __device__ inline int deep_xstride(const ResTiles &t) {
+    return 4 * deep_words(t.TJ * t.nzh) + 4 * deep_words(t.TI * t.nzh) + 4 * deep_words(t.nzh);
+}
 
 // does tiling t fit the deep-halo body <SI, SR, SH, P, QJ, IM> of NT threads
 // (slots per lane: SI interior, SR ring, SH border ring)
@@ -582,54 +593,78 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
 
     // exchange words of this thread: word x = tid + q * NT of a tile's slice,
     // laid out as [im rows d = 0, 1][ip rows][jm rows][jp rows][corners
-    // (0,0), (0,TJ-1), (TI-1,0), (TI-1,TJ-1)]: the face row it publishes
-    // (its LDS slot, low 16 bits of fx) and the border row it receives from
-    // the neighbour's matching word (LDS slot in the high bits; byte offset
-    // of that word in src_b; bit q of fmask: the neighbour exists)
-    const int oIM = 0, oIP = 2 * side_i, oJM = 4 * side_i, oJP = 4 * side_i + 2 * side_j, oC = oJP + 2 * side_j;
-    unsigned src_b[kFetchMaxD], fx[kFetchMaxD], fmask = 0;
+    // (0,0), (0,TJ-1), (TI-1,0), (TI-1,TJ-1)], each section packed three
+    // values to two words (deep_words; sections start at even words, so
+    // word x carries the half x & 1 = tid & 1 of its middle value).  Value v
+    // of a section is element v of its rows, row-major.  Per word: fa and fm
+    // hold the LDS slots of its whole value and of its middle value in the
+    // rows it publishes (low 16 bits) and in the border rows it receives
+    // from the neighbour's matching word (high 16 bits; values past a
+    // section's end map to the sink word E); src_b the byte offset of that
+    // word; bit q of fmask: the neighbour exists.
+    const int wi = deep_words(side_i), wj = deep_words(side_j), wc = deep_words(nzh);
+    const int oIM = 0, oIP = 2 * wi, oJM = 4 * wi, oJP = 4 * wi + 2 * wj, oC = oJP + 2 * wj;
+    unsigned src_b[kFetchMaxD], fa[kFetchMaxD], fm[kFetchMaxD], fmask = 0;
 #pragma unroll
-    for (int q = 0; q < kFetchMaxD; q++) src_b[q] = fx[q] = 0u;
+    for (int q = 0; q < kFetchMaxD; q++) src_b[q] = fa[q] = fm[q] = 0u;
     int nfetch = 0;
     for (int x = tid; x < xstride && nfetch < kFetchMaxD; x += NT) {
-        int off = -1, ps, fd;
-        if (x < 4 * side_i) {
-            const int sec = x / side_i, y = x - sec * side_i, lj = y / nzh, h = y - lj * nzh, d = sec & 1;
+        int off = -1, kind, sec, y;  // kind 0: i face, 1: j face, 2: corner
+        if (x < 4 * wi) {
+            kind = 0;
+            sec = x / wi;
+            y = x - sec * wi;
             if (sec < 2) {  // my rows li = d; I receive li = -1 - d from (pi-1)'s ip rows
-                ps = lrow(d, lj) + h;
-                fd = lrow(-1 - d, lj) + h;
-                if (has_im) off = (t - PJ) * xstride + oIP + d * side_i + y;
+                if (has_im) off = (t - PJ) * xstride + oIP + (sec & 1) * wi + y;
             } else {  // my rows li = TI-1-d; I receive li = TI + d from (pi+1)'s im rows
-                ps = lrow(TI - 1 - d, lj) + h;
-                fd = lrow(TI + d, lj) + h;
-                if (has_ip) off = (t + PJ) * xstride + oIM + d * side_i + y;
+                if (has_ip) off = (t + PJ) * xstride + oIM + (sec & 1) * wi + y;
             }
         } else if (x < oC) {
-            const int x2 = x - oJM, sec = x2 / side_j, y = x2 - sec * side_j, li = y / nzh, h = y - li * nzh,
-                      d = sec & 1;
+            kind = 1;
+            sec = (x - oJM) / wj;
+            y = x - oJM - sec * wj;
             if (sec < 2) {  // my rows lj = d; I receive lj = -1 - d from (pj-1)'s jp rows
-                ps = lrow(li, d) + h;
-                fd = lrow(li, -1 - d) + h;
-                if (has_jm) off = (t - 1) * xstride + oJP + d * side_j + y;
+                if (has_jm) off = (t - 1) * xstride + oJP + (sec & 1) * wj + y;
             } else {  // my rows lj = TJ-1-d; I receive lj = TJ + d from (pj+1)'s jm rows
-                ps = lrow(li, TJ - 1 - d) + h;
-                fd = lrow(li, TJ + d) + h;
-                if (has_jp) off = (t + 1) * xstride + oJM + d * side_j + y;
+                if (has_jp) off = (t + 1) * xstride + oJM + (sec & 1) * wj + y;
             }
         } else {
-            const int c = (x - oC) / nzh, h = x - oC - c * nzh;
             // corner c of mine; corner c of the diagonal tile that it touches
             // lands on my opposite corner
-            const int cli = (c & 2) ? TI - 1 : 0, clj = (c & 1) ? TJ - 1 : 0;
-            ps = lrow(cli, clj) + h;
-            const int di = (c & 2) ? -1 : 1, dj = (c & 1) ? -1 : 1;  // the tile whose corner c faces me
-            fd = lrow(di > 0 ? TI : -1, dj > 0 ? TJ : -1) + h;
+            kind = 2;
+            sec = (x - oC) / wc;
+            y = x - oC - sec * wc;
+            const int di = (sec & 2) ? -1 : 1, dj = (sec & 1) ? -1 : 1;  // the tile whose corner c faces me
             const bool ex = (di > 0 ? has_ip : has_im) && (dj > 0 ? has_jp : has_jm);
-            if (ex) off = (t + di * PJ + dj) * xstride + oC + c * nzh + h;
+            if (ex) off = (t + di * PJ + dj) * xstride + oC + sec * wc + y;
         }
+        // LDS slot of value v of this word's section: in the rows published
+        // (recv 0) or in the border rows received (recv 1)
+        auto slot = [&](int v, int recv) -> unsigned {
+            const int d = sec & 1;
+            int r;
+            if (kind == 0) {
+                if (v >= side_i) return (unsigned)L::E;
+                const int lj = v / nzh;
+                r = (sec < 2 ? (recv ? lrow(-1 - d, lj) : lrow(d, lj))
+                             : (recv ? lrow(TI + d, lj) : lrow(TI - 1 - d, lj))) + v - lj * nzh;
+            } else if (kind == 1) {
+                if (v >= side_j) return (unsigned)L::E;
+                const int li = v / nzh;
+                r = (sec < 2 ? (recv ? lrow(li, -1 - d) : lrow(li, d))
+                             : (recv ? lrow(li, TJ + d) : lrow(li, TJ - 1 - d))) + v - li * nzh;
+            } else {
+                if (v >= nzh) return (unsigned)L::E;
+                const int cli = (sec & 2) ? TI - 1 : 0, clj = (sec & 1) ? TJ - 1 : 0;
+                r = (recv ? lrow((sec & 2) ? -1 : TI, (sec & 1) ? -1 : TJ) : lrow(cli, clj)) + v;
+            }
+            return (unsigned)r;
+        };
+        const int g = y >> 1, va = 3 * g + 2 * (y & 1), vm = 3 * g + 1;
         src_b[nfetch] = off >= 0 ? 8u * (unsigned)off : 0u;
         if (off >= 0) fmask |= 1u << nfetch;
-        fx[nfetch] = (unsigned)ps | (unsigned)fd << 16;
+        fa[nfetch] = slot(va, 0) | slot(va, 1) << 16;
+        fm[nfetch] = slot(vm, 0) | slot(vm, 1) << 16;
         nfetch++;
     }
     __syncthreads();
@@ -675,13 +710,26 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
     using Rg = std::integral_constant<int, 1>;
     using Hr = std::integral_constant<int, 2>;
 
+    const unsigned hsel = (unsigned)tid & 1u;  // which half of its middle value a word carries
     auto publish = [&](unsigned long long *xw, unsigned tag) {
         unsigned b = 8u * (unsigned)(t * xstride + tid);
         asm volatile("" : "+v"(b));
+        const unsigned long long tg = (unsigned long long)(tag & 0xffffu) << 48;
 #pragma unroll
         for (int q = 0; q < kFetchMaxD; q++)
-            if (q < nfetch)
-                st_tagged((unsigned long long *)((char *)xw + (b + 8u * q * NT)), x1[fx[q] & 0xffffu], tag);
+            if (q < nfetch) {
+                const unsigned a = __float_as_uint(x1[fa[q] & 0xffffu]);
+                const unsigned mh = (__float_as_uint(x1[fm[q] & 0xffffu]) >> (16u * hsel)) & 0xffffu;
+                __hip_atomic_store((unsigned long long *)((char *)xw + (b + 8u * q * NT)),
+                                   tg | (unsigned long long)mh << 32 | a, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+    };
+    // a received word into the border rows: its whole value, and its half of
+    // the middle value (16-bit LDS store into that half)
+    auto land = [&](unsigned long long w, int q) {
+        x1[fa[q] >> 16] = __uint_as_float((unsigned)w);
+        ((unsigned short *)(x1 + (fm[q] >> 16)))[hsel] = (unsigned short)(w >> 32);
     };
 
     // Poll this thread's neighbour words until they carry `tag`, into the
@@ -722,8 +770,8 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
                     if (spend & (1u << q)) sv[q] = ld_dev_f(row + lane + q * kWave);
 #pragma unroll
                 for (int q = 0; q < kFetchMaxD; q++) {
-                    if ((pending & (1u << q)) && (unsigned)(w[q] >> 32) >= tag) {
-                        x1[fx[q] >> 16] = __uint_as_float((unsigned)w[q]);
+                    if ((pending & (1u << q)) && (unsigned)(w[q] >> 48) == (tag & 0xffffu)) {
+                        land(w[q], q);
                         pending &= ~(1u << q);
                     }
                 }
@@ -746,8 +794,8 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
                     if (pending & (1u << q)) w[q] = ld_tagged((const unsigned long long *)((const char *)xs + sb[q]));
 #pragma unroll
                 for (int q = 0; q < kFetchMaxD; q++) {
-                    if ((pending & (1u << q)) && (unsigned)(w[q] >> 32) >= tag) {
-                        x1[fx[q] >> 16] = __uint_as_float((unsigned)w[q]);
+                    if ((pending & (1u << q)) && (unsigned)(w[q] >> 48) == (tag & 0xffffu)) {
+                        land(w[q], q);
                         pending &= ~(1u << q);
                     }
                 }
