@@ -1606,7 +1606,8 @@ int hv_set_coulomb_bc(hv_ctx *c, int z) {
 
 int hv_coulomb_sor(hv_ctx *c, double omega, double tol, int max_it, int *iters, double *max_diff) {
     if (!c) return HV_ERR_ARG;
-    if (!(tol > 0.0)) return set_err(c, HV_ERR_ARG, "tol must be positive");
+    // tol 0: exactly max_it iterations (no max |dphi| is below 0)
+    if (!(tol >= 0.0)) return set_err(c, HV_ERR_ARG, "tol must be non-negative");
     return HV_DISPATCH(c, sor_solve<double>(c, (const double *)c->density, omega, tol, max_it, iters, max_diff),
                        sor_solve<float>(c, (const float *)c->density, omega, tol, max_it, iters, max_diff));
 }

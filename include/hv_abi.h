@@ -141,7 +141,8 @@ int hv_skyrme_potentials(hv_ctx *ctx, const hv_skyrme *skyrme);
 int hv_set_coulomb_bc(hv_ctx *ctx, int z);
 /* compute_volumetric_coulomb_potentials_sor (include/physics.hpp:100-157):
  * red-black SOR on HV_FIELD_COULOMB with source = proton density.  iters and
- * max_diff (may be NULL) receive the iterations done and the last max|dphi|. */
+ * max_diff (may be NULL) receive the iterations done and the last max|dphi|.
+ * tol >= 0; tol 0 runs exactly max_it iterations. */
 int hv_coulomb_sor(hv_ctx *ctx, double omega, double tol, int max_it, int *iters, double *max_diff);
 /* merge_volumetric_potentials (include/tools.hpp:378-384): U_p += phi */
 int hv_merge_potentials(hv_ctx *ctx);

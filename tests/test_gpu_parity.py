@@ -453,6 +453,35 @@ def test_resident_f32_equals_sweeps_at_fixed_iterations(n, form):
     _eq(out[0][1], out[1][1])
 
 
+def test_resident_f32_tag_wraps():
+    """The deep-halo body's exchange words carry the low 16 bits of the
+    iteration tag.  Past 65,536 iterations the tag wraps to 0, the value the
+    buffers are cleared to; a word still holds the tag of two iterations
+    earlier then, so the solve must go on exactly as before: 65,600 FP32
+    iterations (tolerance 0 never stops) equal the per-sweep launches bit for
+    bit, with no wait running into its spin limit."""
+    print("resident FP32 tag wrap, 48^3, 65600 iterations", flush=True)
+    n = (48, 48, 48)
+    d = tuple(0.2 * x for x in n)
+    G = int(np.prod(n))
+    rho_p = smooth_field(17, n, 0.09)
+    out = {}
+    for res32 in (0, 1):
+        sim = _sim(n, d, 10, 10, hvamd.HV_PREC_MIXED)
+        sim.set_option(hvamd.HV_OPT_SOR_RESIDENT_F32, res32)
+        sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p, np.zeros(G)]))
+        sim.set_initial_coulomb_boundaries(82)
+        sim.compute_volumetric_coulomb_potentials_sor(1.8, 1e-6, 512)  # cold: FP64
+        sim.upload_field(hvamd.HV_FIELD_DENSITY, np.concatenate([rho_p * 1.02, np.zeros(G)]))
+        it, _ = sim.compute_volumetric_coulomb_potentials_sor(1.8, 0.0, 65600)
+        if res32:
+            assert sim.get_option(hvamd.HV_OPT_SOR_F32_OWNED) == 2
+        out[res32] = (it, sim.download_field(hvamd.HV_FIELD_COULOMB))
+        sim.close()
+    assert out[0][0] == out[1][0] == 65600 + 4
+    _eq(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("res32", [0, 1])
 def test_mixed_sor_matches_fp64_iterations(res32):
     """The mixed form's FP32 defect-correction SOR follows the FP64 iteration:
