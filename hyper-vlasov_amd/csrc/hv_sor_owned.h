@@ -601,10 +601,12 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
     // rows it publishes (low 16 bits) and in the border rows it receives
     // from the neighbour's matching word (high 16 bits; values past a
     // section's end map to the sink word E); src_b the byte offset of that
-    // word; bit q of fmask: the neighbour exists.
+    // word; bit q of fmask: the neighbour exists; bit q of pmask: a neighbour
+    // reads the word this thread publishes (tiles on the grid's faces skip
+    // the stores nobody reads).
     const int wi = deep_words(side_i), wj = deep_words(side_j), wc = deep_words(nzh);
     const int oIM = 0, oIP = 2 * wi, oJM = 4 * wi, oJP = 4 * wi + 2 * wj, oC = oJP + 2 * wj;
-    unsigned src_b[kFetchMaxD], fa[kFetchMaxD], fm[kFetchMaxD], fmask = 0;
+    unsigned src_b[kFetchMaxD], fa[kFetchMaxD], fm[kFetchMaxD], fmask = 0, pmask = 0;
 #pragma unroll
     for (int q = 0; q < kFetchMaxD; q++) src_b[q] = fa[q] = fm[q] = 0u;
     int nfetch = 0;
@@ -637,7 +639,11 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
             const int di = (sec & 2) ? -1 : 1, dj = (sec & 1) ? -1 : 1;  // the tile whose corner c faces me
             const bool ex = (di > 0 ? has_ip : has_im) && (dj > 0 ? has_jp : has_jm);
             if (ex) off = (t + di * PJ + dj) * xstride + oC + sec * wc + y;
+            // my corner c is read by the diagonal tile (pi - di, pj - dj)
+            if ((di > 0 ? has_im : has_ip) && (dj > 0 ? has_jm : has_jp)) pmask |= 1u << nfetch;
         }
+        // a face word is read by the neighbour it is fetched from
+        if (kind < 2 && off >= 0) pmask |= 1u << nfetch;
         // LDS slot of value v of this word's section: in the rows published
         // (recv 0) or in the border rows received (recv 1)
         auto slot = [&](int v, int recv) -> unsigned {
@@ -717,7 +723,7 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
         const unsigned long long tg = (unsigned long long)(tag & 0xffffu) << 48;
 #pragma unroll
         for (int q = 0; q < kFetchMaxD; q++)
-            if (q < nfetch) {
+            if (q < nfetch && ((pmask >> q) & 1u)) {
                 const unsigned a = __float_as_uint(x1[fa[q] & 0xffffu]);
                 const unsigned mh = (__float_as_uint(x1[fm[q] & 0xffffu]) >> (16u * hsel)) & 0xffffu;
                 __hip_atomic_store((unsigned long long *)((char *)xw + (b + 8u * q * NT)),
