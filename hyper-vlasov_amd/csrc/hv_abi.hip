@@ -1030,7 +1030,7 @@ int stats_into(hv_ctx *c, int slot) {
     k_deposit<T><<<nb, kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total, make_geom<T>(c), nullptr,
                                                  c->partials, c->status);
     HV_CHECK_LAUNCH(c);
-    k_stats_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nb, c->slots + slot);
+    k_stats_finalize<<<1, kStatThreads, 0, c->stream>>>(c->partials, (int)nb, c->slots + slot);
     HV_CHECK_LAUNCH(c);
     return HV_OK;
 }
@@ -1242,7 +1242,7 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
             particles<T>(c), c->np, c->total, g, (const T *)c->U, c->acc, kick_fac, drift_fac, n > 0 ? 1 : 0, boost, (T)bp,
             (T)bn, c->partials, c->status);
         HV_CHECK_LAUNCH(c);
-        k_stats_finalize<<<1, 256, 0, c->stream>>>(c->partials, (int)nbp, c->slots + (n + 1));
+        k_stats_finalize<<<1, kStatThreads, 0, c->stream>>>(c->partials, (int)nbp, c->slots + (n + 1));
         HV_CHECK_LAUNCH(c);
         prof_end(c, HV_KCLASS_PARTICLE, pa, 1);
         int it = 0;

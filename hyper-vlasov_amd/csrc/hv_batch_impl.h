@@ -732,7 +732,7 @@ int batch_step_device(hv_batch *b, int end_kick, bool res) {
     {
         BStage st(b, HV_KCLASS_PARTICLE);
         kb_particle_step<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n, end_kick);
-        kb_stats_finalize<T><<<n, 256, 0, b->stream>>>(a, b->dstep);
+        kb_stats_finalize<T><<<n, kStatThreads, 0, b->stream>>>(a, b->dstep);
         kb_step_advance<<<1, 64, 0, b->stream>>>(b->dstep);
         HV_BHIP(b, hipGetLastError());
     }

@@ -86,7 +86,7 @@ kb_particle_step(const BInst<T> *__restrict__ a, const int *__restrict__ first, 
 // one block per instance; the diagnostics slot is the step counter + 1 (the
 // counter lives on the device, so a captured step graph needs no update)
 template <typename T>
-__global__ void __launch_bounds__(256) kb_stats_finalize(const BInst<T> *__restrict__ a, const int *__restrict__ step) {
+__global__ void __launch_bounds__(kStatThreads) kb_stats_finalize(const BInst<T> *__restrict__ a, const int *__restrict__ step) {
     const BInst<T> &A = a[blockIdx.x];
     stats_finalize_block(A.partials, A.part_blocks, A.slots + *step + 1);
 }

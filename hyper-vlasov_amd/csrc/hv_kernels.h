@@ -57,29 +57,38 @@ __device__ __forceinline__ void stat_commit(long long (&v)[6], long long *partia
     }
 }
 
-// One block: exact sum of nblocks rows of partials into *slot (128-bit sums;
-// integer addition, so the order does not matter).
+// One block of kStatThreads: exact sum of nblocks rows of partials into *slot
+// (128-bit sums; integer addition, so the order does not matter: each thread
+// sums a strided share, then a tree over the threads' sums).
+constexpr int kStatThreads = 256;
 __device__ __forceinline__ void stats_finalize_block(const long long *__restrict__ partials, int nblocks,
                                                      StatSlot *__restrict__ slot) {
-    __shared__ unsigned long long lo[6][256], hi[6][256];
+    __shared__ unsigned long long lo[6][kStatThreads], hi[6][kStatThreads];
+    const int tid = threadIdx.x;
     unsigned long long l[6] = {0, 0, 0, 0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll 4
+    for (int b = tid; b < nblocks; b += kStatThreads)
 #pragma unroll
         for (int c = 0; c < 6; c++) add_i128(l[c], h[c], partials[6 * b + c]);
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-        lo[c][threadIdx.x] = l[c];
-        hi[c][threadIdx.x] = h[c];
+        lo[c][tid] = l[c];
+        hi[c][tid] = h[c];
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
-        const int c = threadIdx.x;
-        unsigned long long L = 0, H = 0;
-        for (int t = 0; t < (int)blockDim.x; t++) {
-            unsigned long long nl = L + lo[c][t];
-            H += hi[c][t] + (nl < L ? 1ULL : 0ULL);
-            L = nl;
-        }
+    for (int s = kStatThreads / 2; s > 0; s >>= 1) {
+        if (tid < s)
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                const unsigned long long a = lo[c][tid], nl = a + lo[c][tid + s];
+                hi[c][tid] += hi[c][tid + s] + (nl < a ? 1ULL : 0ULL);
+                lo[c][tid] = nl;
+            }
+        __syncthreads();
+    }
+    if (tid < 6) {
+        const int c = tid;
+        const unsigned long long L = lo[c][0], H = hi[c][0];
         if (c < 4) {
             slot->s[c][0] = L;
             slot->s[c][1] = H;
@@ -89,7 +98,7 @@ __device__ __forceinline__ void stats_finalize_block(const long long *__restrict
     }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kStatThreads)
 k_stats_finalize(const long long *__restrict__ partials, int nblocks, StatSlot *__restrict__ slot) {
     stats_finalize_block(partials, nblocks, slot);
 }
