@@ -9,7 +9,9 @@ diagnostics) over all test particles with inputs resident in HBM.
 
 Prints ONE JSON line.  `value` is the mixed-precision form (FP32 particles and
 grid, FP64 Coulomb potential / SOR / reductions); the plain FP64 form is timed
-the same way and reported beside it.
+the same way and reported beside it.  Each form runs the workload's whole
+2,000 steps (`full_run`); `value` times the `--steps` steps after `--warmup`
+untimed ones in the middle of that run.  Progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -53,14 +55,17 @@ def measured_traffic(rl, workload):
 
 
 def sor_roofline(w, res):
-    """Roofline entry for the SOR kernel that dominates a run's device time.
+    """Roofline entry for the SOR kernel that dominates a run's device time,
+    over every launch of the profiled run.
 
     Algorithmic bytes of one SOR iteration per interior node: read the iterate
     and its source, write the iterate (FP64 potential: 8 + 8 + 8 = 24 B; the
     mixed form's FP32 correction and FP32 residual source: 4 + 4 + 4 = 12 B).
     A per-sweep launch is half an iteration; one launch of the LDS-resident
-    FP32 kernel runs a whole correction solve (its iteration count is read
-    back per solve), so its bytes per launch are iterations x 12 B x nodes."""
+    FP32 kernel runs a whole correction solve, so its bytes per launch are its
+    iterations x 12 B x nodes (the launch keeps its tile of the correction and
+    of its source in LDS, so these bytes are not HBM bytes: `traffic` holds
+    the counter-measured HBM bytes)."""
     interior = (w.grid - 2) ** 3
     prof, cnt = res["prof"], res["sor"]
     ms, n = prof.get("sor_resident_f32", (0.0, 0))
@@ -69,48 +74,35 @@ def sor_roofline(w, res):
         name = {2: "k_sor_resident_f32d", 1: "k_sor_resident_f32o"}.get(res.get("body"), "k_sor_resident_f32")
         return {"kernel": name + " (the mixed form's FP32 SOR correction, one LDS-resident launch per solve)",
                 "bytes_per_launch": it * 12.0 * interior, "avg_launch_us": 1e3 * ms / n,
-                "iterations_per_launch": it, "launches": n}
+                "iterations_per_launch": it, "launches": n,
+                "limiter": "neighbour exchange: the round trip of a tile face between CUs (agent-scope stores "
+                           "and loads) each iteration; the tile's values stay in LDS"}
     ms, n = prof.get("sor_sweep_f32", (0.0, 0))
     if n > 0:
         return {"kernel": "k_sor_colour_f32 (one red or black sweep of the FP32 SOR correction)",
                 "bytes_per_launch": 6.0 * interior, "avg_launch_us": 1e3 * ms / n, "iterations_per_launch": 0.5,
-                "launches": n}
+                "launches": res["issued"]["sor_sweep_f32"], "timed_launches": n}
     ms, n = prof["sor_sweep"]
     return {"kernel": "k_sor_colour (one red or black FP64 SOR sweep)", "bytes_per_launch": 12.0 * interior,
-            "avg_launch_us": 1e3 * ms / max(n, 1), "iterations_per_launch": 0.5, "launches": n}
+            "avg_launch_us": 1e3 * ms / max(n, 1), "iterations_per_launch": 0.5,
+            "launches": res["issued"]["sor_sweep"], "timed_launches": n}
 
 
 def copy_calibration(mib: int):
-    """A plain device copy of `mib` MiB (read mib, write mib), run three times
-    before the workload so that HBM byte counters (FETCH_SIZE / WRITE_SIZE)
-    read for it give the counter's scale against a known size."""
-    import torch
-    a = torch.ones(mib << 20, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    del a, b
+    """A plain device copy of `mib` MiB (read mib, write mib) by the library's
+    copy kernel (k_copy_calibration), run before the workload so that HBM byte
+    counters (FETCH_SIZE / WRITE_SIZE) read for it give the counters' scale
+    against a known size."""
+    from hvamd.sim import copy_bandwidth
+    copy_bandwidth(mib, reps=3)
 
 
 def measured_copy_gbs(mib: int = 1024, reps: int = 5) -> float:
     """HBM bandwidth of a plain device copy on this GPU (read + write bytes
-    per second, best of `reps`), for context beside the 8 TB/s spec."""
-    import torch
-    a = torch.ones(mib << 20, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    best = 0.0
-    for _ in range(reps):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        b.copy_(a)
-        e1.record()
-        e1.synchronize()
-        best = max(best, 2.0 * (mib << 20) / (e0.elapsed_time(e1) * 1e-3) / 1e9)
-    del a, b
-    torch.cuda.synchronize()
-    return best
+    per second, best of `reps`, the library's copy kernel), for context beside
+    the 8 TB/s spec."""
+    from hvamd.sim import copy_bandwidth
+    return copy_bandwidth(mib, reps)[0]
 
 
 def smear_flops_per_call(w) -> float:
@@ -134,47 +126,85 @@ def smear_flops_per_call(w) -> float:
     return 2.0 * n ** 3 * per_node
 
 
-def run_gpu(w, prec: str, steps: int, warmup: int):
-    """warmup steps and `steps` timed steps with no kernel timers; then the
-    same run again (bit for bit the same work) with the per-kernel-class
-    timers on over the same `steps` steps: kernel_ms_per_step and the SOR
-    kernel's launches for the roofline."""
+def window_start(total: int, steps: int, warmup: int) -> int:
+    """First step of the warm-up + timed window: the middle of the run (the
+    SOR iterations per step climb along a run, so its first steps are its
+    cheapest), or as late as the window still fits."""
+    return max(0, min(total // 2, total - warmup - steps))
+
+
+def run_gpu(w, prec: str, steps: int, warmup: int, full: bool = True):
+    """Pass 1 (kernel timers off): every step of the workload's run, timed
+    piece by piece; the headline window is `warmup` untimed steps and then
+    `steps` timed steps starting at window_start (the middle of the run).
+    Pass 2: the same run again (bit for bit the same work; checked by the SOR
+    iteration counts) with the per-kernel-class timers on over all its steps:
+    kernel_ms_per_step and the roofline kernel's launches over the whole run,
+    the launches a kernel trace of the run averages over.  With full=False
+    only the window is run (from step 0) and profiled."""
     import hvamd
     p = hvamd.HV_PREC_F64 if prec == "fp64" else hvamd.HV_PREC_MIXED
     x, y, z, kx, ky, kz = w.initial_state()
-    prm = w.params(warmup + steps)
+    total = max(w.steps, warmup + steps) if full else warmup + steps
+    first = window_start(total, steps, warmup) if full else 0
+    prm = w.params(total)
     sk = w.sky()
+    # pieces: [0, first), [first, first + warmup), the timed window, the rest
+    cuts = [0, first, first + warmup, first + warmup + steps, total]
 
-    def start():
+    def new_sim():
         sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, p)
         sim.upload_particles(x, y, z, kx, ky, kz)
         sim.run_init(sk, prm)
-        if warmup:
-            sim.run_steps(sk, prm, 0, warmup)
         sim.sync()
         return sim
 
-    sim = start()
-    t0 = time.perf_counter()
-    stats, its = sim.run_steps(sk, prm, warmup, steps)
-    sim.sync()
-    wall = time.perf_counter() - t0
-    state = None
-    if prec == "fp64":
-        state = sim.download_particles()
-        state["phi"] = sim.download_field(hvamd.HV_FIELD_COULOMB)
+    sim = new_sim()
+    walls, its_all, state = [], [], None
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        wall = 0.0
+        for c0 in range(a, b, 250):  # pieces of <= 250 steps, one progress line each
+            c1 = min(b, c0 + 250)
+            t0 = time.perf_counter()
+            _, it = sim.run_steps(sk, prm, c0, c1 - c0)
+            sim.sync()
+            wall += time.perf_counter() - t0
+            its_all.append(it)
+            print(f"[bench] {w.name} {prec}: steps {c0}-{c1} {1e3 * (time.perf_counter() - t0) / (c1 - c0):.3f} "
+                  f"ms/step", file=sys.stderr, flush=True)
+        walls.append(wall)
+        if b == cuts[2] and prec == "fp64":
+            # the state at the start of the timed window, for the CPU baseline
+            state = sim.download_particles()
+            state["phi"] = sim.download_field(hvamd.HV_FIELD_COULOMB)
     sim.close()
-    sim = start()
+    its_all = np.concatenate(its_all)
+    win = slice(cuts[2], cuts[3])
+    sim = new_sim()
     sim.profile_enable(True)
-    _, its2 = sim.run_steps(sk, prm, warmup, steps)
+    _, its2 = sim.run_steps(sk, prm, 0, total)
     sim.sync()
     prof = sim.profile_read()
+    issued = sim.profile_issued()
     sor = sim.sor_counts()
     sim.profile_enable(False)
     body = sim.get_option(hvamd.HV_OPT_SOR_F32_OWNED) if prec == "mixed" else None
     sim.close()
-    assert np.array_equal(its, its2), "the profiled replay must do the timed run's work"
-    return dict(wall=wall, its=its, prof=prof, sor=sor, stats=stats, state=state, body=body)
+    assert np.array_equal(its_all, its2), "the profiled run must do the timed run's work"
+    return dict(wall=walls[2], its=its_all[win], full_wall=sum(walls), full_its=its_all, total=total,
+                first=cuts[2], prof=prof, issued=issued, sor=sor, state=state, body=body)
+
+
+def kernel_ms_per_step(res) -> dict:
+    """Device ms per step of each kernel class over the profiled run.  The
+    per-sweep SOR classes time one launch in eight; their total is the
+    sampled time per launch times the launches issued."""
+    out = {}
+    for k, (ms, n) in res["prof"].items():
+        if k in ("sor_sweep", "sor_sweep_f32") and n > 0:
+            ms = ms / n * res["issued"][k]
+        out[k] = ms / res["total"]
+    return out
 
 
 def run_batch(ws, prec: str, steps: int, tune: bool):
@@ -239,11 +269,25 @@ def run_batch(ws, prec: str, steps: int, tune: bool):
                                               for k, v in (timings or {}).items()},
         "stage_ms_per_step": {k: v[0] / prof_steps for k, v in prof.items() if v[1] and k in
                               ("particle_pass", "smear", "sor_stage", "potential")},
-        "roofline": {"bound": "hbm", "kernel": kname,
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "avg_launch_us": 1e3 * avg_ms, "launches": launches,
-                     "bytes_per_launch": node_iters * bpn / max(launches, 1)},
+        "roofline": sweep_or_resident_entry(prec, kname, achieved, avg_ms, launches, node_iters * bpn),
     }
+
+
+def sweep_or_resident_entry(prec, kname, achieved, avg_ms, launches, alg_bytes):
+    """The ensemble's SOR kernel against the HBM peak.  The FP64 per-sweep
+    launches stream the potential through HBM, so their algorithmic rate is
+    an HBM rate.  The mixed form's LDS-resident launch keeps every tile in LDS
+    and moves only tile faces through memory: its algorithmic rate (12 B per
+    node and iteration over the launch time) is what a streaming kernel would
+    need, not HBM traffic, and can exceed the HBM peak."""
+    e = {"kernel": kname, "peak": HBM_PEAK_GBS, "unit": "GB/s", "avg_launch_us": 1e3 * avg_ms,
+         "launches": launches, "bytes_per_launch": alg_bytes / max(launches, 1)}
+    if prec == "mixed" and kname.startswith("kb_sor_resident"):
+        e.update({"bound": "neighbour exchange (LDS-resident tiles)", "algorithmic_equiv_GBs": achieved,
+                  "algorithmic_equiv_over_hbm_peak": achieved / HBM_PEAK_GBS})
+    else:
+        e.update({"bound": "hbm", "achieved": achieved, "frac": achieved / HBM_PEAK_GBS})
+    return e
 
 
 def cpu_model() -> str:
@@ -256,9 +300,12 @@ def cpu_model() -> str:
     return "unknown CPU"
 
 
-def cpu_baseline(w, state, budget_s: float):
-    """The CPU restatement (oracle/liboracle.so, OpenMP) from the same warm
-    mid-run state, on this host's cores; a bounded number of steps."""
+def cpu_baseline(w, state, window_steps: int, window_its, budget_s: float):
+    """The CPU restatement (oracle/liboracle.so, OpenMP) over the GPU's timed
+    window: from the FP64 run's state at the window's first step, as many of
+    the window's steps as fit in about `budget_s` seconds, on this host's
+    cores.  The oracle's FP64 loop equals the GPU's FP64 loop bit for bit, so
+    these steps run the same SOR iterations as the GPU's window (checked)."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_lib import OParams, OSkyrme, Oracle, oworld, p
     o = Oracle()
@@ -282,21 +329,86 @@ def cpu_baseline(w, state, budget_s: float):
         L.hvo_simulate_loop(*[p(st[k]) for k in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz")],
                             w.n_protons, w.n_neutrons, ctypes.byref(world), ctypes.byref(s), ctypes.byref(op),
                             p(phi), p(stats), p(its))
-        return time.perf_counter() - t0, its
+        return time.perf_counter() - t0, its[1:]
 
     t1, _ = run(1)
-    nsteps = int(max(2, min(40, budget_s / max(t1, 1e-3))))
+    nsteps = int(max(2, min(window_steps, budget_s / max(t1, 1e-3))))
     t, its = run(nsteps)
+    same = bool(np.array_equal(its, np.asarray(window_its[:nsteps])))
     return {
         "value": w.total * nsteps / t,
         "unit": "particle-steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"{nsteps} steps of the {w.name} loop from the GPU's warm mid-run state "
-                   f"(mean {its[1:].mean():.0f} SOR iterations/step), oracle/liboracle.so, "
+        "ms_per_step": 1e3 * t / nsteps,
+        "sor_iters_per_step": float(its.mean()),
+        "us_per_sor_iteration_and_step": 1e6 * t / nsteps / float(its.mean()),
+        "same_sor_iterations_as_gpu_window": same,
+        "sample": (f"the first {nsteps} steps of the GPU's timed {w.name} window, from the FP64 GPU run's state "
+                   f"there (mean {its.mean():.0f} SOR iterations/step, the GPU's FP64 iterations "
+                   f"{'exactly' if same else 'NOT'}), oracle/liboracle.so, "
                    f"OpenMP {cores} threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
                    f"OMP_PLACES={os.environ.get('OMP_PLACES')}), -O3 -march=x86-64-v3, {cpu_model()}"),
     }
+
+
+def reference_literal_c0(budget_s: float):
+    """The reference's own step loop (oracle/_ref/libhvref.so: cpu_simulate's
+    sequence of the reference's kernels, src/hyper_vlasov.cpp:53-104,
+    including the O(G^2) smearing of include/physics.hpp:209-229 and the
+    SOR constants it hard-codes, omega 1.5 / tol 1e-4) on the A=40, 32^3
+    workload's initial state, OpenMP on this host's cores: per-step time from
+    the difference of two run lengths (the pre-loop solve cancels), and the
+    oracle's literal and factorised smearing timed on the same counts."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import Oracle, Reference
+    if not Reference.available:
+        return None
+    from hvamd import configs
+    w = configs.c0()
+    x, y, z, kx, ky, kz = w.initial_state()
+    st = dict(x=x, y=y, z=z, kx=kx, ky=ky, kz=kz)
+    ref = Reference()
+    prm = w.params()
+    sk = w.sky()
+    n, d = w.world().n, w.world().d_max
+
+    def run(steps):
+        t0 = time.perf_counter()
+        ref.simulate(st, w.n_protons, w.n_neutrons, n, d, sk.A, sk.B, sk.C, sk.gamma, prm.dt * steps, steps, 0.0,
+                     prm.eta_exc, w.z, w.n, prm.sigma_r, prm.part_per_nucleon)
+        return time.perf_counter() - t0
+
+    t1 = run(1)
+    extra = int(max(1, min(20, budget_s / max(t1, 1e-3) - 1)))
+    t2 = run(1 + extra)
+    per_step = max(t2 - t1, 1e-9) / extra
+    o = Oracle()
+    counts, _, _ = o.deposit(x, y, z, w.n_protons, w.n_neutrons, n, d)
+    t0 = time.perf_counter()
+    o.smear(counts, n, d, prm.sigma_r, prm.part_per_nucleon, literal=True)
+    t_lit = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(10):
+        o.smear(counts, n, d, prm.sigma_r, prm.part_per_nucleon)
+    t_fac = (time.perf_counter() - t0) / 10
+    o.L.hvo_threads.restype = ctypes.c_int
+    return {"value": w.total / per_step, "unit": "particle-steps/s", "kind": "reference",
+            "cores": o.L.hvo_threads(), "ms_per_step": 1e3 * per_step, "steps_timed": extra,
+            "smear_literal_over_factorised": t_lit / t_fac,
+            "sample": (f"{extra} steps of the reference's own loop on the {w.name} state (its hard-coded SOR "
+                       f"omega 1.5 / tol 1e-4, O(G^2) smearing), oracle/_ref/libhvref.so built from the "
+                       f"reference sources, OpenMP; smearing of one 32^3 count field: literal O(G^2) "
+                       f"{1e3 * t_lit:.1f} ms vs factorised {1e3 * t_fac:.2f} ms")}
+
+
+def counter_traffic(rl, workload):
+    """`traffic` (HBM bytes per launch from the committed counter profile of
+    this kernel) and its fraction of the HBM peak over the launch time."""
+    t = measured_traffic(rl, workload)
+    if t is None:
+        return None, None
+    return t, t / (rl["avg_launch_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS
 
 
 def main():
@@ -305,6 +417,8 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--full", type=int, default=1,
+                    help="run every step of the workload (the timed window sits mid-run); 0: the window only")
     ap.add_argument("--ensemble", type=int, default=256, help="instances of the c3 ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=8)
     ap.add_argument("--fp64", type=int, default=1, help="also time the plain FP64 form")
@@ -313,26 +427,28 @@ def main():
                     help="first run a plain MIB-MiB device copy (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the single-GPU number is the "
-              "figure of record for this loop", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build measures the loop on a "
+              "single MI355X and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     # the CPU baseline's OpenMP placement (read when libgomp starts)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
+    import hvamd
     from hvamd import configs
+    hvamd.load()
     w = configs.WORKLOADS[args.workload]()
-    # torch's HIP runtime measures the copy before the library's own runtime
-    # starts in this process (the two do not share the device afterwards)
     copy_gbs = measured_copy_gbs()
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
-    mixed = run_gpu(w, "mixed", args.steps, args.warmup)
-    fp64 = run_gpu(w, "fp64", args.steps, args.warmup) if args.fp64 else None
+    full = bool(args.full)
+    mixed = run_gpu(w, "mixed", args.steps, args.warmup, full)
+    fp64 = run_gpu(w, "fp64", args.steps, args.warmup, full) if args.fp64 else None
 
     value = w.total * args.steps / mixed["wall"]
     rl = sor_roofline(w, mixed)
     achieved = rl["bytes_per_launch"] / (rl["avg_launch_us"] * 1e-6) / 1e9
+    traffic, traffic_frac = counter_traffic(rl, args.workload)
     out = {
         "metric": "particle-steps/s at 1.04M tp, 128^3 (mixed FP32/FP64 form; plain FP64 beside it)",
         "value": value,
@@ -347,13 +463,18 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic (seeded uniform-sphere nucleus)",
         "config": {"workload": w.name, "test_particles": w.total, "grid": w.grid, "dx_fm": w.dx,
-                   "dt_fm_c": w.dt, "sor": [w.omega, w.tol, w.max_it], "parallelism": "single GPU"},
+                   "dt_fm_c": w.dt, "sor": [w.omega, w.tol, w.max_it], "window_first_step": mixed["first"],
+                   "run_steps": mixed["total"], "parallelism": "single GPU"},
         "sor_iters_per_step": float(np.mean(mixed["its"])),
         "sor_counts": mixed["sor"],
-        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in mixed["prof"].items()},
+        "kernel_ms_per_step": kernel_ms_per_step(mixed),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(rl, args.workload), **rl},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_frac_of_peak": traffic_frac,
+                     **rl},
     }
+    out["full_run"] = {"mixed": {"value": w.total * mixed["total"] / mixed["full_wall"],
+                                 "ms_per_step": 1e3 * mixed["full_wall"] / mixed["total"], "steps": mixed["total"],
+                                 "sor_iters_per_step": float(mixed["full_its"].mean())}}
     sm_ms = mixed["prof"]["smear"][0] / max(mixed["prof"]["smear"][1], 1)
     sflop = smear_flops_per_call(w)
     out["smear_flops"] = {"flop_per_call": sflop, "avg_call_us": 1e3 * sm_ms,
@@ -364,12 +485,20 @@ def main():
         r64 = sor_roofline(w, fp64)
         out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["wall"] / args.steps,
                        "sor_iters_per_step": float(np.mean(fp64["its"])),
-                       "kernel_ms_per_step": {k: v[0] / args.steps for k, v in fp64["prof"].items()},
+                       "kernel_ms_per_step": kernel_ms_per_step(fp64),
                        "sor_achieved_GBs": r64["bytes_per_launch"] / (r64["avg_launch_us"] * 1e-6) / 1e9,
-                       "sor_avg_launch_us": r64["avg_launch_us"]}
+                       "sor_avg_launch_us": r64["avg_launch_us"], "sor_launches": r64["launches"],
+                       "sor_timed_launches": r64["timed_launches"]}
         out["mixed_over_fp64"] = value / v64
+        out["full_run"]["fp64"] = {"value": w.total * fp64["total"] / fp64["full_wall"],
+                                   "ms_per_step": 1e3 * fp64["full_wall"] / fp64["total"], "steps": fp64["total"],
+                                   "sor_iters_per_step": float(fp64["full_its"].mean())}
+        out["full_run"]["mixed_over_fp64"] = out["full_run"]["mixed"]["value"] / out["full_run"]["fp64"]["value"]
         if args.cpu_budget > 0:
-            out["cpu_baseline"] = cpu_baseline(w, fp64["state"], args.cpu_budget)
+            out["cpu_baseline"] = cpu_baseline(w, fp64["state"], args.steps, fp64["its"], args.cpu_budget)
+            lit = reference_literal_c0(args.cpu_budget / 3)
+            if lit is not None:
+                out["cpu_baseline_reference_loop_c0"] = lit
     if args.ensemble > 0:
         ens = configs.c3(args.ensemble)
         bm = run_batch(ens, "mixed", args.ensemble_steps, tune=True)

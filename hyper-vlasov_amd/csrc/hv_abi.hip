@@ -131,6 +131,7 @@ struct hv_ctx {
     bool prof = false;
     double prof_ms[HV_KCLASS_COUNT] = {};
     long long prof_launches[HV_KCLASS_COUNT] = {};
+    long long prof_issued[HV_KCLASS_COUNT] = {};  // launches of a sampled class issued, timed or not
     struct Pending {
         int cls;
         hipEvent_t a, b;
@@ -290,6 +291,7 @@ constexpr int kSorSample = 8;
 template <typename K, typename... A>
 void sor_launch(hv_ctx *c, int it, int colour, int kclass, K kernel, dim3 grid, dim3 block, size_t shm,
                 A... args) {
+    if (c->prof) c->prof_issued[kclass]++;
     if (c->prof && ((it * 2 + colour) % kSorSample) == 3) {
         hipEvent_t a = prof_event(c), b = a ? prof_event(c) : nullptr;
         if (a && b) {
@@ -1362,6 +1364,14 @@ int copy_out(hv_ctx *c, double *dst, const void *src, long long n, bool f64) {
 
 }  // namespace
 
+// A plain device copy of n 16-byte words (read n, write n): the bandwidth
+// probe and the known-size kernel that scales HBM byte counters.
+__global__ void __launch_bounds__(256) k_copy_calibration(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                          long long n) {
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // =============================================================================
 extern "C" {
 
@@ -1882,6 +1892,7 @@ int hv_profile_enable(hv_ctx *c, int on) {
     for (int k = 0; k < HV_KCLASS_COUNT; k++) {
         c->prof_ms[k] = 0.0;
         c->prof_launches[k] = 0;
+        c->prof_issued[k] = 0;
     }
     c->cnt_solves = c->cnt_it32 = c->cnt_it64 = 0;
     return HV_OK;
@@ -1904,6 +1915,12 @@ int hv_profile_read(hv_ctx *c, int kclass, double *ms, long long *launches) {
     return HV_OK;
 }
 
+int hv_profile_issued(hv_ctx *c, int kclass, long long *issued) {
+    if (!c || kclass < 0 || kclass >= HV_KCLASS_COUNT || !issued) return HV_ERR_ARG;
+    *issued = c->prof_issued[kclass];
+    return HV_OK;
+}
+
 #ifdef HV_RES_TRACE
 // trace builds only: the phase timestamps of the last resident SOR solve
 int hv_debug_trace(hv_ctx *c, unsigned long long *host256) {
@@ -1912,6 +1929,49 @@ int hv_debug_trace(hv_ctx *c, unsigned long long *host256) {
     return HV_OK;
 }
 #endif
+
+int hv_copy_bandwidth(long long bytes, int reps, double *best_gbs, double *best_us) {
+    if (bytes < 16 || (bytes & 15) || reps < 1) return HV_ERR_ARG;
+    uint4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = HV_OK;
+    float best = 0.0f;
+    const long long n = bytes / 16;
+    const long long want = (n + 255) / 256;
+    const unsigned grid = (unsigned)(want < 256LL * 16 ? want : 256LL * 16);
+    if (hipMalloc((void **)&a, (size_t)bytes) != hipSuccess || hipMalloc((void **)&b, (size_t)bytes) != hipSuccess) {
+        rc = HV_ERR_ALLOC;
+        goto out;
+    }
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(a, 1, (size_t)bytes, st) != hipSuccess) {
+        rc = HV_ERR_HIP;
+        goto out;
+    }
+    // the first copy is not timed (first touch of b)
+    for (int r = 0; r <= reps; r++) {
+        hipEventRecord(e0, st);
+        k_copy_calibration<<<grid, 256, 0, st>>>(a, b, n);
+        hipEventRecord(e1, st);
+        float ms = 0.0f;
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = HV_ERR_HIP;
+            goto out;
+        }
+        if (r > 0 && (best == 0.0f || ms < best)) best = ms;
+    }
+    if (best_gbs) *best_gbs = 2.0 * (double)bytes / (best * 1e-3) / 1e9;
+    if (best_us) *best_us = 1e3 * best;
+out:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
 
 const char *hv_build_info(void) {
     return "libhvamd gfx950 -O3 -ffp-contract=off (HIP " HIP_VERSION_STR ")";

@@ -8,6 +8,8 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
+import warnings
 from pathlib import Path
 
 import numpy as np
@@ -140,6 +142,8 @@ SIGNATURES = {
     "hv_set_option": (_I, [_P, _I, _I]),
     "hv_get_option": (_I, [_P, _I, _IP]),
     "hv_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
+    "hv_profile_issued": (_I, [_P, _I, ctypes.POINTER(ctypes.c_longlong)]),
+    "hv_copy_bandwidth": (_I, [ctypes.c_longlong, _I, _DP, _DP]),
     "hv_total_energy": (_I, [_P, ctypes.POINTER(HvSkyrme), _I, _DP]),
     "hv_sor_counts": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                            ctypes.POINTER(ctypes.c_longlong)]),
@@ -167,6 +171,24 @@ SIGNATURES = {
 
 _lib = None
 
+# The ROCm runtime libhvamd.so is built against, loaded under the bare names
+# that torch's own HIP libraries ask for (their DT_NEEDED entries carry no
+# version suffix), so that a torch imported later in the process binds the
+# same HIP and HSA runtime instead of loading the copies bundled in its wheel.
+RUNTIME_PRELOAD = ("librocprofiler-register.so", "libamd_comgr.so", "libhsa-runtime64.so", "libamdhip64.so")
+
+
+def _preload_runtime() -> None:
+    if "torch" in sys.modules:
+        warnings.warn("torch was imported before hvamd: libhvamd.so binds the HIP runtime torch loaded, not "
+                      "the ROCm runtime it was built against; import hvamd first", RuntimeWarning, stacklevel=3)
+        return
+    for name in RUNTIME_PRELOAD:
+        try:
+            ctypes.CDLL(name, mode=ctypes.RTLD_GLOBAL)
+        except OSError:
+            pass  # not on the search path: the library's own RUNPATH finds the runtime
+
 
 def load() -> ctypes.CDLL:
     """Load libhvamd.so once; raise if it is missing (no fallback)."""
@@ -177,6 +199,7 @@ def load() -> ctypes.CDLL:
         raise RuntimeError(
             f"{LIB_PATH} is not built: run `make -C hyper-vlasov_amd` or __graft_entry__.build(); "
             "the step loop has no CPU fallback")
+    _preload_runtime()
     lib = ctypes.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

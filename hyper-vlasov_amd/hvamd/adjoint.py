@@ -14,6 +14,7 @@ gradient the adjoint is checked against."""
 from __future__ import annotations
 
 import ctypes
+import json
 
 import numpy as np
 
@@ -89,8 +90,8 @@ try:
         @staticmethod
         def forward(ctx, theta, fit):
             loss, grad, _ = fit.loss_grad(theta.detach().cpu().numpy())
-            ctx.save_for_backward(torch.as_tensor(grad, dtype=theta.dtype))
-            return torch.tensor(loss, dtype=theta.dtype)
+            ctx.save_for_backward(torch.as_tensor(grad, dtype=theta.dtype, device=theta.device))
+            return torch.tensor(loss, dtype=theta.dtype, device=theta.device)
 
         @staticmethod
         def backward(ctx, g):
@@ -100,59 +101,70 @@ except ImportError:  # torch is optional for the C-ABI users
     DipoleLoss = None
 
 
-def study(name: str = "c4", nsteps: int | None = None, checkpoint_every: int = 100, h: float = 1e-4,
+def study(name: str = "c4", horizons=(None,), hs=(1e-4,), checkpoint_every: int = 100,
           sor_tol: float | None = None, sor_max_it: int | None = None) -> dict:
-    """The differentiation workload: target from the run at theta * 1.03, the
-    adjoint gradient at theta, and central differences with relative step h."""
+    """The differentiation workload: for each horizon (steps of the run), the
+    target from the run at theta * 1.03, the adjoint gradient at theta, and
+    central differences at each relative step h.  The spread of the finite
+    differences over h separates their own error (the loss is only piecewise
+    smooth: CIC weights are piecewise linear in the positions) from the
+    adjoint's."""
     import time
 
     from . import configs
     w = configs.WORKLOADS[name]()
-    nsteps = w.steps if nsteps is None else nsteps
-    prm = w.params(nsteps)
-    if sor_tol is not None:
-        prm.sor_tol = sor_tol
-    if sor_max_it is not None:
-        prm.sor_max_it = sor_max_it
     sk = w.sky()
     theta = np.array([sk.A, sk.B, sk.C, sk.gamma])
-    t0 = time.perf_counter()
-    target = dipole_series(w, theta * 1.03, nsteps, prm)
-    t_fwd = time.perf_counter() - t0
-    fit = DipoleFit(w, target, nsteps, checkpoint_every, prm)
-    t0 = time.perf_counter()
-    loss, grad, _ = fit.loss_grad(theta)
-    t_adj = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    fd = central_difference(fit, theta, h)
-    t_fd = time.perf_counter() - t0
-    fit.close()
-    return {"workload": w.name, "test_particles": w.total, "steps": nsteps, "checkpoint_every": checkpoint_every,
-            "sor": [prm.sor_omega, prm.sor_tol, prm.sor_max_it], "loss": loss,
-            "gradient_adjoint": dict(zip(PARAM_NAMES, grad.tolist())),
-            "gradient_central_difference": dict(zip(PARAM_NAMES, fd.tolist())), "fd_relative_step": h,
-            "relative_difference": dict(zip(PARAM_NAMES, ((grad - fd) / np.abs(fd)).tolist())),
-            "seconds": {"forward_run": t_fwd, "loss_and_adjoint_gradient": t_adj, "central_differences_8_runs": t_fd}}
+    out = {"workload": w.name, "test_particles": w.total, "checkpoint_every": checkpoint_every, "horizons": []}
+    for nsteps in horizons:
+        nsteps = w.steps if nsteps is None else int(nsteps)
+        prm = w.params(nsteps)
+        if sor_tol is not None:
+            prm.sor_tol = sor_tol
+        if sor_max_it is not None:
+            prm.sor_max_it = sor_max_it
+        t0 = time.perf_counter()
+        target = dipole_series(w, theta * 1.03, nsteps, prm)
+        t_fwd = time.perf_counter() - t0
+        fit = DipoleFit(w, target, nsteps, min(checkpoint_every, nsteps), prm)
+        t0 = time.perf_counter()
+        loss, grad, _ = fit.loss_grad(theta)
+        t_adj = time.perf_counter() - t0
+        fds = {}
+        t0 = time.perf_counter()
+        for h in hs:
+            fd = central_difference(fit, theta, h)
+            fds[f"{h:g}"] = {"gradient": dict(zip(PARAM_NAMES, fd.tolist())),
+                             "relative_difference": dict(zip(PARAM_NAMES, ((grad - fd) / np.abs(fd)).tolist()))}
+        t_fd = time.perf_counter() - t0
+        fit.close()
+        out["horizons"].append({
+            "steps": nsteps, "sor": [prm.sor_omega, prm.sor_tol, prm.sor_max_it], "loss": loss,
+            "gradient_adjoint": dict(zip(PARAM_NAMES, grad.tolist())), "central_differences": fds,
+            "seconds": {"forward_run": t_fwd, "loss_and_adjoint_gradient": t_adj,
+                        "central_differences": t_fd}})
+        print(json.dumps(out["horizons"][-1]), flush=True)
+    return out
 
 
 if __name__ == "__main__":
     import argparse
-    import json
     import sys
 
     ap = argparse.ArgumentParser(description="adjoint gradient vs central differences")
     ap.add_argument("--workload", default="c4")
-    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--steps", default=None, help="comma-separated horizons (default: the workload's steps)")
     ap.add_argument("--checkpoint-every", type=int, default=100)
-    ap.add_argument("--h", type=float, default=1e-4)
+    ap.add_argument("--h", default="1e-4", help="comma-separated relative finite-difference steps")
     ap.add_argument("--sor-tol", type=float, default=None)
     ap.add_argument("--sor-max-it", type=int, default=None)
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
-    res = study(a.workload, a.steps, a.checkpoint_every, a.h, a.sor_tol, a.sor_max_it)
+    horizons = [int(v) for v in a.steps.split(",")] if a.steps else [None]
+    res = study(a.workload, horizons, [float(v) for v in a.h.split(",")], a.checkpoint_every, a.sor_tol,
+                a.sor_max_it)
     txt = json.dumps(res, indent=1)
     if a.out:
         with open(a.out, "w") as f:
             f.write(txt + "\n")
-    print(txt, flush=True)
     sys.exit(0)

@@ -13,7 +13,9 @@ precision change cannot beat).  Reported:
     difference from FP64, over the whole run and over its first `window`
     steps;
   - the same D(t) error over growing horizons, and the first step at which it
-    exceeds 1e-4 of the amplitude.
+    exceeds 1e-4 of the amplitude;
+  - each of these for several perturbed FP64 runs (noise in different
+    directions), as a distribution, and where the mixed form falls in it.
 
 The dynamics of these workloads is chaotic: a relative perturbation of 2^-24
 in the positions grows past 1e-4 of the D(t) amplitude within a few hundred
@@ -35,11 +37,26 @@ from ._lib import HV_PREC_F64, HV_PREC_MIXED
 from .sim import HBARC, Simulation
 
 
-def run(w, prec: int, steps: int, chunk: int, perturb: float = 0.0):
-    """(stats[steps, 4], energies[(step, [E, Ekin, Esky, Ecoul])], sor iterations, seconds)."""
+def perturbed_positions(w, seed: int, size: float = 2.0 ** -24):
+    """The workload's initial positions, each coordinate scaled by 1 + s size
+    with s = +1 for seed 0 (every coordinate moved outward) and otherwise a
+    seeded random sign per particle and coordinate: one FP32 rounding step of
+    position noise, in a different direction for every seed."""
     x, y, z, kx, ky, kz = w.initial_state()
-    if perturb:
-        x, y, z = (a * (1.0 + perturb) for a in (x, y, z))
+    if seed == 0:
+        sx = sy = sz = 1.0
+    else:
+        rng = np.random.default_rng([seed, 2 ** 24])
+        sx, sy, sz = (rng.choice([-1.0, 1.0], size=x.size) for _ in range(3))
+    return x * (1.0 + sx * size), y * (1.0 + sy * size), z * (1.0 + sz * size), kx, ky, kz
+
+
+def run(w, prec: int, steps: int, chunk: int, perturb_seed: int | None = None):
+    """(stats[steps, 4], energies[(step, [E, Ekin, Esky, Ecoul])], sor iterations, seconds)."""
+    if perturb_seed is None:
+        x, y, z, kx, ky, kz = w.initial_state()
+    else:
+        x, y, z, kx, ky, kz = perturbed_positions(w, perturb_seed)
     sim = Simulation(w.world(), w.n_protons, w.n_neutrons, prec)
     try:
         sim.upload_particles(x, y, z, kx, ky, kz)
@@ -92,15 +109,56 @@ def compare(w, ref, other, window: int) -> dict:
     }
 
 
-def study(name: str = "c1", steps: int | None = None, chunk: int = 100, window: int = 1000) -> dict:
+def distribution(values) -> dict:
+    v = np.asarray([x for x in values if x is not None], dtype=np.float64)
+    if v.size == 0:
+        return {"n": 0}
+    return {"n": int(v.size), "min": float(v.min()), "median": float(np.median(v)), "max": float(v.max()),
+            "values": v.tolist()}
+
+
+def study(name: str = "c1", steps: int | None = None, chunk: int = 100, window: int = 1000,
+          n_perturbed: int = 6) -> dict:
+    """FP64, mixed and n_perturbed FP64 runs with 2^-24 position noise in
+    different directions (seeds 0..n-1): the mixed form's errors against FP64
+    beside the distribution of the perturbed runs' errors (the chaos floor)."""
     w = configs.WORKLOADS[name]()
     steps = w.steps if steps is None else steps
     window = min(window, steps)
     f64 = run(w, HV_PREC_F64, steps, chunk)
+    print(f"[precision] {w.name}: FP64 done ({f64[3]:.1f} s)", file=sys.stderr, flush=True)
     mix = run(w, HV_PREC_MIXED, steps, chunk)
-    flo = run(w, HV_PREC_F64, steps, chunk, perturb=2.0 ** -24)
+    print(f"[precision] {w.name}: mixed done ({mix[3]:.1f} s)", file=sys.stderr, flush=True)
+    pert = []
+    for seed in range(n_perturbed):
+        r = run(w, HV_PREC_F64, steps, chunk, perturb_seed=seed)
+        pert.append(compare(w, f64, r, window))
+        print(f"[precision] {w.name}: perturbed seed {seed} done ({r[3]:.1f} s)", file=sys.stderr, flush=True)
     e64 = np.array([e for _, e in f64[1]])
     d64 = spectrum.dipole_moment(f64[0][:, 0], f64[0][:, 1], w.z, w.n)
+    mixed = compare(w, f64, mix, window)
+    floor = {}
+    for key in ("D_max_abs_diff_over_amplitude", "D_rms_diff_over_amplitude", "D_first_step_rel_diff_above_1e-4",
+                "energy_max_abs_diff_over_E0", "centroid_diff_MeV", f"centroid_first_{window}_steps_diff_MeV"):
+        floor[key] = distribution([r[key] for r in pert])
+    floor["abs_centroid_diff_MeV"] = distribution([abs(r["centroid_diff_MeV"]) for r in pert])
+    floor[f"abs_centroid_first_{window}_steps_diff_MeV"] = distribution(
+        [abs(r[f"centroid_first_{window}_steps_diff_MeV"]) for r in pert])
+    floor["D_max_rel_diff_by_horizon"] = {h: distribution([r["D_max_rel_diff_by_horizon"][h] for r in pert])
+                                          for h in mixed["D_max_rel_diff_by_horizon"]}
+
+    def rank(v, vals):
+        # perturbed runs whose value is at most the mixed form's, of all of them
+        return f"{sum(1 for x in vals if x is not None and abs(x) <= abs(v))}/{len(vals)}"
+
+    position = {
+        "abs_centroid_diff": rank(mixed["centroid_diff_MeV"], [r["centroid_diff_MeV"] for r in pert]),
+        f"abs_centroid_first_{window}_steps_diff": rank(
+            mixed[f"centroid_first_{window}_steps_diff_MeV"],
+            [r[f"centroid_first_{window}_steps_diff_MeV"] for r in pert]),
+        "D_max_rel_diff_by_horizon": {h: rank(v, [r["D_max_rel_diff_by_horizon"][h] for r in pert])
+                                      for h, v in mixed["D_max_rel_diff_by_horizon"].items()},
+    }
     return {
         "workload": w.name, "steps": steps, "dt": w.dt, "test_particles": w.total, "energy_every": chunk,
         "fp64": {"seconds": f64[3], "sor_iters_mean": float(f64[2].mean()),
@@ -108,9 +166,10 @@ def study(name: str = "c1", steps: int | None = None, chunk: int = 100, window: 
                  "E0_MeV": float(e64[0, 0]), "D_amplitude": float(np.abs(d64).max()),
                  "energy_series_MeV": [[s_, e[0]] for s_, e in f64[1]],
                  "centroid_MeV": float(spectrum.dipole_response(f64[0], w.dt, w.z, w.n, w.dp / HBARC)[4])},
-        "mixed_vs_fp64": {"seconds": mix[3], "sor_iters_mean": float(mix[2].mean()),
-                          **compare(w, f64, mix, window)},
-        "fp64_perturbed_2^-24_vs_fp64": compare(w, f64, flo, window),
+        "mixed_vs_fp64": {"seconds": mix[3], "sor_iters_mean": float(mix[2].mean()), **mixed},
+        "fp64_perturbed_2^-24_vs_fp64": {"runs": pert, "distribution": floor},
+        "mixed_within_perturbed": {"note": "k/n: perturbed runs whose |difference| is at most the mixed form's",
+                                   **position},
     }
 
 
@@ -120,9 +179,10 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--window", type=int, default=1000, help="steps of the early-window centroid")
+    ap.add_argument("--perturbed", type=int, default=6, help="perturbed FP64 runs (different noise directions)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
-    res = study(a.workload, a.steps, a.chunk, a.window)
+    res = study(a.workload, a.steps, a.chunk, a.window, a.perturbed)
     txt = json.dumps(res, indent=1)
     if a.out:
         with open(a.out, "w") as f:

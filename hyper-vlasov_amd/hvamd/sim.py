@@ -319,12 +319,33 @@ class Simulation:
             out[name] = (ms.value, n.value)
         return out
 
+    def profile_issued(self) -> dict:
+        """{class name: launches issued since profile_enable}: for the sampled
+        SOR sweep classes every launch, timed or not."""
+        out = {}
+        for k, name in enumerate(_lib.KCLASS_NAMES):
+            n = ctypes.c_longlong()
+            self._rc("hv_profile_issued", self.lib.hv_profile_issued(self.ctx, k, ctypes.byref(n)))
+            out[name] = n.value
+        return out
+
     def sor_counts(self) -> dict:
         """SOR solves, FP32 correction iterations and FP64 iterations since
         profile_enable."""
         v = [ctypes.c_longlong() for _ in range(3)]
         self._rc("hv_sor_counts", self.lib.hv_sor_counts(self.ctx, *[ctypes.byref(x) for x in v]))
         return {"solves": v[0].value, "iters_f32": v[1].value, "iters_f64": v[2].value}
+
+
+def copy_bandwidth(mib: int = 1024, reps: int = 5) -> tuple:
+    """(GB/s, microseconds) of the best of `reps` plain device copies of
+    `mib` MiB by the library's copy kernel (read + write bytes per second)."""
+    lib = _lib.load()
+    gbs, us = ctypes.c_double(), ctypes.c_double()
+    rc = lib.hv_copy_bandwidth(mib << 20, reps, ctypes.byref(gbs), ctypes.byref(us))
+    if rc < 0:
+        raise _lib.HvError("hv_copy_bandwidth", rc, "copy probe failed")
+    return gbs.value, us.value
 
 
 def dipole(stats: np.ndarray, z: int, n: int) -> np.ndarray:

@@ -278,6 +278,13 @@ int hv_get_option(hv_ctx *ctx, int option, int *value);
  * hv_timer_stop calls (hipEvents on the context stream). */
 int hv_timer_start(hv_ctx *ctx);
 int hv_timer_stop(hv_ctx *ctx, double *ms);
+/* HBM bandwidth of a plain device copy on the current device: `bytes` (a
+ * multiple of 16) read and written by this library's copy kernel
+ * (k_copy_calibration, 16-byte words), reps timed runs after one untimed;
+ * best_gbs = 2 bytes / best time [GB/s], best_us = best time.  Allocates and
+ * frees its two buffers.  The same kernel is the known-size copy that scales
+ * HBM byte counters (FETCH_SIZE, WRITE_SIZE) of a profiled run. */
+int hv_copy_bandwidth(long long bytes, int reps, double *best_gbs, double *best_us);
 /* Library build string (arch, flags). */
 const char *hv_build_info(void);
 
@@ -298,6 +305,10 @@ const char *hv_build_info(void);
 #define HV_KCLASS_COUNT 7
 int hv_profile_enable(hv_ctx *ctx, int on); /* also resets the counters */
 int hv_profile_read(hv_ctx *ctx, int kclass, double *ms, long long *launches);
+/* Launches of a class issued since the last hv_profile_enable, timed or not
+ * (for the sampled SOR sweep classes: the sampled time per launch times this
+ * count estimates the class's whole device time). */
+int hv_profile_issued(hv_ctx *ctx, int kclass, long long *issued);
 /* SOR work since the last hv_profile_enable: solves, FP32 correction
  * iterations (mixed form) and FP64 iterations. */
 int hv_sor_counts(hv_ctx *ctx, long long *solves, long long *iters_f32, long long *iters_f64);

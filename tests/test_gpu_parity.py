@@ -319,7 +319,7 @@ def test_mixed_precision_tracks_fp64():
     d64, d32 = res[hvamd.HV_PREC_F64], res[hvamd.HV_PREC_MIXED]
     rel = np.abs(d32 - d64).max() / np.abs(d64).max()
     print("mixed vs fp64 D(t) max rel", rel)
-    assert rel < 1e-3
+    assert rel < 1e-4
 
 
 def test_determinism_large():
