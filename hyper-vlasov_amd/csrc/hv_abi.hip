@@ -1984,3 +1984,6 @@ const char *hv_build_info(void) {
 
 // ---- adjoint of the loop (hv_dipole_loss_grad) -------------------------------------
 #include "hv_adjoint_impl.h"
+
+// ---- diagnostics without allocation (hv_stats_noalloc) ----------------------------
+#include "hv_noalloc_impl.h"

@@ -26,6 +26,7 @@ HV_ERR_IO = -5
 HV_WARN_SOR_NOT_CONVERGED = 1
 HV_WARN_PARTICLE_OUTSIDE = 2
 HV_WARN_SMEAR_MARGIN = 4
+HV_NOALLOC_MAX_PARTICLES = 1 << 24
 
 HV_PREC_F64 = 0
 HV_PREC_MIXED = 1
@@ -144,6 +145,7 @@ SIGNATURES = {
     "hv_profile_read": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
     "hv_profile_issued": (_I, [_P, _I, ctypes.POINTER(ctypes.c_longlong)]),
     "hv_copy_bandwidth": (_I, [ctypes.c_longlong, _I, _DP, _DP]),
+    "hv_stats_noalloc": (_I, [ctypes.POINTER(HvWorld), _I, _P, _P, _P, _I, _I, _P, _P]),
     "hv_total_energy": (_I, [_P, ctypes.POINTER(HvSkyrme), _I, _DP]),
     "hv_sor_counts": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                            ctypes.POINTER(ctypes.c_longlong)]),

@@ -360,6 +360,26 @@ int hv_batch_sor_counts(hv_batch *batch, long long *sweeps_f32, long long *sweep
  * the batch was created (HV_OPT_BATCH_GRAPH) */
 int hv_batch_graph_info(hv_batch *batch, long long *graph_steps, long long *captures);
 
+/* ---- diagnostics without allocation (no counterpart in the reference) ----- */
+/* For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the four
+ * values of hv_center_of_mass / hv_mean_squared_radius (include/tools.hpp:
+ * 303-338; protons then neutrons: mean z, mean z, <r^2> of the in-box
+ * particles, <r^2> of the in-box particles) for positions the caller already
+ * holds in device memory, protons first (double arrays for HV_PREC_F64, float
+ * for HV_PREC_MIXED).  `out` is device memory for 4 doubles, written on
+ * `stream` (a hipStream_t; NULL = the default stream); nothing is allocated,
+ * copied to the host or synchronised.  The sums are the exact fixed-point
+ * sums of the context entry points, so the values are the same bits.  Scratch
+ * is one file-scope device array per GPU (about 3 MiB, held until the process
+ * ends) shared by every call on that GPU: calls that overlap on one GPU (other
+ * streams, other threads) race.  At most HV_NOALLOC_MAX_PARTICLES particles;
+ * n_protons, n_neutrons >= 1; larger sizes return HV_ERR_ARG.  A species with
+ * no in-box particle gets a NaN <r^2>. */
+#define HV_NOALLOC_MAX_PARTICLES (1 << 24)
+int hv_stats_noalloc(const hv_world *world, int precision, const void *x, const void *y, const void *z,
+                     int n_protons, int n_neutrons, double *out, void *stream);
+
 /* ---- adjoint (no counterpart in the reference) ---------------------------- */
 /* For fitting the Skyrme coefficients to a dipole response (the use of the
  * reference's fit scripts): starting from the particles the context holds

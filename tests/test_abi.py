@@ -96,3 +96,20 @@ def test_oracle_header_only_used_by_tests():
     for p in (ROOT / "hyper-vlasov_amd").rglob("*"):
         if p.suffix in (".py", ".hip", ".cpp", ".h") or p.name == "Makefile":
             assert "liboracle" not in p.read_text() and "hv_oracle" not in p.read_text(), p
+
+
+def test_stats_noalloc_rejects_sizes_beyond_its_scratch():
+    """hv_stats_noalloc checks its arguments before any device call: more
+    particles than its fixed scratch holds, an empty species or a null pointer
+    return HV_ERR_ARG (no GPU needed for these)."""
+    import hvamd
+    from hvamd import _lib
+    lib = hvamd.load()
+    w = hvamd.World((16, 16, 16), (4.0, 4.0, 4.0)).c()
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the checks fail first
+    big = _lib.HV_NOALLOC_MAX_PARTICLES
+    for np_, nn in ((big, 1), (big // 2, big // 2 + 1), (0, 5), (5, 0)):
+        assert lib.hv_stats_noalloc(ctypes.byref(w), 0, dummy, dummy, dummy, np_, nn, dummy, None) == _lib.HV_ERR_ARG
+    assert lib.hv_stats_noalloc(ctypes.byref(w), 0, None, dummy, dummy, 5, 5, dummy, None) == _lib.HV_ERR_ARG
+    assert lib.hv_stats_noalloc(ctypes.byref(w), 0, dummy, dummy, dummy, 5, 5, None, None) == _lib.HV_ERR_ARG
+    assert lib.hv_stats_noalloc(ctypes.byref(w), 7, dummy, dummy, dummy, 5, 5, dummy, None) == _lib.HV_ERR_ARG
