@@ -50,6 +50,13 @@ struct hv_ctx {
     int mixed_fp64_sweeps = 0;          // FP64 iterations of the last mixed solve
     long long cnt_solves = 0, cnt_it32 = 0, cnt_it64 = 0;  // SOR work since hv_profile_enable
     bool phi_warm = false;              // phi holds a previous solve (mixed SOR corrects it in FP32)
+    // mixed form in the time loop: the potentials of the two solves before the
+    // last (split layout), and how many consecutive converged loop solves phi
+    // and they hold (0..3): the start of the next solve is extrapolated from them
+    double *phi_h[2] = {};
+    int hist_n = 0;
+    int sor_extrap = 2;                 // option HV_OPT_SOR_EXTRAPOLATE (mixed form only)
+    int last_extrap = 0;                // order used by the last solve
     int sor_fused = 0;                  // option: one launch per SOR iteration (red + black fused; slower at 128^3)
     double *phi_alt = nullptr;          // second potential buffer of the fused SOR
     float *sor_delta_alt = nullptr;
@@ -903,10 +910,17 @@ inline int mixed_fp64_budget(int max_it, int iters32) {
 // per-iteration max |dphi|; kernels launched past the converged iteration
 // return immediately (SorCtl), so the result is exactly the reference's
 // iterate: stop after the first iteration whose max |dphi| < tol.
+//
+// loop: a solve of the fused time loop.  In the mixed form its start is then
+// extrapolated from the last converged loop solves (HV_OPT_SOR_EXTRAPOLATE);
+// the FP32 correction and the FP64 finishing sweeps under the reference's
+// stop rule run from that start.  Any other solve (the per-kernel entry
+// point) starts from phi as it is and clears the history.
 template <typename RT>
 int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it, int *iters_out,
-              double *maxdiff_out) {
+              double *maxdiff_out, bool loop = false) {
     if (max_it <= 0) return set_err(c, HV_ERR_ARG, "sor max_it must be positive");
+    c->last_extrap = 0;
     const int nx = c->world.n[0], ny = c->world.n[1], nz = c->world.n[2];
     RT *rho = (RT *)c->rho_s;
     k_rho_split<RT><<<blocks_for(c->G), kThreads, 0, c->stream>>>(rho_nat, rho, c->G, c->sp);
@@ -941,6 +955,14 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
     int iters = 0;
     double md = 0.0;
     if (c->precision == HV_PREC_MIXED && c->phi_warm) {
+        if (loop && c->sor_extrap > 0) {
+            int order = c->hist_n - 1 < c->sor_extrap ? c->hist_n - 1 : c->sor_extrap;
+            if (order < 0) order = 0;
+            k_sor_extrapolate<<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, c->phi_h[0], c->phi_h[1],
+                                                                                  order, 2 * c->sp.Gh);
+            HV_CHECK_LAUNCH(c);
+            c->last_extrap = order;
+        }
         // FP32 defect-correction iterations on the small correction of a warm
         // potential, then FP64 sweeps until the stop rule holds in FP64
         k_sor_defect<RT><<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, rho, c->sor_R, c->sor_delta,
@@ -997,6 +1019,8 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
     }
     c->cnt_solves++;
     c->phi_warm = true;
+    // the history counts consecutive converged loop solves only
+    c->hist_n = (loop && !(md > tol)) ? (c->hist_n < 3 ? c->hist_n + 1 : 3) : 0;
     if (iters_out) *iters_out = iters;
     if (maxdiff_out) *maxdiff_out = md;
     return (md > tol) ? HV_WARN_SOR_NOT_CONVERGED : HV_OK;
@@ -1155,7 +1179,7 @@ int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, i
     rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
     if (rc) return rc;
     double md;
-    rc = sor_solve<T>(c, (const T *)c->density, p->sor_omega, p->sor_tol, p->sor_max_it, iters, &md);
+    rc = sor_solve<T>(c, (const T *)c->density, p->sor_omega, p->sor_tol, p->sor_max_it, iters, &md, true);
     if (rc < 0) return rc;
     *warn |= rc;
     return potential_from_density<T>(c, s, true);
@@ -1180,6 +1204,7 @@ int run_init(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *sor_iters) 
     HV_HIP(c, hipMemcpyAsync(c->phi, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, c->stream));
     HV_HIP(c, hipStreamSynchronize(c->stream));
     c->phi_warm = false;
+    c->hist_n = 0;
     rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
     if (rc) return rc;
     if (c->sort_every > 0) {
@@ -1422,7 +1447,10 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
         ok = ok && hipMalloc((void **)&c->sor_delta_alt, sizeof(float) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
         ok = ok && hipMalloc((void **)&c->sor_R, sizeof(float) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
         ok = ok && hipMalloc((void **)&c->sor_delta, sizeof(float) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
+        for (int q = 0; q < 2; q++)
+            ok = ok && hipMalloc((void **)&c->phi_h[q], sizeof(double) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
     }
+    if (precision != HV_PREC_MIXED) c->sor_extrap = 0;
     ok = ok && hipMalloc((void **)&c->ctl, sizeof(SorCtl)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->status, sizeof(unsigned)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->partials, sizeof(long long) * 6 * (size_t)blocks_for(c->total)) == hipSuccess;
@@ -1465,7 +1493,7 @@ int hv_ctx_destroy(hv_ctx *c) {
     for (void *b : sbufs)
         if (b) (void)hipFree(b);
     void *bufs[] = {c->acc, c->density, c->U, c->phi, c->phi_alt, c->rho_s, c->sor_R, c->sor_delta,
-                    c->sor_delta_alt, c->D, c->F[0], c->F[1], c->F[2],
+                    c->sor_delta_alt, c->phi_h[0], c->phi_h[1], c->D, c->F[0], c->F[1], c->F[2],
                     c->smear_tab, c->maxd, c->maxc, c->ctl, c->slots, c->status, c->partials, c->sflags,
                     c->en_partials, c->en_out};
     for (void *b : bufs)
@@ -1503,6 +1531,7 @@ int hv_upload_particles(hv_ctx *c, const double *x, const double *y, const doubl
     if (!c) return HV_ERR_ARG;
     int rc0 = restore_order(c);
     if (rc0) return rc0;
+    c->hist_n = 0;  // a new state: the potentials of earlier solves do not extrapolate to it
     const double *src[9] = {x, y, z, kx, ky, kz, fx, fy, fz};
     for (int q = 0; q < 9; q++) {
         if (!src[q]) continue;
@@ -1543,6 +1572,7 @@ int hv_upload_field(hv_ctx *c, int which, const double *host) {
         HV_HIP(c, hipMemcpyAsync(c->phi, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, c->stream));
         HV_HIP(c, hipStreamSynchronize(c->stream));
         c->phi_warm = false;
+        c->hist_n = 0;
         return HV_OK;
     }
     void *p;
@@ -1862,6 +1892,12 @@ int hv_set_option(hv_ctx *c, int option, int value) {
         if (value < 1 || value > 64) return set_err(c, HV_ERR_ARG, "the FP32 SOR lag must be in [1, 64]");
         c->res32_lag = value;
         return HV_OK;
+    case HV_OPT_SOR_EXTRAPOLATE:
+        if (value < 0 || value > 2) return set_err(c, HV_ERR_ARG, "the extrapolation order must be 0, 1 or 2");
+        if (value != 0 && c->precision != HV_PREC_MIXED)
+            return set_err(c, HV_ERR_STATE, "the FP64 form starts every solve from the last potential");
+        c->sor_extrap = value;
+        return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -1880,6 +1916,7 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
         return HV_OK;
     case HV_OPT_SMEAR_TILE: *value = c->smear_tk_max; return HV_OK;
     case HV_OPT_SMEAR_FUSED: *value = c->smear_fused; return HV_OK;
+    case HV_OPT_SOR_EXTRAPOLATE: *value = c->sor_extrap; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }
@@ -1951,9 +1988,9 @@ int hv_copy_bandwidth(long long bytes, int reps, double *best_gbs, double *best_
     }
     // the first copy is not timed (first touch of b)
     for (int r = 0; r <= reps; r++) {
-        hipEventRecord(e0, st);
+        (void)hipEventRecord(e0, st);
         k_copy_calibration<<<grid, 256, 0, st>>>(a, b, n);
-        hipEventRecord(e1, st);
+        (void)hipEventRecord(e1, st);
         float ms = 0.0f;
         if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {

@@ -6,7 +6,8 @@
 // rho = smear(deposit(x)).  The vector-Jacobian product of F with a
 // cotangent c_i per particle runs the chain backwards, each stage the exact
 // transpose of its forward kernel:
-//   gather        -> deposit of c_i with the same CIC weights (k_adj_deposit_vec)
+//   gather        -> deposit of c_i with the same CIC weights, summed exactly in
+//                    fixed point (k_adj_deposit_vec)
 //   f = -grad_h U -> minus the transposed difference operator (k_adj_fdm_t)
 //   U = S(rho)    -> pointwise derivative; its theta part reduced (k_adj_skyrme)
 //   phi(rho_p)    -> the same SOR on (-L_h) with zero faces (L_h is symmetric)
@@ -20,15 +21,32 @@
 
 namespace hv {
 
-// transpose of the CIC gather: fbar[a][species][node] += w * cot_a
+// Fixed-point scale of the gathered cotangent sums: 2^k with
+// N max|cot| 2^k < 2^61, so every node's sum of at most N terms
+// |rint(w cot 2^k)| <= w |cot| 2^k + 1/2 stays below 2^62 in int64.  One
+// thread; amax holds max |cot| over the 3N components.
+__global__ void k_adj_fix_scale(const double *__restrict__ amax, int n, double *__restrict__ scale) {
+    const double m = *amax * (double)n;
+    int e = 0;
+    if (m > 0.0) (void)frexp(m, &e);  // m < 2^e
+    int k = 61 - e;
+    k = k > 1000 ? 1000 : (k < -1000 ? -1000 : k);
+    scale[0] = ldexp(1.0, k);
+    scale[1] = ldexp(1.0, -k);
+}
+
+// transpose of the CIC gather: fbar[a][species][node] += w * cot_a, summed
+// exactly as int64 multiples of 1/scale (integer adds: the sum, and so the
+// gradient, does not depend on the order the particles are visited in)
 __global__ void __launch_bounds__(kBlock)
 k_adj_deposit_vec(Particles<double> p, int np, int total, Geom<double> g, const double *__restrict__ cot,
-                  double *__restrict__ fbar) {
+                  const double *__restrict__ scale, unsigned long long *__restrict__ fbar) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     Cell<double> c;
     if (!cic_cell(p.x[i], p.y[i], p.z[i], g, c)) return;
     const long long G2 = 2 * g.G, off = (i < np) ? 0 : g.G;
+    const double sc = scale[0];
     const double cx = cot[i], cy = cot[total + i], cz = cot[2LL * total + i];
 #pragma unroll
     for (int corner = 0; corner < 8; corner++) {
@@ -36,9 +54,9 @@ k_adj_deposit_vec(Particles<double> p, int np, int total, Geom<double> g, const 
         if (ii >= g.nx || jj >= g.ny || kk >= g.nz) continue;
         const double w = corner_weight(c, corner);
         const long long q = off + idx3(ii, jj, kk, g.ny, g.nz);
-        atomicAdd(&fbar[q], w * cx);
-        atomicAdd(&fbar[G2 + q], w * cy);
-        atomicAdd(&fbar[2 * G2 + q], w * cz);
+        atomicAdd(&fbar[q], (unsigned long long)(long long)rint((w * cx) * sc));
+        atomicAdd(&fbar[G2 + q], (unsigned long long)(long long)rint((w * cy) * sc));
+        atomicAdd(&fbar[2 * G2 + q], (unsigned long long)(long long)rint((w * cz) * sc));
     }
 }
 
@@ -57,22 +75,25 @@ __device__ __forceinline__ double fdm_axis_t(int i, int n, double inv_h, double 
     return s;
 }
 
-// Ubar = -D^T fbar (f = -grad_h U), both species
+// Ubar = -D^T fbar (f = -grad_h U), both species; fbar in int64 multiples of
+// 1/scale (k_adj_deposit_vec), read back exactly (scale is a power of two)
 __global__ void __launch_bounds__(kBlock)
-k_adj_fdm_t(const double *__restrict__ fbar, double *__restrict__ Ubar, Geom<double> g) {
+k_adj_fdm_t(const unsigned long long *__restrict__ fbar, const double *__restrict__ scale, double *__restrict__ Ubar,
+            Geom<double> g) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long G = g.G, G2 = 2 * G;
     if (q >= G2) return;
     const long long off = q >= G ? G : 0;
     const long long node = q - off;
+    const double inv = scale[1];
     const int k = (int)(node % g.nz), j = (int)((node / g.nz) % g.ny), i = (int)(node / ((long long)g.ny * g.nz));
-    const double *fx = fbar + off, *fy = fbar + G2 + off, *fz = fbar + 2 * G2 + off;
+    const unsigned long long *fx = fbar + off, *fy = fbar + G2 + off, *fz = fbar + 2 * G2 + off;
     const double sx = fdm_axis_t(i, g.nx, 1.0 / g.dx, 1.0 / g.two_dx,
-                                 [&](int m) { return fx[idx3(m, j, k, g.ny, g.nz)]; });
+                                 [&](int m) { return (double)(long long)fx[idx3(m, j, k, g.ny, g.nz)] * inv; });
     const double sy = fdm_axis_t(j, g.ny, 1.0 / g.dy, 1.0 / g.two_dy,
-                                 [&](int m) { return fy[idx3(i, m, k, g.ny, g.nz)]; });
+                                 [&](int m) { return (double)(long long)fy[idx3(i, m, k, g.ny, g.nz)] * inv; });
     const double sz = fdm_axis_t(k, g.nz, 1.0 / g.dz, 1.0 / g.two_dz,
-                                 [&](int m) { return fz[idx3(i, j, m, g.ny, g.nz)]; });
+                                 [&](int m) { return (double)(long long)fz[idx3(i, j, m, g.ny, g.nz)] * inv; });
     Ubar[q] = -(sx + sy + sz);
 }
 

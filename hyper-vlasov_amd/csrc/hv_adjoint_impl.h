@@ -28,7 +28,8 @@ constexpr int kAdjBlocks = 1024;
 
 struct AdjBuf {
     double *X = nullptr, *Kb = nullptr, *cot = nullptr;  // 3N each
-    double *fbar = nullptr;                              // 3 x 2G
+    unsigned long long *fbar = nullptr;                  // 3 x 2G, fixed point (k_adj_deposit_vec)
+    double *fscale = nullptr;                            // its scale 2^k and 2^-k
     double *Ubar = nullptr, *rhobar = nullptr, *cbar = nullptr;  // 2G
     double *psi = nullptr, *psrc = nullptr;              // 2 Gh (split)
     double *partials = nullptr, *theta = nullptr, *amax = nullptr;
@@ -81,11 +82,18 @@ int adj_force_vjp(hv_ctx *c, AdjBuf &b, const double *x, const double *U, const 
     pp.x = (double *)x;
     pp.y = (double *)x + N;
     pp.z = (double *)x + 2LL * N;
-    int rc = fill<double>(c, b.fbar, 0.0, 3 * G2);
+    int rc = fill<unsigned long long>(c, b.fbar, 0ULL, 3 * G2);
     if (rc) return rc;
-    k_adj_deposit_vec<<<blocks_for(N), kThreads, 0, c->stream>>>(pp, c->np, N, g, b.cot, b.fbar);
+    // fixed-point scale from max |cot| (k_adj_fix_scale)
+    k_adj_absmax<<<kAdjBlocks, kThreads, 0, c->stream>>>(b.cot, 3LL * N, b.amax);
     HV_CHECK_LAUNCH(c);
-    k_adj_fdm_t<<<blocks_for(G2), kThreads, 0, c->stream>>>(b.fbar, b.Ubar, g);
+    k_adj_absmax<<<1, kThreads, 0, c->stream>>>(b.amax, kAdjBlocks, b.amax + kAdjBlocks);
+    HV_CHECK_LAUNCH(c);
+    k_adj_fix_scale<<<1, 1, 0, c->stream>>>(b.amax + kAdjBlocks, N, b.fscale);
+    HV_CHECK_LAUNCH(c);
+    k_adj_deposit_vec<<<blocks_for(N), kThreads, 0, c->stream>>>(pp, c->np, N, g, b.cot, b.fscale, b.fbar);
+    HV_CHECK_LAUNCH(c);
+    k_adj_fdm_t<<<blocks_for(G2), kThreads, 0, c->stream>>>(b.fbar, b.fscale, b.Ubar, g);
     HV_CHECK_LAUNCH(c);
     const unsigned nbs = blocks_for(G);
     k_adj_skyrme<<<nbs, kThreads, 0, c->stream>>>(rho, b.Ubar, b.rhobar, s->A, s->B, s->C, s->gamma, G, b.partials);
@@ -136,7 +144,8 @@ int adj_setup(hv_ctx *c, AdjBuf &b, int seg_len, int nck) {
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.X, sizeof(double) * 3 * N);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.Kb, sizeof(double) * 3 * N);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.cot, sizeof(double) * 3 * N);
-    rc = rc ? rc : adj_alloc(c, b, (void **)&b.fbar, sizeof(double) * 3 * G2);
+    rc = rc ? rc : adj_alloc(c, b, (void **)&b.fbar, sizeof(unsigned long long) * 3 * G2);
+    rc = rc ? rc : adj_alloc(c, b, (void **)&b.fscale, sizeof(double) * 2);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.Ubar, sizeof(double) * G2);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.rhobar, sizeof(double) * G2);
     rc = rc ? rc : adj_alloc(c, b, (void **)&b.cbar, sizeof(double) * G2);
@@ -170,24 +179,26 @@ int dipole_loss_grad(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int nste
     // the particles in the caller's order, never re-sorted during the run
     rc = restore_order(c);
     if (rc) return rc;
-    const int sort_saved = c->sort_every;
-    c->sort_every = 0;
     AdjBuf b;
+    // on every return (the HV_HIP / HV_CHECK_LAUNCH early returns included):
+    // restore the sort cadence, drain the stream, free the adjoint buffers
+    struct Guard {
+        hv_ctx *c;
+        AdjBuf &b;
+        int sort_saved;
+        ~Guard() {
+            c->sort_every = sort_saved;
+            (void)hipStreamSynchronize(c->stream);
+            adj_free(b);
+        }
+    } guard{c, b, c->sort_every};
+    c->sort_every = 0;
     rc = ensure_smear<double>(c, p->sigma_r, p->part_per_nucleon);
     if (!rc) rc = ensure_maxd(c, kAdjMaxIt > p->sor_max_it ? kAdjMaxIt : p->sor_max_it);
     if (!rc) rc = ensure_pinned(c, (size_t)(kAdjMaxIt > p->sor_max_it ? kAdjMaxIt : p->sor_max_it) + 8);
     if (!rc) rc = adj_setup(c, b, ck_every, nseg);
-    if (rc) {
-        c->sort_every = sort_saved;
-        adj_free(b);
-        return rc;
-    }
-    auto done = [&](int r) {
-        c->sort_every = sort_saved;
-        (void)hipStreamSynchronize(c->stream);
-        adj_free(b);
-        return r;
-    };
+    if (rc) return rc;
+    auto done = [](int r) { return r; };
     for (int q = 0; q < 9; q++)
         HV_HIP(c, hipMemcpyAsync(b.init + (size_t)q * N, c->part[q], sizeof(double) * N, hipMemcpyDeviceToDevice,
                                  c->stream));

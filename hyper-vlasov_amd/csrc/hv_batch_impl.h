@@ -43,6 +43,9 @@ struct hv_batch {
     std::vector<int> pred32, pred64;      // per-instance SOR iteration predictions
     std::vector<int> boost;               // per-instance boost flag of the current step
     bool ready = false;
+    int extrap = 0;                       // option HV_OPT_SOR_EXTRAPOLATE (mixed form)
+    int *xorder = nullptr;                // per-instance extrapolation order of the current step
+    int *hxorder = nullptr;               // pinned staging of xorder
     int sort_every = 8;
     int group_kib = 0;  // SOR working-set budget (KiB) of one instance group (0: all instances together)
     // LDS-resident FP32 correction of the whole batch (mixed precision)
@@ -171,6 +174,8 @@ void batch_fill_args(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
         A.G = c->G;
         A.sp = c->sp;
         A.phi = c->phi;
+        A.h0 = c->phi_h[0];
+        A.h1 = c->phi_h[1];
         A.rho_s = (T *)c->rho_s;
         A.R = c->sor_R;
         A.delta = c->sor_delta;
@@ -752,6 +757,10 @@ int batch_step_device(hv_batch *b, int end_kick, bool res) {
     HV_BHIP(b, hipGetLastError());
     if (res) {
         BStage st(b, HV_KCLASS_SOR_STAGE);
+        if (b->extrap) {
+            kb_sor_extrapolate<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n, b->xorder);
+            HV_BHIP(b, hipGetLastError());
+        }
         kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
         HV_BHIP(b, hipGetLastError());
         int rc = batch_res32_enqueue(b);
@@ -779,6 +788,11 @@ int batch_step_finish(hv_batch *b, const hv_params *prm, bool res, int *iters_ou
             if (res) {
                 rc = batch_res32_collect(b, it1);
             } else {
+                if (b->extrap) {
+                    kb_sor_extrapolate<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n,
+                                                                                        b->xorder);
+                    HV_BHIP(b, hipGetLastError());
+                }
                 kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
                 HV_BHIP(b, hipGetLastError());
                 const std::vector<int> cap(n, prm[0].sor_max_it);
@@ -800,6 +814,10 @@ int batch_step_finish(hv_batch *b, const hv_params *prm, bool res, int *iters_ou
                 b->inst[i]->mixed_fp64_sweeps = it2[i];
                 iters_out[i] = it1[i] + it2[i];
                 if (md2[i] > prm[i].sor_tol) *warn |= HV_WARN_SOR_NOT_CONVERGED;
+                // as sor_solve: consecutive converged loop solves
+                hv_ctx *c = b->inst[i];
+                c->last_extrap = b->extrap ? b->hxorder[i] : 0;
+                c->hist_n = md2[i] > prm[i].sor_tol ? 0 : (c->hist_n < 3 ? c->hist_n + 1 : 3);
             }
         } else {
             const std::vector<int> cap(n, prm[0].sor_max_it);
@@ -948,6 +966,15 @@ int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int 
             rc = batch_res32_prepare(b, prm);
             if (rc) return rc;
         }
+        if (b->extrap) {
+            // extrapolation order of each instance's solve from its history
+            for (int i = 0; i < n; i++) {
+                const int h = b->inst[i]->hist_n - 1;
+                b->hxorder[i] = h < 0 ? 0 : (h < b->extrap ? h : b->extrap);
+            }
+            HV_BHIP(b, hipMemcpyAsync(b->xorder, b->hxorder, sizeof(int) * (size_t)n, hipMemcpyHostToDevice,
+                                      b->stream));
+        }
         if (s > 0 && graphs && !b->gexec && batch_capture_step<T>(b, res) != HV_OK) graphs = false;
         if (s > 0 && graphs) {
             HV_BHIP(b, hipGraphLaunch(b->gexec, b->stream));
@@ -1010,7 +1037,9 @@ int hv_batch_create(int n, const hv_world *worlds, const int *n_protons, const i
     b->pred64.assign(n, 16);
     b->boost.assign(n, 0);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc((void **)&b->dstep, sizeof(int)) != hipSuccess) {
+        hipMalloc((void **)&b->dstep, sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&b->xorder, sizeof(int) * (size_t)n) != hipSuccess ||
+        hipHostMalloc((void **)&b->hxorder, sizeof(int) * (size_t)n, hipHostMallocDefault) != hipSuccess) {
         hv_batch_destroy(b);
         return HV_ERR_HIP;
     }
@@ -1032,6 +1061,7 @@ int hv_batch_create(int n, const hv_world *worlds, const int *n_protons, const i
         c->sort_every = b->sort_every;
         b->inst.push_back(c);
     }
+    b->extrap = precision == HV_PREC_MIXED ? b->inst[0]->sor_extrap : 0;
     *out = b;
     return HV_OK;
 }
@@ -1052,6 +1082,8 @@ int hv_batch_destroy(hv_batch *b) {
     if (b->hrres) (void)hipHostFree(b->hrres);
     if (b->hrinst) (void)hipHostFree(b->hrinst);
     if (b->dstep) (void)hipFree(b->dstep);
+    if (b->xorder) (void)hipFree(b->xorder);
+    if (b->hxorder) (void)hipHostFree(b->hxorder);
     batch_drop_graph(b);
     if (b->act) (void)hipFree(b->act);
     if (b->hact) (void)hipHostFree(b->hact);
@@ -1083,6 +1115,7 @@ int hv_batch_get_option(hv_batch *b, int option, int *value) {
         return HV_OK;
     case HV_OPT_SMEAR_FUSED: *value = b->smear_fused; return HV_OK;
     case HV_OPT_BATCH_GRAPH: *value = b->use_graph; return HV_OK;
+    case HV_OPT_SOR_EXTRAPOLATE: *value = b->extrap; return HV_OK;
     default: return batch_err(b, HV_ERR_ARG, "option %d is not a batch option", option);
     }
 }
@@ -1115,6 +1148,16 @@ int hv_batch_set_option(hv_batch *b, int option, int value) {
     }
     if (option == HV_OPT_BATCH_GRAPH) {
         b->use_graph = value ? 1 : 0;
+        return HV_OK;
+    }
+    if (option == HV_OPT_SOR_EXTRAPOLATE) {
+        for (hv_ctx *c : b->inst) {
+            int rc = hv_set_option(c, option, value);
+            if (rc) return batch_fwd(b, c, rc);
+        }
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        batch_drop_graph(b);  // the step's device part gains or loses a kernel
+        b->extrap = value;
         return HV_OK;
     }
     if (option == HV_OPT_SOR_F32_LAG) {

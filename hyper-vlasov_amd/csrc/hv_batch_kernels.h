@@ -40,6 +40,7 @@ struct BInst {
     long long G;
     Split sp;
     double *phi;
+    double *h0, *h1;      // mixed form: potentials of the two solves before the last
     T *rho_s;
     float *R, *delta;
     SorTiles tl;
@@ -144,6 +145,16 @@ __global__ void __launch_bounds__(kBlock)
 kb_sor_defect(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
     sor_defect_body<T>(bid, A.phi, A.rho_s, A.R, A.delta, A.sp, A.cf);
+}
+
+// extrapolated start of every instance's mixed solve (sor_extrapolate_body),
+// order[i] per instance
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_sor_extrapolate(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n,
+                   const int *__restrict__ order) {
+    HV_BATCH_LOCATE();
+    sor_extrapolate_body(bid, A.phi, A.h0, A.h1, order[inst], 2 * A.sp.Gh);
 }
 
 template <typename T>

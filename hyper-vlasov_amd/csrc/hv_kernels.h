@@ -1240,6 +1240,36 @@ __global__ void k_sor_fold(double *__restrict__ phi, const float *__restrict__ d
     sor_fold_body(blockIdx.x, phi, delta, n);
 }
 
+// Start of a mixed solve in a time loop: phi holds the potential of the last
+// solve (phi_n), h0 and h1 the two before it (phi_{n-1}, phi_{n-2}).  The new
+// start is phi_n (order 0), 2 phi_n - phi_{n-1} = phi_n + (phi_n - phi_{n-1})
+// (order 1) or 3 phi_n - 3 phi_{n-1} + phi_{n-2} = phi_{n-2} + 3 (phi_n -
+// phi_{n-1}) (order 2), and the history moves one step: h1 <- h0, h0 <- phi_n.
+// Fixed (Dirichlet) nodes hold the same value in every entry, so each form
+// returns that value exactly.
+__device__ __forceinline__ void sor_extrapolate_body(const int bid, double *__restrict__ phi,
+                                                     double *__restrict__ h0, double *__restrict__ h1, int order,
+                                                     long long n) {
+    const long long q = (long long)bid * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double p0 = phi[q], p1 = h0[q];
+    double g = p0;
+    if (order == 1) {
+        g = p0 + (p0 - p1);
+    } else if (order >= 2) {
+        const double p2 = h1[q];
+        g = p2 + 3.0 * (p0 - p1);
+    }
+    h1[q] = p1;
+    h0[q] = p0;
+    phi[q] = g;
+}
+
+__global__ void k_sor_extrapolate(double *__restrict__ phi, double *__restrict__ h0, double *__restrict__ h1,
+                                  int order, long long n) {
+    sor_extrapolate_body(blockIdx.x, phi, h0, h1, order, n);
+}
+
 // ---- one launch per SOR iteration: red and black fused ------------------------
 // Out of place (in -> out).  A block owns FT x FT grid rows (all k): it first
 // computes the new red values of its rows and of the one-row ring around them

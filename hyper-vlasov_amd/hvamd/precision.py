@@ -33,7 +33,7 @@ import time
 import numpy as np
 
 from . import configs, spectrum
-from ._lib import HV_PREC_F64, HV_PREC_MIXED
+from ._lib import HV_OPT_SOR_EXTRAPOLATE, HV_PREC_F64, HV_PREC_MIXED
 from .sim import HBARC, Simulation
 
 
@@ -51,14 +51,17 @@ def perturbed_positions(w, seed: int, size: float = 2.0 ** -24):
     return x * (1.0 + sx * size), y * (1.0 + sy * size), z * (1.0 + sz * size), kx, ky, kz
 
 
-def run(w, prec: int, steps: int, chunk: int, perturb_seed: int | None = None):
-    """(stats[steps, 4], energies[(step, [E, Ekin, Esky, Ecoul])], sor iterations, seconds)."""
+def run(w, prec: int, steps: int, chunk: int, perturb_seed: int | None = None, extrapolate: int | None = None):
+    """(stats[steps, 4], energies[(step, [E, Ekin, Esky, Ecoul])], sor iterations, seconds);
+    extrapolate: the mixed form's HV_OPT_SOR_EXTRAPOLATE (None: the default)."""
     if perturb_seed is None:
         x, y, z, kx, ky, kz = w.initial_state()
     else:
         x, y, z, kx, ky, kz = perturbed_positions(w, perturb_seed)
     sim = Simulation(w.world(), w.n_protons, w.n_neutrons, prec)
     try:
+        if extrapolate is not None and prec == HV_PREC_MIXED:
+            sim.set_option(HV_OPT_SOR_EXTRAPOLATE, extrapolate)
         sim.upload_particles(x, y, z, kx, ky, kz)
         prm = w.params(steps)
         sk = w.sky()
@@ -129,6 +132,8 @@ def study(name: str = "c1", steps: int | None = None, chunk: int = 100, window: 
     print(f"[precision] {w.name}: FP64 done ({f64[3]:.1f} s)", file=sys.stderr, flush=True)
     mix = run(w, HV_PREC_MIXED, steps, chunk)
     print(f"[precision] {w.name}: mixed done ({mix[3]:.1f} s)", file=sys.stderr, flush=True)
+    mix0 = run(w, HV_PREC_MIXED, steps, chunk, extrapolate=0)
+    print(f"[precision] {w.name}: mixed, no extrapolation, done ({mix0[3]:.1f} s)", file=sys.stderr, flush=True)
     pert = []
     for seed in range(n_perturbed):
         r = run(w, HV_PREC_F64, steps, chunk, perturb_seed=seed)
@@ -167,6 +172,8 @@ def study(name: str = "c1", steps: int | None = None, chunk: int = 100, window: 
                  "energy_series_MeV": [[s_, e[0]] for s_, e in f64[1]],
                  "centroid_MeV": float(spectrum.dipole_response(f64[0], w.dt, w.z, w.n, w.dp / HBARC)[4])},
         "mixed_vs_fp64": {"seconds": mix[3], "sor_iters_mean": float(mix[2].mean()), **mixed},
+        "mixed_no_extrapolation_vs_fp64": {"seconds": mix0[3], "sor_iters_mean": float(mix0[2].mean()),
+                                           **compare(w, f64, mix0, window)},
         "fp64_perturbed_2^-24_vs_fp64": {"runs": pert, "distribution": floor},
         "mixed_within_perturbed": {"note": "k/n: perturbed runs whose |difference| is at most the mixed form's",
                                    **position},

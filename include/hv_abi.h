@@ -270,6 +270,18 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * Same bits for every value.  hv_get_option / hv_batch_get_option report the
  * body in use once the form has solved. */
 #define HV_OPT_SOR_F32_OWNED 10
+/* HV_OPT_SOR_EXTRAPOLATE (mixed form only; 0 for HV_PREC_F64, which starts
+ * every solve from the last potential as the reference does): the start of
+ * each Coulomb solve of the fused loop (hv_run_init / hv_run_steps) is
+ * extrapolated in time from the potentials of the last converged loop solves,
+ * 0 none (the last potential), 1 linear (2 phi_n - phi_(n-1)), 2 quadratic
+ * (3 phi_n - 3 phi_(n-1) + phi_(n-2); default).  The FP32 correction and the
+ * FP64 finishing sweeps then run from that start under the same stop rule
+ * (max |dphi| of an FP64 iteration below tol); the solution differs from the
+ * FP64 form's by the stop rule's own error, not by precision.  Only loop solves
+ * that converged count as history; run_init, uploads of particles or of the
+ * potential and hv_coulomb_sor clear it. */
+#define HV_OPT_SOR_EXTRAPOLATE 11
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
@@ -332,7 +344,7 @@ hv_ctx *hv_batch_instance(hv_batch *batch, int i);
 /* HV_OPT_SORT_EVERY or HV_OPT_SMEAR_TILE (applied to every instance), or
  * HV_OPT_BATCH_GROUP_KIB, HV_OPT_SOR_RESIDENT_F32, HV_OPT_SOR_F32_LAG,
  * HV_OPT_SOR_F32_OWNED, HV_OPT_SMEAR_FUSED (default 1 for a batch),
- * HV_OPT_BATCH_GRAPH */
+ * HV_OPT_BATCH_GRAPH, HV_OPT_SOR_EXTRAPOLATE (every instance) */
 int hv_batch_set_option(hv_batch *batch, int option, int value);
 /* the value of a batch option (HV_OPT_SOR_RESIDENT_F32 and
  * HV_OPT_SOR_F32_OWNED: whether that form runs, once the batch has solved) */

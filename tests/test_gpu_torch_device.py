@@ -36,4 +36,4 @@ def test_dipole_loss_on_gpu_theta():
         _, grad, _ = fit.loss_grad(theta)
     finally:
         fit.close()
-    assert torch.allclose(th.grad.cpu(), torch.as_tensor(3.0 * grad), rtol=1e-9, atol=0)
+    assert torch.equal(th.grad.cpu(), torch.as_tensor(3.0 * grad))
