@@ -50,13 +50,16 @@ struct hv_ctx {
     int mixed_fp64_sweeps = 0;          // FP64 iterations of the last mixed solve
     long long cnt_solves = 0, cnt_it32 = 0, cnt_it64 = 0;  // SOR work since hv_profile_enable
     bool phi_warm = false;              // phi holds a previous solve (mixed SOR corrects it in FP32)
-    // mixed form in the time loop: the potentials of the two solves before the
-    // last (split layout), and how many consecutive converged loop solves phi
-    // and they hold (0..3): the start of the next solve is extrapolated from them
-    double *phi_h[2] = {};
+    // mixed form in the time loop: a ring of the potentials of the five solves
+    // before the last (split layout; newest at phi_h[hist_pos]) and how many
+    // consecutive converged loop solves phi and the ring hold (0..6): the start
+    // of the next solve is extrapolated from them (sor_extrapolate_body)
+    double *phi_h[5] = {};
     int hist_n = 0;
-    int sor_extrap = 2;                 // option HV_OPT_SOR_EXTRAPOLATE (mixed form only)
-    int last_extrap = 0;                // order used by the last solve
+    int hist_pos = 0;
+    int sor_extrap = 3;                 // option HV_OPT_SOR_EXTRAPOLATE (mixed form only)
+    int last_extrap = 0;                // predictor used by the last solve
+    int f32_tol_shift = 0;              // option HV_OPT_SOR_F32_TOL_SHIFT: FP32 stage tolerance tol / 2^shift
     int sor_fused = 0;                  // option: one launch per SOR iteration (red + black fused; slower at 128^3)
     double *phi_alt = nullptr;          // second potential buffer of the fused SOR
     float *sor_delta_alt = nullptr;
@@ -834,6 +837,16 @@ int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double
     return HV_OK;
 }
 
+// The predictor a loop solve uses: the chosen one once the ring holds its
+// points (hist_n counts phi and the ring's converged solves), else a line
+// through the last two, else none.
+inline int sor_predictor_for(int chosen, int hist_n) {
+    double w[5];
+    const int need = sor_predictor(chosen, w);
+    if (hist_n >= need) return chosen;
+    return hist_n >= 2 ? 1 : 0;
+}
+
 // The reference's SOR on c->phi (FP64), one launch per iteration (fused) or
 // per sweep.  The fused form alternates phi and phi_alt; the result is
 // left in c->phi.
@@ -956,12 +969,14 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
     double md = 0.0;
     if (c->precision == HV_PREC_MIXED && c->phi_warm) {
         if (loop && c->sor_extrap > 0) {
-            int order = c->hist_n - 1 < c->sor_extrap ? c->hist_n - 1 : c->sor_extrap;
-            if (order < 0) order = 0;
-            k_sor_extrapolate<<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, c->phi_h[0], c->phi_h[1],
-                                                                                  order, 2 * c->sp.Gh);
+            const int id = sor_predictor_for(c->sor_extrap, c->hist_n);
+            SorHist hs;
+            for (int k = 0; k < 5; k++) hs.h[k] = c->phi_h[k];
+            k_sor_extrapolate<<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, hs, id, c->hist_pos,
+                                                                                  2 * c->sp.Gh);
             HV_CHECK_LAUNCH(c);
-            c->last_extrap = order;
+            c->hist_pos = (c->hist_pos + 4) % 5;
+            c->last_extrap = id;
         }
         // FP32 defect-correction iterations on the small correction of a warm
         // potential, then FP64 sweeps until the stop rule holds in FP64
@@ -970,21 +985,26 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         HV_CHECK_LAUNCH(c);
         float *dbuf[2] = {c->sor_delta, c->sor_delta_alt};
         bool res32 = false;
+        // the FP32 stage may stop below the reference tolerance (its last
+        // potential then carries less of the stop rule's own error into the
+        // next start); the FP64 sweeps apply the reference rule either way
+        SorCoef cf32 = cf;
+        cf32.tol = ldexp(cf.tol, -c->f32_tol_shift);
         if (c->sor_res32 && res32_setup(c) == HV_OK) {
             // its iteration count is read in the FP64 sweeps' first wait
-            rc = sor_resident_f32_enqueue(c, cf, max_it);
+            rc = sor_resident_f32_enqueue(c, cf32, max_it);
             res32 = true;
         } else if (fused) {
-            rc = sor_iterate(c, cf, max_it, &iters, &md, [&](int it) {
+            rc = sor_iterate(c, cf32, max_it, &iters, &md, [&](int it) {
                 sor_launch(c, it, 1, HV_KCLASS_SOR_F32, k_sor_fused<false, float, float>, fgrd, fblk,
                            rows_lds * sizeof(float), dbuf[it & 1], dbuf[(it + 1) & 1], (const float *)c->sor_R, c->sp,
-                           ft, cf, it, c->maxd, c->ctl);
+                           ft, cf32, it, c->maxd, c->ctl);
             });
         } else {
-            rc = sor_iterate(c, cf, max_it, &iters, &md, [&](int it) {
+            rc = sor_iterate(c, cf32, max_it, &iters, &md, [&](int it) {
                 for (int colour = 0; colour < 2; colour++)
                     sor_launch(c, it, colour, HV_KCLASS_SOR_F32, k_sor_colour_f32<RT>, sgrd, sblk, 0, c->sor_delta,
-                               (const float *)c->sor_R, c->sp, tl, cf, colour, it, c->maxd, c->ctl);
+                               (const float *)c->sor_R, c->sp, tl, cf32, colour, it, c->maxd, c->ctl);
             });
         }
         if (rc) return rc;
@@ -1020,7 +1040,7 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
     c->cnt_solves++;
     c->phi_warm = true;
     // the history counts consecutive converged loop solves only
-    c->hist_n = (loop && !(md > tol)) ? (c->hist_n < 3 ? c->hist_n + 1 : 3) : 0;
+    c->hist_n = (loop && !(md > tol)) ? (c->hist_n < 6 ? c->hist_n + 1 : 6) : 0;
     if (iters_out) *iters_out = iters;
     if (maxdiff_out) *maxdiff_out = md;
     return (md > tol) ? HV_WARN_SOR_NOT_CONVERGED : HV_OK;
@@ -1447,7 +1467,7 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
         ok = ok && hipMalloc((void **)&c->sor_delta_alt, sizeof(float) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
         ok = ok && hipMalloc((void **)&c->sor_R, sizeof(float) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
         ok = ok && hipMalloc((void **)&c->sor_delta, sizeof(float) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
-        for (int q = 0; q < 2; q++)
+        for (int q = 0; q < 5; q++)
             ok = ok && hipMalloc((void **)&c->phi_h[q], sizeof(double) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
     }
     if (precision != HV_PREC_MIXED) c->sor_extrap = 0;
@@ -1493,7 +1513,8 @@ int hv_ctx_destroy(hv_ctx *c) {
     for (void *b : sbufs)
         if (b) (void)hipFree(b);
     void *bufs[] = {c->acc, c->density, c->U, c->phi, c->phi_alt, c->rho_s, c->sor_R, c->sor_delta,
-                    c->sor_delta_alt, c->phi_h[0], c->phi_h[1], c->D, c->F[0], c->F[1], c->F[2],
+                    c->sor_delta_alt, c->phi_h[0], c->phi_h[1], c->phi_h[2], c->phi_h[3], c->phi_h[4], c->D,
+                    c->F[0], c->F[1], c->F[2],
                     c->smear_tab, c->maxd, c->maxc, c->ctl, c->slots, c->status, c->partials, c->sflags,
                     c->en_partials, c->en_out};
     for (void *b : bufs)
@@ -1892,8 +1913,12 @@ int hv_set_option(hv_ctx *c, int option, int value) {
         if (value < 1 || value > 64) return set_err(c, HV_ERR_ARG, "the FP32 SOR lag must be in [1, 64]");
         c->res32_lag = value;
         return HV_OK;
+    case HV_OPT_SOR_F32_TOL_SHIFT:
+        if (value < 0 || value > 20) return set_err(c, HV_ERR_ARG, "the FP32 tolerance shift must be in 0..20");
+        c->f32_tol_shift = value;
+        return HV_OK;
     case HV_OPT_SOR_EXTRAPOLATE:
-        if (value < 0 || value > 2) return set_err(c, HV_ERR_ARG, "the extrapolation order must be 0, 1 or 2");
+        if (value < 0 || value > 5) return set_err(c, HV_ERR_ARG, "the SOR start predictor must be in 0..5");
         if (value != 0 && c->precision != HV_PREC_MIXED)
             return set_err(c, HV_ERR_STATE, "the FP64 form starts every solve from the last potential");
         c->sor_extrap = value;
@@ -1917,6 +1942,7 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
     case HV_OPT_SMEAR_TILE: *value = c->smear_tk_max; return HV_OK;
     case HV_OPT_SMEAR_FUSED: *value = c->smear_fused; return HV_OK;
     case HV_OPT_SOR_EXTRAPOLATE: *value = c->sor_extrap; return HV_OK;
+    case HV_OPT_SOR_F32_TOL_SHIFT: *value = c->f32_tol_shift; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }

@@ -44,7 +44,7 @@ struct hv_batch {
     std::vector<int> boost;               // per-instance boost flag of the current step
     bool ready = false;
     int extrap = 0;                       // option HV_OPT_SOR_EXTRAPOLATE (mixed form)
-    int *xorder = nullptr;                // per-instance extrapolation order of the current step
+    int *xorder = nullptr;                // per instance: predictor and ring position of the current step
     int *hxorder = nullptr;               // pinned staging of xorder
     int sort_every = 8;
     int group_kib = 0;  // SOR working-set budget (KiB) of one instance group (0: all instances together)
@@ -174,8 +174,7 @@ void batch_fill_args(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
         A.G = c->G;
         A.sp = c->sp;
         A.phi = c->phi;
-        A.h0 = c->phi_h[0];
-        A.h1 = c->phi_h[1];
+        for (int k = 0; k < 5; k++) A.hist.h[k] = c->phi_h[k];
         A.rho_s = (T *)c->rho_s;
         A.R = c->sor_R;
         A.delta = c->sor_delta;
@@ -816,8 +815,8 @@ int batch_step_finish(hv_batch *b, const hv_params *prm, bool res, int *iters_ou
                 if (md2[i] > prm[i].sor_tol) *warn |= HV_WARN_SOR_NOT_CONVERGED;
                 // as sor_solve: consecutive converged loop solves
                 hv_ctx *c = b->inst[i];
-                c->last_extrap = b->extrap ? b->hxorder[i] : 0;
-                c->hist_n = md2[i] > prm[i].sor_tol ? 0 : (c->hist_n < 3 ? c->hist_n + 1 : 3);
+                c->last_extrap = b->extrap ? b->hxorder[2 * i] : 0;
+                c->hist_n = md2[i] > prm[i].sor_tol ? 0 : (c->hist_n < 6 ? c->hist_n + 1 : 6);
             }
         } else {
             const std::vector<int> cap(n, prm[0].sor_max_it);
@@ -967,12 +966,14 @@ int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int 
             if (rc) return rc;
         }
         if (b->extrap) {
-            // extrapolation order of each instance's solve from its history
+            // predictor and ring position of each instance's solve (as sor_solve)
             for (int i = 0; i < n; i++) {
-                const int h = b->inst[i]->hist_n - 1;
-                b->hxorder[i] = h < 0 ? 0 : (h < b->extrap ? h : b->extrap);
+                hv_ctx *c = b->inst[i];
+                b->hxorder[2 * i] = sor_predictor_for(b->extrap, c->hist_n);
+                b->hxorder[2 * i + 1] = c->hist_pos;
+                c->hist_pos = (c->hist_pos + 4) % 5;
             }
-            HV_BHIP(b, hipMemcpyAsync(b->xorder, b->hxorder, sizeof(int) * (size_t)n, hipMemcpyHostToDevice,
+            HV_BHIP(b, hipMemcpyAsync(b->xorder, b->hxorder, sizeof(int) * 2 * (size_t)n, hipMemcpyHostToDevice,
                                       b->stream));
         }
         if (s > 0 && graphs && !b->gexec && batch_capture_step<T>(b, res) != HV_OK) graphs = false;
@@ -1038,8 +1039,8 @@ int hv_batch_create(int n, const hv_world *worlds, const int *n_protons, const i
     b->boost.assign(n, 0);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void **)&b->dstep, sizeof(int)) != hipSuccess ||
-        hipMalloc((void **)&b->xorder, sizeof(int) * (size_t)n) != hipSuccess ||
-        hipHostMalloc((void **)&b->hxorder, sizeof(int) * (size_t)n, hipHostMallocDefault) != hipSuccess) {
+        hipMalloc((void **)&b->xorder, sizeof(int) * 2 * (size_t)n) != hipSuccess ||
+        hipHostMalloc((void **)&b->hxorder, sizeof(int) * 2 * (size_t)n, hipHostMallocDefault) != hipSuccess) {
         hv_batch_destroy(b);
         return HV_ERR_HIP;
     }

@@ -40,7 +40,7 @@ struct BInst {
     long long G;
     Split sp;
     double *phi;
-    double *h0, *h1;      // mixed form: potentials of the two solves before the last
+    SorHist hist;         // mixed form: ring of the potentials of the five solves before the last
     T *rho_s;
     float *R, *delta;
     SorTiles tl;
@@ -147,14 +147,13 @@ kb_sor_defect(const BInst<T> *__restrict__ a, const int *__restrict__ first, int
     sor_defect_body<T>(bid, A.phi, A.rho_s, A.R, A.delta, A.sp, A.cf);
 }
 
-// extrapolated start of every instance's mixed solve (sor_extrapolate_body),
-// order[i] per instance
+// extrapolated start of every instance's mixed solve (sor_extrapolate_body):
+// xs[2 i] the predictor of instance i, xs[2 i + 1] its ring position
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-kb_sor_extrapolate(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n,
-                   const int *__restrict__ order) {
+kb_sor_extrapolate(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, const int *__restrict__ xs) {
     HV_BATCH_LOCATE();
-    sor_extrapolate_body(bid, A.phi, A.h0, A.h1, order[inst], 2 * A.sp.Gh);
+    sor_extrapolate_body(bid, A.phi, A.hist, xs[2 * inst], xs[2 * inst + 1], 2 * A.sp.Gh);
 }
 
 template <typename T>

@@ -1240,34 +1240,61 @@ __global__ void k_sor_fold(double *__restrict__ phi, const float *__restrict__ d
     sor_fold_body(blockIdx.x, phi, delta, n);
 }
 
-// Start of a mixed solve in a time loop: phi holds the potential of the last
-// solve (phi_n), h0 and h1 the two before it (phi_{n-1}, phi_{n-2}).  The new
-// start is phi_n (order 0), 2 phi_n - phi_{n-1} = phi_n + (phi_n - phi_{n-1})
-// (order 1) or 3 phi_n - 3 phi_{n-1} + phi_{n-2} = phi_{n-2} + 3 (phi_n -
-// phi_{n-1}) (order 2), and the history moves one step: h1 <- h0, h0 <- phi_n.
-// Fixed (Dirichlet) nodes hold the same value in every entry, so each form
-// returns that value exactly.
-__device__ __forceinline__ void sor_extrapolate_body(const int bid, double *__restrict__ phi,
-                                                     double *__restrict__ h0, double *__restrict__ h1, int order,
-                                                     long long n) {
+// Start of a mixed solve in a time loop, extrapolated in time from the
+// potentials of the last solves: P_0 = phi (the last solve, phi_n) and
+// P_k = phi_(n-k), k = 1..4, held in a ring of five history buffers (P_k in
+// hist[(pos + k - 1) % 5]).  Predictor `id` (HV_OPT_SOR_EXTRAPOLATE) gives
+// weights w_k (sum 1) over its first npts entries, the value at t_(n+1) of a
+// polynomial fitted to them:
+//   1  linear through 2 points          2 phi_n - phi_(n-1)
+//   2  quadratic through 3 points       3 phi_n - 3 phi_(n-1) + phi_(n-2)
+//   3  least-squares quadratic, 5 points (1.8, 0, -0.8, -0.6, 0.6)
+//   4  least-squares quadratic, 4 points (2.25, -0.75, -1.25, 0.75)
+//   5  least-squares line, 3 points      (4, 1, -2) / 3
+// Each solve leaves an error of the stop rule's size that differs from step to
+// step; a predictor multiplies that part by sqrt(sum w_k^2) (2.2, 4.4, 2.1,
+// 2.8, 1.5), so the least-squares forms keep the next start close where the
+// interpolating quadratic amplifies the previous solves' errors.
+// The guess is phi_n + sum_k w_k (P_k - phi_n), exactly phi_n where the
+// entries agree (the fixed Dirichlet faces).  phi_n then enters the ring at
+// hist[(pos + 4) % 5] (the oldest entry, which no predictor reads).
+__host__ __device__ __forceinline__ int sor_predictor(int id, double w[5]) {
+    for (int k = 0; k < 5; k++) w[k] = 0.0;
+    switch (id) {
+    case 1: w[0] = 2.0; w[1] = -1.0; return 2;
+    case 2: w[0] = 3.0; w[1] = -3.0; w[2] = 1.0; return 3;
+    case 3: w[0] = 1.8; w[1] = 0.0; w[2] = -0.8; w[3] = -0.6; w[4] = 0.6; return 5;
+    case 4: w[0] = 2.25; w[1] = -0.75; w[2] = -1.25; w[3] = 0.75; return 4;
+    case 5: w[0] = 4.0 / 3.0; w[1] = 1.0 / 3.0; w[2] = -2.0 / 3.0; return 3;
+    default: w[0] = 1.0; return 1;
+    }
+}
+
+struct SorHist {
+    double *h[5];
+};
+
+__device__ __forceinline__ void sor_extrapolate_body(const int bid, double *__restrict__ phi, SorHist hist, int id,
+                                                     int pos, long long n) {
     const long long q = (long long)bid * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    const double p0 = phi[q], p1 = h0[q];
+    double w[5];
+    const int npts = sor_predictor(id, w);
+    const double p0 = phi[q];
     double g = p0;
-    if (order == 1) {
-        g = p0 + (p0 - p1);
-    } else if (order >= 2) {
-        const double p2 = h1[q];
-        g = p2 + 3.0 * (p0 - p1);
+    for (int k = 1; k < npts; k++) {
+        int slot = pos + k - 1;
+        slot -= slot >= 5 ? 5 : 0;
+        g += w[k] * (hist.h[slot][q] - p0);
     }
-    h1[q] = p1;
-    h0[q] = p0;
+    int newest = pos + 4;
+    newest -= newest >= 5 ? 5 : 0;
+    hist.h[newest][q] = p0;
     phi[q] = g;
 }
 
-__global__ void k_sor_extrapolate(double *__restrict__ phi, double *__restrict__ h0, double *__restrict__ h1,
-                                  int order, long long n) {
-    sor_extrapolate_body(blockIdx.x, phi, h0, h1, order, n);
+__global__ void k_sor_extrapolate(double *__restrict__ phi, SorHist hist, int id, int pos, long long n) {
+    sor_extrapolate_body(blockIdx.x, phi, hist, id, pos, n);
 }
 
 // ---- one launch per SOR iteration: red and black fused ------------------------

@@ -56,6 +56,7 @@ HV_OPT_BATCH_GRAPH = 8
 HV_OPT_SOR_F32_TILES_PER_CU = 9
 HV_OPT_SOR_F32_OWNED = 10
 HV_OPT_SOR_EXTRAPOLATE = 11
+HV_OPT_SOR_F32_TOL_SHIFT = 12
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1

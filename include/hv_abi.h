@@ -273,15 +273,22 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
 /* HV_OPT_SOR_EXTRAPOLATE (mixed form only; 0 for HV_PREC_F64, which starts
  * every solve from the last potential as the reference does): the start of
  * each Coulomb solve of the fused loop (hv_run_init / hv_run_steps) is
- * extrapolated in time from the potentials of the last converged loop solves,
- * 0 none (the last potential), 1 linear (2 phi_n - phi_(n-1)), 2 quadratic
- * (3 phi_n - 3 phi_(n-1) + phi_(n-2); default).  The FP32 correction and the
- * FP64 finishing sweeps then run from that start under the same stop rule
- * (max |dphi| of an FP64 iteration below tol); the solution differs from the
- * FP64 form's by the stop rule's own error, not by precision.  Only loop solves
- * that converged count as history; run_init, uploads of particles or of the
- * potential and hv_coulomb_sor clear it. */
+ * extrapolated in time from the potentials of the last converged loop solves
+ * by a polynomial predictor: 0 none (the last potential), 1 line through 2
+ * points, 2 quadratic through 3, 3 least-squares quadratic through 5
+ * (default), 4 least-squares quadratic through 4, 5 least-squares line
+ * through 3.  The FP32 correction and the FP64 finishing sweeps then run from
+ * that start under the same stop rule (max |dphi| of an FP64 iteration below
+ * tol); the solution differs from the FP64 form's by the stop rule's own
+ * error, not by precision.  Until the ring holds a predictor's points a line
+ * (or no extrapolation) is used.  Only loop solves that converged count as
+ * history; run_init, uploads of particles or of the potential and
+ * hv_coulomb_sor clear it. */
 #define HV_OPT_SOR_EXTRAPOLATE 11
+/* HV_OPT_SOR_F32_TOL_SHIFT (mixed form): the FP32 correction stops once its
+ * max |update| is below tol / 2^shift (0..20, default 0); the FP64 finishing
+ * sweeps apply the reference's rule with tol itself either way. */
+#define HV_OPT_SOR_F32_TOL_SHIFT 12
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
