@@ -134,23 +134,26 @@ def window_start(total: int, steps: int, warmup: int) -> int:
 
 
 def run_gpu(w, prec: str, steps: int, warmup: int, full: bool = True):
-    """Pass 1 (kernel timers off): every step of the workload's run, timed
-    piece by piece; the headline window is `warmup` untimed steps and then
-    `steps` timed steps starting at window_start (the middle of the run).
-    Pass 2: the same run again (bit for bit the same work; checked by the SOR
-    iteration counts) with the per-kernel-class timers on over all its steps:
-    kernel_ms_per_step and the roofline kernel's launches over the whole run,
-    the launches a kernel trace of the run averages over.  With full=False
-    only the window is run (from step 0) and profiled."""
+    """Pass 1 (kernel timers off except in the tuner's trials): every step of
+    the workload's run, timed piece by piece; the particle-sort cadence is
+    autotuned on the live run at its start and again just before the headline
+    window (tune_single: the trials are steps of the run); the window is
+    `warmup` untimed steps and then `steps` timed steps starting at
+    window_start (the middle of the run).  Pass 2: the same run again with
+    the window's cadence (sorting changes no bit: the same work, checked by
+    the SOR iteration counts) and the per-kernel-class timers on over all its
+    steps: kernel_ms_per_step and the roofline kernel's launches over the
+    whole run, the launches a kernel trace of the run averages over.  With
+    full=False only the tuning and the window are run (from step 0)."""
     import hvamd
+    from hvamd.autotune import tune_single, tuning_steps
     p = hvamd.HV_PREC_F64 if prec == "fp64" else hvamd.HV_PREC_MIXED
     x, y, z, kx, ky, kz = w.initial_state()
-    total = max(w.steps, warmup + steps) if full else warmup + steps
-    first = window_start(total, steps, warmup) if full else 0
+    need = tuning_steps()
+    total = max(w.steps, 2 * need + warmup + steps) if full else need + warmup + steps
+    first = max(2 * need, window_start(total, steps, warmup)) if full else need
     prm = w.params(total)
     sk = w.sky()
-    # pieces: [0, first), [first, first + warmup), the timed window, the rest
-    cuts = [0, first, first + warmup, first + warmup + steps, total]
 
     def new_sim():
         sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, p)
@@ -159,28 +162,55 @@ def run_gpu(w, prec: str, steps: int, warmup: int, full: bool = True):
         sim.sync()
         return sim
 
+    def log(msg):
+        print(f"[bench] {w.name} {prec}: {msg}", file=sys.stderr, flush=True)
+
     sim = new_sim()
-    walls, its_all, state = [], [], None
-    for a, b in zip(cuts[:-1], cuts[1:]):
-        wall = 0.0
+    its_all, state, tunes = [], None, []
+    walls = {"window": 0.0, "rest": 0.0}
+
+    def plain(a, b, key):
         for c0 in range(a, b, 250):  # pieces of <= 250 steps, one progress line each
             c1 = min(b, c0 + 250)
             t0 = time.perf_counter()
             _, it = sim.run_steps(sk, prm, c0, c1 - c0)
             sim.sync()
-            wall += time.perf_counter() - t0
+            dt = time.perf_counter() - t0
+            walls[key] += dt
             its_all.append(it)
-            print(f"[bench] {w.name} {prec}: steps {c0}-{c1} {1e3 * (time.perf_counter() - t0) / (c1 - c0):.3f} "
-                  f"ms/step", file=sys.stderr, flush=True)
-        walls.append(wall)
-        if b == cuts[2] and prec == "fp64":
-            # the state at the start of the timed window, for the CPU baseline
-            state = sim.download_particles()
-            state["phi"] = sim.download_field(hvamd.HV_FIELD_COULOMB)
+            log(f"steps {c0}-{c1} {1e3 * dt / (c1 - c0):.3f} ms/step")
+
+    def tune(at):
+        from hvamd.autotune import _Steps
+        r = _Steps(sim, sk, prm)
+        t0 = time.perf_counter()
+        best, timings, nxt = tune_single(sim, sk, prm, at, _runner=r)
+        sim.sync()
+        walls["rest"] += time.perf_counter() - t0
+        its_all.extend(r.its)
+        tunes.append({"at_step": at, "best": best,
+                      "particle_pass_ms_per_step": {str(k): v for k, v in timings["particle_pass"].items()}})
+        log(f"sort cadence tuned at step {at}: {best} {timings['particle_pass']}")
+        return nxt
+
+    step = tune(0)
+    if full:
+        plain(step, first - need, "rest")
+        step = tune(first - need)
+        assert step == first
+    plain(first, first + warmup, "rest")
+    if prec == "fp64":
+        # the state at the start of the timed window, for the CPU baseline
+        state = sim.download_particles()
+        state["phi"] = sim.download_field(hvamd.HV_FIELD_COULOMB)
+    plain(first + warmup, first + warmup + steps, "window")
+    plain(first + warmup + steps, total, "rest")
+    sort_every = sim.get_option(hvamd.HV_OPT_SORT_EVERY)
     sim.close()
     its_all = np.concatenate(its_all)
-    win = slice(cuts[2], cuts[3])
+    win = slice(first + warmup, first + warmup + steps)
     sim = new_sim()
+    sim.set_option(hvamd.HV_OPT_SORT_EVERY, sort_every)
     sim.profile_enable(True)
     _, its2 = sim.run_steps(sk, prm, 0, total)
     sim.sync()
@@ -191,19 +221,30 @@ def run_gpu(w, prec: str, steps: int, warmup: int, full: bool = True):
     body = sim.get_option(hvamd.HV_OPT_SOR_F32_OWNED) if prec == "mixed" else None
     sim.close()
     assert np.array_equal(its_all, its2), "the profiled run must do the timed run's work"
-    return dict(wall=walls[2], its=its_all[win], full_wall=sum(walls), full_its=its_all, total=total,
-                first=cuts[2], prof=prof, issued=issued, sor=sor, state=state, body=body)
+    return dict(wall=walls["window"], its=its_all[win], full_wall=walls["window"] + walls["rest"], full_its=its_all,
+                total=total, first=first + warmup, prof=prof, issued=issued, sor=sor, state=state, body=body,
+                tunes=tunes, sort_every=sort_every)
 
 
 def kernel_ms_per_step(res) -> dict:
     """Device ms per step of each kernel class over the profiled run.  The
-    per-sweep SOR classes time one launch in eight; their total is the
-    sampled time per launch times the launches issued."""
-    out = {}
+    per-sweep SOR classes are timed one launch in eight with an event pair
+    around the dispatch, which inflates a few-microsecond launch; for them
+    the sampled time per launch and the launches per step are reported, and
+    the SOR stage's time per step is the timed run's wall time per step minus
+    the other classes (the sweeps with their launch gaps and read-backs)."""
+    out, other = {}, 0.0
+    total = res["total"]
     for k, (ms, n) in res["prof"].items():
-        if k in ("sor_sweep", "sor_sweep_f32") and n > 0:
-            ms = ms / n * res["issued"][k]
-        out[k] = ms / res["total"]
+        if k in ("sor_sweep", "sor_sweep_f32"):
+            if n > 0:
+                out[k + "_sampled_us_per_launch"] = 1e3 * ms / n
+                out[k + "_launches_per_step"] = res["issued"][k] / total
+            continue
+        out[k] = ms / total
+        other += ms / total
+    if res["prof"]["sor_resident_f32"][1] == 0:
+        out["sor_stage_wall_remainder"] = 1e3 * res["full_wall"] / total - other
     return out
 
 
@@ -418,7 +459,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c4"])
     ap.add_argument("--full", type=int, default=1,
-                    help="run every step of the workload (the timed window sits mid-run); 0: the window only")
+                    help="run every step of the workload (the timed window sits mid-run); 0: tuning and the window only")
     ap.add_argument("--ensemble", type=int, default=256, help="instances of the c3 ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=8)
     ap.add_argument("--fp64", type=int, default=1, help="also time the plain FP64 form")
@@ -472,6 +513,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_frac_of_peak": traffic_frac,
                      **rl},
     }
+    out["sort_cadence_autotune"] = {"mixed": mixed["tunes"], "fp64": fp64["tunes"] if fp64 else None}
     out["full_run"] = {"mixed": {"value": w.total * mixed["total"] / mixed["full_wall"],
                                  "ms_per_step": 1e3 * mixed["full_wall"] / mixed["total"], "steps": mixed["total"],
                                  "sor_iters_per_step": float(mixed["full_its"].mean())}}

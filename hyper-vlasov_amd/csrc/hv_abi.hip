@@ -1282,9 +1282,10 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
             c->excited = 1;
             boost = 1;
         }
+        // a re-sort counts in the particle pass's time (its only purpose)
+        hipEvent_t pa = prof_begin(c);
         rc = maybe_sort<T>(c);
         if (rc) return rc;
-        hipEvent_t pa = prof_begin(c);
         k_particle_step<T, T><<<nbp, kThreads, 0, c->stream>>>(
             particles<T>(c), c->np, c->total, g, (const T *)c->U, c->acc, kick_fac, drift_fac, n > 0 ? 1 : 0, boost, (T)bp,
             (T)bn, c->partials, c->status);

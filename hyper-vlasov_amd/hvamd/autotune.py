@@ -1,5 +1,7 @@
-"""Autotuner of an ensemble's smearing tiles and form and its particle-sort
-cadence.
+"""Autotuner of the particle-sort cadence and the smearing tiles and form, for
+an ensemble (tune_batch) and for a single run (tune_single, and
+run_retuned, which re-tunes the sort cadence along a run as the particles'
+order decays).
 
 No knob changes a bit of the results (sorting only reorders particles in
 storage; the deposits are exact integer sums; the smearing tile width and form
@@ -69,3 +71,95 @@ def tune_batch(batch, skyrmes, params, first_step: int, steps_per_trial: int = 2
     batch.set_option(HV_OPT_SORT_EVERY, sort_every)
     timings = {(s, t, f): part[s] + smear[(t, f)] for s in sort_choices for (t, f) in smear}
     return {"sort_every": sort_every, "smear_tile": tile, "smear_fused": fused}, timings, step
+
+
+SINGLE_SORT_CHOICES = (0, 2, 4, 8, 16, 32)
+
+
+class _Steps:
+    """Runs steps of one context, keeping each step's stats and SOR
+    iterations (the tuning trials are steps of the run itself)."""
+
+    def __init__(self, sim, skyrme, params):
+        self.sim, self.skyrme, self.params = sim, skyrme, params
+        self.stats, self.its = [], []
+
+    def run(self, step, n, stages=None):
+        if stages:
+            self.sim.profile_enable(True)
+        s, it = self.sim.run_steps(self.skyrme, self.params, step, n)
+        self.stats.append(s)
+        self.its.append(it)
+        if not stages:
+            return None
+        prof = self.sim.profile_read()
+        self.sim.profile_enable(False)
+        return sum(prof[k][0] for k in stages) / n
+
+
+def tune_single(sim, skyrme, params, first_step: int, steps_per_trial: int = 16,
+                sort_choices=SINGLE_SORT_CHOICES, tile_choices=(), form_choices=(), _runner=None):
+    """The same search for one context (hvamd.Simulation), on the live run:
+    each candidate sort cadence advances the run by max(cadence,
+    steps_per_trial) steps after one untimed step and is compared on the
+    particle pass's device time per step (the pass with its re-sorts, which
+    is all a cadence changes); tile_choices / form_choices, when given, are
+    compared on the smearing's time.  Returns (best {sort_every[, smear_tile,
+    smear_fused]}, timings {knob: {value: ms/step}}, next first_step)."""
+    r = _runner or _Steps(sim, skyrme, params)
+    step = first_step
+    best, timings = {}, {}
+    if tile_choices or form_choices:
+        smear = {}
+        for tile, fused in itertools.product(tile_choices or (64,), form_choices or (0,)):
+            sim.set_option(HV_OPT_SMEAR_TILE, tile)
+            sim.set_option(HV_OPT_SMEAR_FUSED, fused)
+            r.run(step, 1)
+            step += 1
+            smear[(tile, fused)] = r.run(step, steps_per_trial, ("smear",))
+            step += steps_per_trial
+        tile, fused = min(smear, key=smear.get)
+        sim.set_option(HV_OPT_SMEAR_TILE, tile)
+        sim.set_option(HV_OPT_SMEAR_FUSED, fused)
+        best.update(smear_tile=tile, smear_fused=fused)
+        timings["smear"] = {f"tile{t}_fused{f}": v for (t, f), v in smear.items()}
+    part = {}
+    for sort_every in sort_choices:
+        sim.set_option(HV_OPT_SORT_EVERY, sort_every)
+        r.run(step, 1)
+        step += 1
+        n = max(sort_every, steps_per_trial)
+        part[sort_every] = r.run(step, n, ("particle_pass",))
+        step += n
+    sort_every = min(part, key=part.get)
+    sim.set_option(HV_OPT_SORT_EVERY, sort_every)
+    best["sort_every"] = sort_every
+    timings["particle_pass"] = part
+    return best, timings, step
+
+
+def tuning_steps(steps_per_trial: int = 16, sort_choices=SINGLE_SORT_CHOICES) -> int:
+    """Steps one tune_single round of the sort cadence runs."""
+    return sum(1 + max(c, steps_per_trial) for c in sort_choices)
+
+
+def run_retuned(sim, skyrme, params, first_step: int, nsteps: int, retune_every: int = 500, **tune_args):
+    """nsteps steps of the loop from first_step, with the sort cadence tuned
+    (tune_single) at the start and again after every retune_every steps; the
+    tuning trials are steps of the run.  Returns (stats[nsteps, 4],
+    sor_iters[nsteps], [(step, best, timings)] of each tuning)."""
+    import numpy as np
+    r = _Steps(sim, skyrme, params)
+    step, end = first_step, first_step + nsteps
+    need = tuning_steps(tune_args.get("steps_per_trial", 16), tune_args.get("sort_choices", SINGLE_SORT_CHOICES))
+    log = []
+    while step < end:
+        if end - step > need:
+            t0 = step
+            best, timings, step = tune_single(sim, skyrme, params, step, _runner=r, **tune_args)
+            log.append((t0, best, timings))
+        n = min(retune_every, end - step)
+        if n > 0:
+            r.run(step, n)
+            step += n
+    return np.concatenate(r.stats), np.concatenate(r.its), log

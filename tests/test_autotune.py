@@ -63,3 +63,41 @@ def test_tune_batch_picks_fastest_stages_and_leaves_them_set():
     assert firsts == sorted(firsts) and firsts[0] == 5
     # the sort-8 trial's cost includes one sort over its 8 steps
     assert abs(timings[(8, 32, 0)] - (1.0 + 5.0 / 8 + 1.0)) < 1e-12
+
+
+class StandInSim(StandInBatch):
+    """One context: run_steps returns per-step stats and SOR iterations (the
+    step number, so the test can check every step was run once, in order)."""
+
+    def run_steps(self, skyrme, params, first, n):
+        import numpy as np
+        super().run_steps([], [], first, n)
+        steps = np.arange(first, first + n)
+        return np.repeat(steps[:, None], 4, axis=1).astype(float), steps.astype(np.int32)
+
+
+def test_tune_single_picks_the_cheapest_cadence_on_the_particle_pass():
+    from hvamd.autotune import tune_single
+    s = StandInSim()
+    s.set_option(HV_OPT_SMEAR_TILE, 64)
+    s.set_option(HV_OPT_SMEAR_FUSED, 0)
+    best, timings, nxt = tune_single(s, None, None, 10, steps_per_trial=4, sort_choices=(0, 2, 8, 16))
+    # sorting every 16 steps: 1 + 5/16 per step, the cheapest of 3, 1 + 5/4, 1 + 5/8, 1 + 5/16
+    assert best == {"sort_every": 16}
+    assert s.opt[HV_OPT_SORT_EVERY] == 16
+    assert abs(timings["particle_pass"][8] - (1.0 + 5.0 / 8)) < 1e-12
+    assert nxt == 10 + (1 + 4) + (1 + 4) + (1 + 8) + (1 + 16)
+
+
+def test_run_retuned_runs_every_step_once_in_order_and_retunes():
+    from hvamd.autotune import run_retuned, tuning_steps
+    s = StandInSim()
+    s.set_option(HV_OPT_SMEAR_TILE, 64)
+    s.set_option(HV_OPT_SMEAR_FUSED, 0)
+    kw = dict(steps_per_trial=4, sort_choices=(0, 8))
+    need = tuning_steps(4, (0, 8))
+    stats, its, log = run_retuned(s, None, None, 3, 200, retune_every=50, **kw)
+    assert list(its) == list(range(3, 203))
+    assert stats.shape == (200, 4)
+    assert [t for t, _, _ in log][:2] == [3, 3 + need + 50]
+    assert all(b["sort_every"] == 8 for _, b, _ in log)
