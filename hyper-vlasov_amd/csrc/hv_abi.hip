@@ -970,10 +970,19 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
     if (c->precision == HV_PREC_MIXED && c->phi_warm) {
         if (loop && c->sor_extrap > 0) {
             const int id = sor_predictor_for(c->sor_extrap, c->hist_n);
-            SorHist hs;
-            for (int k = 0; k < 5; k++) hs.h[k] = c->phi_h[k];
-            k_sor_extrapolate<<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, hs, id, c->hist_pos,
-                                                                                  2 * c->sp.Gh);
+            SorHist ring;
+            for (int k = 0; k < 5; k++) ring.h[k] = c->phi_h[k];
+            const SorHist hs = sor_hist_order(ring, c->hist_pos);
+            double w[5];
+            const unsigned nb = blocks_for(2 * c->sp.Gh);
+            const long long nn = 2 * c->sp.Gh;
+            switch (sor_predictor(id, w)) {
+            case 1: k_sor_extrapolate<1><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
+            case 2: k_sor_extrapolate<2><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
+            case 3: k_sor_extrapolate<3><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
+            case 4: k_sor_extrapolate<4><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
+            default: k_sor_extrapolate<5><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
+            }
             HV_CHECK_LAUNCH(c);
             c->hist_pos = (c->hist_pos + 4) % 5;
             c->last_extrap = id;

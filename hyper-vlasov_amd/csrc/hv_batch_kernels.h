@@ -153,7 +153,16 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_sor_extrapolate(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, const int *__restrict__ xs) {
     HV_BATCH_LOCATE();
-    sor_extrapolate_body(bid, A.phi, A.hist, xs[2 * inst], xs[2 * inst + 1], 2 * A.sp.Gh);
+    const int id = __builtin_amdgcn_readfirstlane(xs[2 * inst]);
+    const SorHist hs = sor_hist_order(A.hist, __builtin_amdgcn_readfirstlane(xs[2 * inst + 1]));
+    double w[5];
+    switch (sor_predictor(id, w)) {  // wave-uniform: one instance per block
+    case 1: sor_extrapolate_body<1>(bid, A.phi, hs, id, 2 * A.sp.Gh); break;
+    case 2: sor_extrapolate_body<2>(bid, A.phi, hs, id, 2 * A.sp.Gh); break;
+    case 3: sor_extrapolate_body<3>(bid, A.phi, hs, id, 2 * A.sp.Gh); break;
+    case 4: sor_extrapolate_body<4>(bid, A.phi, hs, id, 2 * A.sp.Gh); break;
+    default: sor_extrapolate_body<5>(bid, A.phi, hs, id, 2 * A.sp.Gh); break;
+    }
 }
 
 template <typename T>

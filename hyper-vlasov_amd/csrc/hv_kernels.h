@@ -1274,27 +1274,43 @@ struct SorHist {
     double *h[5];
 };
 
-__device__ __forceinline__ void sor_extrapolate_body(const int bid, double *__restrict__ phi, SorHist hist, int id,
-                                                     int pos, long long n) {
+// hist.h[k] holds P_(k+1) for the predictor's reads (k < npts - 1); hist.h[4]
+// is the slot phi_n moves into (sor_hist_order).  NPTS: the predictor's point
+// count.
+template <int NPTS>
+__device__ __forceinline__ void sor_extrapolate_body(const int bid, double *__restrict__ phi, const SorHist &hist,
+                                                     int id, long long n) {
     const long long q = (long long)bid * blockDim.x + threadIdx.x;
     if (q >= n) return;
     double w[5];
-    const int npts = sor_predictor(id, w);
+    (void)sor_predictor(id, w);
+    double pk[NPTS > 1 ? NPTS - 1 : 1];
+#pragma unroll
+    for (int k = 1; k < NPTS; k++) pk[k - 1] = hist.h[k - 1][q];
     const double p0 = phi[q];
     double g = p0;
-    for (int k = 1; k < npts; k++) {
-        int slot = pos + k - 1;
-        slot -= slot >= 5 ? 5 : 0;
-        g += w[k] * (hist.h[slot][q] - p0);
-    }
-    int newest = pos + 4;
-    newest -= newest >= 5 ? 5 : 0;
-    hist.h[newest][q] = p0;
+#pragma unroll
+    for (int k = 1; k < NPTS; k++) g += w[k] * (pk[k - 1] - p0);
+    hist.h[4][q] = p0;
     phi[q] = g;
 }
 
-__global__ void k_sor_extrapolate(double *__restrict__ phi, SorHist hist, int id, int pos, long long n) {
-    sor_extrapolate_body(blockIdx.x, phi, hist, id, pos, n);
+template <int NPTS>
+__global__ void __launch_bounds__(kBlock) k_sor_extrapolate(double *__restrict__ phi, SorHist hist, int id,
+                                                            long long n) {
+    sor_extrapolate_body<NPTS>(blockIdx.x, phi, hist, id, n);
+}
+
+// the ring's entries in predictor order for position pos: P_k (k = 1..4) at
+// ring[(pos + k - 1) % 5], the newest entry's slot at ring[(pos + 4) % 5]
+__host__ __device__ __forceinline__ SorHist sor_hist_order(const SorHist &ring, int pos) {
+    SorHist o;
+    for (int k = 0; k < 5; k++) {
+        int slot = pos + k;
+        slot -= slot >= 5 ? 5 : 0;
+        o.h[k] = ring.h[slot];
+    }
+    return o;
 }
 
 // ---- one launch per SOR iteration: red and black fused ------------------------
