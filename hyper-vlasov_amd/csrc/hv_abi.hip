@@ -2060,3 +2060,6 @@ const char *hv_build_info(void) {
 
 // ---- diagnostics without allocation (hv_stats_noalloc) ----------------------------
 #include "hv_noalloc_impl.h"
+
+// ---- the reference's Woods-Saxon / Fermi-level initial state (hv_init_woods_saxon) ----
+#include "hv_init_ws_impl.h"

@@ -8,16 +8,22 @@
 //   hv_vlasov --check <input file>       (parse, print the run as JSON, no GPU)
 //
 // --float selects the mixed FP32/FP64 form, otherwise FP64.  The initial
-// nucleus is the seeded uniform-sphere sample of include/hv_init.h (radius
-// r_init A^(1/3), momenta uniform in the Fermi ball of rho0): the reference's
-// Woods-Saxon / Fermi-level fit needs GSL, so V0, a, epsilon_p, epsilon_n and
-// k_fwhm are read (and required, as there) but not used.  Every `substeps`
-// steps it writes the stats line and, as the reference does, the snapshot
-// output_dir/out-%04d.vtk (hv_write_vtk: forces, density, potentials).
+// nucleus is the reference's (initialize_particles, include/physics.hpp:35-97,
+// through hv_init_woods_saxon): Woods-Saxon wells from V0 and a, Fermi levels
+// from epsilon_p / epsilon_n, momentum width from k_fwhm, srand(seed) with the
+// reference's seed 128, the wells fitted by Levenberg-Marquardt where the
+// reference uses GSL; it prints the reference's "MAX TEST PART" and "CHI
+// SQUARED" lines.  With the optional key `init 1` it is instead the seeded
+// uniform-sphere sample of include/hv_init.h (radius r_init A^(1/3), momenta
+// uniform in the Fermi ball of rho0), the benchmark configurations' nucleus.
+// Every `substeps` steps it writes the stats line and, as the reference does,
+// the snapshot output_dir/out-%04d.vtk (hv_write_vtk: forces, density,
+// potentials).
 //
 // Optional keys beyond the reference's 21: sor_omega, sor_tol, sor_max_it
 // (default: the reference's 1.5, 1e-4, 512), boost (0 = the reference's
-// weighted split, 1 = protons +eta, neutrons -eta), seed (128), r_init (1.2),
+// weighted split, 1 = protons +eta, neutrons -eta), init (0 = the reference's
+// Woods-Saxon nucleus, 1 = uniform sphere), seed (128), r_init (1.2),
 // sort_every, vtk (1; 0 writes no snapshots).
 #include <chrono>
 #include <cmath>
@@ -39,7 +45,7 @@ struct Run {
     double t_f = 0, t_exc = 0, eta_exc = 0, d_max_scale = 0;
     int nx = 0, num_test_part = 0, steps = 0, n = 0, z = 0, substeps = 0, use_gpu = 0;
     double sor_omega = 1.5, sor_tol = 1e-4;
-    int sor_max_it = 512, boost = HV_BOOST_REF_WEIGHTED, sort_every = 8, vtk = 1;
+    int sor_max_it = 512, boost = HV_BOOST_REF_WEIGHTED, sort_every = 8, vtk = 1, init = 0;
     unsigned long long seed = 128;
     double r_init = 1.2;
 };
@@ -63,7 +69,8 @@ bool read_input(const char *path, Run &r, std::string &why) {
     while (in >> key) {
         bool known = false;
         for (const char *k : kRequired) known = known || key == k;
-        const char *opt[] = {"sor_omega", "sor_tol", "sor_max_it", "boost", "seed", "r_init", "sort_every", "vtk"};
+        const char *opt[] = {"sor_omega", "sor_tol", "sor_max_it", "boost", "seed", "r_init", "sort_every", "vtk",
+                             "init"};
         for (const char *k : opt) known = known || key == k;
         if (!known) continue;
         std::string v;
@@ -98,14 +105,15 @@ bool read_input(const char *path, Run &r, std::string &why) {
               i("steps", r.steps) && i("n", r.n) && i("z", r.z) && i("substeps", r.substeps) &&
               i("use_gpu", r.use_gpu) && d("sor_omega", r.sor_omega) && d("sor_tol", r.sor_tol) &&
               i("sor_max_it", r.sor_max_it) && i("boost", r.boost) && d("seed", seed) && d("r_init", r.r_init) &&
-              i("sort_every", r.sort_every) && i("vtk", r.vtk);
+              i("sort_every", r.sort_every) && i("vtk", r.vtk) && i("init", r.init);
     r.seed = (unsigned long long)seed;
     if (!ok) {
         why = "a value does not parse";
         return false;
     }
     if (r.steps <= 0 || r.substeps <= 0 || r.nx < 4 || r.num_test_part <= 0 || r.z < 0 || r.n < 0 ||
-        r.z + r.n <= 0 || !(r.t_f > 0) || !(r.d_max_scale > 0) || !(r.r_fwhm > 0)) {
+        r.z + r.n <= 0 || !(r.t_f > 0) || !(r.d_max_scale > 0) || !(r.r_fwhm > 0) || (r.init != 0 && r.init != 1) ||
+        (r.init == 0 && (!(r.a > 0) || !(r.k_fwhm >= 0) || r.z < 1 || r.n < 1))) {
         why = "a value is out of range";
         return false;
     }
@@ -174,10 +182,10 @@ int main(int argc, char **argv) {
         std::printf("{\"n\": %d, \"d_max\": %.17g, \"n_protons\": %d, \"n_neutrons\": %d, \"sigma_r\": %.17g, "
                     "\"dt\": %.17g, \"steps\": %d, \"substeps\": %d, \"t_exc\": %.17g, \"eta_exc\": %.17g, "
                     "\"skyrme\": [%.17g, %.17g, %.17g, %.17g], \"sor\": [%.17g, %.17g, %d], \"boost\": %d, "
-                    "\"seed\": %llu, \"r_init\": %.17g, \"k_fermi\": %.17g, \"vtk\": %d}\n",
+                    "\"seed\": %llu, \"r_init\": %.17g, \"k_fermi\": %.17g, \"vtk\": %d, \"init\": %d}\n",
                     w.n[0], w.d_max[0], np, nn, p.sigma_r, p.t_f / p.steps, p.steps, p.substeps, p.t_exc, p.eta_exc,
                     s.A, s.B, s.C, s.gamma, p.sor_omega, p.sor_tol, p.sor_max_it, p.boost_mode, r.seed, rmax, kmax,
-                    r.vtk);
+                    r.vtk, r.init);
         return 0;
     }
     const char *outdir = mixed ? argv[3] : argv[2];
@@ -185,8 +193,26 @@ int main(int argc, char **argv) {
     const auto t0 = std::chrono::steady_clock::now();
     const int tot = np + nn;
     std::vector<double> x(tot), y(tot), z(tot), kx(tot), ky(tot), kz(tot);
-    if (hv_sample_uniform_spheres(r.seed, np, nn, rmax, kmax, x.data(), y.data(), z.data(), kx.data(), ky.data(),
-                                  kz.data()) != 0) {
+    if (r.init == 0) {
+        // the reference's initialize_particles (src/hyper_vlasov.cpp:139-141)
+        std::printf("MAX TEST PART %i\n", hv_ws_max_particles(nuclear_radius(r.z + r.n), 1.5, r.num_test_part));
+        double fermi[2] = {r.epsilon_p, r.epsilon_n}, chi2[2] = {0.0, 0.0};
+        hv_woods_saxon ws[2];
+        int rounds = 0;
+        const int irc = hv_init_woods_saxon(r.z, r.n, r.num_test_part, &s, p.sigma_r, sigma_from_fwhm(r.k_fwhm),
+                                            r.V0, r.a, fermi, (unsigned)r.seed, x.data(), y.data(), z.data(),
+                                            kx.data(), ky.data(), kz.data(), ws, &rounds, chi2);
+        if (irc < 0) {
+            std::fprintf(stderr, "initial state failed (%d)\n", irc);
+            return 1;
+        }
+        if (irc & HV_WARN_INIT_NOT_CONVERGED) std::fprintf(stderr, "INITIALIZATION DID NOT CONVERGE!\n");
+        std::printf("V0 %0.2lf R12 %0.2lf a %0.2lf\n", ws[0].V0, ws[0].R12, ws[0].a);
+        std::printf("V0 %0.2lf R12 %0.2lf a %0.2lf\n", ws[1].V0, ws[1].R12, ws[1].a);
+        std::printf("FERMI P %0.2lf FERMI N %0.2lf\nITERATIONS %i\n", fermi[0], fermi[1], rounds);
+        std::printf("CHI SQUARED P %0.2lf CHI SQUARED N %0.2lf\n", chi2[0], chi2[1]);
+    } else if (hv_sample_uniform_spheres(r.seed, np, nn, rmax, kmax, x.data(), y.data(), z.data(), kx.data(),
+                                         ky.data(), kz.data()) != 0) {
         std::fprintf(stderr, "initial state: bad sizes\n");
         return 1;
     }

@@ -26,6 +26,7 @@ HV_ERR_IO = -5
 HV_WARN_SOR_NOT_CONVERGED = 1
 HV_WARN_PARTICLE_OUTSIDE = 2
 HV_WARN_SMEAR_MARGIN = 4
+HV_WARN_INIT_NOT_CONVERGED = 8
 HV_NOALLOC_MAX_PARTICLES = 1 << 24
 
 HV_PREC_F64 = 0
@@ -94,6 +95,10 @@ class HvParams(ctypes.Structure):
         ("sor_tol", ctypes.c_double),
         ("sor_max_it", ctypes.c_int),
     ]
+
+
+class HvWoodsSaxon(ctypes.Structure):
+    _fields_ = [("V0", ctypes.c_double), ("R12", ctypes.c_double), ("a", ctypes.c_double)]
 
 
 class HvError(RuntimeError):
@@ -171,6 +176,14 @@ SIGNATURES = {
     "hv_fermi_momentum": (_D, [_D]),
     "hv_sample_uniform_spheres": (_I, [ctypes.c_ulonglong, _I, _I, _D, _D] + [_P] * 6),
     "hv_mix64": (ctypes.c_ulonglong, [ctypes.c_ulonglong]),
+    "hv_ws_nuclear_radius": (_D, [_I]),
+    "hv_ws_max_particles": (_I, [_D, _D, _I]),
+    "hv_ws_sample_round": (_I, [_I, _I, _I, _D, ctypes.POINTER(HvWoodsSaxon), _DP, _DP, _IP] + [_P] * 6),
+    "hv_ws_particle_densities": (_I, [_P, _P, _P, _I, _I, _D, _I, _P, _P]),
+    "hv_ws_fit": (_I, [_P] * 5 + [_I, _I, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvWoodsSaxon), _IP]),
+    "hv_ws_chi_squared": (_I, [_P] * 5 + [_I, _I, ctypes.POINTER(HvSkyrme), ctypes.POINTER(HvWoodsSaxon), _I, _DP]),
+    "hv_init_woods_saxon": (_I, [_I, _I, _I, ctypes.POINTER(HvSkyrme), _D, _D, _D, _D, _DP, ctypes.c_uint] + [_P] * 6
+                            + [ctypes.POINTER(HvWoodsSaxon), _IP, _DP]),
 }
 
 _lib = None

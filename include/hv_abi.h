@@ -50,6 +50,7 @@ extern "C" {
 #define HV_WARN_SOR_NOT_CONVERGED 1  /* SOR hit max_it with max|dphi| > tol */
 #define HV_WARN_PARTICLE_OUTSIDE 2   /* some particles were outside the grid or non-finite */
 #define HV_WARN_SMEAR_MARGIN 4       /* a smearing tap lies within 1e-9 of the cutoff sphere */
+#define HV_WARN_INIT_NOT_CONVERGED 8 /* hv_init_woods_saxon: 32 rounds did not settle the Fermi levels */
 
 /* ---- precision ----------------------------------------------------------- */
 #define HV_PREC_F64 0

@@ -16,6 +16,8 @@
 // Test infrastructure only; the product never loads this library.
 
 #define FIT_ALGORITHM_H
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -269,6 +271,104 @@ int ref_write_vtk(const char *path, const int n[3], const double d_max[3], const
     }
     std::fclose(out);
     return 0;
+}
+
+// ---- initialize_particles (include/physics.hpp:35-97), without its GSL fit ----
+// One sampling round with the reference's own templates: the body of
+// initialize_particles' loop up to the densities (include/physics.hpp:42-72,
+// replayed: the function itself instantiates minim_woods_saxon, which needs
+// GSL).  rand() is not reseeded here.  ws = {V0_p, R12_p, a_p, V0_n, R12_n, a_n}.
+void ref_ws_sample_round(int z, int n, int ppn, double sigma_k, const double ws_in[6], double fermi[2],
+                         double delta[2], int counts[6], double *x, double *y, double *zz, double *kx, double *ky,
+                         double *kz) {
+    Parameters<double> param{};
+    set_parameters(&param, z, n, ppn, 1, 1, false, sigma_k, 0.6, 1.0, 0.0, 0.0);
+    WoodsSaxon<double> ws[2];
+    set_woods_saxon(&ws[0], ws_in[0], ws_in[1], ws_in[2]);
+    set_woods_saxon(&ws[1], ws_in[3], ws_in[4], ws_in[5]);
+    Fermi<double> fermi_levels;
+    set_fermi_levels(&fermi_levels, fermi[0], fermi[1]);
+    const int max_part = param.max_test_part, total_p = z * ppn, total_n = n * ppn;
+    TestParticles<double> part, temp_part;
+    create_particles(&part, total_p, total_n);
+    create_particles(&temp_part, max_part, max_part);
+    generate_random_particles(&temp_part, param.r_max);
+    compute_particle_energies(&temp_part, ws, param);
+    int less_p = 0, equal_p = 0, more_p = 0, less_n = 0, equal_n = 0, more_n = 0;
+    for (int i = 0; i < max_part; i++) {
+        if (temp_part.energy[i] < fermi_levels.epsilon_p) equal_p += 2;
+        if (temp_part.energy[i] < fermi_levels.epsilon_p + 0.5) more_p += 2;
+        if (temp_part.energy[i] < fermi_levels.epsilon_p - 0.5) less_p += 2;
+        if (temp_part.energy[i + max_part] < fermi_levels.epsilon_n) equal_n += 2;
+        if (temp_part.energy[i + max_part] < fermi_levels.epsilon_n + 0.5) more_n += 2;
+        if (temp_part.energy[i + max_part] < fermi_levels.epsilon_n - 0.5) less_n += 2;
+    }
+    double delta_part_n = total_n - equal_n, delta_part_p = total_p - equal_p;
+    double delta_epsilon_n = 0.5 * delta_part_n / (more_n - less_n);
+    double delta_epsilon_p = 0.5 * delta_part_p / (more_p - less_p);
+    if (std::abs(delta_epsilon_p) > 0.5) delta_epsilon_p *= 0.6;
+    if (std::abs(delta_epsilon_n) > 0.5) delta_epsilon_n *= 0.6;
+    fermi_levels.epsilon_p += delta_epsilon_p;
+    fermi_levels.epsilon_n += delta_epsilon_n;
+    generate_checking_particles(&part, ws, param, &fermi_levels);
+    fermi[0] = fermi_levels.epsilon_p;
+    fermi[1] = fermi_levels.epsilon_n;
+    delta[0] = delta_epsilon_p;
+    delta[1] = delta_epsilon_n;
+    const int c[6] = {equal_p, more_p, less_p, equal_n, more_n, less_n};
+    for (int q = 0; q < 6; q++) counts[q] = c[q];
+    const int N = total_p + total_n;
+    for (int i = 0; i < N; i++) {
+        x[i] = part.x[i]; y[i] = part.y[i]; zz[i] = part.z[i];
+        kx[i] = part.kx[i]; ky[i] = part.ky[i]; kz[i] = part.kz[i];
+    }
+    free_particles(&part);
+    free_particles(&temp_part);
+}
+
+// max_particles (include/physics_formulas.hpp:36-42) at the reference's r_max and k_max
+int ref_ws_max_particles(int z, int n, int ppn) {
+    Parameters<double> param{};
+    set_parameters(&param, z, n, ppn, 1, 1, false, 0.1, 0.6, 1.0, 0.0, 0.0);
+    return param.max_test_part;
+}
+
+// compute_particle_densities (include/tools.hpp:171-203)
+void ref_ws_particle_densities(double *x, double *y, double *zz, int np, int nn, double sigma_r, int ppn,
+                               double *rho_p, double *rho_n) {
+    TestParticles<double> part = wrap(x, y, zz, x, y, zz, x, y, zz, np, nn);
+    part.density_p = rho_p;
+    part.density_n = rho_n;
+    Parameters<double> param{};
+    param.sigma_r = sigma_r;
+    param.part_per_nucleon = ppn;
+    compute_particle_densities(&part, param);
+}
+
+// chi_squared (include/tools.hpp:341-366): the two figures it prints, read
+// back from its own output ("%0.2lf")
+int ref_ws_chi_squared(double *x, double *y, double *zz, double *rho_p, double *rho_n, int np, int nn, double A,
+                       double B, double C, double gamma, const double ws_in[6], int ppn, double out[2]) {
+    TestParticles<double> part = wrap(x, y, zz, x, y, zz, x, y, zz, np, nn);
+    part.density_p = rho_p;
+    part.density_n = rho_n;
+    WoodsSaxon<double> ws[2];
+    set_woods_saxon(&ws[0], ws_in[0], ws_in[1], ws_in[2]);
+    set_woods_saxon(&ws[1], ws_in[3], ws_in[4], ws_in[5]);
+    Skyrme<double> skm{A, B, C, gamma};
+    std::fflush(stdout);
+    FILE *tmp = std::tmpfile();
+    if (!tmp) return 1;
+    const int saved = dup(1);
+    dup2(fileno(tmp), 1);
+    chi_squared(part, ws, skm, ppn);
+    std::fflush(stdout);
+    dup2(saved, 1);
+    close(saved);
+    std::rewind(tmp);
+    const int got = std::fscanf(tmp, "CHI SQUARED P %lf CHI SQUARED N %lf", &out[0], &out[1]);
+    std::fclose(tmp);
+    return got == 2 ? 0 : 1;
 }
 
 }  // extern "C"

@@ -85,7 +85,7 @@ def test_run_writes_reference_stats(driver, tmp_path):
     scripts/programs.py:92), equals the library's own step diagnostics."""
     import hvamd
     print("driver run", flush=True)
-    _write(tmp_path / "input.txt", INPUT)
+    _write(tmp_path / "input.txt", INPUT, "init 1\n")
     out = tmp_path / "out"
     out.mkdir()
     r = subprocess.run([str(DRIVER), str(tmp_path / "input.txt"), str(out)], capture_output=True, text=True,
@@ -118,3 +118,23 @@ def test_run_writes_reference_stats(driver, tmp_path):
                             rows[:, 1]])
     assert data.shape == want.shape
     np.testing.assert_allclose(data, want, rtol=5e-6, atol=1e-12)  # %e keeps 7 significant digits
+
+
+@pytest.mark.gpu
+def test_run_from_the_reference_initial_state(driver, tmp_path):
+    """Without `init 1` the driver starts from the reference's Woods-Saxon /
+    Fermi-level nucleus (hv_init_woods_saxon with the input's V0, a,
+    epsilon_p, epsilon_n, k_fwhm and seed 128), prints the reference's
+    initialisation lines, and runs the loop from it."""
+    print("driver run from the reference's initial state", flush=True)
+    _write(tmp_path / "input.txt", INPUT, "vtk 0\n")
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([str(DRIVER), str(tmp_path / "input.txt"), str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "MAX TEST PART" in r.stdout and "CHI SQUARED P" in r.stdout and "Simulation ended." in r.stdout
+    data = np.loadtxt(out / "stats.txt", dtype=np.float64)
+    assert data.shape[1] == 5 and np.isfinite(data).all()
+    # sqrt<r^2> of a bound nucleus of A = 40: a few fm
+    assert 2.0 < data[0, 1] < 5.0 and 2.0 < data[0, 2] < 5.0
