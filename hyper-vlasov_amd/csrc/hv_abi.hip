@@ -73,11 +73,14 @@ struct hv_ctx {
     int tab_warn = 0;
     SmearTiles stl{};                   // two-pass smearing tiles
     size_t stl_lds = 0;
-    unsigned char *sflags = nullptr;    // per-tile "has counts" flags of the two-pass smearing
+    unsigned *sflags = nullptr;         // plane masks of the tile columns of the two-pass smearing
     size_t sflags_cap = 0;
     SmearTiles stf{};                   // fused (plane-walking) smearing tiles
     size_t stf_lds = 0;
     int smear_fused = 0;                // option: single-grid smearing in the fused form
+    int smear_xs_vec = 1;               // two-pass x pass: VW nodes per thread (when nz allows)
+    int xs_rxm = 0;                     // its radius bucket (0: radius too large for it)
+    long long zy_blocks = 1024;         // workgroups of the zy pass (blocks per CU that its LDS allows x CUs)
     unsigned long long *maxd = nullptr; // per-iteration slot maxima of |dphi| (double bits)
     double *maxc = nullptr;             // per-iteration max |dphi|, collected for read-back
     int maxd_cap = 0;
@@ -154,6 +157,9 @@ struct hv_ctx {
 namespace {
 
 constexpr int kThreads = 256;
+
+// u64 words of the counts buffer: 2G fixed-point counts, then the count box
+long long acc_words(const hv_ctx *c) { return 2 * c->G + cbox_acc_words(c->world.n[0]); }
 
 int set_err(hv_ctx *c, int code, const char *fmt, ...) {
     if (c) {
@@ -425,11 +431,22 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
         return set_err(c, HV_ERR_HIP, "smearing: LDS size attribute refused");
     c->stl = stl;
     c->stl_lds = lds;
+    {
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 256;
+        const long long per_cu = std::max<long long>(1, std::min<long long>(8, (long long)(160 * 1024) / (long long)std::max<size_t>(lds, 1)));
+        c->zy_blocks = per_cu * ncu;
+    }
     c->stf = stf;
     c->stf_lds = ldf;
-    const size_t ntiles = (size_t)2 * c->world.n[0] * stl.njt * stl.nkt;
-    int rc = ensure_buf(c, (void **)&c->sflags, &c->sflags_cap, ntiles);
+    // the vector x pass: radius bucket RXM >= rx
+    c->xs_rxm = tb.rx <= 4 ? 4 : tb.rx <= 9 ? 9 : tb.rx <= 14 ? 14 : 0;
+    const size_t nmask = (size_t)2 * stl.njt * stl.nkt * smear_mask_words(c->world.n[0]);
+    int rc = ensure_buf(c, (void **)&c->sflags, &c->sflags_cap, nmask * sizeof(unsigned));
     if (rc) return rc;
+    HV_HIP(c, hipMemsetAsync(c->sflags, 0, nmask * sizeof(unsigned), c->stream));
     rc = ensure_buf(c, &c->D, &c->D_cap, (size_t)(tb.rx + 1) * n2 * sizeof(T));
     if (rc) return rc;
     c->tab_sigma = sigma_r;
@@ -460,14 +477,30 @@ int smear(hv_ctx *c, bool from_acc, double sigma_r, int ppn) {
         }
         HV_CHECK_LAUNCH(c);
     } else {
-        const unsigned ntiles = (unsigned)(2 * nx * c->stl.njt * c->stl.nkt);
+        // a few blocks per CU stride over the zy items
+        const unsigned ntiles = (unsigned)std::min<long long>(2LL * nx * c->stl.njt * c->stl.nkt, c->zy_blocks);
+        // the count box of the deposit (none for counts uploaded as a field)
+        const unsigned *cbox = from_acc ? (const unsigned *)(c->acc + 2 * c->G) : nullptr;
+        const int cw = cbox_species_words(nx);
         if (from_acc)
-            k_smear_zy<T, unsigned long long><<<ntiles, kSmearThreads, c->stl_lds, c->stream>>>(c->acc, D, c->sflags, tb, nx,
-                                                                                     ny, nz, c->stl);
+            k_smear_zy<T, unsigned long long><<<ntiles, kSmearThreads, c->stl_lds, c->stream>>>(
+                c->acc, D, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl);
         else
-            k_smear_zy<T, T><<<ntiles, kSmearThreads, c->stl_lds, c->stream>>>(rho, D, c->sflags, tb, nx, ny, nz, c->stl);
+            k_smear_zy<T, T><<<ntiles, kSmearThreads, c->stl_lds, c->stream>>>(rho, D, c->sflags, nullptr, 0, tb, nx, ny,
+                                                                               nz, c->stl);
         HV_CHECK_LAUNCH(c);
-        k_smear_xs<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, c->sflags, tb, nx, ny, nz, c->stl);
+        constexpr int VW = smear_vw<T>();
+        const long long nq = n2 / VW;
+        if (c->smear_xs_vec && nz % VW == 0 && c->xs_rxm > 0) {
+            const unsigned nb = (unsigned)((nq + kThreads - 1) / kThreads);
+            switch (c->xs_rxm) {
+            case 4: k_smear_xs_vec<T, 4><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl); break;
+            case 9: k_smear_xs_vec<T, 9><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl); break;
+            default: k_smear_xs_vec<T, 14><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl); break;
+            }
+        } else {
+            k_smear_xs<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl);
+        }
         HV_CHECK_LAUNCH(c);
     }
     prof_end(c, HV_KCLASS_SMEAR, pa, 1);
@@ -1205,7 +1238,7 @@ int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, i
     int rc = smear<T>(c, true, p->sigma_r, p->part_per_nucleon);
     if (rc < 0) return rc;
     *warn |= rc;
-    rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
+    rc = fill<unsigned long long>(c, c->acc, 0ULL, acc_words(c));
     if (rc) return rc;
     double md;
     rc = sor_solve<T>(c, (const T *)c->density, p->sor_omega, p->sor_tol, p->sor_max_it, iters, &md, true);
@@ -1234,7 +1267,7 @@ int run_init(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *sor_iters) 
     HV_HIP(c, hipStreamSynchronize(c->stream));
     c->phi_warm = false;
     c->hist_n = 0;
-    rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
+    rc = fill<unsigned long long>(c, c->acc, 0ULL, acc_words(c));
     if (rc) return rc;
     if (c->sort_every > 0) {
         rc = sort_particles<T>(c);
@@ -1467,7 +1500,7 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
         ok = ok && hipMalloc(&c->sort_tmp, bytes + 256) == hipSuccess;
         c->sort_tmp_bytes = bytes + 256;
     }
-    ok = ok && hipMalloc((void **)&c->acc, sizeof(unsigned long long) * (size_t)(2 * c->G)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->acc, sizeof(unsigned long long) * (size_t)acc_words(c)) == hipSuccess;
     ok = ok && hipMalloc(&c->density, ts * (size_t)(2 * c->G)) == hipSuccess;
     ok = ok && hipMalloc(&c->U, ts * (size_t)(2 * c->G)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->phi, sizeof(double) * (size_t)(2 * c->sp.Gh)) == hipSuccess;
@@ -1490,7 +1523,7 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
     }
     // zero state like the reference's allocators (include/init.hpp:97-99, 111, 124-128)
     for (int q = 0; q < 9 && ok; q++) ok = hipMemsetAsync(c->part[q], 0, ts * (size_t)c->total, c->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(c->acc, 0, sizeof(unsigned long long) * (size_t)(2 * c->G), c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->acc, 0, sizeof(unsigned long long) * (size_t)acc_words(c), c->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(c->density, 0, ts * (size_t)(2 * c->G), c->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(c->U, 0, ts * (size_t)(2 * c->G), c->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(c->phi, 0, sizeof(double) * (size_t)(2 * c->sp.Gh), c->stream) == hipSuccess;
@@ -1634,7 +1667,7 @@ int hv_download_field(hv_ctx *c, int which, double *host) {
 // ---- per-kernel entry points --------------------------------------------------
 int hv_cic_deposit(hv_ctx *c) {
     if (!c) return HV_ERR_ARG;
-    int rc = fill<unsigned long long>(c, c->acc, 0ULL, 2 * c->G);
+    int rc = fill<unsigned long long>(c, c->acc, 0ULL, acc_words(c));
     if (rc) return rc;
     if (c->precision == HV_PREC_F64) {
         k_deposit<double><<<blocks_for(c->total), kThreads, 0, c->stream>>>(

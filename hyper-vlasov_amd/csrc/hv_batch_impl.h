@@ -170,7 +170,7 @@ void batch_fill_args(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
         A.st = c->stf;
         A.st2 = c->stl;
         A.D = (T *)c->D;
-        A.flags = c->sflags;
+        A.fmask = c->sflags;
         A.G = c->G;
         A.sp = c->sp;
         A.phi = c->phi;

@@ -35,7 +35,7 @@ struct BInst {
     SmearTiles st;        // fused form
     SmearTiles st2;       // two-pass form
     T *D;                 // two-pass D channels
-    unsigned char *flags; // two-pass tile flags
+    unsigned *fmask;      // two-pass plane masks of the tile columns
     // grid, SOR, potentials
     long long G;
     Split sp;
@@ -100,14 +100,15 @@ template <typename T>
 __global__ void __launch_bounds__(kSmearThreads)
 kb_smear_zy(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
-    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.flags, smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st2);
+    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.fmask, (const unsigned *)(A.acc + 2 * A.G),
+                                         cbox_species_words(A.g.nx), smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st2);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_smear_xs(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
-    smear_xs_body<T>(bid, A.D, A.rho, A.flags, smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st2);
+    smear_xs_body<T>(bid, A.D, A.rho, A.fmask, nullptr, 0, smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st2);
 }
 
 template <typename T>
@@ -117,13 +118,14 @@ kb_smear_fused(const BInst<T> *__restrict__ a, const int *__restrict__ first, in
     smear_fused_body<T, unsigned long long>(bid, A.acc, A.rho, smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st);
 }
 
-// acc = 0 (2G words), blocks of the 2G table
+// acc = 0 (2G words, then the count box), blocks of the 2G table
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_zero_acc(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
     const long long q = (long long)bid * blockDim.x + threadIdx.x;
     if (q < 2 * A.G) A.acc[q] = 0ULL;
+    if (q < cbox_acc_words(A.g.nx)) A.acc[2 * A.G + q] = 0ULL;
 }
 
 template <typename T>

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "hv_device.h"
 
@@ -116,6 +117,47 @@ __device__ __forceinline__ void deposit_particle(const Cell<T> &c, long long off
     }
 }
 
+// ---- count box ------------------------------------------------------------------
+// The node ranges that a deposit may have touched, kept in unsigned words after
+// the 2G fixed-point counts (so every clear of the counts clears it too).  Per
+// species 6 + 4 nx words: [0..2] 0xffff - the least i, j, k; [3..5] the
+// greatest i, j, k + 1; then per plane i: 0xffff - least j, greatest j + 1,
+// 0xffff - least k, greatest k + 1.  Every word only grows (atomicMax), so a
+// zeroed box is empty and a filled one bounds every non-zero count (it may be
+// larger than the tight box, never smaller).  Grids are at most 65535 nodes
+// along an axis.
+__host__ __device__ inline int cbox_species_words(int nx) { return 6 + 4 * nx; }
+// u64 words after the 2G counts that hold both species' boxes
+__host__ __device__ inline long long cbox_acc_words(int nx) { return (2LL * cbox_species_words(nx) + 1) / 2; }
+__device__ __forceinline__ int cbox_lo(unsigned w) { return 0xffff - (int)w; }  // empty: 65535
+__device__ __forceinline__ int cbox_hi(unsigned w) { return (int)w - 1; }        // empty: -1
+
+__device__ __forceinline__ void cbox_raise(unsigned *w, unsigned v) {
+    // (a relaxed read first: once the box has grown, most blocks need no atomic)
+    if (v > __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(w, v);
+}
+
+// the nodes x0..x1, y0..y1, z0..z1 of species s0..s1 may hold counts
+__device__ __forceinline__ void cbox_record(unsigned *__restrict__ cbox, int nx, int s0, int s1, int x0, int x1,
+                                            int y0, int y1, int z0, int z1) {
+    const int nw = 6 + 4 * (x1 - x0 + 1);
+    for (int s = s0; s <= s1; s++) {
+        unsigned *cb = cbox + (long long)s * cbox_species_words(nx);
+        for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+            if (w < 6) {
+                const unsigned v = w == 0 ? 0xffffu - x0 : w == 1 ? 0xffffu - y0 : w == 2 ? 0xffffu - z0
+                                 : w == 3 ? (unsigned)x1 + 1 : w == 4 ? (unsigned)y1 + 1 : (unsigned)z1 + 1;
+                cbox_raise(cb + w, v);
+            } else {
+                const int pl = x0 + (w - 6) / 4, q = (w - 6) & 3;
+                const unsigned v = q == 0 ? 0xffffu - y0 : q == 1 ? (unsigned)y1 + 1 : q == 2 ? 0xffffu - z0
+                                                                                   : (unsigned)z1 + 1;
+                cbox_raise(cb + 6 + 4 * pl + q, v);
+            }
+        }
+    }
+}
+
 // ---- block-local CIC deposit ----------------------------------------------------
 // When the particles of a block are sorted by cell (see k_sort_keys), their
 // cells fill a small box; the block then accumulates the box of nodes in LDS
@@ -171,6 +213,9 @@ __device__ __forceinline__ void block_deposit(bool has, const Cell<T> &c, int sp
     const int x0 = bb[0], y0 = bb[1], z0 = bb[2];
     const int ex = bb[4] - x0 + 2, ey = bb[5] - y0 + 2, ez = bb[6] - z0 + 2;
     if (bb[4] < 0) return;  // no particle of this block is inside the grid
+    // the count box (after the 2G counts) takes the nodes of the block's cells
+    cbox_record((unsigned *)(acc + 2 * g.G), g.nx, bb[3], bb[7], x0, min(bb[4] + 1, g.nx - 1), y0,
+                min(bb[5] + 1, g.ny - 1), z0, min(bb[6] + 1, g.nz - 1));
     const long long off = bb[3] ? g.G : 0;
     const bool lds = (bb[3] == bb[7]) && (long long)ex * ey * ez <= kBoxCap;
     if (!lds) {
@@ -433,31 +478,50 @@ template <typename T>
 __device__ __forceinline__ void smear_h_levels(const T *__restrict__ C, T *__restrict__ Hs,
                                                SmearTabC<T> *__restrict__ tb, int R, int W, int TK, int rz,
                                                int nthr) {
+    // four chains of a thread at a time, two levels per step: the counts of
+    // sixteen taps load together ahead of their sums (a chain past the
+    // R x TK columns runs on a safe column and stores nothing)
     const int lvl = R * TK;
-    for (int e = threadIdx.x; e < lvl; e += nthr) {
-        const int r = e / TK, cc = e - r * TK;
-        const T *cp = C + r * W + cc + rz;
-        T *hp = Hs + e;
-        T h = tb->gz[0] * cp[0];
-        hp[0] = h;
+    const T gz0 = tb->gz[0];
+    for (int e0 = threadIdx.x; e0 < lvl; e0 += 4 * nthr) {
+        const T *cp[4];
+        T h[4];
+        int e[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            e[u] = e0 + u * nthr;
+            const int ee = e[u] < lvl ? e[u] : 0;
+            const int r = ee / TK, cc = ee - r * TK;
+            cp[u] = C + r * W + cc + rz;
+            h[u] = gz0 * cp[u][0];
+            if (e[u] < lvl) Hs[e[u]] = h[u];
+        }
         int K = 1;
-        for (; K + 3 <= rz; K += 4) {
-            T lo[4], hi[4], g[4];
+        for (; K + 1 <= rz; K += 2) {
+            const T g0 = tb->gz[K], g1 = tb->gz[K + 1];
+            T lo0[4], hi0[4], lo1[4], hi1[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                lo[u] = cp[-(K + u)];
-                hi[u] = cp[K + u];
-                g[u] = tb->gz[K + u];
+                lo0[u] = cp[u][-K];
+                hi0[u] = cp[u][K];
+                lo1[u] = cp[u][-(K + 1)];
+                hi1[u] = cp[u][K + 1];
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                h = smear_madd(smear_madd(h, g[u], lo[u]), g[u], hi[u]);
-                hp[(K + u) * lvl] = h;
+                h[u] = smear_madd(smear_madd(h[u], g0, lo0[u]), g0, hi0[u]);
+                if (e[u] < lvl) Hs[K * lvl + e[u]] = h[u];
+                h[u] = smear_madd(smear_madd(h[u], g1, lo1[u]), g1, hi1[u]);
+                if (e[u] < lvl) Hs[(K + 1) * lvl + e[u]] = h[u];
             }
         }
-        for (; K <= rz; K++) {
-            h = smear_madd(smear_madd(h, tb->gz[K], cp[-K]), tb->gz[K], cp[K]);
-            hp[K * lvl] = h;
+        if (K <= rz) {
+            const T g0 = tb->gz[K];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                h[u] = smear_madd(smear_madd(h[u], g0, cp[u][-K]), g0, cp[u][K]);
+                if (e[u] < lvl) Hs[K * lvl + e[u]] = h[u];
+            }
         }
     }
 }
@@ -682,23 +746,140 @@ k_smear_fused(const CT *__restrict__ cnt, T *__restrict__ rho, const SmearTab<T>
     smear_fused_body<T, CT>(blockIdx.x, cnt, rho, smear_tab_c(tb), nx, ny, nz, st);
 }
 
+// f(std::integral_constant<int, 0>) ... f(std::integral_constant<int, N - 1>),
+// in order: indices known at compile time (register arrays stay in registers)
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+template <typename T, typename CT>
+__device__ __forceinline__ T count_of(CT v) {
+    if constexpr (sizeof(CT) == 8 && !std::is_floating_point<CT>::value)
+        return (T)hv_fix_to_double((long long)v);
+    else
+        return (T)v;
+}
+
 // ---- two-pass smearing: z and y factors in LDS, then a sparse x pass ----------
 // The same factor sums in the same order (bit-identical to k_smear_fused),
 // with the D channels stored between two launches; no plane walk, so every
 // (plane, tile) is a block of its own (the faster form for a single grid):
 //   k_smear_zy: one block per (species, plane i, row tile of TJ rows, column
-//     tile of TK columns).  It stages the counts of the tile plus its
-//     (ry rows, rz columns) halo in LDS, builds every H_K of the (TJ + 2 ry) x
-//     TK columns in LDS, and writes the D_a channels of its TJ x TK nodes.  A
-//     tile whose halo holds no count writes nothing and flags itself empty:
-//     its H and D are exactly zero.
-//   k_smear_xs: the x factor, skipping planes whose tile is flagged empty
-//     (their terms are exact zeros, so skipping them changes no bit).
+//     tile of TK columns).  A tile whose halo misses the count box of its
+//     plane (written by the deposit; none: every tile reads its counts) stops
+//     at once.  Otherwise it stages the counts of the tile plus its (ry rows,
+//     rz columns) halo in LDS (four loads per thread in flight, none outside
+//     the count box), builds every H_K of the (TJ + 2 ry) x TK columns in LDS
+//     and writes the D_a channels of its TJ x TK nodes.  Each block sets or
+//     clears bit i of its tile column's plane mask: set when its halo holds a
+//     count.  A cleared tile writes nothing: its H and D are exactly zero.
+//   k_smear_xs / k_smear_xs_vec: the x factor, skipping planes whose bit is
+//     clear (their terms are exact zeros, so skipping them changes no bit);
+//     the vector form takes VW nodes per thread and issues the loads of all
+//     2 RXM + 1 planes at once, and writes zeros outside the count box grown
+//     by the radius without reading anything.
+
+// plane-mask words per tile column
+__host__ __device__ inline int smear_mask_words(int nx) { return (nx + 31) / 32; }
+
+// one (species sp, plane i, tile jt, kt) item of the z and y factors; pw: the
+// four count-box words of plane i of species sp (null: every count is read).
+// Every branch is uniform over the block.
+// A workgroup barrier that orders LDS accesses only: global loads issued
+// before it stay in flight, so a prefetch survives it (__syncthreads fences
+// every address space and waits for them).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// the OR of v over a workgroup of kSmearThreads threads, through LDS-only
+// barriers (slot: kSmearThreads / 64 ints of LDS, not reused before the
+// caller's next barrier)
+__device__ __forceinline__ int block_or_lds(int v, int *slot) {
+    const bool b = __ballot(v) != 0ULL;
+    if ((threadIdx.x & (kWave - 1)) == 0) slot[threadIdx.x / kWave] = b ? 1 : 0;
+    lds_barrier();
+    int r = 0;
+#pragma unroll
+    for (int w = 0; w < kSmearThreads / kWave; w++) r |= slot[w];
+    return r;
+}
+
+// staged count elements per thread with precomputed coordinates
+constexpr int kZyStaged = 8;
+
+template <typename T>
+__device__ __forceinline__ T *smem_zy() {
+    extern __shared__ unsigned char smem[];
+    return (T *)smem;
+}
+
+// (row << 16 | column) of the halo elements thread + 256 m (-1 past the R x W)
+__device__ __forceinline__ void smear_zy_coords(int R, int W, int (&ecoord)[kZyStaged]) {
+#pragma unroll
+    for (int m = 0; m < kZyStaged; m++) {
+        const int e = (int)threadIdx.x + m * kSmearThreads;
+        const int r = e / W;
+        ecoord[m] = e < R * W ? (r << 16 | (e - r * W)) : -1;
+    }
+}
+
+// The H levels and D channels of a staged item (counts in C, `any` set when
+// one of them is non-zero): sets or clears its plane bit.
+template <typename T>
+__device__ __forceinline__ void smear_zy_finish(bool any, int sp, int i, int jt, int kt, T *__restrict__ D,
+                                                unsigned *__restrict__ fmask, SmearTabC<T> *__restrict__ tb, int nx,
+                                                int ny, int nz, const SmearTiles &st) {
+    // (LDS-only barriers: the next item's count loads stay in flight)
+    extern __shared__ unsigned char smem[];
+    const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
+    const int TJ = st.TJ, TK = st.TK;
+    const int R = TJ + 2 * ry, W = TK + 2 * rz;
+    constexpr int VW = smear_vw<T>();
+    T *C = (T *)smem;
+    T *Hs = C + smear_c_words(R, W, VW);
+    const long long G = (long long)nx * ny * nz, n2 = 2 * G;
+    const long long base = sp * G + (long long)i * ny * nz;
+    const int j0 = jt * TJ, k0 = kt * TK;
+    unsigned *mw = fmask + (((long long)sp * st.njt + jt) * st.nkt + kt) * smear_mask_words(nx) + (i >> 5);
+    const unsigned bit = 1u << (i & 31);
+    if (!any) {
+        if (threadIdx.x == 0) atomicAnd(mw, ~bit);
+        return;
+    }
+    smear_h_levels<T>(C, Hs, tb, R, W, TK, rz, kSmearThreads);
+    lds_barrier();
+    if (threadIdx.x == 0) atomicOr(mw, bit);
+    // channel a per wave, VW consecutive columns per lane (the vector tile)
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), nwaves = kSmearThreads / kWave;
+    const int r = lane >> 3, c0 = (lane & 7) * VW;
+    const int j = j0 + r;
+    const long long q = base + (long long)j * nz + k0 + c0;
+    for (int a = wave; a <= rx; a += nwaves) {
+        T acc[VW];
+        smear_channel_vec<T>(Hs, tb, a, (r + ry) * TK + c0, R, TK, acc);
+        if (j < ny) {
+#pragma unroll
+            for (int v = 0; v < VW; v++)
+                if (k0 + c0 + v < nz) D[(long long)a * n2 + q + v] = acc[v];
+        }
+    }
+}
 
 template <typename T, typename CT>
-__device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restrict__ cnt, T *__restrict__ D,
-                                              unsigned char *__restrict__ flags,
-           SmearTabC<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
+__device__ __forceinline__ void smear_zy_item(int sp, int i, int jt, int kt, const CT *__restrict__ cnt,
+                                              T *__restrict__ D, unsigned *__restrict__ fmask,
+                                              const unsigned *__restrict__ pw, SmearTabC<T> *__restrict__ tb,
+                                              int nx, int ny, int nz, const SmearTiles &st,
+                                              const int *ecoord = nullptr) {
     extern __shared__ unsigned char smem[];
     const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
     const int TJ = st.TJ, TK = st.TK;
@@ -706,34 +887,75 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
     constexpr int VW = smear_vw<T>();
     T *C = (T *)smem;                       // R x W counts
     T *Hs = C + smear_c_words(R, W, VW);    // (rz + 1) x R x TK
-    int t = bid;
-    const int kt = t % st.nkt;
-    t /= st.nkt;
-    const int jt = t % st.njt;
-    t /= st.njt;
-    const int i = t % nx;
-    const int sp = t / nx;
     const long long G = (long long)nx * ny * nz, n2 = 2 * G;
     const long long base = sp * G + (long long)i * ny * nz;
     const int j0 = jt * TJ, k0 = kt * TK;
-    int nonzero = 0;
-    for (int e = threadIdx.x; e < R * W; e += kSmearThreads) {
-        const int r = e / W, cc = e - r * W;
-        const int j = j0 - ry + r, k = k0 - rz + cc;
-        T v = T(0.0);
-        if (j >= 0 && j < ny && k >= 0 && k < nz) v = count_value<T, CT>(cnt, base + (long long)j * nz + k);
-        C[e] = v;
-        nonzero |= (v != T(0.0));
+    unsigned *mw = fmask + (((long long)sp * st.njt + jt) * st.nkt + kt) * smear_mask_words(nx) + (i >> 5);
+    const unsigned bit = 1u << (i & 31);
+    // the count box of plane i: rows pj0..pj1, columns pk0..pk1 may hold counts
+    int pj0 = 0, pj1 = ny - 1, pk0 = 0, pk1 = nz - 1;
+    if (pw) {
+        pj0 = cbox_lo(pw[0]);
+        pj1 = cbox_hi(pw[1]);
+        pk0 = cbox_lo(pw[2]);
+        pk1 = cbox_hi(pw[3]);
     }
-    const long long tile = bid;
+    if (pj1 < pj0 || pj0 > j0 + TJ - 1 + ry || pj1 < j0 - ry || pk0 > k0 + TK - 1 + rz || pk1 < k0 - rz) {
+        if (threadIdx.x == 0) atomicAnd(mw, ~bit);
+        return;
+    }
+    int nonzero = 0;
+    const int RW = R * W;
+    if (ecoord && RW <= kZyStaged * kSmearThreads) {
+        // element e = thread + 256 m at (row, column) = ecoord[m] of the halo
+        // (rows and columns relative to (j0 - ry, k0 - rz)), all loads at once
+        const CT *cb = cnt + base + (long long)(j0 - ry) * nz + (k0 - rz);
+        const int rlo = pj0 - (j0 - ry), rhi = pj1 - (j0 - ry), clo = pk0 - (k0 - rz), chi = pk1 - (k0 - rz);
+        CT v[kZyStaged];
+#pragma unroll
+        for (int m = 0; m < kZyStaged; m++) {
+            const int r = ecoord[m] >> 16, cc = ecoord[m] & 0xffff;
+            v[m] = (ecoord[m] >= 0 && r >= rlo && r <= rhi && cc >= clo && cc <= chi) ? cb[r * nz + cc] : CT(0);
+        }
+#pragma unroll
+        for (int m = 0; m < kZyStaged; m++) {
+            if (ecoord[m] >= 0) {
+                const T x = count_of<T, CT>(v[m]);
+                C[(int)threadIdx.x + m * kSmearThreads] = x;
+                nonzero |= (x != T(0.0));
+            }
+        }
+    } else {
+        for (int e0 = 0; e0 < RW; e0 += 4 * kSmearThreads) {
+            CT v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + u * kSmearThreads + (int)threadIdx.x;
+                const int r = e / W, cc = e - r * W;
+                const int j = j0 - ry + r, k = k0 - rz + cc;
+                v[u] = (e < RW && j >= pj0 && j <= pj1 && k >= pk0 && k <= pk1) ? cnt[base + (long long)j * nz + k]
+                                                                                : CT(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + u * kSmearThreads + (int)threadIdx.x;
+                if (e < RW) {
+                    const T x = count_of<T, CT>(v[u]);
+                    C[e] = x;
+                    nonzero |= (x != T(0.0));
+                }
+            }
+        }
+    }
     // (read back as a wave-uniform value: the channel loop below then stays
     // scalar, with the tap tables in scalar loads)
     if (!__builtin_amdgcn_readfirstlane(__syncthreads_or(nonzero))) {
-        if (threadIdx.x == 0) flags[tile] = 0;
+        if (threadIdx.x == 0) atomicAnd(mw, ~bit);
         return;
     }
     smear_h_levels<T>(C, Hs, tb, R, W, TK, rz, kSmearThreads);
     __syncthreads();
+    if (threadIdx.x == 0) atomicOr(mw, bit);
     if (smear_vec_tile<T>(TJ, TK)) {
         // channel a per wave, VW consecutive columns per lane
         const int lane = threadIdx.x & (kWave - 1);
@@ -750,7 +972,6 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
                     if (k0 + c0 + v < nz) D[(long long)a * n2 + q + v] = acc[v];
             }
         }
-        if (threadIdx.x == 0) flags[tile] = 1;
         return;
     }
     for (int e = threadIdx.x; e < TJ * TK; e += kSmearThreads) {
@@ -768,19 +989,144 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
             D[(long long)a * n2 + q] = acc;
         }
     }
-    if (threadIdx.x == 0) flags[tile] = 1;
 }
 
+// one block per (species, plane, tile) of the whole grid (ensembles)
+template <typename T, typename CT>
+__device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restrict__ cnt, T *__restrict__ D,
+                                              unsigned *__restrict__ fmask, const unsigned *__restrict__ cbox, int cw,
+           SmearTabC<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
+    int t = bid;
+    const int kt = t % st.nkt;
+    t /= st.nkt;
+    const int jt = t % st.njt;
+    t /= st.njt;
+    const int i = t % nx;
+    const int sp = t / nx;
+    smear_zy_item<T, CT>(sp, i, jt, kt, cnt, D, fmask, cbox ? cbox + (long long)sp * cw + 6 + 4 * i : nullptr, tb,
+                         nx, ny, nz, st);
+}
+
+// The items of the count box only (planes that hold counts times the tiles
+// whose halo meets the box; without a box every item), a grid of a few
+// blocks per CU striding over them: no block is spent on empty space, and
+// the plane masks of exactly these items are what the x pass reads.
 template <typename T, typename CT>
 __global__ void __launch_bounds__(kSmearThreads)
-k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned char *__restrict__ flags,
-           const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
-    smear_zy_body<T, CT>(blockIdx.x, cnt, D, flags, smear_tab_c(tb), nx, ny, nz, st);
+k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned *__restrict__ fmask,
+           const unsigned *__restrict__ cbox, int cw, const SmearTab<T> *__restrict__ tbp, int nx, int ny, int nz,
+           SmearTiles st) {
+    SmearTabC<T> *tb = smear_tab_c(tbp);
+    const int ry = tb->ry, rz = tb->rz;
+    int i0[2], jt0[2], kt0[2], ni[2], njt[2], nkt[2], nit[2];
+#pragma unroll
+    for (int sp = 0; sp < 2; sp++) {
+        int ci0 = 0, ci1 = nx - 1, cj0 = 0, cj1 = ny - 1, ck0 = 0, ck1 = nz - 1;
+        if (cbox) {
+            const unsigned *cb = cbox + (long long)sp * cw;
+            ci0 = cbox_lo(cb[0]);
+            cj0 = cbox_lo(cb[1]);
+            ck0 = cbox_lo(cb[2]);
+            ci1 = cbox_hi(cb[3]);
+            cj1 = cbox_hi(cb[4]);
+            ck1 = cbox_hi(cb[5]);
+        }
+        const int ja = cj0 - ry > 0 ? (cj0 - ry) / st.TJ : 0, jb = min(st.njt - 1, (cj1 + ry) / st.TJ);
+        const int ka = ck0 - rz > 0 ? (ck0 - rz) / st.TK : 0, kb = min(st.nkt - 1, (ck1 + rz) / st.TK);
+        i0[sp] = ci0;
+        jt0[sp] = ja;
+        kt0[sp] = ka;
+        ni[sp] = ci1 >= ci0 ? ci1 - ci0 + 1 : 0;
+        njt[sp] = jb - ja + 1;
+        nkt[sp] = kb - ka + 1;
+        nit[sp] = ni[sp] * njt[sp] * nkt[sp];  // (at most 2 nx njt nkt items: 32-bit)
+    }
+    const int nall = nit[0] + nit[1];
+    auto decode = [&](int it, int &sp, int &i, int &jt, int &kt) {
+        sp = it >= nit[0] ? 1 : 0;
+        const int u = it - (sp ? nit[0] : 0);
+        const int ul = u / nkt[sp];
+        kt = kt0[sp] + (u - ul * nkt[sp]);
+        const int uj = ul / njt[sp];
+        jt = jt0[sp] + (ul - uj * njt[sp]);
+        i = i0[sp] + uj;
+    };
+    const int R = st.TJ + 2 * ry, W = st.TK + 2 * rz;
+    if (!smear_vec_tile<T>(st.TJ, st.TK) || R * W > kZyStaged * kSmearThreads) {
+        int ecoord[kZyStaged];
+        smear_zy_coords(R, W, ecoord);
+        for (int it = blockIdx.x; it < nall; it += gridDim.x) {
+            int sp, i, jt, kt;
+            decode(it, sp, i, jt, kt);
+            smear_zy_item<T, CT>(sp, i, jt, kt, cnt, D, fmask, cbox ? cbox + (long long)sp * cw + 6 + 4 * i : nullptr,
+                                 tb, nx, ny, nz, st, ecoord);
+            __syncthreads();  // the next item's counts overwrite C and H
+        }
+        return;
+    }
+    // vector tiles with at most 8 staged counts per thread: the counts of the
+    // next item load into registers while this item builds H and D
+    int ecoord[kZyStaged];
+    smear_zy_coords(R, W, ecoord);
+    T *C = (T *)smem_zy<T>();
+    __shared__ int orslot[kSmearThreads / kWave];
+    CT pre[kZyStaged];
+    // stage item it: its coordinates, whether its halo meets the count box of
+    // its plane (hit), and the loads of its counts into pre
+    auto stage = [&](int it, int &sp, int &i, int &jt, int &kt, bool &hit) {
+        decode(it, sp, i, jt, kt);
+        const int j0 = jt * st.TJ, k0 = kt * st.TK;
+        int pj0 = 0, pj1 = ny - 1, pk0 = 0, pk1 = nz - 1;
+        if (cbox) {
+            const unsigned *pw = cbox + (long long)sp * cw + 6 + 4 * i;
+            pj0 = cbox_lo(pw[0]);
+            pj1 = cbox_hi(pw[1]);
+            pk0 = cbox_lo(pw[2]);
+            pk1 = cbox_hi(pw[3]);
+        }
+        hit = !(pj1 < pj0 || pj0 > j0 + st.TJ - 1 + ry || pj1 < j0 - ry || pk0 > k0 + st.TK - 1 + rz ||
+                pk1 < k0 - rz);
+        const long long G = (long long)nx * ny * nz;
+        const CT *cb = cnt + sp * G + (long long)i * ny * nz + (long long)(j0 - ry) * nz + (k0 - rz);
+        const int rlo = pj0 - (j0 - ry), rhi = pj1 - (j0 - ry), clo = pk0 - (k0 - rz), chi = pk1 - (k0 - rz);
+#pragma unroll
+        for (int m = 0; m < kZyStaged; m++) {
+            const int r = ecoord[m] >> 16, cc = ecoord[m] & 0xffff;
+            pre[m] = (hit && ecoord[m] >= 0 && r >= rlo && r <= rhi && cc >= clo && cc <= chi) ? cb[r * nz + cc]
+                                                                                                : CT(0);
+        }
+    };
+    int sp, i, jt, kt;
+    bool hit = false;
+    int it = blockIdx.x;
+    if (it < nall) stage(it, sp, i, jt, kt, hit);
+    for (; it < nall; it += gridDim.x) {
+        int nonzero = 0;
+        if (hit) {
+#pragma unroll
+            for (int m = 0; m < kZyStaged; m++) {
+                if (ecoord[m] >= 0) {
+                    const T x = count_of<T, CT>(pre[m]);
+                    C[(int)threadIdx.x + m * kSmearThreads] = x;
+                    nonzero |= (x != T(0.0));
+                }
+            }
+        }
+        const int csp = sp, ci = i, cjt = jt, ckt = kt;
+        const bool chit = hit;
+        if (it + (int)gridDim.x < nall) stage(it + gridDim.x, sp, i, jt, kt, hit);
+        const bool any = chit && __builtin_amdgcn_readfirstlane(block_or_lds(nonzero, orslot));
+        smear_zy_finish<T>(any, csp, ci, cjt, ckt, D, fmask, tb, nx, ny, nz, st);
+        lds_barrier();  // the next item's counts overwrite C and H
+    }
 }
 
+// one node per thread, any radius; with a count box (cbox, cw words per
+// species) only the planes of the box are read, whose bits the zy pass wrote
 template <typename T>
 __device__ __forceinline__ void smear_xs_body(const int bid, const T *__restrict__ D, T *__restrict__ rho,
-                                              const unsigned char *__restrict__ flags,
+                                              const unsigned *__restrict__ fmask,
+                                              const unsigned *__restrict__ cbox, int cw,
            SmearTabC<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
     const long long G = (long long)nx * ny * nz, n2 = 2 * G;
     const long long q = (long long)bid * blockDim.x + threadIdx.x;
@@ -789,15 +1135,20 @@ __device__ __forceinline__ void smear_xs_body(const int bid, const T *__restrict
     const int sp = q >= G ? 1 : 0;
     const long long node = q - sp * G;
     const int i = (int)(node / plane), j = (int)((node / nz) % ny), k = (int)(node % nz);
-    const long long trow = ((long long)sp * nx) * st.njt * st.nkt + (long long)(j / st.TJ) * st.nkt + k / st.TK;
-    const long long tstride = (long long)st.njt * st.nkt;
+    const unsigned *mw = fmask + (((long long)sp * st.njt + j / st.TJ) * st.nkt + k / st.TK) * smear_mask_words(nx);
+    int plo = 0, phi = nx - 1;
+    if (cbox) {
+        const unsigned *cb = cbox + (long long)sp * cw;
+        plo = cbox_lo(cb[0]);
+        phi = cbox_hi(cb[3]);
+    }
     T acc = T(0.0);
     for (int a = -tb->rx; a <= tb->rx; a++) {
         const int aa = a < 0 ? -a : a;
         if (tb->kb[aa] < 0) continue;
         const int ia = i + a;
-        if (ia < 0 || ia >= nx) continue;
-        if (!flags[trow + ia * tstride]) continue;
+        if (ia < plo || ia > phi) continue;
+        if (!((mw[ia >> 5] >> (ia & 31)) & 1u)) continue;
         acc = smear_madd(acc, tb->gx[aa], D[(long long)aa * n2 + q + (long long)a * plane]);
     }
     rho[q] = acc * tb->term;
@@ -805,9 +1156,87 @@ __device__ __forceinline__ void smear_xs_body(const int bid, const T *__restrict
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_smear_xs(const T *__restrict__ D, T *__restrict__ rho, const unsigned char *__restrict__ flags,
-           const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz, SmearTiles st) {
-    smear_xs_body<T>(blockIdx.x, D, rho, flags, smear_tab_c(tb), nx, ny, nz, st);
+k_smear_xs(const T *__restrict__ D, T *__restrict__ rho, const unsigned *__restrict__ fmask,
+           const unsigned *__restrict__ cbox, int cw, const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz,
+           SmearTiles st) {
+    smear_xs_body<T>(blockIdx.x, D, rho, fmask, cbox, cw, smear_tab_c(tb), nx, ny, nz, st);
+}
+
+// VW nodes per thread (nz a multiple of VW), radius at most RXM <= 16: the
+// plane bits i - RXM .. i + RXM of the tile column come from two mask words
+// (planes outside the count box, or outside 0..nx-1 without one, are excluded
+// by their index: the zy pass writes the bits of the box's planes only),
+// the loads of all its planes are issued before the sum, and a term whose
+// plane is skipped is an exact zero (acc + gx 0 = acc for acc >= +0)
+template <typename T, int RXM>
+__device__ __forceinline__ void smear_xs_vec_body(const int bid, const T *__restrict__ D, T *__restrict__ rho,
+                                                  const unsigned *__restrict__ fmask,
+                                                  const unsigned *__restrict__ cbox, int cw,
+                                                  SmearTabC<T> *__restrict__ tb, int nx, int ny, int nz,
+                                                  SmearTiles st) {
+    constexpr int VW = smear_vw<T>(), NA = 2 * RXM + 1;
+    static_assert(RXM <= 16, "two mask words cover the planes of a node");
+    typedef T V __attribute__((ext_vector_type(VW)));
+    const int nzq = nz / VW;
+    const long long t = (long long)bid * blockDim.x + threadIdx.x;
+    if (t >= 2LL * nx * ny * nzq) return;
+    const int kq = (int)(t % nzq);
+    long long rr = t / nzq;
+    const int j = (int)(rr % ny);
+    rr /= ny;
+    const int i = (int)(rr % nx), sp = (int)(rr / nx);
+    const int k0 = kq * VW;
+    const long long G = (long long)nx * ny * nz, n2 = 2 * G, plane = (long long)ny * nz;
+    const long long q = sp * G + (long long)i * plane + (long long)j * nz + k0;
+    const int rx = tb->rx;
+    V *out = (V *)(rho + q);
+    int plo = 0, phi = nx - 1;  // planes that may hold counts
+    if (cbox) {
+        const unsigned *cb = cbox + (long long)sp * cw;
+        const int ry = tb->ry, rz = tb->rz;
+        const int ci0 = cbox_lo(cb[0]), cj0 = cbox_lo(cb[1]), ck0 = cbox_lo(cb[2]);
+        const int ci1 = cbox_hi(cb[3]), cj1 = cbox_hi(cb[4]), ck1 = cbox_hi(cb[5]);
+        if (ci1 < ci0 || i < ci0 - rx || i > ci1 + rx || j < cj0 - ry || j > cj1 + ry || k0 + VW - 1 < ck0 - rz ||
+            k0 > ck1 + rz) {
+            *out = V(T(0.0));
+            return;
+        }
+        plo = ci0;
+        phi = ci1;
+    }
+    const unsigned *mw = fmask + (((long long)sp * st.njt + j / st.TJ) * st.nkt + k0 / st.TK) * smear_mask_words(nx);
+    // bits of planes i - RXM ..: word wlo = floor((i - RXM) / 32) and the next
+    const int p0 = i - RXM, wlo = p0 >= 0 ? p0 >> 5 : -1, nmw = smear_mask_words(nx);
+    const unsigned long long m = (wlo >= 0 ? (unsigned long long)mw[wlo] : 0ULL) |
+                                 (wlo + 1 < nmw ? (unsigned long long)mw[wlo + 1] << 32 : 0ULL);
+    const unsigned long long win = m >> (p0 - 32 * wlo);  // bit d: plane i - RXM + d
+    V d[NA];
+    static_for<NA>([&](auto dd) {
+        constexpr int a = decltype(dd)::value - RXM, aa = a < 0 ? -a : a;
+        const bool take = aa <= rx && i + a >= plo && i + a <= phi && ((win >> decltype(dd)::value) & 1ULL);
+        d[decltype(dd)::value] = take ? *(const V *)(D + (long long)aa * n2 + q + (long long)a * plane) : V(T(0.0));
+    });
+    V acc = V(T(0.0));
+    static_for<NA>([&](auto dd) {
+        constexpr int a = decltype(dd)::value - RXM, aa = a < 0 ? -a : a;
+        if (aa <= rx) {
+            const T g = tb->gx[aa];
+#pragma unroll
+            for (int v = 0; v < VW; v++) acc[v] = smear_madd(acc[v], g, d[decltype(dd)::value][v]);
+        }
+    });
+    const T term = tb->term;
+#pragma unroll
+    for (int v = 0; v < VW; v++) acc[v] = acc[v] * term;
+    *out = acc;
+}
+
+template <typename T, int RXM>
+__global__ void __launch_bounds__(kBlock)
+k_smear_xs_vec(const T *__restrict__ D, T *__restrict__ rho, const unsigned *__restrict__ fmask,
+               const unsigned *__restrict__ cbox, int cw, const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz,
+               SmearTiles st) {
+    smear_xs_vec_body<T, RXM>(blockIdx.x, D, rho, fmask, cbox, cw, smear_tab_c(tb), nx, ny, nz, st);
 }
 
 // ---- Skyrme potentials (include/physics.hpp:273-281), optionally + phi ------

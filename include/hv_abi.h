@@ -244,8 +244,10 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
 #define HV_OPT_BATCH_GROUP_KIB 6
 /* HV_OPT_SMEAR_FUSED: smear a single grid with the fused plane-walking kernel
  * (the D channels stay in LDS) instead of the two-pass form (default 0: the
- * two-pass form, faster on one grid; ensembles always use the fused kernel).
- * Same bits either way. */
+ * two-pass form, faster on one grid: after a deposit it visits only the count
+ * box that the deposit records, the nodes that can hold a count, and writes
+ * exact zeros beyond that box grown by the smearing radius; ensembles default
+ * to the fused kernel).  Same bits either way. */
 #define HV_OPT_SMEAR_FUSED 7
 /* HV_OPT_BATCH_GRAPH (ensembles only): from the second step of a
  * hv_batch_run_steps call on, replay one captured hipGraph of the step's

@@ -51,35 +51,46 @@ def test_deposit_bitwise(oracle, n, d):
     assert skipped > 0 and (warn & 2)
 
 
-@pytest.mark.parametrize("fused", [0, 1])
-@pytest.mark.parametrize("n,d", GRIDS + [((64, 60, 64), (16.0, 15.0, 16.0))])  # last: chunks along i
-def test_smear_bitwise(oracle, n, d, fused):
+# smearing forms: HV_OPT_SMEAR_FUSED
+FORMS = {"two-pass": 0, "fused": 1}
+
+
+def _set_form(sim, form):
+    sim.set_option(hvamd.HV_OPT_SMEAR_FUSED, FORMS[form])
+    assert sim.get_option(hvamd.HV_OPT_SMEAR_FUSED) == FORMS[form]
+
+
+@pytest.mark.parametrize("form", ["two-pass", "fused"])
+@pytest.mark.parametrize("n,d", GRIDS + [((64, 60, 64), (16.0, 15.0, 16.0)),   # chunks along i
+                                         ((24, 20, 18), (6.0, 5.0, 4.5))])      # nz not a multiple of 4
+def test_smear_bitwise(oracle, n, d, form):
     """Both smearing forms (two-pass; fused plane walk) against the oracle."""
-    print(f"smear grid={n} fused={fused}", flush=True)
+    print(f"smear grid={n} form={form}", flush=True)
     st = particles(12, 500, 600, 0.5 * min(d))
     counts, _, _ = oracle.deposit(st["x"], st["y"], st["z"], 500, 600, n, d)
     sim = _sim(n, d, 500, 600)
-    sim.set_option(hvamd.HV_OPT_SMEAR_FUSED, fused)
+    _set_form(sim, form)
     sim.upload_field(hvamd.HV_FIELD_DENSITY, counts)
     sim.compute_volumetric_densities(SIGMA_R, 100)
     got = sim.download_field(hvamd.HV_FIELD_DENSITY)
     _eq(got, oracle.smear(counts, n, d, SIGMA_R, 100))
 
 
-@pytest.mark.parametrize("n,d", GRIDS + [((64, 60, 64), (16.0, 15.0, 16.0)), ((96, 96, 96), (19.2, 19.2, 19.2))])
+@pytest.mark.parametrize("n,d", GRIDS + [((64, 60, 64), (16.0, 15.0, 16.0)), ((96, 96, 96), (19.2, 19.2, 19.2)),
+                                         ((24, 20, 18), (6.0, 5.0, 4.5))])
 def test_smear_mixed_near_oracle(oracle, n, d):
     """The mixed form smears in FP32 with one rounding per tap (a fused
     multiply-add): within 2e-5 of the largest value of the oracle's FP64
     smearing (a few hundred taps per node at FP32's 6e-8 unit roundoff), and
-    the two forms give the same bits."""
+    every form gives the same bits."""
     print(f"smear mixed grid={n}", flush=True)
     st = particles(12, 500, 600, 0.5 * min(d))
     counts, _, _ = oracle.deposit(st["x"], st["y"], st["z"], 500, 600, n, d)
     want = oracle.smear(counts, n, d, SIGMA_R, 100)
     got = []
-    for fused in (0, 1):
+    for form in ("two-pass", "fused"):
         sim = _sim(n, d, 500, 600, hvamd.HV_PREC_MIXED)
-        sim.set_option(hvamd.HV_OPT_SMEAR_FUSED, fused)
+        _set_form(sim, form)
         sim.upload_field(hvamd.HV_FIELD_DENSITY, counts)
         sim.compute_volumetric_densities(SIGMA_R, 100)
         got.append(sim.download_field(hvamd.HV_FIELD_DENSITY))
@@ -88,6 +99,40 @@ def test_smear_mixed_near_oracle(oracle, n, d):
     err = np.abs(got[0] - want).max() / np.abs(want).max()
     print("max error / max", err)
     assert err < 2e-5
+
+
+@pytest.mark.parametrize("prec", [hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED])
+@pytest.mark.parametrize("n,d,radius,spill,sort_every", [
+    ((32, 32, 32), (8.0, 8.0, 8.0), 4.0, 0.0, 8),          # the c0 nucleus
+    ((48, 40, 56), (12.0, 10.0, 14.0), 3.0, 0.01, 8),     # small nucleus, large anisotropic box, some outside
+    ((48, 40, 56), (12.0, 10.0, 14.0), 3.0, 0.01, 0),     # unsorted: the deposit's per-particle atomics
+    ((20, 16, 24), (6.0, 5.0, 7.5), 5.5, 0.0, 8),         # nucleus over the whole box, counts on its faces
+    ((40, 36, 30), (10.0, 9.0, 7.5), 3.5, 0.0, 8),        # nz not a multiple of 4: the one-node x pass (FP32)
+])
+def test_smear_count_box_loop_equals_fused(n, d, radius, spill, sort_every, prec):
+    """The two-pass form reads only the count box that the deposit records (its
+    z and y pass visits the box's planes and tiles, its x pass writes zeros
+    outside the box grown by the radius); the fused form reads every count: a
+    few steps of the loop give the same bits in both."""
+    w = configs.c0()
+    st = particles(21, w.n_protons, w.n_neutrons, radius, spill=spill)
+    outs = []
+    for form in ("fused", "two-pass"):
+        print(f"count-box loop grid={n} prec={prec} sort={sort_every} form={form}", flush=True)
+        sim = _sim(n, d, w.n_protons, w.n_neutrons, prec)
+        _set_form(sim, form)
+        sim.set_option(hvamd._lib.HV_OPT_SORT_EVERY, sort_every)
+        sim.upload_particles(**st)
+        stats, its = sim.simulate(w.sky(), w.params(6))
+        outs.append((stats, list(its), sim.download_particles(), sim.download_field(hvamd.HV_FIELD_DENSITY)))
+        sim.close()
+    assert np.abs(outs[0][3]).max() > 0
+    for o in outs[1:]:
+        _eq(o[0], outs[0][0])
+        assert o[1] == outs[0][1]
+        for k in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz"):
+            _eq(o[2][k], outs[0][2][k])
+        _eq(o[3], outs[0][3])
 
 
 @pytest.mark.parametrize("n,d", GRIDS)
