@@ -81,6 +81,7 @@ struct hv_ctx {
     int smear_xs_vec = 1;               // two-pass x pass: VW nodes per thread (when nz allows)
     int xs_rxm = 0;                     // its radius bucket (0: radius too large for it)
     long long zy_blocks = 1024;         // workgroups of the zy pass (blocks per CU that its LDS allows x CUs)
+    unsigned clear_blocks = 512;        // workgroups of k_clear_counts
     unsigned long long *maxd = nullptr; // per-iteration slot maxima of |dphi| (double bits)
     double *maxc = nullptr;             // per-iteration max |dphi|, collected for read-back
     int maxd_cap = 0;
@@ -1238,8 +1239,9 @@ int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, i
     int rc = smear<T>(c, true, p->sigma_r, p->part_per_nucleon);
     if (rc < 0) return rc;
     *warn |= rc;
-    rc = fill<unsigned long long>(c, c->acc, 0ULL, acc_words(c));
-    if (rc) return rc;
+    // counts and their box back to zero: the box's rows only
+    k_clear_counts<<<c->clear_blocks, 256, 0, c->stream>>>(c->acc, c->world.n[0], c->world.n[1], c->world.n[2]);
+    HV_CHECK_LAUNCH(c);
     double md;
     rc = sor_solve<T>(c, (const T *)c->density, p->sor_omega, p->sor_tol, p->sor_max_it, iters, &md, true);
     if (rc < 0) return rc;

@@ -127,8 +127,9 @@ __device__ __forceinline__ void deposit_particle(const Cell<T> &c, long long off
 // larger than the tight box, never smaller).  Grids are at most 65535 nodes
 // along an axis.
 __host__ __device__ inline int cbox_species_words(int nx) { return 6 + 4 * nx; }
-// u64 words after the 2G counts that hold both species' boxes
-__host__ __device__ inline long long cbox_acc_words(int nx) { return (2LL * cbox_species_words(nx) + 1) / 2; }
+// u64 words after the 2G counts that hold both species' boxes and, after
+// them, the arrival counter of k_clear_counts
+__host__ __device__ inline long long cbox_acc_words(int nx) { return (2LL * cbox_species_words(nx) + 2) / 2; }
 __device__ __forceinline__ int cbox_lo(unsigned w) { return 0xffff - (int)w; }  // empty: 65535
 __device__ __forceinline__ int cbox_hi(unsigned w) { return (int)w - 1; }        // empty: -1
 
@@ -418,6 +419,41 @@ __global__ void k_fill(V *__restrict__ p, V v, long long n) {
     long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (; q < n; q += stride) p[q] = v;
+}
+
+// counts = 0 after a smearing: only the count box of each species can hold a
+// non-zero count, so only its rows are cleared; the last block to finish (all
+// others have read the box by then) zeroes the box words and the counter.
+__global__ void __launch_bounds__(256)
+k_clear_counts(unsigned long long *__restrict__ acc, int nx, int ny, int nz) {
+    const long long G = (long long)nx * ny * nz;
+    unsigned *cbox = (unsigned *)(acc + 2 * G);
+    const int cw = cbox_species_words(nx);
+    unsigned *counter = cbox + 2 * cw;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+#pragma unroll
+    for (int sp = 0; sp < 2; sp++) {
+        const unsigned *cb = cbox + sp * cw;
+        const int i0 = cbox_lo(cb[0]), j0 = cbox_lo(cb[1]), k0 = cbox_lo(cb[2]);
+        const int i1 = cbox_hi(cb[3]), j1 = cbox_hi(cb[4]), k1 = cbox_hi(cb[5]);
+        if (i1 < i0) continue;
+        const int nj = j1 - j0 + 1, rows = (i1 - i0 + 1) * nj;
+        // one row (i, j) per wave, its columns k0..k1 over the lanes
+        for (int row = blockIdx.x * nw + wave; row < rows; row += gridDim.x * nw) {
+            const int i = i0 + row / nj, j = j0 + row % nj;
+            unsigned long long *rp = acc + sp * G + ((long long)i * ny + j) * nz;
+            for (int k = k0 + lane; k <= k1; k += kWave) rp[k] = 0ULL;
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    __shared__ bool last;
+    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (last) {
+        for (int w = threadIdx.x; w < 2 * cw; w += blockDim.x) cbox[w] = 0u;
+        if (threadIdx.x == 0) *counter = 0u;
+    }
 }
 
 // ---- Gaussian smearing (include/physics.hpp:203-234) ----------------------------
