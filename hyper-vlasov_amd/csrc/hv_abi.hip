@@ -81,7 +81,10 @@ struct hv_ctx {
     int smear_xs_vec = 1;               // two-pass x pass: VW nodes per thread (when nz allows)
     int xs_rxm = 0;                     // its radius bucket (0: radius too large for it)
     long long zy_blocks = 1024;         // workgroups of the zy pass (blocks per CU that its LDS allows x CUs)
-    unsigned clear_blocks = 512;        // workgroups of k_clear_counts
+    unsigned clear_blocks = 2048;       // workgroups of k_clear_counts and k_force_grid (one box row per wave)
+    unsigned *fbox = nullptr;           // the count boxes of the last smearing (2 x 6 words)
+    void *fgrid = nullptr;              // force grid (fx, fy, fz, 0) per node, T, within fbox (run_steps)
+    size_t fgrid_cap = 0;
     unsigned long long *maxd = nullptr; // per-iteration slot maxima of |dphi| (double bits)
     double *maxc = nullptr;             // per-iteration max |dphi|, collected for read-back
     int maxd_cap = 0;
@@ -1222,12 +1225,15 @@ int take_status(hv_ctx *c, int *bits) {
 }
 
 // ---- the fused loop ------------------------------------------------------------
+// zero_box: also zero the count box (the counts were cleared after the
+// smearing of this field solve)
 template <typename T>
-int potential_from_density(hv_ctx *c, const hv_skyrme *s, bool with_phi) {
+int potential_from_density(hv_ctx *c, const hv_skyrme *s, bool with_phi, bool zero_box = false) {
     hipEvent_t pa = prof_begin(c);
-    k_skyrme<T><<<blocks_for(c->G), kThreads, 0, c->stream>>>((const T *)c->density, (T *)c->U,
-                                                             with_phi ? c->phi : nullptr, s->A, s->B, s->C,
-                                                             s->gamma, c->G, c->sp);
+    const int nzero = 2 * cbox_species_words(c->world.n[0]);
+    k_skyrme<T><<<std::max(blocks_for(c->G), blocks_for(nzero)), kThreads, 0, c->stream>>>(
+        (const T *)c->density, (T *)c->U, with_phi ? c->phi : nullptr, s->A, s->B, s->C, s->gamma, c->G, c->sp,
+        zero_box ? (unsigned *)(c->acc + 2 * c->G) : nullptr, nzero);
     HV_CHECK_LAUNCH(c);
     prof_end(c, HV_KCLASS_POTENTIAL, pa, 1);
     return HV_OK;
@@ -1239,14 +1245,16 @@ int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, i
     int rc = smear<T>(c, true, p->sigma_r, p->part_per_nucleon);
     if (rc < 0) return rc;
     *warn |= rc;
-    // counts and their box back to zero: the box's rows only
-    k_clear_counts<<<c->clear_blocks, 256, 0, c->stream>>>(c->acc, c->world.n[0], c->world.n[1], c->world.n[2]);
+    // counts and their box back to zero: the box's rows only (the global
+    // boxes kept in fbox for the force grid)
+    k_clear_counts<<<c->clear_blocks, 256, 0, c->stream>>>(c->acc, c->world.n[0], c->world.n[1], c->world.n[2],
+                                                           c->fbox);
     HV_CHECK_LAUNCH(c);
     double md;
     rc = sor_solve<T>(c, (const T *)c->density, p->sor_omega, p->sor_tol, p->sor_max_it, iters, &md, true);
     if (rc < 0) return rc;
     *warn |= rc;
-    return potential_from_density<T>(c, s, true);
+    return potential_from_density<T>(c, s, true, true);
 }
 
 int check_params(hv_ctx *c, const hv_skyrme *s, const hv_params *p) {
@@ -1319,6 +1327,13 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
     if (rc) return rc;
     const unsigned nbp = blocks_for(c->total);
     int warn = 0;
+    // the force grid is valid only from a field solve of this call to the next
+    // particle pass (nothing else can have moved a particle or changed U then)
+    bool fgrid_ok = false;
+    if (nsteps > 1) {
+        rc = ensure_buf(c, &c->fgrid, &c->fgrid_cap, 4 * sizeof(T) * (size_t)(2 * c->G));
+        if (rc) return rc;
+    }
     for (int n = 0; n < nsteps; n++) {
         const int step = first + n;
         int boost = 0;
@@ -1330,10 +1345,18 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
         hipEvent_t pa = prof_begin(c);
         rc = maybe_sort<T>(c);
         if (rc) return rc;
-        k_particle_step<T, T><<<nbp, kThreads, 0, c->stream>>>(
-            particles<T>(c), c->np, c->total, g, (const T *)c->U, c->acc, kick_fac, drift_fac, n > 0 ? 1 : 0, boost, (T)bp,
-            (T)bn, c->partials, c->status);
+        // the force grid of the previous step's field solve when this call made
+        // it, else the force from U directly
+        if (fgrid_ok)
+            k_particle_step<T, T, true><<<nbp, kThreads, 0, c->stream>>>(
+                particles<T>(c), c->np, c->total, g, (const T *)c->U, (const T *)c->fgrid, c->acc, kick_fac,
+                drift_fac, n > 0 ? 1 : 0, boost, (T)bp, (T)bn, c->partials, c->status);
+        else
+            k_particle_step<T, T, false><<<nbp, kThreads, 0, c->stream>>>(
+                particles<T>(c), c->np, c->total, g, (const T *)c->U, nullptr, c->acc, kick_fac, drift_fac,
+                n > 0 ? 1 : 0, boost, (T)bp, (T)bn, c->partials, c->status);
         HV_CHECK_LAUNCH(c);
+        fgrid_ok = false;
         k_stats_finalize<<<1, kStatThreads, 0, c->stream>>>(c->partials, (int)nbp, c->slots + (n + 1));
         HV_CHECK_LAUNCH(c);
         prof_end(c, HV_KCLASS_PARTICLE, pa, 1);
@@ -1341,6 +1364,15 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
         rc = field_solve<T>(c, s, p, &it, &warn);
         if (rc < 0) return rc;
         if (sor_iters) sor_iters[n] = it;
+        if (n + 1 < nsteps) {
+            // F = -grad U at the cells the deposit above recorded: the nodes the
+            // next particle pass gathers from
+            hipEvent_t pf = prof_begin(c);
+            k_force_grid<T><<<c->clear_blocks, 256, 0, c->stream>>>((const T *)c->U, (T *)c->fgrid, c->fbox, g);
+            HV_CHECK_LAUNCH(c);
+            prof_end(c, HV_KCLASS_POTENTIAL, pf, 1);
+            fgrid_ok = true;
+        }
     }
     // end-of-step half kick of the last step, with the force stored (:101-103)
     k_gather_u<T, T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total, g,
@@ -1519,6 +1551,7 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
     ok = ok && hipMalloc((void **)&c->ctl, sizeof(SorCtl)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->status, sizeof(unsigned)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->partials, sizeof(long long) * 6 * (size_t)blocks_for(c->total)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->fbox, 12 * sizeof(unsigned)) == hipSuccess;
     if (!ok) {
         hv_ctx_destroy(c);
         return HV_ERR_ALLOC;
@@ -1561,7 +1594,7 @@ int hv_ctx_destroy(hv_ctx *c) {
                     c->sor_delta_alt, c->phi_h[0], c->phi_h[1], c->phi_h[2], c->phi_h[3], c->phi_h[4], c->D,
                     c->F[0], c->F[1], c->F[2],
                     c->smear_tab, c->maxd, c->maxc, c->ctl, c->slots, c->status, c->partials, c->sflags,
-                    c->en_partials, c->en_out};
+                    c->en_partials, c->en_out, c->fbox, c->fgrid};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->pinned) (void)hipHostFree(c->pinned);

@@ -80,7 +80,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_particle_step(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, int end_kick) {
     HV_BATCH_LOCATE();
-    particle_step_body<T, T>(bid, A.p, A.np, A.total, A.g, A.U, A.acc, A.kick_fac, A.drift_fac, end_kick, A.boost,
+    particle_step_body<T, T, false>(bid, A.p, A.np, A.total, A.g, A.U, nullptr, A.acc, A.kick_fac, A.drift_fac, end_kick, A.boost,
                              A.boost_p, A.boost_n, A.partials, A.status);
 }
 

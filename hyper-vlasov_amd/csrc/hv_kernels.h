@@ -127,9 +127,8 @@ __device__ __forceinline__ void deposit_particle(const Cell<T> &c, long long off
 // larger than the tight box, never smaller).  Grids are at most 65535 nodes
 // along an axis.
 __host__ __device__ inline int cbox_species_words(int nx) { return 6 + 4 * nx; }
-// u64 words after the 2G counts that hold both species' boxes and, after
-// them, the arrival counter of k_clear_counts
-__host__ __device__ inline long long cbox_acc_words(int nx) { return (2LL * cbox_species_words(nx) + 2) / 2; }
+// u64 words after the 2G counts that hold both species' boxes
+__host__ __device__ inline long long cbox_acc_words(int nx) { return (2LL * cbox_species_words(nx) + 1) / 2; }
 __device__ __forceinline__ int cbox_lo(unsigned w) { return 0xffff - (int)w; }  // empty: 65535
 __device__ __forceinline__ int cbox_hi(unsigned w) { return (int)w - 1; }        // empty: -1
 
@@ -299,6 +298,67 @@ __global__ void k_iota(int *__restrict__ a, int n) {
     if (s < n) a[s] = s;
 }
 
+// ---- force grid: F = -grad U at the nodes a particle pass gathers from ---------
+// fdm_force (the reference's compute_volumetric_forces_fdm,
+// include/physics.hpp:160-200) at every node of each species' count box (fbox:
+// the deposit's box of the particles' cells, copied by k_clear_counts), stored
+// as (fx, fy, fz, 0) per node so that a gather takes one aligned vector load
+// per corner.  Nodes outside the box are left as they are: no particle of the
+// pass that follows has a corner there.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_force_grid(const T *__restrict__ U, T *__restrict__ F4, const unsigned *__restrict__ fbox, Geom<T> g) {
+    typedef T V4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+#pragma unroll
+    for (int sp = 0; sp < 2; sp++) {
+        const unsigned *cb = fbox + 6 * sp;
+        const int i0 = cbox_lo(cb[0]), j0 = cbox_lo(cb[1]), k0 = cbox_lo(cb[2]);
+        const int i1 = cbox_hi(cb[3]), j1 = cbox_hi(cb[4]), k1 = cbox_hi(cb[5]);
+        if (i1 < i0) continue;
+        const int nj = j1 - j0 + 1, rows = (i1 - i0 + 1) * nj;
+        const T *u = U + sp * g.G;
+        V4 *f = (V4 *)F4 + sp * g.G;
+        for (int row = blockIdx.x * nw + wave; row < rows; row += gridDim.x * nw) {
+            const int i = i0 + row / nj, j = j0 + row % nj;
+            for (int k = k0 + lane; k <= k1; k += kWave) {
+                T fx, fy, fz;
+                fdm_force<T, T>(u, i, j, k, g, fx, fy, fz);
+                f[idx3(i, j, k, g.ny, g.nz)] = V4{fx, fy, fz, T(0.0)};
+            }
+        }
+    }
+}
+
+// CIC gather of the force from the force grid: the values and the order of
+// gather_force_from_u (F4s: the species' part of the grid)
+template <typename T>
+__device__ __forceinline__ void gather_force_from_f(const T *__restrict__ F4s, const Cell<T> &c, const Geom<T> &g,
+                                                    T &ax, T &ay, T &az) {
+    typedef T V4 __attribute__((ext_vector_type(4)));
+    const V4 *f = (const V4 *)F4s;
+    ax = T(0.0);
+    ay = T(0.0);
+    az = T(0.0);
+    V4 fv[8];
+    // the eight loads first (32-bit node offsets: G < 2^31)
+#pragma unroll
+    for (int corner = 0; corner < 8; corner++) {
+        const int i = c.x0 + (corner & 1), j = c.y0 + ((corner >> 1) & 1), k = c.z0 + ((corner >> 2) & 1);
+        const bool in = i < g.nx && j < g.ny && k < g.nz;
+        fv[corner] = in ? f[(unsigned)((i * g.ny + j) * g.nz + k)] : V4{T(0.0), T(0.0), T(0.0), T(0.0)};
+    }
+#pragma unroll
+    for (int corner = 0; corner < 8; corner++) {
+        const int i = c.x0 + (corner & 1), j = c.y0 + ((corner >> 1) & 1), k = c.z0 + ((corner >> 2) & 1);
+        if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
+        const T w = corner_weight(c, corner);
+        ax += w * fv[corner].x;
+        ay += w * fv[corner].y;
+        az += w * fv[corner].z;
+    }
+}
+
 // ---- fused particle pass --------------------------------------------------
 // One step of the particle side of the loop, for positions x_s and the field U
 // built from x_s (src/hyper_vlasov.cpp:86-93 and :101-103):
@@ -310,9 +370,11 @@ __global__ void k_iota(int *__restrict__ a, int n) {
 //   deposit x_{s+1} into acc, diagnostics of x_{s+1} into *slot.
 // The intermediate k of the two half kicks is rounded exactly as in the
 // reference's two separate updates.
-template <typename T, typename UT>
+// FG: the force comes from the force grid F4 (computed from the same U for the
+// particles' cells), else from U directly; the same bits either way.
+template <typename T, typename UT, bool FG = false>
 __device__ __forceinline__ void particle_step_body(const int bid, Particles<T> p, int np, int total, Geom<T> g,
-                                                   const UT *__restrict__ U,
+                                                   const UT *__restrict__ U, const T *__restrict__ F4,
                 unsigned long long *__restrict__ acc, T kick_fac, T drift_fac, int end_kick,
                 int boost, T boost_p, T boost_n, long long *partials, unsigned *status) {
     const int i = bid * blockDim.x + threadIdx.x;
@@ -325,7 +387,10 @@ __device__ __forceinline__ void particle_step_body(const int bid, Particles<T> p
         T x = p.x[i], y = p.y[i], z = p.z[i];
         T kx = p.kx[i], ky = p.ky[i], kz = p.kz[i];
         T fx = T(0.0), fy = T(0.0), fz = T(0.0);
-        if (cic_cell(x, y, z, g, c)) gather_force_from_u<T, UT>(U + off, c, g, fx, fy, fz);
+        if (cic_cell(x, y, z, g, c)) {
+            if constexpr (FG) gather_force_from_f<T>(F4 + 4 * off, c, g, fx, fy, fz);
+            else gather_force_from_u<T, UT>(U + off, c, g, fx, fy, fz);
+        }
         if (end_kick) {
             kx += kick_fac * fx;
             ky += kick_fac * fy;
@@ -351,13 +416,13 @@ __device__ __forceinline__ void particle_step_body(const int bid, Particles<T> p
     stat_commit(v, partials, bid);
 }
 
-template <typename T, typename UT>
+template <typename T, typename UT, bool FG>
 __global__ void __launch_bounds__(kBlock)
-k_particle_step(Particles<T> p, int np, int total, Geom<T> g, const UT *__restrict__ U,
+k_particle_step(Particles<T> p, int np, int total, Geom<T> g, const UT *__restrict__ U, const T *__restrict__ F4,
                 unsigned long long *__restrict__ acc, T kick_fac, T drift_fac, int end_kick,
                 int boost, T boost_p, T boost_n, long long *partials, unsigned *status) {
-    particle_step_body<T, UT>(blockIdx.x, p, np, total, g, U, acc, kick_fac, drift_fac, end_kick, boost, boost_p,
-                              boost_n, partials, status);
+    particle_step_body<T, UT, FG>(blockIdx.x, p, np, total, g, U, F4, acc, kick_fac, drift_fac, end_kick, boost,
+                                  boost_p, boost_n, partials, status);
 }
 
 // deposit (and optionally diagnostics) of the current positions, no motion
@@ -422,14 +487,15 @@ __global__ void k_fill(V *__restrict__ p, V v, long long n) {
 }
 
 // counts = 0 after a smearing: only the count box of each species can hold a
-// non-zero count, so only its rows are cleared; the last block to finish (all
-// others have read the box by then) zeroes the box words and the counter.
+// non-zero count, so only its rows are cleared (block 0 first copies the
+// global boxes to fbox when given).  The box words themselves are zeroed by a
+// later kernel of the step (k_skyrme's zero_words): every block here reads them.
 __global__ void __launch_bounds__(256)
-k_clear_counts(unsigned long long *__restrict__ acc, int nx, int ny, int nz) {
+k_clear_counts(unsigned long long *__restrict__ acc, int nx, int ny, int nz, unsigned *__restrict__ fbox) {
     const long long G = (long long)nx * ny * nz;
-    unsigned *cbox = (unsigned *)(acc + 2 * G);
+    const unsigned *cbox = (const unsigned *)(acc + 2 * G);
     const int cw = cbox_species_words(nx);
-    unsigned *counter = cbox + 2 * cw;
+    if (fbox && blockIdx.x == 0 && threadIdx.x < 12) fbox[threadIdx.x] = cbox[(threadIdx.x / 6) * cw + threadIdx.x % 6];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
 #pragma unroll
     for (int sp = 0; sp < 2; sp++) {
@@ -444,15 +510,6 @@ k_clear_counts(unsigned long long *__restrict__ acc, int nx, int ny, int nz) {
             unsigned long long *rp = acc + sp * G + ((long long)i * ny + j) * nz;
             for (int k = k0 + lane; k <= k1; k += kWave) rp[k] = 0ULL;
         }
-    }
-    __threadfence();
-    __syncthreads();
-    __shared__ bool last;
-    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (last) {
-        for (int w = threadIdx.x; w < 2 * cw; w += blockDim.x) cbox[w] = 0u;
-        if (threadIdx.x == 0) *counter = 0u;
     }
 }
 
@@ -1296,10 +1353,16 @@ __device__ __forceinline__ void skyrme_body(const int bid, const T *__restrict__
     U[i + G] = (T)un;
 }
 
+// zero_words (optional): n_zero words zeroed on the side (the count box after
+// the counts were cleared, ready for the next deposit)
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_skyrme(const T *__restrict__ rho, T *__restrict__ U, const double *__restrict__ phi, double A, double B,
-         double C, double gamma, long long G, Split sp) {
+         double C, double gamma, long long G, Split sp, unsigned *__restrict__ zero_words, int n_zero) {
+    if (zero_words) {
+        const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (q < n_zero) zero_words[q] = 0u;
+    }
     skyrme_body<T>(blockIdx.x, rho, U, phi, A, B, C, gamma, G, sp);
 }
 
