@@ -83,6 +83,7 @@ struct hv_ctx {
     long long zy_blocks = 1024;         // workgroups of the zy pass (blocks per CU that its LDS allows x CUs)
     unsigned clear_blocks = 2048;       // workgroups of k_clear_counts and k_force_grid (one box row per wave)
     unsigned *fbox = nullptr;           // the count boxes of the last smearing (2 x 6 words)
+    int *brec = nullptr;                // per-block box records of the last deposit (kBoxRec ints per block)
     void *fgrid = nullptr;              // force grid (fx, fy, fz, 0) per node, T, within fbox (run_steps)
     size_t fgrid_cap = 0;
     unsigned long long *maxd = nullptr; // per-iteration slot maxima of |dphi| (double bits)
@@ -1225,15 +1226,12 @@ int take_status(hv_ctx *c, int *bits) {
 }
 
 // ---- the fused loop ------------------------------------------------------------
-// zero_box: also zero the count box (the counts were cleared after the
-// smearing of this field solve)
 template <typename T>
-int potential_from_density(hv_ctx *c, const hv_skyrme *s, bool with_phi, bool zero_box = false) {
+int potential_from_density(hv_ctx *c, const hv_skyrme *s, bool with_phi) {
     hipEvent_t pa = prof_begin(c);
-    const int nzero = 2 * cbox_species_words(c->world.n[0]);
-    k_skyrme<T><<<std::max(blocks_for(c->G), blocks_for(nzero)), kThreads, 0, c->stream>>>(
-        (const T *)c->density, (T *)c->U, with_phi ? c->phi : nullptr, s->A, s->B, s->C, s->gamma, c->G, c->sp,
-        zero_box ? (unsigned *)(c->acc + 2 * c->G) : nullptr, nzero);
+    k_skyrme<T><<<blocks_for(c->G), kThreads, 0, c->stream>>>((const T *)c->density, (T *)c->U,
+                                                             with_phi ? c->phi : nullptr, s->A, s->B, s->C,
+                                                             s->gamma, c->G, c->sp);
     HV_CHECK_LAUNCH(c);
     prof_end(c, HV_KCLASS_POTENTIAL, pa, 1);
     return HV_OK;
@@ -1242,6 +1240,10 @@ int potential_from_density(hv_ctx *c, const hv_skyrme *s, bool with_phi, bool ze
 // deposit already in c->acc -> rho -> SOR -> U (src/hyper_vlasov.cpp:94-98)
 template <typename T>
 int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, int *warn) {
+    // the count box of the deposit, from its per-block records
+    k_cbox_build<<<2, kCboxThreads, 0, c->stream>>>(c->brec, (int)blocks_for(c->total),
+                                                     (unsigned *)(c->acc + 2 * c->G));
+    HV_CHECK_LAUNCH(c);
     int rc = smear<T>(c, true, p->sigma_r, p->part_per_nucleon);
     if (rc < 0) return rc;
     *warn |= rc;
@@ -1254,7 +1256,7 @@ int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, i
     rc = sor_solve<T>(c, (const T *)c->density, p->sor_omega, p->sor_tol, p->sor_max_it, iters, &md, true);
     if (rc < 0) return rc;
     *warn |= rc;
-    return potential_from_density<T>(c, s, true, true);
+    return potential_from_density<T>(c, s, true);
 }
 
 int check_params(hv_ctx *c, const hv_skyrme *s, const hv_params *p) {
@@ -1284,7 +1286,7 @@ int run_init(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *sor_iters) 
         if (rc) return rc;
     }
     k_deposit<T><<<blocks_for(c->total), kThreads, 0, c->stream>>>(particles<T>(c), c->np, c->total, make_geom<T>(c),
-                                                                  c->acc, nullptr, c->status);
+                                                                  c->acc, nullptr, c->status, c->brec);
     HV_CHECK_LAUNCH(c);
     int warn = 0, it = 0;
     rc = field_solve<T>(c, s, p, &it, &warn);
@@ -1349,12 +1351,12 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
         // it, else the force from U directly
         if (fgrid_ok)
             k_particle_step<T, T, true><<<nbp, kThreads, 0, c->stream>>>(
-                particles<T>(c), c->np, c->total, g, (const T *)c->U, (const T *)c->fgrid, c->acc, kick_fac,
-                drift_fac, n > 0 ? 1 : 0, boost, (T)bp, (T)bn, c->partials, c->status);
+                particles<T>(c), c->np, c->total, g, (const T *)c->U, (const T *)c->fgrid, c->acc, c->brec,
+                kick_fac, drift_fac, n > 0 ? 1 : 0, boost, (T)bp, (T)bn, c->partials, c->status);
         else
             k_particle_step<T, T, false><<<nbp, kThreads, 0, c->stream>>>(
-                particles<T>(c), c->np, c->total, g, (const T *)c->U, nullptr, c->acc, kick_fac, drift_fac,
-                n > 0 ? 1 : 0, boost, (T)bp, (T)bn, c->partials, c->status);
+                particles<T>(c), c->np, c->total, g, (const T *)c->U, nullptr, c->acc, c->brec, kick_fac,
+                drift_fac, n > 0 ? 1 : 0, boost, (T)bp, (T)bn, c->partials, c->status);
         HV_CHECK_LAUNCH(c);
         fgrid_ok = false;
         k_stats_finalize<<<1, kStatThreads, 0, c->stream>>>(c->partials, (int)nbp, c->slots + (n + 1));
@@ -1552,6 +1554,7 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
     ok = ok && hipMalloc((void **)&c->status, sizeof(unsigned)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->partials, sizeof(long long) * 6 * (size_t)blocks_for(c->total)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->fbox, 12 * sizeof(unsigned)) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->brec, kBoxRec * sizeof(int) * (size_t)blocks_for(c->total)) == hipSuccess;
     if (!ok) {
         hv_ctx_destroy(c);
         return HV_ERR_ALLOC;
@@ -1594,7 +1597,7 @@ int hv_ctx_destroy(hv_ctx *c) {
                     c->sor_delta_alt, c->phi_h[0], c->phi_h[1], c->phi_h[2], c->phi_h[3], c->phi_h[4], c->D,
                     c->F[0], c->F[1], c->F[2],
                     c->smear_tab, c->maxd, c->maxc, c->ctl, c->slots, c->status, c->partials, c->sflags,
-                    c->en_partials, c->en_out, c->fbox, c->fgrid};
+                    c->en_partials, c->en_out, c->fbox, c->fgrid, c->brec};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->pinned) (void)hipHostFree(c->pinned);

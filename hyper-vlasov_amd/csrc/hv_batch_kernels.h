@@ -80,7 +80,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_particle_step(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, int end_kick) {
     HV_BATCH_LOCATE();
-    particle_step_body<T, T, false>(bid, A.p, A.np, A.total, A.g, A.U, nullptr, A.acc, A.kick_fac, A.drift_fac, end_kick, A.boost,
+    particle_step_body<T, T, false>(bid, A.p, A.np, A.total, A.g, A.U, nullptr, A.acc, nullptr, A.kick_fac, A.drift_fac, end_kick, A.boost,
                              A.boost_p, A.boost_n, A.partials, A.status);
 }
 
@@ -100,8 +100,8 @@ template <typename T>
 __global__ void __launch_bounds__(kSmearThreads)
 kb_smear_zy(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
     HV_BATCH_LOCATE();
-    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.fmask, (const unsigned *)(A.acc + 2 * A.G),
-                                         cbox_species_words(A.g.nx), smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st2);
+    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.fmask, nullptr, 0, smear_tab_c(A.tab), A.g.nx,
+                                         A.g.ny, A.g.nz, A.st2);
 }
 
 template <typename T>

@@ -119,42 +119,79 @@ __device__ __forceinline__ void deposit_particle(const Cell<T> &c, long long off
 
 // ---- count box ------------------------------------------------------------------
 // The node ranges that a deposit may have touched, kept in unsigned words after
-// the 2G fixed-point counts (so every clear of the counts clears it too).  Per
-// species 6 + 4 nx words: [0..2] 0xffff - the least i, j, k; [3..5] the
-// greatest i, j, k + 1; then per plane i: 0xffff - least j, greatest j + 1,
-// 0xffff - least k, greatest k + 1.  Every word only grows (atomicMax), so a
-// zeroed box is empty and a filled one bounds every non-zero count (it may be
-// larger than the tight box, never smaller).  Grids are at most 65535 nodes
-// along an axis.
-__host__ __device__ inline int cbox_species_words(int nx) { return 6 + 4 * nx; }
+// the 2G fixed-point counts.  Per species kCboxWords = 6 words: 0xffff - the
+// least i, j, k, then the greatest i, j, k + 1; all zero for an empty box.  The
+// box bounds every non-zero count (it may be larger than the tight box, never
+// smaller).  Grids are at most 65535 nodes along an axis.
+constexpr int kCboxWords = 6;
+__host__ __device__ inline int cbox_species_words(int) { return kCboxWords; }
 // u64 words after the 2G counts that hold both species' boxes
-__host__ __device__ inline long long cbox_acc_words(int nx) { return (2LL * cbox_species_words(nx) + 1) / 2; }
+__host__ __device__ inline long long cbox_acc_words(int) { return kCboxWords; }
 __device__ __forceinline__ int cbox_lo(unsigned w) { return 0xffff - (int)w; }  // empty: 65535
 __device__ __forceinline__ int cbox_hi(unsigned w) { return (int)w - 1; }        // empty: -1
 
-__device__ __forceinline__ void cbox_raise(unsigned *w, unsigned v) {
-    // (a relaxed read first: once the box has grown, most blocks need no atomic)
-    if (v > __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(w, v);
-}
+// A deposit writes one record per block, [s0, s1, x0, x1, y0, y1, z0, z1]:
+// the species s0..s1 and the node box x0..x1, y0..y1, z0..z1 its particles'
+// cells touch (x1 < x0: none); k_cbox_build folds the records into the box
+// words before the smearing, rewriting them.
+constexpr int kBoxRec = 8;
 
-// the nodes x0..x1, y0..y1, z0..z1 of species s0..s1 may hold counts
-__device__ __forceinline__ void cbox_record(unsigned *__restrict__ cbox, int nx, int s0, int s1, int x0, int x1,
-                                            int y0, int y1, int z0, int z1) {
-    const int nw = 6 + 4 * (x1 - x0 + 1);
-    for (int s = s0; s <= s1; s++) {
-        unsigned *cb = cbox + (long long)s * cbox_species_words(nx);
-        for (int w = threadIdx.x; w < nw; w += blockDim.x) {
-            if (w < 6) {
-                const unsigned v = w == 0 ? 0xffffu - x0 : w == 1 ? 0xffffu - y0 : w == 2 ? 0xffffu - z0
-                                 : w == 3 ? (unsigned)x1 + 1 : w == 4 ? (unsigned)y1 + 1 : (unsigned)z1 + 1;
-                cbox_raise(cb + w, v);
-            } else {
-                const int pl = x0 + (w - 6) / 4, q = (w - 6) & 3;
-                const unsigned v = q == 0 ? 0xffffu - y0 : q == 1 ? (unsigned)y1 + 1 : q == 2 ? 0xffffu - z0
-                                                                                   : (unsigned)z1 + 1;
-                cbox_raise(cb + 6 + 4 * pl + q, v);
-            }
+// One block of kCboxThreads per species: the box of every record of it, four
+// record loads in flight per thread.
+constexpr int kCboxThreads = 1024;
+
+__global__ void __launch_bounds__(kCboxThreads)
+k_cbox_build(const int *__restrict__ brec, int nrec, unsigned *__restrict__ cbox) {
+    const int sp = blockIdx.x;
+    constexpr int kBig = 0x7fffffff;
+    int lo[3] = {kBig, kBig, kBig}, hi[3] = {-1, -1, -1};
+    for (int r0 = threadIdx.x; r0 < nrec; r0 += 4 * kCboxThreads) {
+        int4 ra[4], rb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int r = r0 + u * kCboxThreads;
+            ra[u] = r < nrec ? *(const int4 *)(brec + kBoxRec * r) : int4{0, -1, 0, -1};
+            rb[u] = r < nrec ? *(const int4 *)(brec + kBoxRec * r + 4) : int4{0, -1, 0, -1};
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int4 a = ra[u], c = rb[u];
+            if (a.w < a.z || sp < a.x || sp > a.y) continue;
+            lo[0] = min(lo[0], a.z);
+            hi[0] = max(hi[0], a.w);
+            lo[1] = min(lo[1], c.x);
+            hi[1] = max(hi[1], c.y);
+            lo[2] = min(lo[2], c.z);
+            hi[2] = max(hi[2], c.w);
+        }
+    }
+    __shared__ int red[6][kCboxThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        int l = lo[q], h = hi[q];
+#pragma unroll
+        for (int s = kWave / 2; s > 0; s >>= 1) {
+            l = min(l, __shfl_xor(l, s, kWave));
+            h = max(h, __shfl_xor(h, s, kWave));
+        }
+        if (lane == 0) {
+            red[q][wave] = l;
+            red[3 + q][wave] = h;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int q = threadIdx.x;
+        int v = red[q][0];
+        for (int w = 1; w < kCboxThreads / kWave; w++) v = q < 3 ? min(v, red[q][w]) : max(v, red[q][w]);
+        red[q][0] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int q = threadIdx.x;
+        const bool empty = red[3][0] < red[0][0];
+        cbox[sp * kCboxWords + q] = empty ? 0u : q < 3 ? 0xffffu - (unsigned)red[q][0] : (unsigned)red[q][0] + 1u;
     }
 }
 
@@ -181,7 +218,8 @@ __device__ __forceinline__ int wave_max_i(int v) {
 
 template <typename T>
 __device__ __forceinline__ void block_deposit(bool has, const Cell<T> &c, int species, const Geom<T> &g,
-                                              unsigned long long *__restrict__ acc) {
+                                              unsigned long long *__restrict__ acc, int *__restrict__ brec,
+                                              int bid) {
     __shared__ unsigned long long box[kBoxCap];
     __shared__ int red[8][16];
     __shared__ int bb[8];
@@ -212,10 +250,18 @@ __device__ __forceinline__ void block_deposit(bool has, const Cell<T> &c, int sp
     __syncthreads();
     const int x0 = bb[0], y0 = bb[1], z0 = bb[2];
     const int ex = bb[4] - x0 + 2, ey = bb[5] - y0 + 2, ez = bb[6] - z0 + 2;
+    // the block's record: its species and the nodes of its cells (none: x1 < x0)
+    if (brec && threadIdx.x == 0) {
+        int4 *r = (int4 *)(brec + kBoxRec * bid);
+        if (bb[4] < 0) {
+            r[0] = int4{0, -1, 0, -1};
+            r[1] = int4{0, -1, 0, -1};
+        } else {
+            r[0] = int4{bb[3], bb[7], x0, min(bb[4] + 1, g.nx - 1)};
+            r[1] = int4{y0, min(bb[5] + 1, g.ny - 1), z0, min(bb[6] + 1, g.nz - 1)};
+        }
+    }
     if (bb[4] < 0) return;  // no particle of this block is inside the grid
-    // the count box (after the 2G counts) takes the nodes of the block's cells
-    cbox_record((unsigned *)(acc + 2 * g.G), g.nx, bb[3], bb[7], x0, min(bb[4] + 1, g.nx - 1), y0,
-                min(bb[5] + 1, g.ny - 1), z0, min(bb[6] + 1, g.nz - 1));
     const long long off = bb[3] ? g.G : 0;
     const bool lds = (bb[3] == bb[7]) && (long long)ex * ey * ez <= kBoxCap;
     if (!lds) {
@@ -375,8 +421,8 @@ __device__ __forceinline__ void gather_force_from_f(const T *__restrict__ F4s, c
 template <typename T, typename UT, bool FG = false>
 __device__ __forceinline__ void particle_step_body(const int bid, Particles<T> p, int np, int total, Geom<T> g,
                                                    const UT *__restrict__ U, const T *__restrict__ F4,
-                unsigned long long *__restrict__ acc, T kick_fac, T drift_fac, int end_kick,
-                int boost, T boost_p, T boost_n, long long *partials, unsigned *status) {
+                unsigned long long *__restrict__ acc, int *__restrict__ brec, T kick_fac, T drift_fac,
+                int end_kick, int boost, T boost_p, T boost_n, long long *partials, unsigned *status) {
     const int i = bid * blockDim.x + threadIdx.x;
     long long v[6] = {0, 0, 0, 0, 0, 0};
     Cell<T> c;
@@ -412,24 +458,24 @@ __device__ __forceinline__ void particle_step_body(const int bid, Particles<T> p
         if (!has) atomicOr(status, kStatusOutside);
         stat_values(x, y, z, proton, g, v);
     }
-    block_deposit(has, c, i < np ? 0 : 1, g, acc);
+    block_deposit(has, c, i < np ? 0 : 1, g, acc, brec, bid);
     stat_commit(v, partials, bid);
 }
 
 template <typename T, typename UT, bool FG>
 __global__ void __launch_bounds__(kBlock)
 k_particle_step(Particles<T> p, int np, int total, Geom<T> g, const UT *__restrict__ U, const T *__restrict__ F4,
-                unsigned long long *__restrict__ acc, T kick_fac, T drift_fac, int end_kick,
+                unsigned long long *__restrict__ acc, int *__restrict__ brec, T kick_fac, T drift_fac, int end_kick,
                 int boost, T boost_p, T boost_n, long long *partials, unsigned *status) {
-    particle_step_body<T, UT, FG>(blockIdx.x, p, np, total, g, U, F4, acc, kick_fac, drift_fac, end_kick, boost,
-                                  boost_p, boost_n, partials, status);
+    particle_step_body<T, UT, FG>(blockIdx.x, p, np, total, g, U, F4, acc, brec, kick_fac, drift_fac, end_kick,
+                                  boost, boost_p, boost_n, partials, status);
 }
 
 // deposit (and optionally diagnostics) of the current positions, no motion
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_deposit(Particles<T> p, int np, int total, Geom<T> g, unsigned long long *__restrict__ acc,
-          long long *partials, unsigned *status) {
+          long long *partials, unsigned *status, int *__restrict__ brec = nullptr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     long long v[6] = {0, 0, 0, 0, 0, 0};
     Cell<T> c;
@@ -441,7 +487,7 @@ k_deposit(Particles<T> p, int np, int total, Geom<T> g, unsigned long long *__re
         if (acc && !has) atomicOr(status, kStatusOutside);
         stat_values(x, y, z, proton, g, v);
     }
-    if (acc) block_deposit(has, c, i < np ? 0 : 1, g, acc);
+    if (acc) block_deposit(has, c, i < np ? 0 : 1, g, acc, brec, blockIdx.x);
     if (partials) stat_commit(v, partials, blockIdx.x);
 }
 
@@ -488,8 +534,7 @@ __global__ void k_fill(V *__restrict__ p, V v, long long n) {
 
 // counts = 0 after a smearing: only the count box of each species can hold a
 // non-zero count, so only its rows are cleared (block 0 first copies the
-// global boxes to fbox when given).  The box words themselves are zeroed by a
-// later kernel of the step (k_skyrme's zero_words): every block here reads them.
+// global boxes to fbox when given).
 __global__ void __launch_bounds__(256)
 k_clear_counts(unsigned long long *__restrict__ acc, int nx, int ny, int nz, unsigned *__restrict__ fbox) {
     const long long G = (long long)nx * ny * nz;
@@ -881,7 +926,7 @@ __device__ __forceinline__ T count_of(CT v) {
 __host__ __device__ inline int smear_mask_words(int nx) { return (nx + 31) / 32; }
 
 // one (species sp, plane i, tile jt, kt) item of the z and y factors; pw: the
-// four count-box words of plane i of species sp (null: every count is read).
+// count-box words of species sp (null: every count is read).
 // Every branch is uniform over the block.
 // A workgroup barrier that orders LDS accesses only: global loads issued
 // before it stay in flight, so a prefetch survives it (__syncthreads fences
@@ -985,13 +1030,13 @@ __device__ __forceinline__ void smear_zy_item(int sp, int i, int jt, int kt, con
     const int j0 = jt * TJ, k0 = kt * TK;
     unsigned *mw = fmask + (((long long)sp * st.njt + jt) * st.nkt + kt) * smear_mask_words(nx) + (i >> 5);
     const unsigned bit = 1u << (i & 31);
-    // the count box of plane i: rows pj0..pj1, columns pk0..pk1 may hold counts
+    // the count box of the species: rows pj0..pj1, columns pk0..pk1 may hold counts
     int pj0 = 0, pj1 = ny - 1, pk0 = 0, pk1 = nz - 1;
     if (pw) {
-        pj0 = cbox_lo(pw[0]);
-        pj1 = cbox_hi(pw[1]);
+        pj0 = cbox_lo(pw[1]);
+        pj1 = cbox_hi(pw[4]);
         pk0 = cbox_lo(pw[2]);
-        pk1 = cbox_hi(pw[3]);
+        pk1 = cbox_hi(pw[5]);
     }
     if (pj1 < pj0 || pj0 > j0 + TJ - 1 + ry || pj1 < j0 - ry || pk0 > k0 + TK - 1 + rz || pk1 < k0 - rz) {
         if (threadIdx.x == 0) atomicAnd(mw, ~bit);
@@ -1096,8 +1141,8 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
     t /= st.njt;
     const int i = t % nx;
     const int sp = t / nx;
-    smear_zy_item<T, CT>(sp, i, jt, kt, cnt, D, fmask, cbox ? cbox + (long long)sp * cw + 6 + 4 * i : nullptr, tb,
-                         nx, ny, nz, st);
+    smear_zy_item<T, CT>(sp, i, jt, kt, cnt, D, fmask, cbox ? cbox + (long long)sp * cw : nullptr, tb, nx, ny, nz,
+                         st);
 }
 
 // The items of the count box only (planes that hold counts times the tiles
@@ -1151,8 +1196,8 @@ k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned *__restrict__
         for (int it = blockIdx.x; it < nall; it += gridDim.x) {
             int sp, i, jt, kt;
             decode(it, sp, i, jt, kt);
-            smear_zy_item<T, CT>(sp, i, jt, kt, cnt, D, fmask, cbox ? cbox + (long long)sp * cw + 6 + 4 * i : nullptr,
-                                 tb, nx, ny, nz, st, ecoord);
+            smear_zy_item<T, CT>(sp, i, jt, kt, cnt, D, fmask, cbox ? cbox + (long long)sp * cw : nullptr, tb, nx,
+                                 ny, nz, st, ecoord);
             __syncthreads();  // the next item's counts overwrite C and H
         }
         return;
@@ -1171,11 +1216,11 @@ k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned *__restrict__
         const int j0 = jt * st.TJ, k0 = kt * st.TK;
         int pj0 = 0, pj1 = ny - 1, pk0 = 0, pk1 = nz - 1;
         if (cbox) {
-            const unsigned *pw = cbox + (long long)sp * cw + 6 + 4 * i;
-            pj0 = cbox_lo(pw[0]);
-            pj1 = cbox_hi(pw[1]);
+            const unsigned *pw = cbox + (long long)sp * cw;
+            pj0 = cbox_lo(pw[1]);
+            pj1 = cbox_hi(pw[4]);
             pk0 = cbox_lo(pw[2]);
-            pk1 = cbox_hi(pw[3]);
+            pk1 = cbox_hi(pw[5]);
         }
         hit = !(pj1 < pj0 || pj0 > j0 + st.TJ - 1 + ry || pj1 < j0 - ry || pk0 > k0 + st.TK - 1 + rz ||
                 pk1 < k0 - rz);
@@ -1353,16 +1398,10 @@ __device__ __forceinline__ void skyrme_body(const int bid, const T *__restrict__
     U[i + G] = (T)un;
 }
 
-// zero_words (optional): n_zero words zeroed on the side (the count box after
-// the counts were cleared, ready for the next deposit)
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_skyrme(const T *__restrict__ rho, T *__restrict__ U, const double *__restrict__ phi, double A, double B,
-         double C, double gamma, long long G, Split sp, unsigned *__restrict__ zero_words, int n_zero) {
-    if (zero_words) {
-        const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-        if (q < n_zero) zero_words[q] = 0u;
-    }
+         double C, double gamma, long long G, Split sp) {
     skyrme_body<T>(blockIdx.x, rho, U, phi, A, B, C, gamma, G, sp);
 }
 
