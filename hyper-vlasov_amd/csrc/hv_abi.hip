@@ -875,15 +875,6 @@ int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double
     return HV_OK;
 }
 
-// The predictor a loop solve uses: the chosen one once the ring holds its
-// points (hist_n counts phi and the ring's converged solves), else a line
-// through the last two, else none.
-inline int sor_predictor_for(int chosen, int hist_n) {
-    double w[5];
-    const int need = sor_predictor(chosen, w);
-    if (hist_n >= need) return chosen;
-    return hist_n >= 2 ? 1 : 0;
-}
 
 // The reference's SOR on c->phi (FP64), one launch per iteration (fused) or
 // per sweep.  The fused form alternates phi and phi_alt; the result is
@@ -945,15 +936,6 @@ SorTiles make_sor_tiles(const hv_ctx *c, int planes = kSorPlanes) {
     const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
     tl.per_xcd = (ntiles + kXcds - 1) / kXcds;
     return tl;
-}
-
-// Iteration budget of the FP64 sweeps that end a mixed solve: what is left of
-// max_it after the FP32 correction iterations, and at least a few sweeps, so
-// that a solve that already spent its budget in FP32 stops (not converged)
-// about where the FP64 form would.
-inline int mixed_fp64_budget(int max_it, int iters32) {
-    const int left = max_it - iters32;
-    return left > 4 ? left : 4;
 }
 
 // Red-black SOR on c->phi with the proton density (first G values of `rho`).

@@ -19,6 +19,7 @@
 
 #include "hv_batch_kernels.h"
 #include "hv_batch_resident.h"
+#include "hv_batch_steer.h"
 
 enum { kBPart, kBSmear, kBZy, kBN2, kBG, kBGh2, kBSor, kBKinds };
 
@@ -70,6 +71,20 @@ struct hv_batch {
     int *hrjob = nullptr;                 // pinned staging of rjob
     float *hrres = nullptr;               // pinned read-back of rres and rerr
     void *hrinst = nullptr;               // pinned staging of rinst
+    // device-steered mixed step (the FP64 finishing sweeps and the solve's
+    // bookkeeping on the device; the host reads back every kReplanSteps steps)
+    int steer = 1;                        // option HV_OPT_BATCH_DEVICE_STEER
+    int fin_state = 0;                    // 0: not set up, 1: ready, -1: unavailable
+    int fin_nblk = 0;                     // blocks of the cooperative finishing launch
+    int fin_stride = 0;                   // slot row length (>= sor_max_it and >= 4)
+    int cur_max_it = 0;                   // sor_max_it of the running call
+    int *ffirst = nullptr;                // prefix table of the instances' block groups (n + 1)
+    BFin *fin = nullptr;                  // per instance
+    unsigned long long *fslots = nullptr; // n x fin_stride iteration maxima
+    BLog *log = nullptr;                  // per step and instance
+    size_t log_cap = 0;                   // records
+    int *dhist = nullptr;                 // per instance: consecutive converged loop solves
+    int *serr = nullptr;                  // sticky error word of the call
     // step graphs
     int use_graph = 1;                    // option: replay a captured graph of each step's device part
     int *dstep = nullptr;                 // device step counter (diagnostics slot of the step - 1)
@@ -673,11 +688,24 @@ int batch_res32_prepare(hv_batch *b, const hv_params *prm) {
     return HV_OK;
 }
 
-int batch_res32_enqueue(hv_batch *b) {
-    const int n = b->n, nb = b->res_nblk;
-    const size_t jw = res32_jobs_at(nb) + 2 * b->rjobs_n;
-    HV_BHIP(b, hipMemcpyAsync(b->rinst, b->hrinst, sizeof(BRes32) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+// the instance arguments and job plan from their pinned staging (the host
+// rewrites the staging only after the stream has passed these copies)
+int batch_res32_upload(hv_batch *b) {
+    const size_t jw = res32_jobs_at(b->res_nblk) + 2 * b->rjobs_n;
+    HV_BHIP(b, hipMemcpyAsync(b->rinst, b->hrinst, sizeof(BRes32) * (size_t)b->n, hipMemcpyHostToDevice, b->stream));
     HV_BHIP(b, hipMemcpyAsync(b->rjob, b->hrjob, sizeof(int) * jw, hipMemcpyHostToDevice, b->stream));
+    return HV_OK;
+}
+
+// host_io: upload the plan first and copy the results to pinned memory after
+// (the host-steered step); without it the launch uses the plan already on the
+// device and leaves its results there (the device-steered step)
+int batch_res32_enqueue(hv_batch *b, bool host_io = true) {
+    const int n = b->n, nb = b->res_nblk;
+    if (host_io) {
+        int rc = batch_res32_upload(b);
+        if (rc) return rc;
+    }
     k_fill<float><<<4096, kThreads, 0, b->stream>>>(b->rslots, -1.0f, (long long)b->rslot_words);
     k_fill<unsigned long long><<<4096, kThreads, 0, b->stream>>>(b->rx, 0ULL, (long long)b->rx_words);
     k_fill<int><<<1, 64, 0, b->stream>>>(b->rerr, 0, 4);
@@ -699,14 +727,14 @@ int batch_res32_enqueue(hv_batch *b) {
                                           (unsigned)b->res_lds, b->stream));
     if (ee) (void)hipEventRecord(ee, b->stream);
     if (ea) b->samples.push_back({HV_KCLASS_SOR_RES_F32, ea, ee});
-    HV_BHIP(b, hipMemcpyAsync(b->hrres, b->rres, sizeof(float) * (2 * (size_t)n + 4), hipMemcpyDeviceToHost,
-                              b->stream));
+    if (host_io)
+        HV_BHIP(b, hipMemcpyAsync(b->hrres, b->rres, sizeof(float) * (2 * (size_t)n + 4), hipMemcpyDeviceToHost,
+                                  b->stream));
     return HV_OK;
 }
 
-int batch_res32_collect(hv_batch *b, std::vector<int> &iters) {
-    const int n = b->n;
-    HV_BHIP(b, hipStreamSynchronize(b->stream));
+// the sampled launch times into the stage totals (the stream has passed them)
+void batch_fold_samples(hv_batch *b) {
     for (auto &smp : b->samples) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, smp.a, smp.e) == hipSuccess) {
@@ -717,6 +745,12 @@ int batch_res32_collect(hv_batch *b, std::vector<int> &iters) {
         (void)hipEventDestroy(smp.e);
     }
     b->samples.clear();
+}
+
+int batch_res32_collect(hv_batch *b, std::vector<int> &iters) {
+    const int n = b->n;
+    HV_BHIP(b, hipStreamSynchronize(b->stream));
+    batch_fold_samples(b);
     if (*(const int *)(b->hrres + 2 * (size_t)n))
         return batch_err(b, HV_ERR_HIP, "batched LDS-resident FP32 SOR: a tile wait exceeded its spin limit");
     iters.assign(n, 0);
@@ -724,13 +758,106 @@ int batch_res32_collect(hv_batch *b, std::vector<int> &iters) {
     return HV_OK;
 }
 
+// ---- device-steered end of the mixed solve (hv_batch_steer.h) ---------------
+constexpr int kReplanSteps = 16;  // steps between the host's read-backs of a steered call
+
+// The cooperative finishing launch: every instance gets blocks in proportion
+// to its nodes (at least one, at most its tile count) out of the resident
+// blocks at up to four per CU; nonzero (the batch steers from the host) when
+// the device has no cooperative launch or the instances outnumber the blocks.
+template <typename T>
+int batch_fin_setup(hv_batch *b, int max_it) {
+    const int stride = max_it > 4 ? max_it : 4;
+    if (b->fin_state < 0) return 1;
+    if (b->fin_state > 0 && b->fin_stride >= stride) return HV_OK;
+    b->fin_state = -1;
+    int dev = 0, ncu = 0, coop = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kb_sor_finish<T>, kFinThreads, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        return 1;
+    const int n = b->n;
+    const long long cap = (long long)ncu * (per_cu < 4 ? per_cu : 4);
+    if (n > cap) return 1;
+    std::vector<double> w(n);
+    std::vector<long long> nbk(n), most(n);
+    double tot = 0.0;
+    for (int i = 0; i < n; i++) {
+        const SorTiles tl = make_sor_tiles(b->inst[i], kSorPlanesBatch);
+        most[i] = (long long)tl.n_ic * tl.n_jb * tl.n_hb;
+        w[i] = (double)b->inst[i]->sp.Gh;
+        tot += w[i];
+    }
+    long long sum = 0;
+    for (int i = 0; i < n; i++) {
+        long long k = (long long)((double)cap * w[i] / tot);
+        k = k < 1 ? 1 : (k > most[i] ? most[i] : k);
+        nbk[i] = k;
+        sum += k;
+    }
+    while (sum > cap) {  // (each instance keeps at least one block: n <= cap)
+        int j = 0;
+        for (int i = 1; i < n; i++)
+            if (nbk[i] > nbk[j]) j = i;
+        nbk[j]--;
+        sum--;
+    }
+    std::vector<int> hf(n + 1, 0);
+    for (int i = 0; i < n; i++) hf[i + 1] = hf[i] + (int)nbk[i];
+    if (b->ffirst) (void)hipFree(b->ffirst);
+    if (b->fin) (void)hipFree(b->fin);
+    if (b->fslots) (void)hipFree(b->fslots);
+    if (b->dhist) (void)hipFree(b->dhist);
+    if (b->serr) (void)hipFree(b->serr);
+    b->ffirst = nullptr;
+    b->fin = nullptr;
+    b->fslots = nullptr;
+    b->dhist = nullptr;
+    b->serr = nullptr;
+    if (hipMalloc((void **)&b->ffirst, sizeof(int) * (size_t)(n + 1)) != hipSuccess ||
+        hipMalloc((void **)&b->fin, sizeof(BFin) * (size_t)n) != hipSuccess ||
+        hipMalloc((void **)&b->fslots, sizeof(unsigned long long) * (size_t)n * stride) != hipSuccess ||
+        hipMalloc((void **)&b->dhist, sizeof(int) * (size_t)n) != hipSuccess ||
+        hipMalloc((void **)&b->serr, sizeof(int)) != hipSuccess)
+        return 1;
+    if (hipMemcpy(b->ffirst, hf.data(), sizeof(int) * hf.size(), hipMemcpyHostToDevice) != hipSuccess) return 1;
+    b->fin_nblk = hf[n];
+    b->fin_stride = stride;
+    b->fin_state = 1;
+    return HV_OK;
+}
+
+// the FP64 finishing sweeps of every instance (device stop rule), after the fold
+template <typename T>
+int batch_fin_enqueue(hv_batch *b) {
+    const int n = b->n;
+    kb_fin_reset<<<n, 256, 0, b->stream>>>(b->fin, b->fslots, b->fin_stride);
+    HV_BHIP(b, hipGetLastError());
+    const BInst<T> *a = (const BInst<T> *)b->args;
+    const int *ff = b->ffirst;
+    const float *rr = b->rres;
+    BFin *fi = b->fin;
+    unsigned long long *sl = b->fslots;
+    int st = b->fin_stride, mi = b->cur_max_it;
+    int *er = b->serr;
+    int nn = n;
+    void *args[] = {(void *)&a, (void *)&ff, (void *)&nn, (void *)&rr, (void *)&fi, (void *)&sl, &st, &mi, (void *)&er};
+    HV_BHIP(b, hipLaunchCooperativeKernel((const void *)kb_sor_finish<T>, dim3((unsigned)b->fin_nblk),
+                                          dim3(64, kSorRows), args, 0u, b->stream));
+    return HV_OK;
+}
+
 // The device part of one step (no host decision, no host wait: the part a
 // graph captures): particle pass with the start-of-step kick (end_kick) and
 // the diagnostics of its positions, smearing, the colour-split source and,
 // with the resident FP32 correction (res), the defect, that solve and the
-// fold of its correction.
+// fold of its correction; steered (steer), also the FP64 finishing sweeps,
+// the Skyrme + Coulomb potentials and the step's record, i.e. the whole step.
 template <typename T>
-int batch_step_device(hv_batch *b, int end_kick, bool res) {
+int batch_step_device(hv_batch *b, int end_kick, bool res, bool steer = false) {
     const int n = b->n;
     const BInst<T> *a = (const BInst<T> *)b->args;
     {
@@ -762,9 +889,23 @@ int batch_step_device(hv_batch *b, int end_kick, bool res) {
         }
         kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
         HV_BHIP(b, hipGetLastError());
-        int rc = batch_res32_enqueue(b);
+        int rc = batch_res32_enqueue(b, !steer);
         if (rc) return rc;
         kb_sor_fold<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
+        HV_BHIP(b, hipGetLastError());
+        if (steer) {
+            rc = batch_fin_enqueue<T>(b);
+            if (rc) return rc;
+        }
+    }
+    if (steer) {
+        {
+            BStage st(b, HV_KCLASS_POTENTIAL);
+            kb_skyrme<T><<<b->blocks[kBG], kThreads, 0, b->stream>>>(a, btab(b, kBG), n);
+            HV_BHIP(b, hipGetLastError());
+        }
+        kb_step_record<T><<<(unsigned)((n + 255) / 256), 256, 0, b->stream>>>(
+            a, n, b->rres, b->fin, b->log, b->dstep, b->extrap, b->dhist, b->xorder, b->rerr, b->serr);
         HV_BHIP(b, hipGetLastError());
     }
     return HV_OK;
@@ -842,13 +983,13 @@ int batch_step_finish(hv_batch *b, const hv_params *prm, bool res, int *iters_ou
 // Captures batch_step_device into b->gexec; nonzero (and no graph) when the
 // runtime refuses any part of it, the stream then back in normal mode.
 template <typename T>
-int batch_capture_step(hv_batch *b, bool res) {
+int batch_capture_step(hv_batch *b, bool res, bool steer) {
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         (void)hipGetLastError();
         return 1;
     }
-    const int rc = batch_step_device<T>(b, 1, res);
+    const int rc = batch_step_device<T>(b, 1, res, steer);
     const hipError_t ec = hipStreamEndCapture(b->stream, &g);
     (void)hipGetLastError();
     if (rc || ec != hipSuccess || !g) {
@@ -903,6 +1044,21 @@ int batch_run_init(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int *
     return warn;
 }
 
+// Reads back the log rows [r0, r1) of a steered call (the stream is synchronised).
+int batch_read_log(hv_batch *b, int r0, int r1, std::vector<BLog> &out) {
+    const size_t cnt = (size_t)(r1 - r0) * b->n;
+    out.resize(cnt);
+    if (!cnt) return HV_OK;
+    HV_BHIP(b, hipMemcpyAsync(out.data(), b->log + (size_t)r0 * b->n, sizeof(BLog) * cnt, hipMemcpyDeviceToHost,
+                              b->stream));
+    HV_BHIP(b, hipStreamSynchronize(b->stream));
+    batch_fold_samples(b);
+    int e = 0;
+    HV_BHIP(b, hipMemcpy(&e, b->serr, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) return batch_err(b, HV_ERR_HIP, "steered ensemble SOR: a wait exceeded its spin limit");
+    return HV_OK;
+}
+
 template <typename T>
 int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int first, int nsteps, double *stats,
                     int *sor_iters) {
@@ -923,15 +1079,43 @@ int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int 
     if (rc) return rc;
     const BInst<T> *a = (const BInst<T> *)b->args;
     const bool res = b->precision == HV_PREC_MIXED && b->res32 && batch_res32_setup(b, prm[0].sor_max_it) == HV_OK;
+    // the whole mixed step on the device: the host reads back every
+    // kReplanSteps steps (to re-plan the resident launch) and at the end
+    const bool steer = res && b->steer && batch_fin_setup<T>(b, prm[0].sor_max_it) == HV_OK;
     k_fill<int><<<1, 64, 0, b->stream>>>(b->dstep, 0, 1);  // diagnostics slot of the first step: 1
     HV_BHIP(b, hipGetLastError());
+    if (steer) {
+        b->cur_max_it = prm[0].sor_max_it;
+        if (b->log_cap < (size_t)n * nsteps) {
+            if (b->log) (void)hipFree(b->log);
+            b->log = nullptr;
+            b->log_cap = 0;
+            HV_BHIP(b, hipMalloc((void **)&b->log, sizeof(BLog) * (size_t)n * nsteps));
+            b->log_cap = (size_t)n * nsteps;
+        }
+        // the history as the host holds it, and the first step's predictor and
+        // ring position (the device advances them after every step)
+        std::vector<int> hh(n);
+        for (int i = 0; i < n; i++) {
+            hv_ctx *c = b->inst[i];
+            hh[i] = c->hist_n;
+            b->hxorder[2 * i] = b->extrap ? sor_predictor_for(b->extrap, c->hist_n) : 0;
+            b->hxorder[2 * i + 1] = c->hist_pos;
+        }
+        HV_BHIP(b, hipMemcpyAsync(b->dhist, hh.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+        HV_BHIP(b, hipMemcpyAsync(b->xorder, b->hxorder, sizeof(int) * 2 * (size_t)n, hipMemcpyHostToDevice,
+                                  b->stream));
+        k_fill<int><<<1, 64, 0, b->stream>>>(b->serr, 0, 1);
+        HV_BHIP(b, hipGetLastError());
+        HV_BHIP(b, hipStreamSynchronize(b->stream));
+    }
     // steps after the first of this call (start-of-step kick) replay one
     // captured graph of the device part, unless stages are being timed
     bool graphs = b->use_graph && !b->prof;
     batch_drop_graph(b);
     std::vector<int> its(n);
-    std::vector<std::vector<int>> iters_all;
     int warn = 0;
+    std::vector<BLog> lg;
     for (int s = 0; s < nsteps; s++) {
         const int step = first + s;
         bool dirty = false;
@@ -961,35 +1145,80 @@ int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int 
             rc = batch_upload_args<T>(b);
             if (rc) return rc;
         }
-        if (res) {
-            rc = batch_res32_prepare(b, prm);
-            if (rc) return rc;
-        }
-        if (b->extrap) {
-            // predictor and ring position of each instance's solve (as sor_solve)
-            for (int i = 0; i < n; i++) {
-                hv_ctx *c = b->inst[i];
-                b->hxorder[2 * i] = sor_predictor_for(b->extrap, c->hist_n);
-                b->hxorder[2 * i + 1] = c->hist_pos;
-                c->hist_pos = (c->hist_pos + 4) % 5;
+        if (steer) {
+            if (s % kReplanSteps == 0) {
+                // the job plan of the resident launch from the last step's counts
+                if (s > 0) {
+                    rc = batch_read_log(b, s - 1, s, lg);
+                    if (rc) return rc;
+                    for (int i = 0; i < n; i++) b->pred32[i] = lg[i].it32;
+                } else {
+                    HV_BHIP(b, hipStreamSynchronize(b->stream));  // the staging is free
+                }
+                rc = batch_res32_prepare(b, prm);
+                if (rc) return rc;
+                rc = batch_res32_upload(b);
+                if (rc) return rc;
             }
-            HV_BHIP(b, hipMemcpyAsync(b->xorder, b->hxorder, sizeof(int) * 2 * (size_t)n, hipMemcpyHostToDevice,
-                                      b->stream));
+        } else {
+            if (res) {
+                rc = batch_res32_prepare(b, prm);
+                if (rc) return rc;
+            }
+            if (b->extrap) {
+                // predictor and ring position of each instance's solve (as sor_solve)
+                for (int i = 0; i < n; i++) {
+                    hv_ctx *c = b->inst[i];
+                    b->hxorder[2 * i] = sor_predictor_for(b->extrap, c->hist_n);
+                    b->hxorder[2 * i + 1] = c->hist_pos;
+                    c->hist_pos = (c->hist_pos + 4) % 5;
+                }
+                HV_BHIP(b, hipMemcpyAsync(b->xorder, b->hxorder, sizeof(int) * 2 * (size_t)n,
+                                          hipMemcpyHostToDevice, b->stream));
+            }
         }
-        if (s > 0 && graphs && !b->gexec && batch_capture_step<T>(b, res) != HV_OK) graphs = false;
+        if (s > 0 && graphs && !b->gexec && batch_capture_step<T>(b, res, steer) != HV_OK) graphs = false;
         if (s > 0 && graphs) {
             HV_BHIP(b, hipGraphLaunch(b->gexec, b->stream));
             b->graph_steps++;
         } else {
-            rc = batch_step_device<T>(b, s > 0 ? 1 : 0, res);
+            rc = batch_step_device<T>(b, s > 0 ? 1 : 0, res, steer);
             if (rc) return rc;
         }
-        rc = batch_step_finish<T>(b, prm, res, its.data(), &warn);
-        if (rc) return rc;
-        if (sor_iters)
-            for (int i = 0; i < n; i++) sor_iters[(size_t)i * nsteps + s] = its[i];
+        if (!steer) {
+            rc = batch_step_finish<T>(b, prm, res, its.data(), &warn);
+            if (rc) return rc;
+            if (sor_iters)
+                for (int i = 0; i < n; i++) sor_iters[(size_t)i * nsteps + s] = its[i];
+        }
     }
     batch_drop_graph(b);
+    if (steer) {
+        // the steps' records: iteration counts, warnings, the solve counters and
+        // the history the host keeps for the next call
+        rc = batch_read_log(b, 0, nsteps, lg);
+        if (rc) return rc;
+        std::vector<int> hh(n);
+        HV_BHIP(b, hipMemcpy(hh.data(), b->dhist, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+        HV_BHIP(b, hipMemcpy(b->hxorder, b->xorder, sizeof(int) * 2 * (size_t)n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++) {
+            hv_ctx *c = b->inst[i];
+            for (int s = 0; s < nsteps; s++) {
+                const BLog &L = lg[(size_t)s * n + i];
+                if (sor_iters) sor_iters[(size_t)i * nsteps + s] = L.it32 + L.it64;
+                if (L.md > prm[i].sor_tol) warn |= HV_WARN_SOR_NOT_CONVERGED;
+                c->cnt_it32 += L.it32;
+                c->cnt_it64 += L.it64;
+                c->cnt_solves++;
+            }
+            const BLog &last = lg[(size_t)(nsteps - 1) * n + i];
+            b->pred32[i] = last.it32;
+            c->mixed_fp64_sweeps = last.it64;
+            c->last_extrap = last.pred;
+            c->hist_n = hh[i];
+            if (b->extrap) c->hist_pos = b->hxorder[2 * i + 1];
+        }
+    }
     // closing half kick of the last step, with the force stored (src/hyper_vlasov.cpp:101-103)
     kb_gather_kick<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n);
     HV_BHIP(b, hipGetLastError());
@@ -1085,6 +1314,13 @@ int hv_batch_destroy(hv_batch *b) {
     if (b->dstep) (void)hipFree(b->dstep);
     if (b->xorder) (void)hipFree(b->xorder);
     if (b->hxorder) (void)hipHostFree(b->hxorder);
+    if (b->ffirst) (void)hipFree(b->ffirst);
+    if (b->fin) (void)hipFree(b->fin);
+    if (b->fslots) (void)hipFree(b->fslots);
+    if (b->log) (void)hipFree(b->log);
+    if (b->dhist) (void)hipFree(b->dhist);
+    if (b->serr) (void)hipFree(b->serr);
+    batch_fold_samples(b);
     batch_drop_graph(b);
     if (b->act) (void)hipFree(b->act);
     if (b->hact) (void)hipHostFree(b->hact);
@@ -1117,6 +1353,9 @@ int hv_batch_get_option(hv_batch *b, int option, int *value) {
     case HV_OPT_SMEAR_FUSED: *value = b->smear_fused; return HV_OK;
     case HV_OPT_BATCH_GRAPH: *value = b->use_graph; return HV_OK;
     case HV_OPT_SOR_EXTRAPOLATE: *value = b->extrap; return HV_OK;
+    case HV_OPT_BATCH_DEVICE_STEER:  // once the batch has stepped: whether the steered step runs
+        *value = b->steer && b->fin_state >= 0 && b->precision == HV_PREC_MIXED && b->res32 && b->res_state >= 0;
+        return HV_OK;
     default: return batch_err(b, HV_ERR_ARG, "option %d is not a batch option", option);
     }
 }
@@ -1149,6 +1388,10 @@ int hv_batch_set_option(hv_batch *b, int option, int value) {
     }
     if (option == HV_OPT_BATCH_GRAPH) {
         b->use_graph = value ? 1 : 0;
+        return HV_OK;
+    }
+    if (option == HV_OPT_BATCH_DEVICE_STEER) {
+        b->steer = value ? 1 : 0;
         return HV_OK;
     }
     if (option == HV_OPT_SOR_EXTRAPOLATE) {

@@ -1610,23 +1610,12 @@ struct SorTiles {
 // with fewer, longer blocks).  All P planes' operands are loaded before the
 // first update (the colour arrays do not alias), so the loads are in flight
 // together.
-template <typename RT, int P = kSorPlanes>
-__device__ __forceinline__ void sor_colour_body(const int bid, double *__restrict__ phi, const RT *__restrict__ rho,
-                                                Split sp, SorTiles tl, SorCoef cf, int colour, int it,
-                                                unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
-    if (ctl->done) return;
-    if (colour == 0 && it > 0) {
-        const double prev = sor_row_max(maxd, it - 1);
-        if (prev < cf.tol) {
-            if (threadIdx.x == 0 && threadIdx.y == 0 && bid == 0) {
-                ctl->done = 1;
-                ctl->iters = it;
-            }
-            return;
-        }
-    }
-    const int b = bid;
-    const int t = (b % kXcds) * tl.per_xcd + b / kXcds;
+// One colour of tile t (P planes x kSorRows rows x 64 k-slots, one node per
+// thread and plane): the thread's max |dphi|.
+template <typename RT, int P>
+__device__ __forceinline__ double sor_tile_sweep(const int t, double *__restrict__ phi, const RT *__restrict__ rho,
+                                                 const Split &sp, const SorTiles &tl, const SorCoef &cf,
+                                                 int colour) {
     const int ntiles = tl.n_ic * tl.n_jb * tl.n_hb;
     double diff = 0.0;
     if (t < ntiles) {
@@ -1671,6 +1660,27 @@ __device__ __forceinline__ void sor_colour_body(const int bid, double *__restric
             diff = d > diff ? d : diff;
         }
     }
+    return diff;
+}
+
+template <typename RT, int P = kSorPlanes>
+__device__ __forceinline__ void sor_colour_body(const int bid, double *__restrict__ phi, const RT *__restrict__ rho,
+                                                Split sp, SorTiles tl, SorCoef cf, int colour, int it,
+                                                unsigned long long *__restrict__ maxd, SorCtl *__restrict__ ctl) {
+    if (ctl->done) return;
+    if (colour == 0 && it > 0) {
+        const double prev = sor_row_max(maxd, it - 1);
+        if (prev < cf.tol) {
+            if (threadIdx.x == 0 && threadIdx.y == 0 && bid == 0) {
+                ctl->done = 1;
+                ctl->iters = it;
+            }
+            return;
+        }
+    }
+    const int b = bid;
+    const int t = (b % kXcds) * tl.per_xcd + b / kXcds;
+    const double diff = sor_tile_sweep<RT, P>(t, phi, rho, sp, tl, cf, colour);
     block_max_atomic(diff, &maxd[((long long)it * kMaxSlots + (b % kMaxSlots)) * kSlotStride]);
 }
 
@@ -1835,6 +1845,25 @@ __host__ __device__ __forceinline__ int sor_predictor(int id, double w[5]) {
     case 5: w[0] = 4.0 / 3.0; w[1] = 1.0 / 3.0; w[2] = -2.0 / 3.0; return 3;
     default: w[0] = 1.0; return 1;
     }
+}
+
+// Iteration budget of the FP64 sweeps that end a mixed solve: what is left of
+// max_it after the FP32 correction iterations, and at least a few sweeps, so
+// that a solve that already spent its budget in FP32 stops (not converged)
+// about where the FP64 form would.
+__host__ __device__ __forceinline__ int mixed_fp64_budget(int max_it, int iters32) {
+    const int left = max_it - iters32;
+    return left > 4 ? left : 4;
+}
+
+// The predictor a loop solve uses: the chosen one once the ring holds its
+// points (hist_n counts phi and the ring's converged solves), else a line
+// through the last two, else none.
+__host__ __device__ __forceinline__ int sor_predictor_for(int chosen, int hist_n) {
+    double w[5];
+    const int need = sor_predictor(chosen, w);
+    if (hist_n >= need) return chosen;
+    return hist_n >= 2 ? 1 : 0;
 }
 
 struct SorHist {

@@ -253,8 +253,9 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * hv_batch_run_steps call on, replay one captured hipGraph of the step's
  * device part (particle pass, diagnostics, smearing, source split and, in the
  * mixed form, the defect, the persistent LDS-resident FP32 SOR launch and the
- * fold) instead of launching its kernels one by one (default 1; off while
- * stages are timed).  Same bits either way. */
+ * fold; with HV_OPT_BATCH_DEVICE_STEER the rest of the step as well) instead
+ * of launching its kernels one by one (default 1; off while stages are
+ * timed).  Same bits either way. */
 #define HV_OPT_BATCH_GRAPH 8
 /* HV_OPT_SOR_F32_TILES_PER_CU: LDS-resident FP32 correction tiles per CU of a
  * single grid: 1 (1024-thread blocks) or 2 (512-thread blocks, half-size
@@ -292,6 +293,16 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * max |update| is below tol / 2^shift (0..20, default 0); the FP64 finishing
  * sweeps apply the reference's rule with tol itself either way. */
 #define HV_OPT_SOR_F32_TOL_SHIFT 12
+/* HV_OPT_BATCH_DEVICE_STEER (ensembles, mixed form with the LDS-resident FP32
+ * correction): 1 (default) runs the whole step on the device: the FP64
+ * finishing sweeps of every instance in one cooperative launch that applies
+ * the stop rule itself, then the potentials and a per-step record that keeps
+ * each instance's extrapolation history; the step graph then holds the
+ * entire step and the host reads back only every 16 steps (to re-plan the
+ * resident launch) and at the end of the call.  0: the host reads each
+ * step's FP32 iteration counts and steers the FP64 sweeps.  Same bits either
+ * way. */
+#define HV_OPT_BATCH_DEVICE_STEER 13
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
@@ -354,7 +365,8 @@ hv_ctx *hv_batch_instance(hv_batch *batch, int i);
 /* HV_OPT_SORT_EVERY or HV_OPT_SMEAR_TILE (applied to every instance), or
  * HV_OPT_BATCH_GROUP_KIB, HV_OPT_SOR_RESIDENT_F32, HV_OPT_SOR_F32_LAG,
  * HV_OPT_SOR_F32_OWNED, HV_OPT_SMEAR_FUSED (default 1 for a batch),
- * HV_OPT_BATCH_GRAPH, HV_OPT_SOR_EXTRAPOLATE (every instance) */
+ * HV_OPT_BATCH_GRAPH, HV_OPT_BATCH_DEVICE_STEER, HV_OPT_SOR_EXTRAPOLATE (every
+ * instance) */
 int hv_batch_set_option(hv_batch *batch, int option, int value);
 /* the value of a batch option (HV_OPT_SOR_RESIDENT_F32 and
  * HV_OPT_SOR_F32_OWNED: whether that form runs, once the batch has solved) */

@@ -196,3 +196,39 @@ def test_batch_resident_shape_bodies_equal_solo(big, owned):
         assert np.array_equal(s1[i], s_stats)
         for k in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz"):
             assert np.array_equal(parts[i][k], s_part[k]), (i, k)
+
+
+def test_batch_device_steered_step_equals_host_steered():
+    """The mixed step with the FP64 finishing sweeps, the stop rule and the
+    extrapolation history on the device (one cooperative launch, the whole
+    step in the graph, read-backs every 16 steps) gives the bits of the step
+    the host steers, across a re-plan and across two calls (the history
+    handed back to the host and up again), with sorting and a boost."""
+    print("batch device-steered vs host-steered", flush=True)
+    ws, params = _instances_res()
+    outs = []
+    for steer in (0, 1):
+        b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws],
+                  hvamd.HV_PREC_MIXED)
+        b.set_option(hvamd.HV_OPT_SORT_EVERY, 3)
+        b.set_option(hvamd.HV_OPT_BATCH_DEVICE_STEER, steer)
+        for i, w in enumerate(ws):
+            b.instance(i).upload_particles(*w.initial_state())
+        sks = [w.sky() for w in ws]
+        b.run_init(sks, params)
+        s1, i1 = b.run_steps(sks, params, 0, 19)
+        s2, i2 = b.run_steps(sks, params, 19, 4)
+        used = b.get_option(hvamd.HV_OPT_BATCH_DEVICE_STEER)
+        counts = b.sor_counts()
+        parts = [b.instance(i).download_particles() for i in range(len(ws))]
+        b.close()
+        print("steer", steer, "used", used, "iters", i1[:, -3:].tolist(), i2.tolist(), flush=True)
+        assert used == steer
+        outs.append((s1, i1, s2, i2, parts, counts["node_iters_f32"], counts["node_iters_f64"]))
+    h, d = outs
+    for k in range(4):
+        assert np.array_equal(h[k], d[k]), k
+    for ph, pd in zip(h[4], d[4]):
+        for key in ("x", "y", "z", "kx", "ky", "kz", "fx", "fy", "fz"):
+            assert np.array_equal(ph[key], pd[key]), key
+    assert h[5] == d[5] and h[6] == d[6]
