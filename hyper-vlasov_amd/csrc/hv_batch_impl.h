@@ -48,6 +48,11 @@ struct hv_batch {
     int *xorder = nullptr;                // per instance: predictor and ring position of the current step
     int *hxorder = nullptr;               // pinned staging of xorder
     int sort_every = 8;
+    int since_sort = 1 << 30;             // steps since the last batched sort (as maybe_sort)
+    unsigned *sbuck = nullptr;            // key counts / first slots of every instance's keys
+    long long *sfirst = nullptr;          // per instance: its first key bucket (n + 1)
+    long long sbuck_n = 0;
+    std::vector<long long> hsfirst;       // host copy of sfirst
     int group_kib = 0;  // SOR working-set budget (KiB) of one instance group (0: all instances together)
     // LDS-resident FP32 correction of the whole batch (mixed precision)
     int res32 = 1;                        // option: use it when every instance tiles
@@ -88,7 +93,8 @@ struct hv_batch {
     // step graphs
     int use_graph = 1;                    // option: replay a captured graph of each step's device part
     int *dstep = nullptr;                 // device step counter (diagnostics slot of the step - 1)
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec = nullptr;       // the step without a sort
+    hipGraphExec_t gexec_sort = nullptr;  // the step that sorts first
     long long graph_steps = 0, captures = 0;
 #ifdef HV_BRES_TRACE
     unsigned long long *jtrace = nullptr;
@@ -157,6 +163,12 @@ void batch_fill_args(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
         A.p = particles<T>(c);
         A.np = c->np;
         A.total = c->total;
+        A.palt = particles_alt<T>(c);
+        A.pid = c->pid;
+        A.pid_alt = c->pid_alt;
+        A.skey = c->keys[0];
+        A.srank = c->ord[0];
+        A.sbase = b->hsfirst.empty() ? 0 : b->hsfirst[i];
         A.g = make_geom<T>(c);
         A.U = (T *)c->U;
         A.acc = c->acc;
@@ -260,6 +272,28 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
                                    (int)kSmear2LdsVec));
     if (!b->first) HV_BHIP(b, hipMalloc((void **)&b->first, sizeof(int) * hf.size()));
     HV_BHIP(b, hipMemcpyAsync(b->first, hf.data(), sizeof(int) * hf.size(), hipMemcpyHostToDevice, b->stream));
+    // key buckets of the batched sort: 2 (bricks + 1) per instance
+    {
+        std::vector<long long> sf(b->n + 1, 0);
+        for (int i = 0; i < b->n; i++) {
+            const hv_ctx *c = b->inst[i];
+            const long long nb = (long long)((c->world.n[0] + 3) >> kBrickShift) *
+                                 ((c->world.n[1] + 3) >> kBrickShift) * ((c->world.n[2] + 3) >> kBrickShift);
+            sf[i + 1] = sf[i] + 2 * (nb + 1);
+        }
+        if (b->hsfirst != sf) {
+            if (b->sbuck) (void)hipFree(b->sbuck);
+            if (b->sfirst) (void)hipFree(b->sfirst);
+            b->sbuck = nullptr;
+            b->sfirst = nullptr;
+            b->hsfirst.clear();
+            HV_BHIP(b, hipMalloc((void **)&b->sbuck, sizeof(unsigned) * (size_t)sf[b->n]));
+            HV_BHIP(b, hipMalloc((void **)&b->sfirst, sizeof(long long) * sf.size()));
+            HV_BHIP(b, hipMemcpy(b->sfirst, sf.data(), sizeof(long long) * sf.size(), hipMemcpyHostToDevice));
+            b->hsfirst = sf;
+            b->sbuck_n = sf[b->n];
+        }
+    }
     if (!b->args) HV_BHIP(b, hipMalloc(&b->args, sizeof(BInst<T>) * (size_t)b->n));
     batch_fill_args<T>(b, sk, prm);
     return batch_upload_args<T>(b);
@@ -857,11 +891,21 @@ int batch_fin_enqueue(hv_batch *b) {
 // fold of its correction; steered (steer), also the FP64 finishing sweeps,
 // the Skyrme + Coulomb potentials and the step's record, i.e. the whole step.
 template <typename T>
-int batch_step_device(hv_batch *b, int end_kick, bool res, bool steer = false) {
+int batch_step_device(hv_batch *b, int end_kick, bool res, bool steer = false, bool sort = false) {
     const int n = b->n;
     const BInst<T> *a = (const BInst<T> *)b->args;
     {
         BStage st(b, HV_KCLASS_PARTICLE);
+        if (sort) {
+            // every instance's particles by key into the other buffer set,
+            // which then becomes the current one
+            k_fill<unsigned><<<1024, kThreads, 0, b->stream>>>(b->sbuck, 0u, b->sbuck_n);
+            kb_sort_count<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n, b->sbuck);
+            kb_sort_scan<<<n, kSortScanThreads, 0, b->stream>>>(b->sfirst, b->sbuck);
+            kb_sort_scatter<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n, b->sbuck);
+            kb_sort_swap<T><<<(unsigned)((n + 255) / 256), 256, 0, b->stream>>>((BInst<T> *)b->args, n);
+            HV_BHIP(b, hipGetLastError());
+        }
         kb_particle_step<T><<<b->blocks[kBPart], kThreads, 0, b->stream>>>(a, btab(b, kBPart), n, end_kick);
         kb_stats_finalize<T><<<n, kStatThreads, 0, b->stream>>>(a, b->dstep);
         kb_step_advance<<<1, 64, 0, b->stream>>>(b->dstep);
@@ -983,13 +1027,13 @@ int batch_step_finish(hv_batch *b, const hv_params *prm, bool res, int *iters_ou
 // Captures batch_step_device into b->gexec; nonzero (and no graph) when the
 // runtime refuses any part of it, the stream then back in normal mode.
 template <typename T>
-int batch_capture_step(hv_batch *b, bool res, bool steer) {
+int batch_capture_step(hv_batch *b, bool res, bool steer, bool sort) {
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         (void)hipGetLastError();
         return 1;
     }
-    const int rc = batch_step_device<T>(b, 1, res, steer);
+    const int rc = batch_step_device<T>(b, 1, res, steer, sort);
     const hipError_t ec = hipStreamEndCapture(b->stream, &g);
     (void)hipGetLastError();
     if (rc || ec != hipSuccess || !g) {
@@ -1004,14 +1048,16 @@ int batch_capture_step(hv_batch *b, bool res, bool steer) {
         (void)hipGetLastError();
         return 1;
     }
-    b->gexec = ge;
+    (sort ? b->gexec_sort : b->gexec) = ge;
     b->captures++;
     return HV_OK;
 }
 
 void batch_drop_graph(hv_batch *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->gexec_sort) (void)hipGraphExecDestroy(b->gexec_sort);
     b->gexec = nullptr;
+    b->gexec_sort = nullptr;
 }
 
 int batch_check(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
@@ -1119,15 +1165,15 @@ int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int 
     for (int s = 0; s < nsteps; s++) {
         const int step = first + s;
         bool dirty = false;
-        {
-            // sorting counts in the particle stage's time
-            BStage st(b, HV_KCLASS_PARTICLE);
-            for (int i = 0; i < n; i++) {
-                hv_ctx *c = b->inst[i];
-                void *before = c->part[0];
-                rc = maybe_sort<T>(c);
-                if (rc) return batch_fwd(b, c, rc);
-                if (c->part[0] != before) dirty = true;
+        // the batched sort's cadence (as maybe_sort: the first step, then every
+        // sort_every steps); the sort runs at the head of the step's particle stage
+        bool sort_now = false;
+        if (b->sort_every > 0) {
+            if (b->since_sort < b->sort_every) {
+                b->since_sort++;
+            } else {
+                sort_now = true;
+                b->since_sort = 1;
             }
         }
         for (int i = 0; i < n; i++) {
@@ -1177,13 +1223,24 @@ int batch_run_steps(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int 
                                           hipMemcpyHostToDevice, b->stream));
             }
         }
-        if (s > 0 && graphs && !b->gexec && batch_capture_step<T>(b, res, steer) != HV_OK) graphs = false;
+        if (s > 0 && graphs && !(sort_now ? b->gexec_sort : b->gexec) &&
+            batch_capture_step<T>(b, res, steer, sort_now) != HV_OK)
+            graphs = false;
         if (s > 0 && graphs) {
-            HV_BHIP(b, hipGraphLaunch(b->gexec, b->stream));
+            HV_BHIP(b, hipGraphLaunch(sort_now ? b->gexec_sort : b->gexec, b->stream));
             b->graph_steps++;
         } else {
-            rc = batch_step_device<T>(b, s > 0 ? 1 : 0, res, steer);
+            rc = batch_step_device<T>(b, s > 0 ? 1 : 0, res, steer, sort_now);
             if (rc) return rc;
+        }
+        if (sort_now) {
+            // the host's view of the buffer sets follows the device's swap
+            for (hv_ctx *c : b->inst) {
+                for (int q = 0; q < 9; q++) std::swap(c->part[q], c->part_alt[q]);
+                std::swap(c->pid, c->pid_alt);
+                c->permuted = true;
+            }
+            batch_fill_args<T>(b, sk, prm);
         }
         if (!steer) {
             rc = batch_step_finish<T>(b, prm, res, its.data(), &warn);
@@ -1314,6 +1371,8 @@ int hv_batch_destroy(hv_batch *b) {
     if (b->dstep) (void)hipFree(b->dstep);
     if (b->xorder) (void)hipFree(b->xorder);
     if (b->hxorder) (void)hipHostFree(b->hxorder);
+    if (b->sbuck) (void)hipFree(b->sbuck);
+    if (b->sfirst) (void)hipFree(b->sfirst);
     if (b->ffirst) (void)hipFree(b->ffirst);
     if (b->fin) (void)hipFree(b->fin);
     if (b->fslots) (void)hipFree(b->fslots);
@@ -1415,7 +1474,10 @@ int hv_batch_set_option(hv_batch *b, int option, int value) {
         int rc = hv_set_option(c, option, value);
         if (rc) return batch_fwd(b, c, rc);
     }
-    if (option == HV_OPT_SORT_EVERY) b->sort_every = value;
+    if (option == HV_OPT_SORT_EVERY) {
+        b->sort_every = value;
+        b->since_sort = 1 << 30;
+    }
     return HV_OK;
 }
 

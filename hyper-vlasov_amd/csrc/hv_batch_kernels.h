@@ -20,6 +20,13 @@ struct BInst {
     // particle pass
     Particles<T> p;
     int np, total;
+    // batched sort: the other buffer set, the storage-slot ids of both, each
+    // particle's key and rank within its key, the instance's first key bucket
+    Particles<T> palt;
+    int *pid, *pid_alt;
+    unsigned *skey;
+    int *srank;
+    long long sbase;
     Geom<T> g;
     T *U;
     unsigned long long *acc;
@@ -224,6 +231,86 @@ __global__ void kb_sor_collect(const BInst<T> *__restrict__ a, double *__restric
     const int m = it1 - it0;
     const int inst = blockIdx.x / m, r = blockIdx.x - inst * m;
     sor_collect_body(r, a[inst].maxd, out + (long long)inst * width, it0, it1);
+}
+
+// ---- batched particle sort ---------------------------------------------------
+// A counting sort of every instance's particles by sort_key_of in three
+// launches (counts with ranks, an exclusive scan per instance, the scatter
+// into the other buffer set) and a pointer swap on the device, so that a
+// captured step graph can hold it (no memset, no host decision).  The order
+// within a key follows the order of the count atomics; no result depends on
+// the storage order (the deposits and diagnostics are exact integer sums,
+// every other operation is per particle), and pid keeps each slot's
+// original index for the download.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_sort_count(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, unsigned *__restrict__ buck) {
+    HV_BATCH_LOCATE();
+    const int q = bid * (int)blockDim.x + (int)threadIdx.x;
+    if (q >= A.total) return;
+    const unsigned k = sort_key_of(A.p, q, A.np, A.g);
+    A.skey[q] = k;
+    A.srank[q] = (int)atomicAdd(&buck[A.sbase + k], 1u);
+}
+
+// one block of kSortScanThreads per instance: its key counts (sfirst[i] ..
+// sfirst[i + 1]) become the first slot of each key
+constexpr int kSortScanThreads = 1024;
+
+__global__ void __launch_bounds__(kSortScanThreads)
+kb_sort_scan(const long long *__restrict__ sfirst, unsigned *__restrict__ buck) {
+    __shared__ unsigned wsum[kSortScanThreads / kWave];
+    unsigned *b = buck + sfirst[blockIdx.x];
+    const int m = (int)(sfirst[blockIdx.x + 1] - sfirst[blockIdx.x]);
+    const int per = (m + kSortScanThreads - 1) / kSortScanThreads;
+    const int lo = (int)threadIdx.x * per, hi = min(m, lo + per);
+    unsigned s = 0;
+    for (int q = lo; q < hi; q++) s += b[q];
+    // inclusive scan over the wave, then over the waves' totals
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    unsigned inc = s;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned v = __shfl_up(inc, d, kWave);
+        if (lane >= d) inc += v;
+    }
+    if (lane == kWave - 1) wsum[wave] = inc;
+    __syncthreads();
+    unsigned before = 0;
+    for (int w = 0; w < wave; w++) before += wsum[w];
+    unsigned run = before + inc - s;
+    for (int q = lo; q < hi; q++) {
+        const unsigned v = b[q];
+        b[q] = run;
+        run += v;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_sort_scatter(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n,
+                const unsigned *__restrict__ buck) {
+    HV_BATCH_LOCATE();
+    const int q = bid * (int)blockDim.x + (int)threadIdx.x;
+    if (q >= A.total) return;
+    const int d = (int)buck[A.sbase + A.skey[q]] + A.srank[q];
+    A.palt.x[d] = A.p.x[q]; A.palt.y[d] = A.p.y[q]; A.palt.z[d] = A.p.z[q];
+    A.palt.kx[d] = A.p.kx[q]; A.palt.ky[d] = A.p.ky[q]; A.palt.kz[d] = A.p.kz[q];
+    A.palt.fx[d] = A.p.fx[q]; A.palt.fy[d] = A.p.fy[q]; A.palt.fz[d] = A.p.fz[q];
+    A.pid_alt[d] = A.pid[q];
+}
+
+// the sorted set becomes the current one (one thread per instance)
+template <typename T>
+__global__ void kb_sort_swap(BInst<T> *__restrict__ a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Particles<T> t = a[i].p;
+    a[i].p = a[i].palt;
+    a[i].palt = t;
+    int *u = a[i].pid;
+    a[i].pid = a[i].pid_alt;
+    a[i].pid_alt = u;
 }
 
 // closing half kick of a run of steps (gather of the final U)

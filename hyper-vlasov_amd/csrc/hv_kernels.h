@@ -303,18 +303,26 @@ __device__ __forceinline__ void block_deposit(bool has, const Cell<T> &c, int sp
 // keep [0, np) because the species is the leading part of the key.
 constexpr int kBrickShift = 2;
 
+// Sort key of particle i: its species (protons first), then the 4^3-node
+// brick of its cell (particles outside the grid last within their species);
+// keys lie in [0, 2 (bricks + 1)).
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_sort_keys(Particles<T> p, int np, int total, Geom<T> g, unsigned *__restrict__ keys, int *__restrict__ idx) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+__device__ __forceinline__ unsigned sort_key_of(const Particles<T> &p, int i, int np, const Geom<T> &g) {
     const int bx = (g.nx + 3) >> kBrickShift, by = (g.ny + 3) >> kBrickShift, bz = (g.nz + 3) >> kBrickShift;
     const unsigned nb = (unsigned)(bx * by * bz);
     Cell<T> c;
     unsigned b = nb;
     if (cic_cell(p.x[i], p.y[i], p.z[i], g, c))
         b = (unsigned)(((c.x0 >> kBrickShift) * by + (c.y0 >> kBrickShift)) * bz + (c.z0 >> kBrickShift));
-    keys[i] = (i < np ? 0u : nb + 1u) + b;
+    return (i < np ? 0u : nb + 1u) + b;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_sort_keys(Particles<T> p, int np, int total, Geom<T> g, unsigned *__restrict__ keys, int *__restrict__ idx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    keys[i] = sort_key_of(p, i, np, g);
     idx[i] = i;
 }
 
