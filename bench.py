@@ -248,40 +248,117 @@ def kernel_ms_per_step(res) -> dict:
     return out
 
 
-def run_batch(ws, prec: str, steps: int, tune: bool):
-    """The ensemble in one batch: init, autotuned warm-up (smearing tiles and
-    form, sort cadence), `steps` timed steps, then a few profiled steps for the
-    SOR kernel's time and work counters (the mixed form: the ensemble's
-    LDS-resident FP32 launch; FP64: the per-sweep launches)."""
+ENS_SORT_CHOICES = (0, 4, 8, 16)
+RETUNE_EVERY = 500  # steps between re-tunings of the ensemble's sort cadence
+
+
+def run_batch(ws, prec: str, steps: int, warmup: int, mid_steps: int, full: bool):
+    """The ensemble in one batch.  Both forms: init, the autotuner at step 0
+    (smearing tile and form, sort cadence; tune_batch, its trials are steps
+    of the run), `warmup` untimed steps and `steps` timed steps: the early
+    window, the same steps in both forms.  The mixed form with `full` then
+    continues over the workload's whole run, re-tuning the sort cadence
+    every RETUNE_EVERY steps (retune_batch_sort; the particles' order decays
+    along a run), with `mid_steps` timed steps in the middle of the run (the
+    mixed form's headline window) and every step outside the tuner's trials
+    timed for the whole-run rate; a few profiled steps after that window give
+    the stage breakdown and the SOR kernel's time and work counters (the
+    mixed form: the ensemble's LDS-resident FP32 launch; FP64: the per-sweep
+    launches)."""
     import hvamd
-    from hvamd.autotune import tune_batch
+    from hvamd.autotune import retune_batch_sort, tune_batch
     from hvamd.batch import Batch
     p = hvamd.HV_PREC_F64 if prec == "fp64" else hvamd.HV_PREC_MIXED
     prof_steps = 4
-    total_steps = 16 + steps + prof_steps
+    run_len = ws[0].steps if full else 0
     b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws], p)
     for i, w in enumerate(ws):
         b.instance(i).upload_particles(*w.initial_state())
     sks = [w.sky() for w in ws]
-    prm = [w.params(total_steps + 32) for w in ws]
+    prm = [w.params(max(run_len, 256)) for w in ws]
     b.run_init(sks, prm)
-    step = 0
-    best, timings = None, None
-    if tune:
-        best, timings, step = tune_batch(b, sks, prm, 0, steps_per_trial=1, sort_choices=(0, 8),
-                                         tile_choices=(16, 64), form_choices=(0, 1))
-    b.run_steps(sks, prm, step, 2)
-    step += 2
+
+    def log(msg):
+        print(f"[bench] c3 ensemble {prec}: {msg}", file=sys.stderr, flush=True)
+
+    best, timings, step = tune_batch(b, sks, prm, 0, steps_per_trial=2, sort_choices=ENS_SORT_CHOICES,
+                                     tile_choices=(16, 64), form_choices=(0, 1))
+    tunes = [{"at_step": 0, "best": best,
+              "stage_ms_per_step": {f"sort{k[0]}_tile{k[1]}_fused{k[2]}": v for k, v in timings.items()}}]
+    log(f"tuned at step 0: {best}")
+    b.run_steps(sks, prm, step, warmup)
+    step += warmup
     t0 = time.perf_counter()
     _, its = b.run_steps(sks, prm, step, steps)
     wall = time.perf_counter() - t0
+    early = {"first_step": step, "steps": steps, "ms_per_step": 1e3 * wall / steps,
+             "value": sum(w.total for w in ws) * steps / wall,
+             "sor_iters_per_step": {"mean": float(its.mean()), "max": int(its.max())}}
+    log(f"steps {step}-{step + steps} {1e3 * wall / steps:.2f} ms/step")
     step += steps
-    b.profile_enable(True)
-    b.run_steps(sks, prm, step, prof_steps)
-    prof = b.profile_read()
-    cnt = b.sor_counts()
-    b.profile_enable(False)
+    counted = {"steps": steps, "wall": wall}
+    mid = None
+    if full:
+        mid_first = run_len // 2
+
+        def plain(a, z):
+            for c0 in range(a, z, 100):
+                c1 = min(z, c0 + 100)
+                t = time.perf_counter()
+                _, it = b.run_steps(sks, prm, c0, c1 - c0)
+                dt = time.perf_counter() - t
+                counted["steps"] += c1 - c0
+                counted["wall"] += dt
+                log(f"steps {c0}-{c1} {1e3 * dt / (c1 - c0):.2f} ms/step, SOR iterations {it.mean():.1f}")
+
+        def retune():
+            nonlocal step
+            t = step
+            best_s, part, step = retune_batch_sort(b, sks, prm, step, sort_choices=ENS_SORT_CHOICES)
+            tunes.append({"at_step": t, "best": {"sort_every": best_s},
+                          "particle_stage_ms_per_step": {str(k): v for k, v in part.items()}})
+            log(f"sort cadence re-tuned at step {t}: {best_s} {part}")
+
+        for at in range(RETUNE_EVERY, run_len, RETUNE_EVERY):
+            if at > mid_first and step <= mid_first:
+                plain(step, mid_first - 80)
+                step = mid_first - 80
+                retune()
+                plain(step, mid_first)
+                step = mid_first
+                t = time.perf_counter()
+                _, its_m = b.run_steps(sks, prm, step, mid_steps)
+                dtm = time.perf_counter() - t
+                counted["steps"] += mid_steps
+                counted["wall"] += dtm
+                mid = {"first_step": step, "steps": mid_steps, "ms_per_step": 1e3 * dtm / mid_steps,
+                       "value": sum(w.total for w in ws) * mid_steps / dtm,
+                       "sor_iters_per_step": {"mean": float(its_m.mean()), "max": int(its_m.max())}}
+                log(f"mid-run window steps {step}-{step + mid_steps} {1e3 * dtm / mid_steps:.2f} ms/step")
+                step += mid_steps
+                # the stage breakdown and the SOR kernel right after the window
+                b.profile_enable(True)
+                b.run_steps(sks, prm, step, prof_steps)
+                prof = b.profile_read()
+                cnt = b.sor_counts()
+                b.profile_enable(False)
+                step += prof_steps
+            if step < at:
+                plain(step, at)
+                step = at
+            retune()
+        if step < run_len:
+            plain(step, run_len)
+            step = run_len
+    if mid is None:
+        b.profile_enable(True)
+        b.run_steps(sks, prm, step, prof_steps)
+        prof = b.profile_read()
+        cnt = b.sor_counts()
+        b.profile_enable(False)
     body = b.get_option(hvamd.HV_OPT_SOR_F32_OWNED) if prec == "mixed" else None
+    steered = b.get_option(hvamd.HV_OPT_BATCH_DEVICE_STEER)
+    graphs = b.graph_info()
     b.close()
     n_tot = sum(w.total for w in ws)
     node_iters = cnt["node_iters_f32"] if prec == "mixed" else cnt["node_iters_f64"]
@@ -302,14 +379,22 @@ def run_batch(ws, prec: str, steps: int, tune: bool):
         achieved = node_iters * bpn / (launches * avg_ms * 1e-3) / 1e9 if launches else 0.0
         kname = ("kb_sor_colour_f32" if prec == "mixed" else "kb_sor_colour") + \
             " (one red or black sweep over every instance still iterating)"
+    head = mid if mid is not None else early
     return {
-        "value": n_tot * steps / wall, "ms_per_step": 1e3 * wall / steps, "test_particles": n_tot,
-        "instances": len(ws), "grid_nodes": sum(w.grid ** 3 for w in ws),
-        "sor_iters_per_step": {"mean": float(its.mean()), "max": int(its.max())},
-        "tuned": best, "tuning_ms_per_step": {f"sort{k[0]}_tile{k[1]}_fused{k[2]}": v
-                                              for k, v in (timings or {}).items()},
+        "value": head["value"], "ms_per_step": head["ms_per_step"],
+        "window": "mid-run" if mid is not None else "early",
+        "early_window": early, "mid_run_window": mid,
+        "full_run": ({"run_steps": run_len, "steps_timed": counted["steps"],
+                      "ms_per_step": 1e3 * counted["wall"] / counted["steps"],
+                      "value": n_tot * counted["steps"] / counted["wall"],
+                      "note": "every step of the run outside the tuner's trials and the profiled steps"}
+                     if full else None),
+        "test_particles": n_tot, "instances": len(ws), "grid_nodes": sum(w.grid ** 3 for w in ws),
+        "device_steered_step": bool(steered), "graph_steps": graphs["graph_steps"],
+        "tunes": tunes,
         "stage_ms_per_step": {k: v[0] / prof_steps for k, v in prof.items() if v[1] and k in
                               ("particle_pass", "smear", "sor_stage", "potential")},
+        "stage_steps": f"{prof_steps} profiled steps after the {'mid-run' if mid is not None else 'early'} window",
         "roofline": sweep_or_resident_entry(prec, kname, achieved, avg_ms, launches, node_iters * bpn),
     }
 
@@ -461,7 +546,9 @@ def main():
     ap.add_argument("--full", type=int, default=1,
                     help="run every step of the workload (the timed window sits mid-run); 0: tuning and the window only")
     ap.add_argument("--ensemble", type=int, default=256, help="instances of the c3 ensemble (0: skip)")
-    ap.add_argument("--ensemble-steps", type=int, default=8)
+    ap.add_argument("--ensemble-steps", type=int, default=8, help="timed steps of the early ensemble window")
+    ap.add_argument("--ensemble-mid-steps", type=int, default=40,
+                    help="timed steps of the mixed ensemble's mid-run window (with --full 1)")
     ap.add_argument("--fp64", type=int, default=1, help="also time the plain FP64 form")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -543,11 +630,13 @@ def main():
                 out["cpu_baseline_reference_loop_c0"] = lit
     if args.ensemble > 0:
         ens = configs.c3(args.ensemble)
-        bm = run_batch(ens, "mixed", args.ensemble_steps, tune=True)
-        b64 = run_batch(ens, "fp64", args.ensemble_steps, tune=True) if args.fp64 else None
-        out["ensemble_c3"] = {"mixed": bm, "fp64": b64,
-                              "mixed_over_fp64": bm["value"] / b64["value"] if b64 else None,
-                              "mixed_over_single_run_fp64": bm["value"] / out["fp64"]["value"] if args.fp64 else None}
+        bm = run_batch(ens, "mixed", args.ensemble_steps, 2, args.ensemble_mid_steps, full)
+        b64 = run_batch(ens, "fp64", args.ensemble_steps, 2, 0, False) if args.fp64 else None
+        out["ensemble_c3"] = {
+            "mixed": bm, "fp64": b64,
+            # like for like: the same early steps in both forms (FP64 runs only those)
+            "mixed_over_fp64_early_window": bm["early_window"]["value"] / b64["value"] if b64 else None,
+            "mixed_over_single_run_fp64": bm["value"] / out["fp64"]["value"] if args.fp64 else None}
     out["roofline"]["measured_copy_GBs"] = copy_gbs
     print(json.dumps(out), flush=True)
 

@@ -413,6 +413,16 @@ __device__ __forceinline__ void gather_force_from_f(const T *__restrict__ F4s, c
     }
 }
 
+// Sets the status word's outside bit when a lane of the wave holds a particle
+// outside the grid: one atomic per wave, none once the bit is set (particles
+// that left the grid would otherwise all hit the one word).  Called by every
+// lane of the wave.
+__device__ __forceinline__ void flag_outside(bool out, unsigned *status) {
+    if (__ballot(out) != 0ULL && (threadIdx.x & (kWave - 1)) == 0 &&
+        !(__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kStatusOutside))
+        atomicOr(status, kStatusOutside);
+}
+
 // ---- fused particle pass --------------------------------------------------
 // One step of the particle side of the loop, for positions x_s and the field U
 // built from x_s (src/hyper_vlasov.cpp:86-93 and :101-103):
@@ -463,9 +473,9 @@ __device__ __forceinline__ void particle_step_body(const int bid, Particles<T> p
         p.x[i] = x; p.y[i] = y; p.z[i] = z;
         p.kx[i] = kx; p.ky[i] = ky; p.kz[i] = kz;
         has = cic_cell(x, y, z, g, c);
-        if (!has) atomicOr(status, kStatusOutside);
         stat_values(x, y, z, proton, g, v);
     }
+    flag_outside(i < total && !has, status);
     block_deposit(has, c, i < np ? 0 : 1, g, acc, brec, bid);
     stat_commit(v, partials, bid);
 }
@@ -492,9 +502,9 @@ k_deposit(Particles<T> p, int np, int total, Geom<T> g, unsigned long long *__re
         const bool proton = i < np;
         T x = p.x[i], y = p.y[i], z = p.z[i];
         has = cic_cell(x, y, z, g, c);
-        if (acc && !has) atomicOr(status, kStatusOutside);
         stat_values(x, y, z, proton, g, v);
     }
+    if (acc) flag_outside(i < total && !has, status);
     if (acc) block_deposit(has, c, i < np ? 0 : 1, g, acc, brec, blockIdx.x);
     if (partials) stat_commit(v, partials, blockIdx.x);
 }

@@ -73,6 +73,33 @@ def tune_batch(batch, skyrmes, params, first_step: int, steps_per_trial: int = 2
     return {"sort_every": sort_every, "smear_tile": tile, "smear_fused": fused}, timings, step
 
 
+def retune_batch_sort(batch, skyrmes, params, first_step: int, steps_per_trial: int = 8,
+                      sort_choices=(0, 4, 8, 16)):
+    """The sort cadence alone, again, on a running ensemble (the particles'
+    order decays along a run, so the cadence that is fastest at its start is
+    not the one for later steps): each candidate runs one untimed step, then
+    max(cadence, steps_per_trial) steps compared on the particle stage with
+    its sorts.  Returns (best sort_every, {cadence: ms/step}, next
+    first_step)."""
+    step = first_step
+    part = {}
+    for sort_every in sort_choices:
+        batch.set_option(HV_OPT_SORT_EVERY, sort_every)
+        batch.run_steps(skyrmes, params, step, 1)
+        step += 1
+        n = max(sort_every, steps_per_trial)
+        part[sort_every] = _stage_ms(batch, ("particle_pass",), skyrmes, params, step, n)
+        step += n
+    best = min(part, key=part.get)
+    batch.set_option(HV_OPT_SORT_EVERY, best)
+    return best, part, step
+
+
+def retune_batch_steps(steps_per_trial: int = 8, sort_choices=(0, 4, 8, 16)) -> int:
+    """Steps one retune_batch_sort round runs."""
+    return sum(1 + max(c, steps_per_trial) for c in sort_choices)
+
+
 SINGLE_SORT_CHOICES = (0, 2, 4, 8, 16, 32)
 
 
