@@ -21,7 +21,7 @@
 #include "hv_batch_resident.h"
 #include "hv_batch_steer.h"
 
-enum { kBPart, kBSmear, kBZy, kBN2, kBG, kBGh2, kBSor, kBKinds };
+enum { kBPart, kBSmear, kBZy, kBN2, kBG, kBGh2, kBSor, kBN2V, kBKinds };
 
 struct hv_batch {
     int n = 0;
@@ -34,6 +34,7 @@ struct hv_batch {
     unsigned blocks[kBKinds] = {};
     size_t smear_lds = 0, smear2_lds = 0;
     int smear_fused = 1;                  // option: fused smearing (else two-pass)
+    bool xs_vec = false;                  // two-pass: the vector x pass fits every instance
     double *maxc = nullptr;               // collected SOR maxima, n x maxc_width
     int maxc_width = 0;
     int *act = nullptr;                   // SOR launch subset: prefix table (n + 1) then ids (n)
@@ -228,6 +229,7 @@ template <typename T>
 int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslots) {
     std::vector<int> hf((size_t)kBKinds * (b->n + 1), 0);
     size_t lds = 0, lds2 = 0;
+    bool xs_vec = true;  // every instance fits the vector x pass
     for (int i = 0; i < b->n; i++) {
         hv_ctx *c = b->inst[i];
         int rc = ensure_smear<T>(c, prm[i].sigma_r, prm[i].part_per_nucleon);
@@ -248,7 +250,9 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
             (long long)blocks_for(c->G),
             (long long)blocks_for(2 * c->sp.Gh),
             (long long)tl.per_xcd * kXcds,  // a multiple of 8: local block b runs on XCD b % 8
+            (long long)blocks_for(2 * c->G / smear_vw<T>()),
         };
+        xs_vec = xs_vec && c->world.n[2] % smear_vw<T>() == 0 && c->xs_rxm > 0;
         for (int k = 0; k < kBKinds; k++) {
             const long long nx = hf[(size_t)k * (b->n + 1) + i] + cnt[k];
             if (nx > 0x7fffffffLL) return batch_err(b, HV_ERR_ARG, "batch too large for one launch");
@@ -258,6 +262,7 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
         lds2 = c->stl_lds > lds2 ? c->stl_lds : lds2;
     }
     for (int k = 0; k < kBKinds; k++) b->blocks[k] = (unsigned)hf[(size_t)k * (b->n + 1) + b->n];
+    b->xs_vec = xs_vec;
     b->sor_blocks.resize(b->n);
     for (int i = 0; i < b->n; i++)
         b->sor_blocks[i] = hf[(size_t)kBSor * (b->n + 1) + i + 1] - hf[(size_t)kBSor * (b->n + 1) + i];
@@ -918,7 +923,10 @@ int batch_step_device(hv_batch *b, int end_kick, bool res, bool steer = false, b
                                                                                               n);
         } else {
             kb_smear_zy<T><<<b->blocks[kBZy], kSmearThreads, b->smear2_lds, b->stream>>>(a, btab(b, kBZy), n);
-            kb_smear_xs<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
+            if (b->xs_vec)
+                kb_smear_xs_vec<T><<<b->blocks[kBN2V], kThreads, 0, b->stream>>>(a, btab(b, kBN2V), n);
+            else
+                kb_smear_xs<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
         }
         kb_zero_acc<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
         HV_BHIP(b, hipGetLastError());

@@ -118,6 +118,24 @@ kb_smear_xs(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n
     smear_xs_body<T>(bid, A.D, A.rho, A.fmask, nullptr, 0, smear_tab_c(A.tab), A.g.nx, A.g.ny, A.g.nz, A.st2);
 }
 
+// The vector x pass (smear_xs_vec_body: VW nodes per thread, the loads of
+// all planes issued together) with each instance's radius bucket; blocks of
+// the 2G / VW table.  The host uses it when every instance has nz % VW == 0
+// and a radius of at most 14.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+kb_smear_xs_vec(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    HV_BATCH_LOCATE();
+    SmearTabC<T> *tb = smear_tab_c(A.tab);
+    const int rx = tb->rx;  // wave-uniform: one instance per block
+    if (rx <= 4)
+        smear_xs_vec_body<T, 4>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2);
+    else if (rx <= 9)
+        smear_xs_vec_body<T, 9>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2);
+    else
+        smear_xs_vec_body<T, 14>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kSmearThreads)
 kb_smear_fused(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {

@@ -1159,8 +1159,18 @@ __device__ __forceinline__ void smear_zy_body(const int bid, const CT *__restric
     t /= st.njt;
     const int i = t % nx;
     const int sp = t / nx;
-    smear_zy_item<T, CT>(sp, i, jt, kt, cnt, D, fmask, cbox ? cbox + (long long)sp * cw : nullptr, tb, nx, ny, nz,
-                         st);
+    // the halo's coordinates once per item when the staged form fits (no
+    // division per count element)
+    const int R = st.TJ + 2 * tb->ry, W = st.TK + 2 * tb->rz;
+    if (smear_vec_tile<T>(st.TJ, st.TK) && R * W <= kZyStaged * kSmearThreads) {
+        int ecoord[kZyStaged];
+        smear_zy_coords(R, W, ecoord);
+        smear_zy_item<T, CT>(sp, i, jt, kt, cnt, D, fmask, cbox ? cbox + (long long)sp * cw : nullptr, tb, nx, ny,
+                             nz, st, ecoord);
+    } else {
+        smear_zy_item<T, CT>(sp, i, jt, kt, cnt, D, fmask, cbox ? cbox + (long long)sp * cw : nullptr, tb, nx, ny,
+                             nz, st);
+    }
 }
 
 // The items of the count box only (planes that hold counts times the tiles

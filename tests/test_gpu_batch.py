@@ -104,12 +104,14 @@ def test_batch_equals_solo_runs(form):
 
 def test_batch_options_do_not_change_bits():
     """Sort cadence, smearing tile width (the autotuned knobs), the smearing
-    form and the SOR instance grouping (1 KiB: every instance a group of its
-    own) leave every bit of a batched run unchanged."""
+    form (fused; two-pass with the vector tile and staged halo coordinates;
+    two-pass with narrow tiles) and the SOR instance grouping (1 KiB: every
+    instance a group of its own) leave every bit of a batched run
+    unchanged."""
     print("batch options", flush=True)
     ws, params = _instances()
     outs = []
-    for sort_every, tile, group_kib, fused in ((0, 64, 0, 1), (3, 16, 1, 0)):
+    for sort_every, tile, group_kib, fused in ((0, 64, 0, 1), (3, 16, 1, 0), (4, 64, 0, 0)):
         b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws],
                   hvamd.HV_PREC_MIXED)
         b.set_option(hvamd.HV_OPT_SORT_EVERY, sort_every)
@@ -123,10 +125,11 @@ def test_batch_options_do_not_change_bits():
         s, it = b.run_steps(sks, params, 0, 6)
         outs.append((s, it, [b.instance(i).download_particles()["kz"] for i in range(len(ws))]))
         b.close()
-    assert np.array_equal(outs[0][0], outs[1][0])
-    assert np.array_equal(outs[0][1], outs[1][1])
-    for a, c in zip(outs[0][2], outs[1][2]):
-        assert np.array_equal(a, c)
+    for o in outs[1:]:
+        assert np.array_equal(outs[0][0], o[0])
+        assert np.array_equal(outs[0][1], o[1])
+        for a, c in zip(outs[0][2], o[2]):
+            assert np.array_equal(a, c)
 
 
 @pytest.mark.parametrize("prec", [hvamd.HV_PREC_F64, hvamd.HV_PREC_MIXED])

@@ -176,17 +176,24 @@ def _solo(w, prm, prec, steps, resident):
         sim.close()
 
 
-@pytest.mark.parametrize("form", ["f64", "mixed_resident"])
+@pytest.mark.parametrize("form", ["f64", "mixed_resident", "mixed_twopass_sorted"])
 def test_c3_real_instances_batch_equals_solo(form):
     """Five real ensemble instances (32^3, 64^3 and 96^3 grids, 5.7k-127k test
     particles, their own Skyrme sets and boosts) in one batch with the
-    ensemble's default options: each equals its solo run bit for bit."""
+    ensemble's default options, or (mixed_twopass_sorted) with the options
+    its autotuner picks on the full ensemble (the two-pass smearing with the
+    widest tiles, the batched sort every 4 steps): each equals its solo run
+    bit for bit."""
     steps = 12 if form == "f64" else 24
     prec = hvamd.HV_PREC_F64 if form == "f64" else hvamd.HV_PREC_MIXED
     ws, prms = _c3_instances(steps)
     print(f"c3 real instances form={form}: " + ", ".join(f"{w.name} {w.total} tp" for w in ws), flush=True)
     b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws], prec)
     try:
+        if form == "mixed_twopass_sorted":
+            b.set_option(hvamd.HV_OPT_SMEAR_FUSED, 0)
+            b.set_option(hvamd.HV_OPT_SMEAR_TILE, 64)
+            b.set_option(hvamd.HV_OPT_SORT_EVERY, 4)
         for i, w in enumerate(ws):
             b.instance(i).upload_particles(*w.initial_state())
         sks = [w.sky() for w in ws]
