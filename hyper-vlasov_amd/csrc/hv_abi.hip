@@ -16,7 +16,6 @@
 #include <new>
 #include <vector>
 
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "hv_abi.h"
 #include "hv_kernels.h"
@@ -106,10 +105,8 @@ struct hv_ctx {
     // caller's order once sorted; pid maps storage slot -> original index
     int *pid = nullptr, *pid_alt = nullptr;
     void *part_alt[9] = {};
-    unsigned *keys[2] = {};
-    int *ord[2] = {};
-    void *sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
+    unsigned *keys[2] = {};             // [0]: sort keys; [1]: key counts / first slots (sort_buckets)
+    int *ord[2] = {};                   // [0]: ranks within a key
     int sort_every = 8;  // steps between sorts; 0 = never
     int smear_tk_max = 64;  // widest smearing tile (columns) tried
     bool own_stream = true; // false for the instances of a batch (the batch's stream)
@@ -142,6 +139,7 @@ struct hv_ctx {
     int *res32_err = nullptr;
     float *res32_result = nullptr;
     float *res32_hres = nullptr;        // pinned: iterations, tested max, error word
+    double xread[3] = {0.0, 0.0, 0.0};  // the resident FP32 solve's result read back with the FP64 sweeps' first chunk
     int since_sort = 1 << 30;
     bool permuted = false;
 
@@ -523,7 +521,7 @@ int ensure_maxd(hv_ctx *c, int max_it) {
     c->maxc = nullptr;
     c->maxd_cap = 0;
     HV_HIP(c, hipMalloc((void **)&c->maxd, sizeof(unsigned long long) * maxd_words(max_it)));
-    HV_HIP(c, hipMalloc((void **)&c->maxc, sizeof(double) * (size_t)max_it));
+    HV_HIP(c, hipMalloc((void **)&c->maxc, sizeof(double) * ((size_t)max_it + 4)));  // + the resident solve's result
     c->maxd_cap = max_it;
     return HV_OK;
 }
@@ -706,7 +704,9 @@ int res32_setup(hv_ctx *c) {
 // c->sor_R, in one cooperative launch, enqueued with the copy of its
 // iteration count and error word to pinned memory; sor_resident_f32_collect
 // reads them after a wait on the stream.
-int sor_resident_f32_enqueue(hv_ctx *c, const SorCoef &cf, int max_it) {
+// Slot rows for solves of up to max_it iterations, and the words the solve
+// expects cleared (the defect launch before it clears them).
+int sor_resident_f32_clears(hv_ctx *c, int max_it, ResClears *cl) {
     const int nT = c->rtl32.PI * c->rtl32.PJ;
     if (c->res32_slots_cap < max_it) {
         if (c->res32_slots) (void)hipFree(c->res32_slots);
@@ -715,12 +715,18 @@ int sor_resident_f32_enqueue(hv_ctx *c, const SorCoef &cf, int max_it) {
         HV_HIP(c, hipMalloc((void **)&c->res32_slots, sizeof(float) * (size_t)max_it * nT));
         c->res32_slots_cap = max_it;
     }
-    int rc = fill<float>(c, c->res32_slots, -1.0f, (long long)max_it * nT);
-    if (rc) return rc;
-    rc = fill<unsigned long long>(c, c->res32_x, 0ULL, 2LL * nT * c->rtl32.xstride);
-    if (rc) return rc;
-    rc = fill<int>(c, c->res32_err, 0, 4);
-    if (rc) return rc;
+    cl->slots = c->res32_slots;
+    cl->nslots = (long long)max_it * nT;
+    cl->x = c->res32_x;
+    cl->nx = 2LL * nT * c->rtl32.xstride;
+    cl->err = c->res32_err;
+    return HV_OK;
+}
+
+// (after the clears of sor_resident_f32_clears; its iteration count, tested
+// max and error word are read back with the FP64 sweeps' first chunk)
+int sor_resident_f32_enqueue(hv_ctx *c, const SorCoef &cf, int max_it) {
+    const int nT = c->rtl32.PI * c->rtl32.PJ;
     ResSync32 sy;
 #ifdef HV_RES_TRACE
     static unsigned long long *trace32 = nullptr;
@@ -749,17 +755,13 @@ int sor_resident_f32_enqueue(hv_ctx *c, const SorCoef &cf, int max_it) {
         if (pb) (void)hipEventRecord(pb, c->stream);
     }
     if (pa && pb) c->pending.push_back({HV_KCLASS_SOR_RES_F32, pa, pb, 1});
-    HV_HIP(c, hipMemcpyAsync(c->res32_hres, c->res32_result, 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    HV_HIP(c, hipMemcpyAsync(c->res32_hres + 2, c->res32_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     return HV_OK;
 }
 
+// the resident solve's result, as read back by the FP64 sweeps' first chunk
 int sor_resident_f32_collect(hv_ctx *c, int *iters_out) {
-    HV_HIP(c, hipStreamSynchronize(c->stream));
-    int err = 0;
-    std::memcpy(&err, c->res32_hres + 2, sizeof err);
-    if (err) return set_err(c, HV_ERR_HIP, "LDS-resident FP32 SOR: a tile wait exceeded its spin limit");
-    *iters_out = (int)c->res32_hres[0];
+    if (c->xread[2] != 0.0) return set_err(c, HV_ERR_HIP, "LDS-resident FP32 SOR: a tile wait exceeded its spin limit");
+    *iters_out = (int)c->xread[0];
     prof_flush(c);
     return HV_OK;
 }
@@ -833,29 +835,34 @@ struct NoHook {
     int operator()(int &) const { return HV_OK; }
 };
 
+// xres / xerr (optional): the resident FP32 solve's result and error word,
+// read back with the first chunk into c->xread before after_first_wait.
 template <typename Launch, typename Hook = NoHook>
 int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double *md_out, Launch launch,
-                Hook after_first_wait = Hook()) {
+                Hook after_first_wait = Hook(), const float *xres = nullptr, const int *xerr = nullptr) {
     const double tol = cf.tol;
-    int rc = fill<unsigned long long>(c, c->maxd, 0ULL, (long long)maxd_words(max_it));
-    if (rc) return rc;
-    rc = fill<int>(c, (int *)c->ctl, 0, 2);
-    if (rc) return rc;
+    int rc = HV_OK;
     int it = 0, iters = -1, start = 0;
     int chunk = c->sor_pred > 2 ? c->sor_pred - 1 : 1;
     bool first = true;
     while (it < max_it) {
         start = it;
         int end = it + chunk < max_it ? it + chunk : max_it;
+        // this chunk's rows of the slot maxima (and SorCtl at the start)
+        k_sor_chunk_reset<<<(unsigned)(((end - start) * kMaxSlots + 255) / 256), 256, 0, c->stream>>>(
+            c->maxd, start, end, c->ctl, first ? 1 : 0);
         for (; it < end; it++) launch(it);
         HV_CHECK_LAUNCH(c);
-        k_sor_collect<<<(unsigned)(end - start), 64, 0, c->stream>>>(c->maxd, c->maxc, start, end);
+        const float *xr = first ? xres : nullptr;
+        k_sor_collect_x<<<(unsigned)(end - start), 64, 0, c->stream>>>(c->maxd, c->maxc, start, end, xr, xerr);
         HV_CHECK_LAUNCH(c);
-        HV_HIP(c, hipMemcpyAsync(c->pinned, c->maxc, sizeof(double) * (size_t)(end - start),
+        HV_HIP(c, hipMemcpyAsync(c->pinned, c->maxc, sizeof(double) * (size_t)(end - start + (xr ? 3 : 0)),
                                  hipMemcpyDeviceToHost, c->stream));
         HV_HIP(c, hipStreamSynchronize(c->stream));
         if (first) {
             first = false;
+            if (xr)
+                for (int q = 0; q < 3; q++) c->xread[q] = c->pinned[end - start + q];
             rc = after_first_wait(max_it);
             if (rc) return rc;
         }
@@ -882,7 +889,7 @@ int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double
 template <typename RT, typename Hook = NoHook>
 int sor_exact(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, bool fused, dim3 sgrd, dim3 sblk,
               const SorTiles &tl, dim3 fgrd, dim3 fblk, const FusedTiles &ft, size_t rows_lds, int *iters,
-              double *md, Hook after_first_wait = Hook()) {
+              double *md, Hook after_first_wait = Hook(), const float *xres = nullptr, const int *xerr = nullptr) {
     if (fused) {
         double *pbuf[2] = {c->phi, c->phi_alt};
         int rc = sor_iterate(
@@ -892,7 +899,7 @@ int sor_exact(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, bool fuse
                            rows_lds * sizeof(double), pbuf[it & 1], pbuf[(it + 1) & 1], (const RT *)rho, c->sp, ft, cf,
                            it, c->maxd, c->ctl);
             },
-            after_first_wait);
+            after_first_wait, xres, xerr);
         if (rc) return rc;
         if (*iters & 1) std::swap(c->phi, c->phi_alt);
         return HV_OK;
@@ -904,7 +911,7 @@ int sor_exact(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, bool fuse
                 sor_launch(c, it, colour, HV_KCLASS_SOR, k_sor_colour<RT>, sgrd, sblk, 0, c->phi, (const RT *)rho,
                            c->sp, tl, cf, colour, it, c->maxd, c->ctl);
         },
-        after_first_wait);
+        after_first_wait, xres, xerr);
 }
 
 // SOR coefficients of the context's grid (include/physics.hpp:101-103)
@@ -1009,20 +1016,28 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         }
         // FP32 defect-correction iterations on the small correction of a warm
         // potential, then FP64 sweeps until the stop rule holds in FP64
-        k_sor_defect<RT><<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, rho, c->sor_R, c->sor_delta,
-                                                                              c->sp, cf);
-        HV_CHECK_LAUNCH(c);
         float *dbuf[2] = {c->sor_delta, c->sor_delta_alt};
-        bool res32 = false;
+        const bool res32 = c->sor_res32 && res32_setup(c) == HV_OK;
+        if (res32) {
+            // the defect launch also clears the resident solve's words
+            ResClears cl;
+            rc = sor_resident_f32_clears(c, max_it, &cl);
+            if (rc) return rc;
+            k_sor_defect_clear<RT><<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(
+                c->phi, rho, c->sor_R, c->sor_delta, c->sp, cf, cl);
+        } else {
+            k_sor_defect<RT><<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, rho, c->sor_R,
+                                                                                  c->sor_delta, c->sp, cf);
+        }
+        HV_CHECK_LAUNCH(c);
         // the FP32 stage may stop below the reference tolerance (its last
         // potential then carries less of the stop rule's own error into the
         // next start); the FP64 sweeps apply the reference rule either way
         SorCoef cf32 = cf;
         cf32.tol = ldexp(cf.tol, -c->f32_tol_shift);
-        if (c->sor_res32 && res32_setup(c) == HV_OK) {
-            // its iteration count is read in the FP64 sweeps' first wait
+        if (res32) {
+            // its iteration count is read back with the FP64 sweeps' first chunk
             rc = sor_resident_f32_enqueue(c, cf32, max_it);
-            res32 = true;
         } else if (fused) {
             rc = sor_iterate(c, cf32, max_it, &iters, &md, [&](int it) {
                 sor_launch(c, it, 1, HV_KCLASS_SOR_F32, k_sor_fused<false, float, float>, fgrd, fblk,
@@ -1053,7 +1068,8 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
             return HV_OK;
         };
         rc = sor_exact(c, rho, cf, res32 ? max_it : mixed_fp64_budget(max_it, iters), fused, sgrd, sblk, tl, fgrd,
-                       fblk, ft, rows_lds, &it64, &md, hook);
+                       fblk, ft, rows_lds, &it64, &md, hook, res32 ? (const float *)c->res32_result : nullptr,
+                       res32 ? (const int *)c->res32_err : nullptr);
         if (rc) return rc;
         c->mixed_fp64_sweeps = it64;
         c->sor_pred = iters;
@@ -1133,27 +1149,26 @@ Particles<T> particles_alt(const hv_ctx *c) {
     return p;
 }
 
-int sort_key_bits(const hv_ctx *c) {
+// key buckets of the counting sort: 2 (bricks + 1)
+long long sort_buckets(const hv_ctx *c) {
     const long long nb = (long long)((c->world.n[0] + 3) >> kBrickShift) * ((c->world.n[1] + 3) >> kBrickShift) *
                          ((c->world.n[2] + 3) >> kBrickShift);
-    const long long kmax = 2 * (nb + 1);
-    int bits = 1;
-    while ((1LL << bits) < kmax) bits++;
-    return bits;
+    return 2 * (nb + 1);
 }
 
-// Reorders the particle arrays by (species, brick of the cell), stable.
+// Reorders the particle arrays by (species, brick of the cell): the counting
+// sort of hv_kernels.h (counts with ranks, exclusive scan, scatter into the
+// other buffer set), then the buffer sets swap.
 template <typename T>
 int sort_particles(hv_ctx *c) {
     const int n = c->total;
-    k_sort_keys<T><<<blocks_for(n), kThreads, 0, c->stream>>>(particles<T>(c), c->np, n, make_geom<T>(c), c->keys[0],
-                                                              c->ord[0]);
-    HV_CHECK_LAUNCH(c);
-    size_t bytes = c->sort_tmp_bytes;
-    HV_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp, bytes, c->keys[0], c->keys[1], c->ord[0], c->ord[1],
-                                        (unsigned)n, 0, sort_key_bits(c), c->stream));
-    k_permute<T><<<blocks_for(n), kThreads, 0, c->stream>>>(particles<T>(c), particles_alt<T>(c), c->pid,
-                                                            c->pid_alt, c->ord[1], n);
+    const long long m = sort_buckets(c);
+    k_fill<unsigned><<<(unsigned)((m + kThreads - 1) / kThreads), kThreads, 0, c->stream>>>(c->keys[1], 0u, m);
+    k_sort_count<T><<<blocks_for(n), kThreads, 0, c->stream>>>(particles<T>(c), c->np, n, make_geom<T>(c),
+                                                               c->keys[1], c->keys[0], c->ord[0]);
+    k_sort_scan<<<1, kSortScanThreads, 0, c->stream>>>(c->keys[1], (int)m);
+    k_sort_scatter<T><<<blocks_for(n), kThreads, 0, c->stream>>>(particles<T>(c), particles_alt<T>(c), c->pid,
+                                                                 c->pid_alt, n, c->keys[1], c->keys[0], c->ord[0]);
     HV_CHECK_LAUNCH(c);
     for (int q = 0; q < 9; q++) std::swap(c->part[q], c->part_alt[q]);
     std::swap(c->pid, c->pid_alt);
@@ -1507,17 +1522,9 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
     for (int q = 0; q < 9 && ok; q++) ok = hipMalloc(&c->part_alt[q], ts * (size_t)c->total) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->pid, sizeof(int) * (size_t)c->total) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->pid_alt, sizeof(int) * (size_t)c->total) == hipSuccess;
-    for (int q = 0; q < 2 && ok; q++) {
-        ok = hipMalloc((void **)&c->keys[q], sizeof(unsigned) * (size_t)c->total) == hipSuccess;
-        ok = ok && hipMalloc((void **)&c->ord[q], sizeof(int) * (size_t)c->total) == hipSuccess;
-    }
-    if (ok) {
-        size_t bytes = 0;
-        ok = rocprim::radix_sort_pairs(nullptr, bytes, c->keys[0], c->keys[1], c->ord[0], c->ord[1],
-                                       (unsigned)c->total, 0, sort_key_bits(c), c->stream) == hipSuccess;
-        ok = ok && hipMalloc(&c->sort_tmp, bytes + 256) == hipSuccess;
-        c->sort_tmp_bytes = bytes + 256;
-    }
+    ok = ok && hipMalloc((void **)&c->keys[0], sizeof(unsigned) * (size_t)c->total) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->ord[0], sizeof(int) * (size_t)c->total) == hipSuccess;
+    ok = ok && hipMalloc((void **)&c->keys[1], sizeof(unsigned) * (size_t)sort_buckets(c)) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->acc, sizeof(unsigned long long) * (size_t)acc_words(c)) == hipSuccess;
     ok = ok && hipMalloc(&c->density, ts * (size_t)(2 * c->G)) == hipSuccess;
     ok = ok && hipMalloc(&c->U, ts * (size_t)(2 * c->G)) == hipSuccess;
@@ -1570,7 +1577,7 @@ int hv_ctx_destroy(hv_ctx *c) {
         if (c->part[q]) (void)hipFree(c->part[q]);
         if (c->part_alt[q]) (void)hipFree(c->part_alt[q]);
     }
-    void *sbufs[] = {c->pid, c->pid_alt, c->keys[0], c->keys[1], c->ord[0], c->ord[1], c->sort_tmp,
+    void *sbufs[] = {c->pid, c->pid_alt, c->keys[0], c->keys[1], c->ord[0],
                      c->res_x, c->res_flag, c->res_slots, c->res_err, c->res_result,
                      c->res32_x, c->res32_slots, c->res32_err, c->res32_result};
     for (void *b : sbufs)

@@ -252,56 +252,21 @@ __global__ void kb_sor_collect(const BInst<T> *__restrict__ a, double *__restric
 }
 
 // ---- batched particle sort ---------------------------------------------------
-// A counting sort of every instance's particles by sort_key_of in three
-// launches (counts with ranks, an exclusive scan per instance, the scatter
-// into the other buffer set) and a pointer swap on the device, so that a
-// captured step graph can hold it (no memset, no host decision).  The order
-// within a key follows the order of the count atomics; no result depends on
-// the storage order (the deposits and diagnostics are exact integer sums,
-// every other operation is per particle), and pid keeps each slot's
-// original index for the download.
+// The counting sort of hv_kernels.h (sort_count_body, sort_scan_body,
+// sort_scatter_body) over every instance, each with its own run of key
+// buckets from sbase, and a pointer swap on the device.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_sort_count(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n, unsigned *__restrict__ buck) {
     HV_BATCH_LOCATE();
     const int q = bid * (int)blockDim.x + (int)threadIdx.x;
-    if (q >= A.total) return;
-    const unsigned k = sort_key_of(A.p, q, A.np, A.g);
-    A.skey[q] = k;
-    A.srank[q] = (int)atomicAdd(&buck[A.sbase + k], 1u);
+    sort_count_body(A.p, A.np, A.g, q, q < A.total, buck + A.sbase, A.skey, A.srank);
 }
 
-// one block of kSortScanThreads per instance: its key counts (sfirst[i] ..
-// sfirst[i + 1]) become the first slot of each key
-constexpr int kSortScanThreads = 1024;
-
+// one block per instance: its key counts (sfirst[i] .. sfirst[i + 1])
 __global__ void __launch_bounds__(kSortScanThreads)
 kb_sort_scan(const long long *__restrict__ sfirst, unsigned *__restrict__ buck) {
-    __shared__ unsigned wsum[kSortScanThreads / kWave];
-    unsigned *b = buck + sfirst[blockIdx.x];
-    const int m = (int)(sfirst[blockIdx.x + 1] - sfirst[blockIdx.x]);
-    const int per = (m + kSortScanThreads - 1) / kSortScanThreads;
-    const int lo = (int)threadIdx.x * per, hi = min(m, lo + per);
-    unsigned s = 0;
-    for (int q = lo; q < hi; q++) s += b[q];
-    // inclusive scan over the wave, then over the waves' totals
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    unsigned inc = s;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const unsigned v = __shfl_up(inc, d, kWave);
-        if (lane >= d) inc += v;
-    }
-    if (lane == kWave - 1) wsum[wave] = inc;
-    __syncthreads();
-    unsigned before = 0;
-    for (int w = 0; w < wave; w++) before += wsum[w];
-    unsigned run = before + inc - s;
-    for (int q = lo; q < hi; q++) {
-        const unsigned v = b[q];
-        b[q] = run;
-        run += v;
-    }
+    sort_scan_body(buck + sfirst[blockIdx.x], (int)(sfirst[blockIdx.x + 1] - sfirst[blockIdx.x]));
 }
 
 template <typename T>
@@ -310,12 +275,7 @@ kb_sort_scatter(const BInst<T> *__restrict__ a, const int *__restrict__ first, i
                 const unsigned *__restrict__ buck) {
     HV_BATCH_LOCATE();
     const int q = bid * (int)blockDim.x + (int)threadIdx.x;
-    if (q >= A.total) return;
-    const int d = (int)buck[A.sbase + A.skey[q]] + A.srank[q];
-    A.palt.x[d] = A.p.x[q]; A.palt.y[d] = A.p.y[q]; A.palt.z[d] = A.p.z[q];
-    A.palt.kx[d] = A.p.kx[q]; A.palt.ky[d] = A.p.ky[q]; A.palt.kz[d] = A.p.kz[q];
-    A.palt.fx[d] = A.p.fx[q]; A.palt.fy[d] = A.p.fy[q]; A.palt.fz[d] = A.p.fz[q];
-    A.pid_alt[d] = A.pid[q];
+    if (q < A.total) sort_scatter_body(A.p, A.palt, A.pid, A.pid_alt, q, buck + A.sbase, A.skey, A.srank);
 }
 
 // the sorted set becomes the current one (one thread per instance)

@@ -196,7 +196,7 @@ k_cbox_build(const int *__restrict__ brec, int nrec, unsigned *__restrict__ cbox
 }
 
 // ---- block-local CIC deposit ----------------------------------------------------
-// When the particles of a block are sorted by cell (see k_sort_keys), their
+// When the particles of a block are sorted by cell (see sort_key_of), their
 // cells fill a small box; the block then accumulates the box of nodes in LDS
 // (exact int64 adds) and flushes only the non-zero nodes to HBM, one global
 // atomic per node instead of eight per particle.  Blocks whose box is too big
@@ -317,27 +317,112 @@ __device__ __forceinline__ unsigned sort_key_of(const Particles<T> &p, int i, in
     return (i < np ? 0u : nb + 1u) + b;
 }
 
+// ---- counting sort by key ------------------------------------------------------
+// Three launches (counts with ranks, an exclusive scan of the key counts, the
+// scatter into the other buffer set); no memset and no host decision, so a
+// captured graph can hold it.  The order within a key follows the order of
+// the count atomics: no result depends on the storage order (the deposits and
+// diagnostics are exact integer sums, every other operation is per particle),
+// and pid keeps each slot's original index for the download.
+
+// Particle q's key into skey[q] and its rank within the key into srank[q];
+// buck[k] counts key k.  Runs of equal keys in consecutive lanes (particles
+// are mostly in key order; those outside the grid share one key per species)
+// take one atomic per run: its head adds the run's length and the lanes rank
+// after it.  Called by every lane of the wave (valid: q is a particle).
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_sort_keys(Particles<T> p, int np, int total, Geom<T> g, unsigned *__restrict__ keys, int *__restrict__ idx) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    keys[i] = sort_key_of(p, i, np, g);
-    idx[i] = i;
+__device__ __forceinline__ void sort_count_body(const Particles<T> &p, int np, const Geom<T> &g, int q, bool valid,
+                                                unsigned *__restrict__ buck, unsigned *__restrict__ skey,
+                                                int *__restrict__ srank) {
+    const unsigned k = valid ? sort_key_of(p, q, np, g) : 0xffffffffu;
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const unsigned kp = __shfl_up(k, 1, kWave);
+    const unsigned long long heads = __ballot(lane == 0 || kp != k);
+    const unsigned long long upto = (2ULL << lane) - 1ULL;  // lanes 0..lane
+    const int h = 63 - __clzll((long long)(heads & upto));
+    const unsigned long long above = heads & ~upto;
+    const int next = above ? __ffsll((long long)above) - 1 : kWave;
+    unsigned base = 0;
+    if (valid && lane == h) base = atomicAdd(&buck[k], (unsigned)(next - h));
+    base = __shfl(base, h, kWave);
+    if (!valid) return;
+    skey[q] = k;
+    srank[q] = (int)(base + (unsigned)(lane - h));
 }
 
-// gather the particle arrays (and ids) in sorted order into the other buffer set
+// The m key counts at b become the first slot of each key (exclusive scan),
+// by one block of kSortScanThreads: tiles of 4 kSortScanThreads counts, each
+// thread four consecutive ones (coalesced), scanned over the block with the
+// running total of the earlier tiles carried.
+constexpr int kSortScanThreads = 1024;
+
+__device__ __forceinline__ void sort_scan_body(unsigned *__restrict__ b, int m) {
+    __shared__ unsigned wsum[kSortScanThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    unsigned carry = 0;
+    for (int base = 0; base < m; base += 4 * kSortScanThreads) {
+        const int e0 = base + 4 * (int)threadIdx.x;
+        unsigned v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = e0 + u < m ? b[e0 + u] : 0u;
+        const unsigned s = v[0] + v[1] + v[2] + v[3];
+        // inclusive scan over the wave, then over the waves' totals
+        unsigned inc = s;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const unsigned t = __shfl_up(inc, d, kWave);
+            if (lane >= d) inc += t;
+        }
+        if (lane == kWave - 1) wsum[wave] = inc;
+        __syncthreads();
+        unsigned before = 0, total = 0;
+        for (int w = 0; w < kSortScanThreads / kWave; w++) {
+            const unsigned t = wsum[w];
+            before += w < wave ? t : 0u;
+            total += t;
+        }
+        unsigned run = carry + before + inc - s;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (e0 + u < m) b[e0 + u] = run;
+            run += v[u];
+        }
+        carry += total;
+        __syncthreads();  // wsum is rewritten by the next tile
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void sort_scatter_body(const Particles<T> &src, const Particles<T> &dst,
+                                                  const int *__restrict__ pid_src, int *__restrict__ pid_dst, int q,
+                                                  const unsigned *__restrict__ buck, const unsigned *__restrict__ skey,
+                                                  const int *__restrict__ srank) {
+    const int d = (int)buck[skey[q]] + srank[q];
+    dst.x[d] = src.x[q]; dst.y[d] = src.y[q]; dst.z[d] = src.z[q];
+    dst.kx[d] = src.kx[q]; dst.ky[d] = src.ky[q]; dst.kz[d] = src.kz[q];
+    dst.fx[d] = src.fx[q]; dst.fy[d] = src.fy[q]; dst.fz[d] = src.fz[q];
+    pid_dst[d] = pid_src[q];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_permute(Particles<T> src, Particles<T> dst, const int *__restrict__ pid_src, int *__restrict__ pid_dst,
-          const int *__restrict__ order, int total) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= total) return;
-    const int o = order[s];
-    dst.x[s] = src.x[o]; dst.y[s] = src.y[o]; dst.z[s] = src.z[o];
-    dst.kx[s] = src.kx[o]; dst.ky[s] = src.ky[o]; dst.kz[s] = src.kz[o];
-    dst.fx[s] = src.fx[o]; dst.fy[s] = src.fy[o]; dst.fz[s] = src.fz[o];
-    pid_dst[s] = pid_src[o];
+k_sort_count(Particles<T> p, int np, int total, Geom<T> g, unsigned *__restrict__ buck, unsigned *__restrict__ skey,
+             int *__restrict__ srank) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    sort_count_body(p, np, g, q, q < total, buck, skey, srank);
+}
+
+__global__ void __launch_bounds__(kSortScanThreads) k_sort_scan(unsigned *__restrict__ buck, int m) {
+    sort_scan_body(buck, m);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_sort_scatter(Particles<T> src, Particles<T> dst, const int *__restrict__ pid_src, int *__restrict__ pid_dst,
+               int total, const unsigned *__restrict__ buck, const unsigned *__restrict__ skey,
+               const int *__restrict__ srank) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < total) sort_scatter_body(src, dst, pid_src, pid_dst, q, buck, skey, srank);
 }
 
 // storage order -> original particle order
@@ -1763,6 +1848,29 @@ k_sor_defect(const double *__restrict__ phi, const RT *__restrict__ rho, float *
     sor_defect_body<RT>(blockIdx.x, phi, rho, R, delta, sp, cf);
 }
 
+// The words the LDS-resident FP32 solve expects cleared (its slot rows to
+// -1, its exchange words and error word to 0), cleared by the defect launch
+// that precedes it instead of by launches of their own.
+struct ResClears {
+    float *slots;
+    long long nslots;
+    unsigned long long *x;
+    long long nx;
+    int *err;
+};
+
+template <typename RT>
+__global__ void __launch_bounds__(kBlock)
+k_sor_defect_clear(const double *__restrict__ phi, const RT *__restrict__ rho, float *__restrict__ R,
+                   float *__restrict__ delta, Split sp, SorCoef cf, ResClears cl) {
+    const long long q0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = q0; e < cl.nslots; e += stride) cl.slots[e] = -1.0f;
+    for (long long e = q0; e < cl.nx; e += stride) cl.x[e] = 0ULL;
+    if (q0 < 4) cl.err[q0] = 0;
+    sor_defect_body<RT>(blockIdx.x, phi, rho, R, delta, sp, cf);
+}
+
 template <typename RT, int P = kSorPlanes>
 __device__ __forceinline__ void sor_colour_f32_body(const int bid, float *__restrict__ delta,
                                                     const float *__restrict__ R, Split sp, SorTiles tl, SorCoef cf,
@@ -2073,6 +2181,32 @@ __device__ __forceinline__ void sor_collect_body(const int bid, const unsigned l
 __global__ void k_sor_collect(const unsigned long long *__restrict__ maxd, double *__restrict__ out, int first,
                               int last) {
     sor_collect_body(blockIdx.x, maxd, out, first, last);
+}
+
+// The same, and (xres non-null) the LDS-resident FP32 solve's iteration
+// count, tested max and error word after the chunk's maxima, so that one
+// read-back carries both.
+__global__ void k_sor_collect_x(const unsigned long long *__restrict__ maxd, double *__restrict__ out, int first,
+                                int last, const float *__restrict__ xres, const int *__restrict__ xerr) {
+    if (xres && blockIdx.x == 0 && threadIdx.x == 0) {
+        out[last - first] = (double)xres[0];
+        out[last - first + 1] = (double)xres[1];
+        out[last - first + 2] = (double)xerr[0];
+    }
+    sor_collect_body(blockIdx.x, maxd, out, first, last);
+}
+
+// Start of a chunk of SOR iterations [r0, r1): their rows of the slot maxima
+// zeroed (one word per slot), and at the start of a solve (reset_ctl) SorCtl
+// cleared; grid of at least (r1 - r0) kMaxSlots threads.
+__global__ void k_sor_chunk_reset(unsigned long long *__restrict__ maxd, int r0, int r1, SorCtl *__restrict__ ctl,
+                                  int reset_ctl) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < (long long)(r1 - r0) * kMaxSlots) maxd[((long long)r0 * kMaxSlots + e) * kSlotStride] = 0ULL;
+    if (reset_ctl && e == 0) {
+        ctl->done = 0;
+        ctl->iters = 0;
+    }
 }
 
 }  // namespace hv
