@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <cstdarg>
 #include <new>
@@ -356,10 +357,12 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
         h.gz[a] = (T)tb.gz[a];
     }
     h.term = (T)tb.term;
+    // tap offsets for the vector tile (8 + 2 ry rows of 8 VW columns)
+    const int vR = 8 + 2 * tb.ry, vTK = 8 * smear_vw<T>();
     for (int a = 0; a <= tb.rx; a++)
         for (int t = 0; t <= 2 * tb.kb[a]; t++) {
             const int ab = t > tb.kb[a] ? t - tb.kb[a] : tb.kb[a] - t;
-            h.tl[a][t] = tb.kc[a][ab];
+            h.toff[a][t] = (tb.kc[a][ab] * vR + t) * vTK;
             h.tg[a][t] = (T)tb.gy[ab];
         }
     if (!c->smear_tab) HV_HIP(c, hipMalloc(&c->smear_tab, sizeof(SmearTab<T>)));

@@ -35,6 +35,19 @@ struct hv_batch {
     size_t smear_lds = 0, smear2_lds = 0;
     int smear_fused = 1;                  // option: fused smearing (else two-pass)
     bool xs_vec = false;                  // two-pass: the vector x pass fits every instance
+    // two-pass zy launches by LDS class: the instances whose tiles allow the
+    // same number of blocks per CU run in one launch sized for them (one
+    // launch sized for the largest radius would hold every instance to its
+    // occupancy)
+    struct ZyClass {
+        size_t lds;    // dynamic LDS of the launch
+        unsigned blocks;
+        size_t off;    // int offset of its prefix table (m + 1) then ids (m) in zytab
+        int m;         // instances
+    };
+    std::vector<ZyClass> zycls;
+    int *zytab = nullptr;
+    size_t zytab_cap = 0;
     double *maxc = nullptr;               // collected SOR maxima, n x maxc_width
     int maxc_width = 0;
     int *act = nullptr;                   // SOR launch subset: prefix table (n + 1) then ids (n)
@@ -271,9 +284,54 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
         HV_BHIP(b, hipHostMalloc((void **)&b->hact, sizeof(int) * (2 * (size_t)b->n + 1), hipHostMallocDefault));
     b->smear_lds = lds;
     b->smear2_lds = lds2;
+    // zy launch classes: blocks per CU that the instance's LDS allows (at most 8)
+    {
+        std::vector<int> cls(b->n);
+        std::vector<int> kinds;
+        for (int i = 0; i < b->n; i++) {
+            const size_t l = b->inst[i]->stl_lds > 0 ? b->inst[i]->stl_lds : 1;
+            int k = (int)((size_t)(160 * 1024) / l);
+            k = k < 1 ? 1 : (k > 8 ? 8 : k);
+            cls[i] = k;
+            if (std::find(kinds.begin(), kinds.end(), k) == kinds.end()) kinds.push_back(k);
+        }
+        std::sort(kinds.begin(), kinds.end());
+        std::vector<int> tab;
+        b->zycls.clear();
+        for (int k : kinds) {
+            hv_batch::ZyClass zc{};
+            std::vector<int> ids;
+            for (int i = 0; i < b->n; i++)
+                if (cls[i] == k) ids.push_back(i);
+            zc.m = (int)ids.size();
+            zc.off = tab.size();
+            long long acc = 0;
+            tab.push_back(0);
+            for (int i : ids) {
+                const hv_ctx *c = b->inst[i];
+                acc += 2LL * c->world.n[0] * c->stl.njt * c->stl.nkt;
+                if (acc > 0x7fffffffLL) return batch_err(b, HV_ERR_ARG, "batch too large for one launch");
+                tab.push_back((int)acc);
+                zc.lds = c->stl_lds > zc.lds ? c->stl_lds : zc.lds;
+            }
+            for (int i : ids) tab.push_back(i);
+            zc.blocks = (unsigned)acc;
+            b->zycls.push_back(zc);
+        }
+        if (b->zytab_cap < tab.size()) {
+            if (b->zytab) (void)hipFree(b->zytab);
+            b->zytab = nullptr;
+            b->zytab_cap = 0;
+            HV_BHIP(b, hipMalloc((void **)&b->zytab, sizeof(int) * tab.size()));
+            b->zytab_cap = tab.size();
+        }
+        HV_BHIP(b, hipMemcpy(b->zytab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+    }
     HV_BHIP(b, hipFuncSetAttribute((const void *)kb_smear_fused<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kSmearLdsVec));
     HV_BHIP(b, hipFuncSetAttribute((const void *)kb_smear_zy<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kSmear2LdsVec));
+    HV_BHIP(b, hipFuncSetAttribute((const void *)kb_smear_zy_ids<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kSmear2LdsVec));
     if (!b->first) HV_BHIP(b, hipMalloc((void **)&b->first, sizeof(int) * hf.size()));
     HV_BHIP(b, hipMemcpyAsync(b->first, hf.data(), sizeof(int) * hf.size(), hipMemcpyHostToDevice, b->stream));
@@ -922,7 +980,11 @@ int batch_step_device(hv_batch *b, int end_kick, bool res, bool steer = false, b
             kb_smear_fused<T><<<b->blocks[kBSmear], kSmearThreads, b->smear_lds, b->stream>>>(a, btab(b, kBSmear),
                                                                                               n);
         } else {
-            kb_smear_zy<T><<<b->blocks[kBZy], kSmearThreads, b->smear2_lds, b->stream>>>(a, btab(b, kBZy), n);
+            // one launch per LDS class of the instances
+            for (const auto &zc : b->zycls) {
+                const int *zf = b->zytab + zc.off;
+                kb_smear_zy_ids<T><<<zc.blocks, kSmearThreads, zc.lds, b->stream>>>(a, zf, zf + zc.m + 1, zc.m);
+            }
             if (b->xs_vec)
                 kb_smear_xs_vec<T><<<b->blocks[kBN2V], kThreads, 0, b->stream>>>(a, btab(b, kBN2V), n);
             else
@@ -1381,6 +1443,7 @@ int hv_batch_destroy(hv_batch *b) {
     if (b->hxorder) (void)hipHostFree(b->hxorder);
     if (b->sbuck) (void)hipFree(b->sbuck);
     if (b->sfirst) (void)hipFree(b->sfirst);
+    if (b->zytab) (void)hipFree(b->zytab);
     if (b->ffirst) (void)hipFree(b->ffirst);
     if (b->fin) (void)hipFree(b->fin);
     if (b->fslots) (void)hipFree(b->fslots);

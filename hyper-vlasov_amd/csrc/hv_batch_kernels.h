@@ -111,6 +111,16 @@ kb_smear_zy(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n
                                          A.g.ny, A.g.nz, A.st2);
 }
 
+// the same over a subset of the instances (ids: one LDS class of them, so
+// that the launch's LDS size fits its instances and not the largest radius)
+template <typename T>
+__global__ void __launch_bounds__(kSmearThreads)
+kb_smear_zy_ids(const BInst<T> *__restrict__ a, const int *__restrict__ first, const int *__restrict__ ids, int n) {
+    HV_BATCH_LOCATE_IDS();
+    smear_zy_body<T, unsigned long long>(bid, A.acc, A.D, A.fmask, nullptr, 0, smear_tab_c(A.tab), A.g.nx,
+                                         A.g.ny, A.g.nz, A.st2);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_smear_xs(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {

@@ -671,9 +671,10 @@ struct SmearTab {
     T gx[HV_SMEAR_MAXR + 1], gy[HV_SMEAR_MAXR + 1], gz[HV_SMEAR_MAXR + 1];
     T term;
     // channel a's taps in summation order, t = 0..2 kb(a) for b = t - kb(a):
-    // the H level kc(a, |b|) and the weight gy(|b|) (consecutive, so that a
-    // run of taps loads with a few wide scalar loads)
-    int tl[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
+    // the element offset of its H row in the vector tile, (kc(a, |b|) R + t)
+    // TK with R = 8 + 2 ry rows and TK = 8 VW columns, and the weight gy(|b|)
+    // (consecutive, so that a run of taps loads with a few wide scalar loads)
+    int toff[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
     T tg[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
 };
 
@@ -774,29 +775,32 @@ __device__ __forceinline__ void smear_h_levels(const T *__restrict__ C, T *__res
 template <typename T>
 constexpr int smear_vw() { return 16 / (int)sizeof(T); }
 
+// (the vector tile: TK = 8 VW columns and R = 8 + 2 ry rows, the shape the
+// table's tap offsets are built for)
 template <typename T>
 __device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, SmearTabC<T> *__restrict__ tb, int a,
                                                   int h0, int R, int TK, T (&acc)[smear_vw<T>()]) {
     constexpr int VW = smear_vw<T>();
     typedef T V __attribute__((ext_vector_type(VW)));
+    (void)R;
     const int kb = tb->kb[a], nt = 2 * kb + 1;
 #pragma unroll
     for (int v = 0; v < VW; v++) acc[v] = T(0.0);
-    const T *hb = Hs + h0 - kb * TK;  // tap t reads row kc * R + t of this base
+    const T *hb = Hs + h0 - kb * TK;  // tap t reads element toff[a][t] of this base
     // runs of U taps: their table entries, then their H vectors, load
     // together ahead of the sums (one load latency per run, not per tap)
     auto run = [&](auto uu, int t) {
         constexpr int U = decltype(uu)::value;
-        int L[U];
+        int o[U];
         T g[U];
         V h[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            L[u] = tb->tl[a][t + u];
+            o[u] = tb->toff[a][t + u];
             g[u] = tb->tg[a][t + u];
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) h[u] = *(const V *)(hb + (L[u] * R + t + u) * TK);
+        for (int u = 0; u < U; u++) h[u] = *(const V *)(hb + o[u]);
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
