@@ -102,3 +102,37 @@ def test_adjoint_c4_workload_200_steps():
     print("loss", loss, "adjoint", grad, "fd", fd, "relative", rel)
     assert loss > 0
     assert rel.max() < 1e-3, rel
+
+
+def test_adjoint_c4_full_horizon_windowed():
+    """The differentiation workload over its whole 1,500 steps as a windowed
+    misfit (hvamd.adjoint.WindowedDipoleFit: 15 windows of 100 steps, each
+    started from the target trajectory's state, multiple shooting; SOR to
+    1e-11): the adjoint gradient's derivative along theta itself against the
+    central difference of the same summed misfit at the workload's relative
+    step 1e-4, within 1e-3 (per component, measured 7e-5 at h = 1e-5 and 3e-5,
+    3e-4 at 1e-4)."""
+    from hvamd.adjoint import WindowedDipoleFit, target_run
+    print("adjoint c4 1500 steps windowed vs directional FD h=1e-4", flush=True)
+    w = configs.c4()
+    nsteps, window = w.steps, 100
+    prm = w.params(nsteps)
+    prm.sor_tol = 1e-11
+    prm.sor_max_it = 4000
+    sk = w.sky()
+    theta = np.array([sk.A, sk.B, sk.C, sk.gamma])
+    target, states = target_run(w, theta * 1.03, nsteps, window, prm)
+    fit = WindowedDipoleFit(w, target, states, window, 100, prm)
+    try:
+        loss, grad, _ = fit.loss_grad(theta)
+        print("loss", loss, "adjoint", grad, flush=True)
+        h = 1e-4
+        lp = fit.loss_grad(theta * (1.0 + h), grad=False)[0]
+        lm = fit.loss_grad(theta * (1.0 - h), grad=False)[0]
+    finally:
+        fit.close()
+    fd = (lp - lm) / (2.0 * h)
+    ad = float(np.dot(grad, theta))
+    print("directional: adjoint", ad, "fd", fd, "relative", (ad - fd) / abs(fd))
+    assert len(states) == 15 and loss > 0
+    assert abs(ad - fd) < 1e-3 * abs(fd)
