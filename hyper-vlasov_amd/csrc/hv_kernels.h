@@ -852,6 +852,73 @@ constexpr size_t kSmearLds = 81920;     // LDS per block: two blocks per CU
 constexpr size_t kSmearLdsVec = 114688;  // fused form, vector tile: one block per CU
 constexpr size_t kSmear2LdsVec = 98304;  // two-pass form, vector tile
 
+// Two-pass form: staged counts in rows of smear_zy_wp = TK + 2 RZB words, the
+// halo's column cc (k = k0 - rz + cc) at word RZB - rz + cc of its row, RZB
+// the radius bucket of rz; tile column j's counts at j +- K then sit at word
+// RZB + j +- K whatever rz is, 16-byte aligned for j a multiple of VW.
+__host__ __device__ constexpr int smear_rzb(int rz) {
+    return rz <= 4 ? 4 : rz <= 8 ? 8 : rz <= 12 ? 12 : rz <= 16 ? 16 : 32;
+}
+__host__ __device__ inline int smear_zy_wp(int TK, int rz) { return TK + 2 * smear_rzb(rz); }
+
+// The z factor of the two-pass form: H_0..H_rz as smear_h_levels (the same
+// operations in the same order per column), a thread taking VW adjacent
+// columns of a row at a time with their counts j - RZB .. j + VW - 1 + RZB in
+// registers (2 RZB / VW + 1 aligned vector reads) and each level stored as one
+// vector.
+template <typename T, int RZB>
+__device__ __forceinline__ void smear_h_levels_win(const T *__restrict__ C, T *__restrict__ Hs,
+                                                   SmearTabC<T> *__restrict__ tb, int R, int TK, int rz) {
+    constexpr int VW = 16 / (int)sizeof(T), NW = 2 * RZB / VW + 1;
+    static_assert(RZB % VW == 0, "the window starts on a vector boundary");
+    typedef T V __attribute__((ext_vector_type(VW)));
+    const int Wp = TK + 2 * RZB, gpr = TK / VW, groups = R * gpr, lvl = R * TK;
+    const T gz0 = tb->gz[0];
+    for (int gi = threadIdx.x; gi < groups; gi += kSmearThreads) {
+        const int r = gi / gpr, j0 = (gi - r * gpr) * VW;
+        const V *src = (const V *)(C + r * Wp + j0);
+        T w[NW * VW];
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const V t = src[q];
+#pragma unroll
+            for (int v = 0; v < VW; v++) w[q * VW + v] = t[v];
+        }
+        V h;
+#pragma unroll
+        for (int v = 0; v < VW; v++) h[v] = gz0 * w[RZB + v];
+        T *hp = Hs + r * TK + j0;
+        *(V *)hp = h;
+#pragma unroll
+        for (int K = 1; K <= RZB; K++) {
+            if (K <= rz) {  // (predicated, not a loop exit: the window stays in registers)
+                const T g = tb->gz[K];
+#pragma unroll
+                for (int v = 0; v < VW; v++)
+                    h[v] = smear_madd(smear_madd(h[v], g, w[RZB + v - K]), g, w[RZB + v + K]);
+                *(V *)(hp + K * lvl) = h;
+            }
+        }
+    }
+}
+
+// (radii beyond 16: smear_h_levels on the same layout, its rows Wp words
+// apart from word RZB - rz)
+template <typename T>
+__device__ __forceinline__ void smear_h_levels_zy(const T *__restrict__ C, T *__restrict__ Hs,
+                                                  SmearTabC<T> *__restrict__ tb, int R, int TK, int rz) {
+    switch (smear_rzb(rz)) {  // uniform: one instance per block
+    case 4: smear_h_levels_win<T, 4>(C, Hs, tb, R, TK, rz); break;
+    case 8: smear_h_levels_win<T, 8>(C, Hs, tb, R, TK, rz); break;
+    case 12: smear_h_levels_win<T, 12>(C, Hs, tb, R, TK, rz); break;
+    case 16: smear_h_levels_win<T, 16>(C, Hs, tb, R, TK, rz); break;
+    default:
+        smear_h_levels<T>(C + (smear_rzb(rz) - rz), Hs, tb, R, smear_zy_wp(TK, rz), TK, rz, kSmearThreads);
+        break;
+    }
+}
+
+
 inline size_t smear_lds_bytes(const SmearTiles &st, int rx, int ry, int rz, size_t tsize) {
     const int R = st.TJ + 2 * ry, W = st.TK + 2 * rz;
     return tsize * ((size_t)smear_c_words(R, W, (int)(16 / tsize)) + (size_t)(rz + 1) * R * st.TK +
@@ -860,8 +927,8 @@ inline size_t smear_lds_bytes(const SmearTiles &st, int rx, int ry, int rz, size
 
 // the two-pass form's zy kernel: counts and H levels
 inline size_t smear2_lds_bytes(const SmearTiles &st, int ry, int rz, size_t tsize) {
-    const int R = st.TJ + 2 * ry, W = st.TK + 2 * rz;
-    return tsize * ((size_t)smear_c_words(R, W, (int)(16 / tsize)) + (size_t)(rz + 1) * R * st.TK);
+    const int R = st.TJ + 2 * ry, Wp = smear_zy_wp(st.TK, rz);
+    return tsize * ((size_t)smear_c_words(R, Wp, (int)(16 / tsize)) + (size_t)(rz + 1) * R * st.TK);
 }
 
 inline long long smear_blocks(const SmearTiles &st) { return 2LL * st.njt * st.nkt * st.nic; }
@@ -1086,10 +1153,10 @@ __device__ __forceinline__ void smear_zy_finish(bool any, int sp, int i, int jt,
     extern __shared__ unsigned char smem[];
     const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
     const int TJ = st.TJ, TK = st.TK;
-    const int R = TJ + 2 * ry, W = TK + 2 * rz;
+    const int R = TJ + 2 * ry, Wp = smear_zy_wp(TK, rz);
     constexpr int VW = smear_vw<T>();
     T *C = (T *)smem;
-    T *Hs = C + smear_c_words(R, W, VW);
+    T *Hs = C + smear_c_words(R, Wp, VW);
     const long long G = (long long)nx * ny * nz, n2 = 2 * G;
     const long long base = sp * G + (long long)i * ny * nz;
     const int j0 = jt * TJ, k0 = kt * TK;
@@ -1099,7 +1166,7 @@ __device__ __forceinline__ void smear_zy_finish(bool any, int sp, int i, int jt,
         if (threadIdx.x == 0) atomicAnd(mw, ~bit);
         return;
     }
-    smear_h_levels<T>(C, Hs, tb, R, W, TK, rz, kSmearThreads);
+    smear_h_levels_zy<T>(C, Hs, tb, R, TK, rz);
     lds_barrier();
     if (threadIdx.x == 0) atomicOr(mw, bit);
     // channel a per wave, VW consecutive columns per lane (the vector tile)
@@ -1129,9 +1196,11 @@ __device__ __forceinline__ void smear_zy_item(int sp, int i, int jt, int kt, con
     const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
     const int TJ = st.TJ, TK = st.TK;
     const int R = TJ + 2 * ry, W = TK + 2 * rz;
+    // counts: R rows of Wp words, halo column cc at word pad + cc
+    const int Wp = smear_zy_wp(TK, rz), pad = smear_rzb(rz) - rz;
     constexpr int VW = smear_vw<T>();
-    T *C = (T *)smem;                       // R x W counts
-    T *Hs = C + smear_c_words(R, W, VW);    // (rz + 1) x R x TK
+    T *C = (T *)smem;
+    T *Hs = C + smear_c_words(R, Wp, VW);   // (rz + 1) x R x TK
     const long long G = (long long)nx * ny * nz, n2 = 2 * G;
     const long long base = sp * G + (long long)i * ny * nz;
     const int j0 = jt * TJ, k0 = kt * TK;
@@ -1166,7 +1235,7 @@ __device__ __forceinline__ void smear_zy_item(int sp, int i, int jt, int kt, con
         for (int m = 0; m < kZyStaged; m++) {
             if (ecoord[m] >= 0) {
                 const T x = count_of<T, CT>(v[m]);
-                C[(int)threadIdx.x + m * kSmearThreads] = x;
+                C[(ecoord[m] >> 16) * Wp + pad + (ecoord[m] & 0xffff)] = x;
                 nonzero |= (x != T(0.0));
             }
         }
@@ -1186,7 +1255,8 @@ __device__ __forceinline__ void smear_zy_item(int sp, int i, int jt, int kt, con
                 const int e = e0 + u * kSmearThreads + (int)threadIdx.x;
                 if (e < RW) {
                     const T x = count_of<T, CT>(v[u]);
-                    C[e] = x;
+                    const int r = e / W;
+                    C[r * Wp + pad + (e - r * W)] = x;
                     nonzero |= (x != T(0.0));
                 }
             }
@@ -1198,7 +1268,7 @@ __device__ __forceinline__ void smear_zy_item(int sp, int i, int jt, int kt, con
         if (threadIdx.x == 0) atomicAnd(mw, ~bit);
         return;
     }
-    smear_h_levels<T>(C, Hs, tb, R, W, TK, rz, kSmearThreads);
+    smear_h_levels_zy<T>(C, Hs, tb, R, TK, rz);
     __syncthreads();
     if (threadIdx.x == 0) atomicOr(mw, bit);
     if (smear_vec_tile<T>(TJ, TK)) {
@@ -1324,6 +1394,7 @@ k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned *__restrict__
     int ecoord[kZyStaged];
     smear_zy_coords(R, W, ecoord);
     T *C = (T *)smem_zy<T>();
+    const int Wp = smear_zy_wp(st.TK, rz), pad = smear_rzb(rz) - rz;
     __shared__ int orslot[kSmearThreads / kWave];
     CT pre[kZyStaged];
     // stage item it: its coordinates, whether its halo meets the count box of
@@ -1362,7 +1433,7 @@ k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned *__restrict__
             for (int m = 0; m < kZyStaged; m++) {
                 if (ecoord[m] >= 0) {
                     const T x = count_of<T, CT>(pre[m]);
-                    C[(int)threadIdx.x + m * kSmearThreads] = x;
+                    C[(ecoord[m] >> 16) * Wp + pad + (ecoord[m] & 0xffff)] = x;
                     nonzero |= (x != T(0.0));
                 }
             }
