@@ -252,7 +252,7 @@ ENS_SORT_CHOICES = (0, 4, 8, 16)
 RETUNE_EVERY = 500  # steps between re-tunings of the ensemble's sort cadence
 
 
-def run_batch(ws, prec: str, steps: int, warmup: int, mid_steps: int, full: bool):
+def run_batch(ws, prec: str, steps: int, warmup: int, mid_steps: int, full: bool, graph: bool = True):
     """The ensemble in one batch.  Both forms: init, the autotuner at step 0
     (smearing tile and form, sort cadence; tune_batch, its trials are steps
     of the run), `warmup` untimed steps and `steps` timed steps: the early
@@ -272,6 +272,8 @@ def run_batch(ws, prec: str, steps: int, warmup: int, mid_steps: int, full: bool
     prof_steps = 4
     run_len = ws[0].steps if full else 0
     b = Batch([w.world() for w in ws], [w.n_protons for w in ws], [w.n_neutrons for w in ws], p)
+    if not graph:
+        b.set_option(hvamd.HV_OPT_BATCH_GRAPH, 0)
     for i, w in enumerate(ws):
         b.instance(i).upload_particles(*w.initial_state())
     sks = [w.sky() for w in ws]
@@ -553,6 +555,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain MIB-MiB device copy (HBM counter scale)")
+    ap.add_argument("--single", type=int, default=1,
+                    help="0: the ensemble only (a profiling run; its line carries no headline value)")
+    ap.add_argument("--ensemble-graph", type=int, default=1,
+                    help="0: launch the ensemble's steps one kernel at a time instead of replaying step graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build measures the loop on a "
@@ -570,6 +576,13 @@ def main():
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
     full = bool(args.full)
+    if not args.single:
+        ens = configs.c3(args.ensemble)
+        g = bool(args.ensemble_graph)
+        bm = run_batch(ens, "mixed", args.ensemble_steps, 2, args.ensemble_mid_steps, full, g)
+        b64 = run_batch(ens, "fp64", args.ensemble_steps, 2, 0, False, g) if args.fp64 else None
+        print(json.dumps({"ensemble_c3": {"mixed": bm, "fp64": b64}, "measured_copy_GBs": copy_gbs}), flush=True)
+        return
     mixed = run_gpu(w, "mixed", args.steps, args.warmup, full)
     fp64 = run_gpu(w, "fp64", args.steps, args.warmup, full) if args.fp64 else None
 
@@ -630,8 +643,9 @@ def main():
                 out["cpu_baseline_reference_loop_c0"] = lit
     if args.ensemble > 0:
         ens = configs.c3(args.ensemble)
-        bm = run_batch(ens, "mixed", args.ensemble_steps, 2, args.ensemble_mid_steps, full)
-        b64 = run_batch(ens, "fp64", args.ensemble_steps, 2, 0, False) if args.fp64 else None
+        g = bool(args.ensemble_graph)
+        bm = run_batch(ens, "mixed", args.ensemble_steps, 2, args.ensemble_mid_steps, full, g)
+        b64 = run_batch(ens, "fp64", args.ensemble_steps, 2, 0, False, g) if args.fp64 else None
         out["ensemble_c3"] = {
             "mixed": bm, "fp64": b64,
             # like for like: the same early steps in both forms (FP64 runs only those)

@@ -1152,11 +1152,11 @@ Particles<T> particles_alt(const hv_ctx *c) {
     return p;
 }
 
-// key buckets of the counting sort: 2 (bricks + 1)
+// key buckets of the counting sort: 2 (bricks + kSortOutside)
 long long sort_buckets(const hv_ctx *c) {
     const long long nb = (long long)((c->world.n[0] + 3) >> kBrickShift) * ((c->world.n[1] + 3) >> kBrickShift) *
                          ((c->world.n[2] + 3) >> kBrickShift);
-    return 2 * (nb + 1);
+    return 2 * (nb + kSortOutside);
 }
 
 // Reorders the particle arrays by (species, brick of the cell): the counting

@@ -335,15 +335,10 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
                                    (int)kSmear2LdsVec));
     if (!b->first) HV_BHIP(b, hipMalloc((void **)&b->first, sizeof(int) * hf.size()));
     HV_BHIP(b, hipMemcpyAsync(b->first, hf.data(), sizeof(int) * hf.size(), hipMemcpyHostToDevice, b->stream));
-    // key buckets of the batched sort: 2 (bricks + 1) per instance
+    // key buckets of the batched sort: sort_buckets per instance
     {
         std::vector<long long> sf(b->n + 1, 0);
-        for (int i = 0; i < b->n; i++) {
-            const hv_ctx *c = b->inst[i];
-            const long long nb = (long long)((c->world.n[0] + 3) >> kBrickShift) *
-                                 ((c->world.n[1] + 3) >> kBrickShift) * ((c->world.n[2] + 3) >> kBrickShift);
-            sf[i + 1] = sf[i] + 2 * (nb + 1);
-        }
+        for (int i = 0; i < b->n; i++) sf[i + 1] = sf[i] + sort_buckets(b->inst[i]);
         if (b->hsfirst != sf) {
             if (b->sbuck) (void)hipFree(b->sbuck);
             if (b->sfirst) (void)hipFree(b->sfirst);

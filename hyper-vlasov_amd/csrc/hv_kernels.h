@@ -304,17 +304,21 @@ __device__ __forceinline__ void block_deposit(bool has, const Cell<T> &c, int sp
 constexpr int kBrickShift = 2;
 
 // Sort key of particle i: its species (protons first), then the 4^3-node
-// brick of its cell (particles outside the grid last within their species);
-// keys lie in [0, 2 (bricks + 1)).
+// brick of its cell; particles outside the grid come last within their
+// species, spread over kSortOutside keys by slot (they all share one place in
+// the order, and one key would make every count atomic for them hit one word);
+// keys lie in [0, 2 (bricks + kSortOutside)).
+constexpr int kSortOutside = 64;
+
 template <typename T>
 __device__ __forceinline__ unsigned sort_key_of(const Particles<T> &p, int i, int np, const Geom<T> &g) {
     const int bx = (g.nx + 3) >> kBrickShift, by = (g.ny + 3) >> kBrickShift, bz = (g.nz + 3) >> kBrickShift;
     const unsigned nb = (unsigned)(bx * by * bz);
     Cell<T> c;
-    unsigned b = nb;
+    unsigned b = nb + ((unsigned)i & (kSortOutside - 1));
     if (cic_cell(p.x[i], p.y[i], p.z[i], g, c))
         b = (unsigned)(((c.x0 >> kBrickShift) * by + (c.y0 >> kBrickShift)) * bz + (c.z0 >> kBrickShift));
-    return (i < np ? 0u : nb + 1u) + b;
+    return (i < np ? 0u : nb + (unsigned)kSortOutside) + b;
 }
 
 // ---- counting sort by key ------------------------------------------------------
