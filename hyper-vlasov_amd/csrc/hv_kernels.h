@@ -811,7 +811,44 @@ __device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, Smea
             for (int v = 0; v < VW; v++) acc[v] = smear_madd(acc[v], g[u], h[u][v]);
     };
     int t = 0;
-    for (; t + 8 <= nt; t += 8) run(std::integral_constant<int, 8>{}, t);
+    // runs of 8: the next run's table entries load while this run's H vectors
+    // do (the sums wait once for both, not for one and then the other)
+    constexpr int U8 = 8;
+    if (t + U8 <= nt) {
+        int o[U8];
+        T g[U8];
+#pragma unroll
+        for (int u = 0; u < U8; u++) {
+            o[u] = tb->toff[a][u];
+            g[u] = tb->tg[a][u];
+        }
+        for (;;) {
+            V h[U8];
+#pragma unroll
+            for (int u = 0; u < U8; u++) h[u] = *(const V *)(hb + o[u]);
+            const bool more = t + 2 * U8 <= nt;
+            int on[U8];
+            T gn[U8];
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U8; u++) {
+                    on[u] = tb->toff[a][t + U8 + u];
+                    gn[u] = tb->tg[a][t + U8 + u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U8; u++)
+#pragma unroll
+                for (int v = 0; v < VW; v++) acc[v] = smear_madd(acc[v], g[u], h[u][v]);
+            t += U8;
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < U8; u++) {
+                o[u] = on[u];
+                g[u] = gn[u];
+            }
+        }
+    }
     if (t + 4 <= nt) {
         run(std::integral_constant<int, 4>{}, t);
         t += 4;
@@ -877,7 +914,12 @@ __device__ __forceinline__ void smear_h_levels_win(const T *__restrict__ C, T *_
     static_assert(RZB % VW == 0, "the window starts on a vector boundary");
     typedef T V __attribute__((ext_vector_type(VW)));
     const int Wp = TK + 2 * RZB, gpr = TK / VW, groups = R * gpr, lvl = R * TK;
-    const T gz0 = tb->gz[0];
+    // the weights of all levels up front (scalar loads issued together, not
+    // one wait per level)
+    T gz[RZB + 1];
+#pragma unroll
+    for (int K = 0; K <= RZB; K++) gz[K] = tb->gz[K];
+    const T gz0 = gz[0];
     for (int gi = threadIdx.x; gi < groups; gi += kSmearThreads) {
         const int r = gi / gpr, j0 = (gi - r * gpr) * VW;
         const V *src = (const V *)(C + r * Wp + j0);
@@ -896,7 +938,7 @@ __device__ __forceinline__ void smear_h_levels_win(const T *__restrict__ C, T *_
 #pragma unroll
         for (int K = 1; K <= RZB; K++) {
             if (K <= rz) {  // (predicated, not a loop exit: the window stays in registers)
-                const T g = tb->gz[K];
+                const T g = gz[K];
 #pragma unroll
                 for (int v = 0; v < VW; v++)
                     h[v] = smear_madd(smear_madd(h[v], g, w[RZB + v - K]), g, w[RZB + v + K]);
