@@ -365,6 +365,25 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
             h.toff[a][t] = (tb.kc[a][ab] * vR + t) * vTK;
             h.tg[a][t] = (T)tb.gy[ab];
         }
+    // channels to the 4 waves of a smearing block: longest first (2 kb + 1
+    // taps and a store), each to the least loaded wave, ties to the lower
+    // channel and wave
+    {
+        static_assert(kSmearThreads / kWave == 4, "wave table sized for 4 waves");
+        int order[HV_SMEAR_MAXR + 1], nch[4] = {0, 0, 0, 0}, load[4] = {0, 0, 0, 0};
+        for (int a = 0; a <= tb.rx; a++) order[a] = a;
+        std::stable_sort(order, order + tb.rx + 1, [&](int x, int y) { return tb.kb[x] > tb.kb[y]; });
+        for (int w = 0; w < 4; w++)
+            for (int n = 0; n < HV_SMEAR_MAXR + 2; n++) h.wch[w][n] = -1;
+        for (int m = 0; m <= tb.rx; m++) {
+            const int a = order[m];
+            int w = 0;
+            for (int v = 1; v < 4; v++)
+                if (load[v] < load[w]) w = v;
+            h.wch[w][nch[w]++] = a;
+            load[w] += 2 * tb.kb[a] + 3;
+        }
+    }
     if (!c->smear_tab) HV_HIP(c, hipMalloc(&c->smear_tab, sizeof(SmearTab<T>)));
     HV_HIP(c, hipMemcpyAsync(c->smear_tab, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HV_HIP(c, hipStreamSynchronize(c->stream));

@@ -680,6 +680,10 @@ struct SmearTab {
     // (consecutive, so that a run of taps loads with a few wide scalar loads)
     int toff[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
     T tg[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
+    // the channels wave w of a 256-thread block sums in the vector tile, -1
+    // after the last (longest first to the least loaded wave: the channels'
+    // tap counts differ, so a round robin leaves waves idle)
+    int wch[4][HV_SMEAR_MAXR + 2];
 };
 
 // One tap of a smearing sum, acc + g h: FP64 rounds the product and the sum
@@ -860,6 +864,25 @@ __device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, Smea
 // the tile's 64 VW nodes of one channel.
 template <typename T>
 __host__ __device__ constexpr bool smear_vec_tile(int TJ, int TK) { return TJ == 8 && TK == 8 * smear_vw<T>(); }
+
+// The VW sums of a lane into D at dp: one 16-byte store when the row length
+// nz is a multiple of VW (dp is then aligned), else element by element up to
+// the row's end (n valid elements).
+template <typename T>
+__device__ __forceinline__ void smear_store_vec(T *__restrict__ dp, const T (&acc)[smear_vw<T>()], int nz, int n) {
+    constexpr int VW = smear_vw<T>();
+    typedef T V __attribute__((ext_vector_type(VW)));
+    if (nz % VW == 0 && n >= VW) {
+        V o;
+#pragma unroll
+        for (int v = 0; v < VW; v++) o[v] = acc[v];
+        *(V *)dp = o;
+    } else {
+#pragma unroll
+        for (int v = 0; v < VW; v++)
+            if (v < n) dp[v] = acc[v];
+    }
+}
 
 // counts staged in LDS take R x W values rounded up to a multiple of VW, so
 // that the H levels after them are 16-byte aligned
@@ -1048,7 +1071,9 @@ __device__ __forceinline__ void smear_fused_body(const int bid, const CT *__rest
                 // channel a per wave, VW consecutive columns per lane
                 const int r = lane >> 3, c0 = (lane & 7) * VW;
                 const int h0 = (r + ry) * TK + c0, n0 = r * TK + c0;
-                for (int a = wave; a <= rx; a += nwaves) {
+                for (int n = 0; n <= HV_SMEAR_MAXR; n++) {
+                    const int a = tb->wch[wave][n];
+                    if (a < 0) break;
                     T acc[VW];
                     smear_channel_vec<T>(Hs, tb, a, h0, R, TK, acc);
                     const T gxa = tb->gx[a];
@@ -1197,7 +1222,7 @@ __device__ __forceinline__ void smear_zy_finish(bool any, int sp, int i, int jt,
                                                 int ny, int nz, const SmearTiles &st) {
     // (LDS-only barriers: the next item's count loads stay in flight)
     extern __shared__ unsigned char smem[];
-    const int rx = tb->rx, ry = tb->ry, rz = tb->rz;
+    const int ry = tb->ry, rz = tb->rz;
     const int TJ = st.TJ, TK = st.TK;
     const int R = TJ + 2 * ry, Wp = smear_zy_wp(TK, rz);
     constexpr int VW = smear_vw<T>();
@@ -1221,14 +1246,13 @@ __device__ __forceinline__ void smear_zy_finish(bool any, int sp, int i, int jt,
     const int r = lane >> 3, c0 = (lane & 7) * VW;
     const int j = j0 + r;
     const long long q = base + (long long)j * nz + k0 + c0;
-    for (int a = wave; a <= rx; a += nwaves) {
+    (void)nwaves;
+    for (int n = 0; n <= HV_SMEAR_MAXR; n++) {
+        const int a = tb->wch[wave][n];
+        if (a < 0) break;
         T acc[VW];
         smear_channel_vec<T>(Hs, tb, a, (r + ry) * TK + c0, R, TK, acc);
-        if (j < ny) {
-#pragma unroll
-            for (int v = 0; v < VW; v++)
-                if (k0 + c0 + v < nz) D[(long long)a * n2 + q + v] = acc[v];
-        }
+        if (j < ny) smear_store_vec<T>(D + (long long)a * n2 + q, acc, nz, nz - (k0 + c0));
     }
 }
 
@@ -1324,14 +1348,13 @@ __device__ __forceinline__ void smear_zy_item(int sp, int i, int jt, int kt, con
         const int r = lane >> 3, c0 = (lane & 7) * VW;
         const int j = j0 + r;
         const long long q = base + (long long)j * nz + k0 + c0;
-        for (int a = wave; a <= rx; a += nwaves) {
+        (void)nwaves;
+        for (int n = 0; n <= HV_SMEAR_MAXR; n++) {
+            const int a = tb->wch[wave][n];
+            if (a < 0) break;
             T acc[VW];
             smear_channel_vec<T>(Hs, tb, a, (r + ry) * TK + c0, R, TK, acc);
-            if (j < ny) {
-#pragma unroll
-                for (int v = 0; v < VW; v++)
-                    if (k0 + c0 + v < nz) D[(long long)a * n2 + q + v] = acc[v];
-            }
+            if (j < ny) smear_store_vec<T>(D + (long long)a * n2 + q, acc, nz, nz - (k0 + c0));
         }
         return;
     }
