@@ -81,6 +81,7 @@ struct hv_ctx {
     int smear_xs_vec = 1;               // two-pass x pass: VW nodes per thread (when nz allows)
     int xs_rxm = 0;                     // its radius bucket (0: radius too large for it)
     long long zy_blocks = 1024;         // workgroups of the zy pass (blocks per CU that its LDS allows x CUs)
+    int zy_staged = 0;                  // counts a zy item stages: (TJ + 2 ry)(TK + 2 rz) of stl
     unsigned clear_blocks = 2048;       // workgroups of k_clear_counts and k_force_grid (one box row per wave)
     unsigned *fbox = nullptr;           // the count boxes of the last smearing (2 x 6 words)
     int *brec = nullptr;                // per-block box records of the last deposit (kBoxRec ints per block)
@@ -456,6 +457,7 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
         return set_err(c, HV_ERR_HIP, "smearing: LDS size attribute refused");
     c->stl = stl;
     c->stl_lds = lds;
+    c->zy_staged = (stl.TJ + 2 * tb.ry) * (stl.TK + 2 * tb.rz);
     {
         int dev = 0, ncu = 256;
         if (hipGetDevice(&dev) != hipSuccess ||

@@ -44,6 +44,8 @@ struct hv_batch {
         unsigned blocks;
         size_t off;    // int offset of its prefix table (m + 1) then ids (m) in zytab
         int m;         // instances
+        bool loop;     // every instance fits kb_smear_zy_loop (then launched with `grid` blocks)
+        unsigned grid;
     };
     std::vector<ZyClass> zycls;
     int *zytab = nullptr;
@@ -286,6 +288,10 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
     b->smear2_lds = lds2;
     // zy launch classes: blocks per CU that the instance's LDS allows (at most 8)
     {
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 256;
         std::vector<int> cls(b->n);
         std::vector<int> kinds;
         for (int i = 0; i < b->n; i++) {
@@ -307,15 +313,21 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
             zc.off = tab.size();
             long long acc = 0;
             tab.push_back(0);
+            zc.loop = true;
             for (int i : ids) {
                 const hv_ctx *c = b->inst[i];
                 acc += 2LL * c->world.n[0] * c->stl.njt * c->stl.nkt;
                 if (acc > 0x7fffffffLL) return batch_err(b, HV_ERR_ARG, "batch too large for one launch");
                 tab.push_back((int)acc);
                 zc.lds = c->stl_lds > zc.lds ? c->stl_lds : zc.lds;
+                zc.loop = zc.loop && smear_vec_tile<T>(c->stl.TJ, c->stl.TK) &&
+                          c->zy_staged <= kZyStagedB * kSmearThreads;
             }
             for (int i : ids) tab.push_back(i);
             zc.blocks = (unsigned)acc;
+            // the loop form: k blocks per CU (its class) on every CU, each
+            // striding over the items
+            zc.grid = (unsigned)std::min<long long>(acc, (long long)k * ncu);
             b->zycls.push_back(zc);
         }
         if (b->zytab_cap < tab.size()) {
@@ -332,6 +344,8 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
     HV_BHIP(b, hipFuncSetAttribute((const void *)kb_smear_zy<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kSmear2LdsVec));
     HV_BHIP(b, hipFuncSetAttribute((const void *)kb_smear_zy_ids<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kSmear2LdsVec));
+    HV_BHIP(b, hipFuncSetAttribute((const void *)kb_smear_zy_loop<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kSmear2LdsVec));
     if (!b->first) HV_BHIP(b, hipMalloc((void **)&b->first, sizeof(int) * hf.size()));
     HV_BHIP(b, hipMemcpyAsync(b->first, hf.data(), sizeof(int) * hf.size(), hipMemcpyHostToDevice, b->stream));
@@ -978,7 +992,10 @@ int batch_step_device(hv_batch *b, int end_kick, bool res, bool steer = false, b
             // one launch per LDS class of the instances
             for (const auto &zc : b->zycls) {
                 const int *zf = b->zytab + zc.off;
-                kb_smear_zy_ids<T><<<zc.blocks, kSmearThreads, zc.lds, b->stream>>>(a, zf, zf + zc.m + 1, zc.m);
+                if (zc.loop)
+                    kb_smear_zy_loop<T><<<zc.grid, kSmearThreads, zc.lds, b->stream>>>(a, zf, zf + zc.m + 1, zc.m);
+                else
+                    kb_smear_zy_ids<T><<<zc.blocks, kSmearThreads, zc.lds, b->stream>>>(a, zf, zf + zc.m + 1, zc.m);
             }
             if (b->xs_vec)
                 kb_smear_xs_vec<T><<<b->blocks[kBN2V], kThreads, 0, b->stream>>>(a, btab(b, kBN2V), n);

@@ -121,6 +121,87 @@ kb_smear_zy_ids(const BInst<T> *__restrict__ a, const int *__restrict__ first, c
                                          A.g.ny, A.g.nz, A.st2);
 }
 
+// The zy pass of one LDS class of instances as a loop over its items
+// (instance, species, plane, tile), a few blocks per CU striding over them:
+// the next item's instance is located and its counts load into registers
+// while this item builds its H levels and D channels (one block per item
+// waits for its counts with nothing of its own to overlap).  Every instance
+// of the class has the vector tile and at most kZyStagedB x kSmearThreads
+// staged counts; an item's results are those of smear_zy_item (the same
+// staging, smear_zy_finish).
+constexpr int kZyStagedB = 12;
+
+template <typename T>
+__global__ void __launch_bounds__(kSmearThreads)
+kb_smear_zy_loop(const BInst<T> *__restrict__ a, const int *__restrict__ first, const int *__restrict__ ids, int n) {
+    const int nall = first[n];
+    T *C = smem_zy<T>();
+    __shared__ int orslot[kSmearThreads / kWave];
+    int ecoord[kZyStagedB];
+    unsigned long long pre[kZyStagedB];
+    int cR = -1, cW = -1;  // the halo shape ecoord was computed for
+    int s_inst = 0, s_sp = 0, s_i = 0, s_jt = 0, s_kt = 0;
+    // locate item it, and load its counts (zero outside the grid) into pre
+    auto stage = [&](int it) {
+        const int slot = __builtin_amdgcn_readfirstlane(batch_instance(first, n, it));
+        int t = it - first[slot];
+        const int inst = __builtin_amdgcn_readfirstlane(ids[slot]);
+        const BInst<T> &A = a[inst];
+        SmearTabC<T> *tb = smear_tab_c(A.tab);
+        const int ry = tb->ry, rz = tb->rz;
+        const SmearTiles st = A.st2;
+        const int nx = A.g.nx, ny = A.g.ny, nz = A.g.nz;
+        const int kt = t % st.nkt;
+        t /= st.nkt;
+        const int jt = t % st.njt;
+        t /= st.njt;
+        const int i = t % nx, sp = t / nx;
+        const int R = st.TJ + 2 * ry, W = st.TK + 2 * rz;
+        if (R != cR || W != cW) {
+            smear_zy_coords(R, W, ecoord);
+            cR = R;
+            cW = W;
+        }
+        const int j0 = jt * st.TJ, k0 = kt * st.TK;
+        const long long G = (long long)nx * ny * nz;
+        const unsigned long long *cb =
+            A.acc + sp * G + (long long)i * ny * nz + (long long)(j0 - ry) * nz + (k0 - rz);
+        const int rlo = ry - j0, rhi = ny - 1 - (j0 - ry), clo = rz - k0, chi = nz - 1 - (k0 - rz);
+#pragma unroll
+        for (int m = 0; m < kZyStagedB; m++) {
+            const int r = ecoord[m] >> 16, cc = ecoord[m] & 0xffff;
+            pre[m] = (ecoord[m] >= 0 && r >= rlo && r <= rhi && cc >= clo && cc <= chi) ? cb[r * nz + cc] : 0ULL;
+        }
+        s_inst = inst;
+        s_sp = sp;
+        s_i = i;
+        s_jt = jt;
+        s_kt = kt;
+    };
+    int it = blockIdx.x;
+    if (it < nall) stage(it);
+    for (; it < nall; it += gridDim.x) {
+        const int inst = s_inst, sp = s_sp, i = s_i, jt = s_jt, kt = s_kt;
+        const BInst<T> &A = a[inst];
+        SmearTabC<T> *tb = smear_tab_c(A.tab);
+        const int rz = tb->rz;
+        const int Wp = smear_zy_wp(A.st2.TK, rz), pad = smear_rzb(rz) - rz;
+        int nonzero = 0;
+#pragma unroll
+        for (int m = 0; m < kZyStagedB; m++) {
+            if (ecoord[m] >= 0) {
+                const T x = count_of<T, unsigned long long>(pre[m]);
+                C[(ecoord[m] >> 16) * Wp + pad + (ecoord[m] & 0xffff)] = x;
+                nonzero |= (x != T(0.0));
+            }
+        }
+        if (it + (int)gridDim.x < nall) stage(it + (int)gridDim.x);
+        const bool any = __builtin_amdgcn_readfirstlane(block_or_lds(nonzero, orslot));
+        smear_zy_finish<T>(any, sp, i, jt, kt, A.D, A.fmask, tb, A.g.nx, A.g.ny, A.g.nz, A.st2);
+        lds_barrier();  // the next item's counts overwrite C and H
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_smear_xs(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {

@@ -1205,9 +1205,10 @@ __device__ __forceinline__ T *smem_zy() {
 }
 
 // (row << 16 | column) of the halo elements thread + 256 m (-1 past the R x W)
-__device__ __forceinline__ void smear_zy_coords(int R, int W, int (&ecoord)[kZyStaged]) {
+template <int NS>
+__device__ __forceinline__ void smear_zy_coords(int R, int W, int (&ecoord)[NS]) {
 #pragma unroll
-    for (int m = 0; m < kZyStaged; m++) {
+    for (int m = 0; m < NS; m++) {
         const int e = (int)threadIdx.x + m * kSmearThreads;
         const int r = e / W;
         ecoord[m] = e < R * W ? (r << 16 | (e - r * W)) : -1;
