@@ -60,6 +60,7 @@ struct hv_ctx {
     int sor_extrap = 3;                 // option HV_OPT_SOR_EXTRAPOLATE (mixed form only)
     int last_extrap = 0;                // predictor used by the last solve
     int f32_tol_shift = 0;              // option HV_OPT_SOR_F32_TOL_SHIFT: FP32 stage tolerance tol / 2^shift
+    int f32_omega = 0;                  // option HV_OPT_SOR_F32_OMEGA: 0 grid rule, 1 the run's omega, else /1000
     int sor_fused = 0;                  // option: one launch per SOR iteration (red + black fused; slower at 128^3)
     double *phi_alt = nullptr;          // second potential buffer of the fused SOR
     float *sor_delta_alt = nullptr;
@@ -957,6 +958,30 @@ SorCoef make_sor_coef(const hv_ctx *c, double omega, double tol) {
     return cf;
 }
 
+// The relaxation factor of the mixed form's FP32 correction (option
+// HV_OPT_SOR_F32_OMEGA).  The grid rule: the Jacobi iteration of the 7-point
+// Laplacian with Dirichlet faces has spectral radius mu = sum_d cos(pi /
+// (n_d - 1)) / h_d^2 over sum_d 1 / h_d^2, and SOR's optimal factor is
+// 2 / (1 + s), s = sqrt(1 - mu^2); the correction's error after an
+// extrapolated start is not the slowest mode alone, and 2 / (1 + 2.2 s) took
+// the fewest iterations on 64^3..128^3 grids.  Never below the run's own
+// omega: at a smaller factor the slowest mode decays more slowly, so the
+// correction stops (at the same tolerance on its updates) further from the
+// fixed point and the FP64 sweeps take more iterations.
+double mixed_omega32(const hv_ctx *c, double omega) {
+    if (c->f32_omega == 1) return omega;
+    if (c->f32_omega > 1) return c->f32_omega / 1000.0;
+    double num = 0.0, den = 0.0;
+    for (int d = 0; d < 3; d++) {
+        const double h = 2.0 * c->world.d_max[d] / c->world.n[d];
+        num += cos(HV_PI / (c->world.n[d] - 1)) / (h * h);
+        den += 1.0 / (h * h);
+    }
+    const double mu = num / den;
+    const double w = 2.0 / (1.0 + 2.2 * sqrt(1.0 - mu * mu));
+    return w > omega ? w : omega;
+}
+
 // tiles of the per-sweep SOR kernels (`planes` planes x kSorRows rows x 64
 // k-slots per block, XCD slabs)
 SorTiles make_sor_tiles(const hv_ctx *c, int planes = kSorPlanes) {
@@ -1059,6 +1084,8 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         // next start); the FP64 sweeps apply the reference rule either way
         SorCoef cf32 = cf;
         cf32.tol = ldexp(cf.tol, -c->f32_tol_shift);
+        cf32.omega = mixed_omega32(c, omega);
+        cf32.one_minus_omega = 1.0 - cf32.omega;
         if (res32) {
             // its iteration count is read back with the FP64 sweeps' first chunk
             rc = sor_resident_f32_enqueue(c, cf32, max_it);
@@ -2011,6 +2038,11 @@ int hv_set_option(hv_ctx *c, int option, int value) {
         if (value < 0 || value > 20) return set_err(c, HV_ERR_ARG, "the FP32 tolerance shift must be in 0..20");
         c->f32_tol_shift = value;
         return HV_OK;
+    case HV_OPT_SOR_F32_OMEGA:
+        if (value != 0 && value != 1 && (value < 1001 || value > 1999))
+            return set_err(c, HV_ERR_ARG, "the FP32 SOR factor must be 0, 1 or 1001..1999 (thousandths)");
+        c->f32_omega = value;
+        return HV_OK;
     case HV_OPT_SOR_EXTRAPOLATE:
         if (value < 0 || value > 5) return set_err(c, HV_ERR_ARG, "the SOR start predictor must be in 0..5");
         if (value != 0 && c->precision != HV_PREC_MIXED)
@@ -2037,6 +2069,7 @@ int hv_get_option(hv_ctx *c, int option, int *value) {
     case HV_OPT_SMEAR_FUSED: *value = c->smear_fused; return HV_OK;
     case HV_OPT_SOR_EXTRAPOLATE: *value = c->sor_extrap; return HV_OK;
     case HV_OPT_SOR_F32_TOL_SHIFT: *value = c->f32_tol_shift; return HV_OK;
+    case HV_OPT_SOR_F32_OMEGA: *value = c->f32_omega; return HV_OK;
     default: return set_err(c, HV_ERR_ARG, "unknown option %d", option);
     }
 }

@@ -223,6 +223,10 @@ void batch_fill_args(hv_batch *b, const hv_skyrme *sk, const hv_params *prm) {
         A.delta = c->sor_delta;
         A.tl = make_sor_tiles(c, kSorPlanesBatch);
         A.cf = make_sor_coef(c, p.sor_omega, p.sor_tol);
+        A.cf32 = A.cf;
+        A.cf32.tol = ldexp(A.cf.tol, -c->f32_tol_shift);
+        A.cf32.omega = mixed_omega32(c, p.sor_omega);
+        A.cf32.one_minus_omega = 1.0 - A.cf32.omega;
         A.maxd = c->maxd;
         A.ctl = c->ctl;
         A.A = sk[i].A;
@@ -515,8 +519,10 @@ int batch_sor_iterate(hv_batch *b, const hv_params *prm, bool f32, const std::ve
             for (int i = 0; i < n; i++) {
                 if (iters[i] >= 0 || grp[i] != g) continue;
                 const double *row = b->pinned + (size_t)i * len;
+                // (the FP32 correction stops at its own tolerance, as in sor_solve)
+                const double tl = f32 ? ldexp(prm[i].sor_tol, -b->inst[i]->f32_tol_shift) : prm[i].sor_tol;
                 for (int q = 0; q < len; q++) {
-                    if (row[q] < prm[i].sor_tol) {
+                    if (row[q] < tl) {
                         iters[i] = start + q + 1;
                         lastmax[i] = row[q];
                         break;
@@ -776,6 +782,9 @@ int batch_res32_prepare(hv_batch *b, const hv_params *prm) {
         I.sp = c->sp;
         I.tl = b->rtl[i];
         I.cf = make_sor_coef(c, prm[i].sor_omega, prm[i].sor_tol);
+        I.cf.tol = ldexp(I.cf.tol, -c->f32_tol_shift);
+        I.cf.omega = mixed_omega32(c, prm[i].sor_omega);
+        I.cf.one_minus_omega = 1.0 - I.cf.omega;
         I.max_it = max_it;
         const size_t nT = (size_t)I.tl.PI * I.tl.PJ;
         I.sy.xr = b->rx + b->rx_off[i];
@@ -1495,6 +1504,9 @@ int hv_batch_get_option(hv_batch *b, int option, int *value) {
     case HV_OPT_SMEAR_FUSED: *value = b->smear_fused; return HV_OK;
     case HV_OPT_BATCH_GRAPH: *value = b->use_graph; return HV_OK;
     case HV_OPT_SOR_EXTRAPOLATE: *value = b->extrap; return HV_OK;
+    case HV_OPT_SOR_F32_OMEGA:
+    case HV_OPT_SOR_F32_TOL_SHIFT:
+        return b->n > 0 ? hv_get_option(b->inst[0], option, value) : HV_OK;
     case HV_OPT_BATCH_DEVICE_STEER:  // once the batch has stepped: whether the steered step runs
         *value = b->steer && b->fin_state >= 0 && b->precision == HV_PREC_MIXED && b->res32 && b->res_state >= 0;
         return HV_OK;
@@ -1545,6 +1557,13 @@ int hv_batch_set_option(hv_batch *b, int option, int value) {
         batch_drop_graph(b);  // the step's device part gains or loses a kernel
         b->extrap = value;
         return HV_OK;
+    }
+    if (option == HV_OPT_SOR_F32_OMEGA || option == HV_OPT_SOR_F32_TOL_SHIFT) {
+        for (hv_ctx *c : b->inst) {
+            int rc = hv_set_option(c, option, value);
+            if (rc) return batch_fwd(b, c, rc);
+        }
+        return HV_OK;  // (the coefficients are rebuilt at the start of every call)
     }
     if (option == HV_OPT_SOR_F32_LAG) {
         if (value < 1 || value > 64) return batch_err(b, HV_ERR_ARG, "lag must be in 1..64");

@@ -52,6 +52,7 @@ struct BInst {
     float *R, *delta;
     SorTiles tl;
     SorCoef cf;
+    SorCoef cf32;  // the FP32 correction's (its relaxation factor, mixed_omega32)
     unsigned long long *maxd;
     SorCtl *ctl;
     double A, B, C, gamma;
@@ -329,7 +330,7 @@ kb_sor_colour_f32(const BInst<T> *__restrict__ a, const int *__restrict__ first,
                   int colour, int it, int max_it) {
     HV_BATCH_LOCATE_IDS();
     zero_next_row(A.maxd, colour, bid, it, max_it);
-    sor_colour_f32_body<T, kSorPlanesBatch>(bid, A.delta, A.R, A.sp, A.tl, A.cf, colour, it, A.maxd, A.ctl);
+    sor_colour_f32_body<T, kSorPlanesBatch>(bid, A.delta, A.R, A.sp, A.tl, A.cf32, colour, it, A.maxd, A.ctl);
 }
 
 // per-iteration maxima of iterations [it0, it1) of every instance into

@@ -59,6 +59,7 @@ HV_OPT_SOR_F32_OWNED = 10
 HV_OPT_SOR_EXTRAPOLATE = 11
 HV_OPT_SOR_F32_TOL_SHIFT = 12
 HV_OPT_BATCH_DEVICE_STEER = 13
+HV_OPT_SOR_F32_OMEGA = 14
 
 HV_KCLASS_SOR = 0
 HV_KCLASS_PARTICLE = 1

@@ -303,6 +303,16 @@ int hv_simulate(hv_ctx *ctx, const hv_skyrme *skyrme, const hv_params *params, d
  * step's FP32 iteration counts and steers the FP64 sweeps.  Same bits either
  * way. */
 #define HV_OPT_BATCH_DEVICE_STEER 13
+/* HV_OPT_SOR_F32_OMEGA (mixed form): the relaxation factor of the FP32
+ * correction.  The FP64 finishing sweeps apply the reference's omega and stop
+ * rule whatever the correction used, so the correction may relax at any
+ * convergent factor: 0 (default) takes 2 / (1 + 2.2 s) from the grid, with s
+ * = sqrt(1 - mu^2) and mu the Jacobi spectral radius of the grid's Laplacian
+ * (s = sin(pi / (n - 1)) on an n^3 grid: 1.80 at 64^3, 1.86 at 96^3, 1.90 at
+ * 128^3, the factors that took the fewest iterations on those grids), but
+ * never less than the run's own sor_omega; 1 takes the run's sor_omega;
+ * 1001..1999 take value / 1000. */
+#define HV_OPT_SOR_F32_OMEGA 14
 int hv_set_option(hv_ctx *ctx, int option, int value);
 int hv_get_option(hv_ctx *ctx, int option, int *value);
 
@@ -365,8 +375,8 @@ hv_ctx *hv_batch_instance(hv_batch *batch, int i);
 /* HV_OPT_SORT_EVERY or HV_OPT_SMEAR_TILE (applied to every instance), or
  * HV_OPT_BATCH_GROUP_KIB, HV_OPT_SOR_RESIDENT_F32, HV_OPT_SOR_F32_LAG,
  * HV_OPT_SOR_F32_OWNED, HV_OPT_SMEAR_FUSED (default 1 for a batch),
- * HV_OPT_BATCH_GRAPH, HV_OPT_BATCH_DEVICE_STEER, HV_OPT_SOR_EXTRAPOLATE (every
- * instance) */
+ * HV_OPT_BATCH_GRAPH, HV_OPT_BATCH_DEVICE_STEER, HV_OPT_SOR_EXTRAPOLATE and
+ * HV_OPT_SOR_F32_OMEGA (every instance) */
 int hv_batch_set_option(hv_batch *batch, int option, int value);
 /* the value of a batch option (HV_OPT_SOR_RESIDENT_F32 and
  * HV_OPT_SOR_F32_OWNED: whether that form runs, once the batch has solved) */
