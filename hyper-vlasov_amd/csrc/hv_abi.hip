@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <functional>
 #include <cstring>
 #include <cstdarg>
 #include <new>
@@ -36,6 +37,14 @@ struct hv_ctx {
     long long G = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // The work after a loop solve (potential, force grid), enqueued ahead of
+    // the FP64 sweeps' first read-back so that the device runs it while the
+    // host waits; spec_valid: that first chunk met the stop rule, so the
+    // enqueued work saw the final phi (else it is enqueued again after the
+    // solve and overwrites it).  ev_copy: the read-back's completion.
+    std::function<int()> spec;
+    bool spec_valid = false;
+    hipEvent_t ev_copy = nullptr;
     char err[512] = {0};
 
     // particle SoA (T = double or float): x y z kx ky kz fx fy fz
@@ -864,7 +873,8 @@ struct NoHook {
 // read back with the first chunk into c->xread before after_first_wait.
 template <typename Launch, typename Hook = NoHook>
 int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double *md_out, Launch launch,
-                Hook after_first_wait = Hook(), const float *xres = nullptr, const int *xerr = nullptr) {
+                Hook after_first_wait = Hook(), const float *xres = nullptr, const int *xerr = nullptr,
+                bool final_solve = false) {
     const double tol = cf.tol;
     int rc = HV_OK;
     int it = 0, iters = -1, start = 0;
@@ -883,7 +893,17 @@ int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double
         HV_CHECK_LAUNCH(c);
         HV_HIP(c, hipMemcpyAsync(c->pinned, c->maxc, sizeof(double) * (size_t)(end - start + (xr ? 3 : 0)),
                                  hipMemcpyDeviceToHost, c->stream));
-        HV_HIP(c, hipStreamSynchronize(c->stream));
+        // the solve's final chunk of a loop step: the step's later work goes in
+        // the queue first, and only the read-back is waited for
+        const bool spec_now = first && final_solve && c->spec;
+        if (spec_now) {
+            HV_HIP(c, hipEventRecord(c->ev_copy, c->stream));
+            rc = c->spec();
+            if (rc) return rc;
+            HV_HIP(c, hipEventSynchronize(c->ev_copy));
+        } else {
+            HV_HIP(c, hipStreamSynchronize(c->stream));
+        }
         if (first) {
             first = false;
             if (xr)
@@ -897,6 +917,7 @@ int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double
                 break;
             }
         }
+        if (spec_now) c->spec_valid = iters > 0;
         prof_flush(c);
         if (iters > 0) break;
         chunk = 4;
@@ -929,6 +950,8 @@ int sor_exact(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, bool fuse
         if (*iters & 1) std::swap(c->phi, c->phi_alt);
         return HV_OK;
     }
+    // (in place on c->phi: a loop step's later work may be enqueued ahead of
+    // the first read-back)
     return sor_iterate(
         c, cf, max_it, iters, md,
         [&](int it) {
@@ -936,7 +959,7 @@ int sor_exact(hv_ctx *c, const RT *rho, const SorCoef &cf, int max_it, bool fuse
                 sor_launch(c, it, colour, HV_KCLASS_SOR, k_sor_colour<RT>, sgrd, sblk, 0, c->phi, (const RT *)rho,
                            c->sp, tl, cf, colour, it, c->maxd, c->ctl);
         },
-        after_first_wait, xres, xerr);
+        after_first_wait, xres, xerr, true);
 }
 
 // SOR coefficients of the context's grid (include/physics.hpp:101-103)
@@ -1304,6 +1327,7 @@ int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, i
     rc = sor_solve<T>(c, (const T *)c->density, p->sor_omega, p->sor_tol, p->sor_max_it, iters, &md, true);
     if (rc < 0) return rc;
     *warn |= rc;
+    if (c->spec_valid) return HV_OK;  // the potential is already enqueued, from the final phi
     return potential_from_density<T>(c, s, true);
 }
 
@@ -1411,16 +1435,37 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
         HV_CHECK_LAUNCH(c);
         prof_end(c, HV_KCLASS_PARTICLE, pa, 1);
         int it = 0;
-        rc = field_solve<T>(c, s, p, &it, &warn);
-        if (rc < 0) return rc;
-        if (sor_iters) sor_iters[n] = it;
-        if (n + 1 < nsteps) {
-            // F = -grad U at the cells the deposit above recorded: the nodes the
-            // next particle pass gathers from
+        // F = -grad U at the cells the deposit above recorded: the nodes the
+        // next particle pass gathers from
+        auto force_grid = [&]() -> int {
             hipEvent_t pf = prof_begin(c);
             k_force_grid<T><<<c->clear_blocks, 256, 0, c->stream>>>((const T *)c->U, (T *)c->fgrid, c->fbox, g);
             HV_CHECK_LAUNCH(c);
             prof_end(c, HV_KCLASS_POTENTIAL, pf, 1);
+            return HV_OK;
+        };
+        // the potential and the force grid go in the queue ahead of the solve's
+        // last read-back (mixed form: its FP64 sweeps nearly always meet the
+        // stop rule in their first chunk); when they did not, both are
+        // enqueued again after the solve
+        c->spec_valid = false;
+        if (c->precision == HV_PREC_MIXED)
+            c->spec = [&]() -> int {
+                int r = potential_from_density<T>(c, s, true);
+                if (r) return r;
+                return n + 1 < nsteps ? force_grid() : HV_OK;
+            };
+        rc = field_solve<T>(c, s, p, &it, &warn);
+        c->spec = nullptr;
+        const bool tail_done = c->spec_valid;
+        c->spec_valid = false;
+        if (rc < 0) return rc;
+        if (sor_iters) sor_iters[n] = it;
+        if (n + 1 < nsteps) {
+            if (!tail_done) {
+                rc = force_grid();
+                if (rc) return rc;
+            }
             fgrid_ok = true;
         }
     }
@@ -1568,7 +1613,8 @@ int hv_ctx_create(const hv_world *world, int n_protons, int n_neutrons, int prec
     c->sp = make_split(*world);
     const size_t ts = tsize(c);
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
-    ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+    ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
+         hipEventCreateWithFlags(&c->ev_copy, hipEventDisableTiming) == hipSuccess;
     for (int q = 0; q < 9 && ok; q++) ok = hipMalloc(&c->part[q], ts * (size_t)c->total) == hipSuccess;
     for (int q = 0; q < 9 && ok; q++) ok = hipMalloc(&c->part_alt[q], ts * (size_t)c->total) == hipSuccess;
     ok = ok && hipMalloc((void **)&c->pid, sizeof(int) * (size_t)c->total) == hipSuccess;
@@ -1646,6 +1692,7 @@ int hv_ctx_destroy(hv_ctx *c) {
     for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev_copy) (void)hipEventDestroy(c->ev_copy);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return HV_OK;

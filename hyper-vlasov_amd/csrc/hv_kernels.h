@@ -305,9 +305,10 @@ constexpr int kBrickShift = 2;
 
 // Sort key of particle i: its species (protons first), then the 4^3-node
 // brick of its cell; particles outside the grid come last within their
-// species, spread over kSortOutside keys by slot (they all share one place in
-// the order, and one key would make every count atomic for them hit one word);
-// keys lie in [0, 2 (bricks + kSortOutside)).
+// species, spread over kSortOutside keys by the wave of their slot (they all
+// share one place in the order: one key would make every count atomic for
+// them hit one word, and a key per slot would leave each lane an atomic of its
+// own); keys lie in [0, 2 (bricks + kSortOutside)).
 constexpr int kSortOutside = 64;
 
 template <typename T>
@@ -315,7 +316,7 @@ __device__ __forceinline__ unsigned sort_key_of(const Particles<T> &p, int i, in
     const int bx = (g.nx + 3) >> kBrickShift, by = (g.ny + 3) >> kBrickShift, bz = (g.nz + 3) >> kBrickShift;
     const unsigned nb = (unsigned)(bx * by * bz);
     Cell<T> c;
-    unsigned b = nb + ((unsigned)i & (kSortOutside - 1));
+    unsigned b = nb + (((unsigned)i / kWave) & (kSortOutside - 1));
     if (cic_cell(p.x[i], p.y[i], p.z[i], g, c))
         b = (unsigned)(((c.x0 >> kBrickShift) * by + (c.y0 >> kBrickShift)) * bz + (c.z0 >> kBrickShift));
     return (i < np ? 0u : nb + (unsigned)kSortOutside) + b;
