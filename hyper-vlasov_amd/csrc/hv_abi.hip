@@ -146,6 +146,7 @@ struct hv_ctx {
     size_t res32_lds = 0;
     unsigned long long *res32_x = nullptr;  // tagged exchange words (2 colours)
     const void *res32_kernel = nullptr;     // instance for the chosen tile shape
+    const void *res32_kernel_fold = nullptr;  // the deep-halo body with the defect and fold in it (or none)
     float *res32_slots = nullptr;
     int res32_slots_cap = 0;
     int *res32_err = nullptr;
@@ -659,6 +660,7 @@ int res32_setup(hv_ctx *c) {
         return 1;
     // a fixed-shape instance for the common tilings (same LDS size and block)
     c->res32_kernel = generic;
+    c->res32_kernel_fold = nullptr;
     if (per == 2) {
         if (best.TI == 8 && best.TJ == 4 && best.nzh == 64)
             c->res32_kernel = (const void *)k_sor_resident_f32<8, 4, 64, 512>;
@@ -705,6 +707,9 @@ int res32_setup(hv_ctx *c) {
             c->res32_kernel = k;
             lds = l;
             deep = true;
+            const void *kf = per == 2 ? (const void *)k_sor_resident_f32d_fold<HV_DEEP_SOLO2, 512>
+                                      : (const void *)k_sor_resident_f32d_fold<HV_DEEP_SOLO>;
+            c->res32_kernel_fold = try_kernel(kf, l) ? kf : nullptr;
         }
     }
     if (!deep && per == 1 && c->res32_owned >= 1 && owned_fits<HV_OWNED_SOLO, kRes32Threads>(best)) {
@@ -759,7 +764,7 @@ int sor_resident_f32_clears(hv_ctx *c, int max_it, ResClears *cl) {
 
 // (after the clears of sor_resident_f32_clears; its iteration count, tested
 // max and error word are read back with the FP64 sweeps' first chunk)
-int sor_resident_f32_enqueue(hv_ctx *c, const SorCoef &cf, int max_it) {
+int sor_resident_f32_enqueue(hv_ctx *c, const SorCoef &cf, int max_it, const float *rho_fold = nullptr) {
     const int nT = c->rtl32.PI * c->rtl32.PJ;
     ResSync32 sy;
 #ifdef HV_RES_TRACE
@@ -779,9 +784,15 @@ int sor_resident_f32_enqueue(hv_ctx *c, const SorCoef &cf, int max_it) {
     ResTiles tl = c->rtl32;
     SorCoef cfv = cf;
     int mi = max_it, lag = c->res32_lag;
+    double *phi = c->phi;
+    const float *rho = rho_fold;
     void *args[] = {&delta, (void *)&R, &sp, &tl, &cfv, &mi, &lag, &sy};
+    // (rho_fold: the variant that computes its residual from phi and adds its
+    // correction to phi, k_sor_resident_f32d_fold)
+    void *args_fold[] = {&phi, (void *)&rho, &sp, &tl, &cfv, &mi, &lag, &sy};
     hipEvent_t pa = prof_begin(c);
-    HV_HIP(c, hipLaunchCooperativeKernel(c->res32_kernel, dim3((unsigned)nT), dim3((unsigned)c->res32_threads), args,
+    HV_HIP(c, hipLaunchCooperativeKernel(rho_fold ? c->res32_kernel_fold : c->res32_kernel, dim3((unsigned)nT),
+                                         dim3((unsigned)c->res32_threads), rho_fold ? args_fold : args,
                                          (unsigned)c->res32_lds, c->stream));
     hipEvent_t pb = nullptr;
     if (pa) {
@@ -1067,6 +1078,16 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
     int iters = 0;
     double md = 0.0;
     if (c->precision == HV_PREC_MIXED && c->phi_warm) {
+        const bool res32 = c->sor_res32 && res32_setup(c) == HV_OK;
+        // the resident correction computes its own residual and folds its
+        // result into phi when a launch ahead of it (the extrapolation) can
+        // clear its words: no defect and no fold launch (the same bits)
+        const bool infold = res32 && loop && c->sor_extrap > 0 && c->res32_kernel_fold && std::is_same<RT, float>::value;
+        ResClears cl{};
+        if (res32) {
+            rc = sor_resident_f32_clears(c, max_it, &cl);
+            if (rc) return rc;
+        }
         if (loop && c->sor_extrap > 0) {
             const int id = sor_predictor_for(c->sor_extrap, c->hist_n);
             SorHist ring;
@@ -1075,12 +1096,13 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
             double w[5];
             const unsigned nb = blocks_for(2 * c->sp.Gh);
             const long long nn = 2 * c->sp.Gh;
+            const ResClears xc = infold ? cl : ResClears{};
             switch (sor_predictor(id, w)) {
-            case 1: k_sor_extrapolate<1><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
-            case 2: k_sor_extrapolate<2><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
-            case 3: k_sor_extrapolate<3><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
-            case 4: k_sor_extrapolate<4><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
-            default: k_sor_extrapolate<5><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn); break;
+            case 1: k_sor_extrapolate<1><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn, xc); break;
+            case 2: k_sor_extrapolate<2><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn, xc); break;
+            case 3: k_sor_extrapolate<3><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn, xc); break;
+            case 4: k_sor_extrapolate<4><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn, xc); break;
+            default: k_sor_extrapolate<5><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn, xc); break;
             }
             HV_CHECK_LAUNCH(c);
             c->hist_pos = (c->hist_pos + 4) % 5;
@@ -1089,12 +1111,9 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         // FP32 defect-correction iterations on the small correction of a warm
         // potential, then FP64 sweeps until the stop rule holds in FP64
         float *dbuf[2] = {c->sor_delta, c->sor_delta_alt};
-        const bool res32 = c->sor_res32 && res32_setup(c) == HV_OK;
-        if (res32) {
+        if (infold) {
+        } else if (res32) {
             // the defect launch also clears the resident solve's words
-            ResClears cl;
-            rc = sor_resident_f32_clears(c, max_it, &cl);
-            if (rc) return rc;
             k_sor_defect_clear<RT><<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(
                 c->phi, rho, c->sor_R, c->sor_delta, c->sp, cf, cl);
         } else {
@@ -1111,7 +1130,7 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
         cf32.one_minus_omega = 1.0 - cf32.omega;
         if (res32) {
             // its iteration count is read back with the FP64 sweeps' first chunk
-            rc = sor_resident_f32_enqueue(c, cf32, max_it);
+            rc = sor_resident_f32_enqueue(c, cf32, max_it, infold ? (const float *)rho : nullptr);
         } else if (fused) {
             rc = sor_iterate(c, cf32, max_it, &iters, &md, [&](int it) {
                 sor_launch(c, it, 1, HV_KCLASS_SOR_F32, k_sor_fused<false, float, float>, fgrd, fblk,
@@ -1126,9 +1145,11 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
             });
         }
         if (rc) return rc;
-        const float *dres = (fused && !res32) ? dbuf[iters & 1] : c->sor_delta;
-        k_sor_fold<<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, dres, 2 * c->sp.Gh);
-        HV_CHECK_LAUNCH(c);
+        if (!infold) {
+            const float *dres = (fused && !res32) ? dbuf[iters & 1] : c->sor_delta;
+            k_sor_fold<<<blocks_for(2 * c->sp.Gh), kThreads, 0, c->stream>>>(c->phi, dres, 2 * c->sp.Gh);
+            HV_CHECK_LAUNCH(c);
+        }
         int it64 = 0;
         // FP64 sweeps: the first chunk is one iteration (prediction 2), within
         // every budget (at least 4), so it is queued before the resident

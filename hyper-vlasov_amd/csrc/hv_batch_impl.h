@@ -74,6 +74,7 @@ struct hv_batch {
     int res32 = 1;                        // option: use it when every instance tiles
     int res32_owned = 2;                  // option: 2 deep-halo, 1 lane-owned, 0 per-row (when every instance fits)
     int res32_is_owned = 0;               // the body the chosen kernel runs (same values)
+    bool res32_fold = false;              // that kernel computes its residual and folds its correction
     int res32_lag = 2;                    // option: lag of its convergence test
     int res_state = 0;                    // 0: not set up, 1: ready, -1: unavailable
     int res_max_it = 0;
@@ -655,8 +656,9 @@ int batch_res32_setup(hv_batch *b, int max_it) {
     bool owned = !deep && b->res32_owned >= 1;
     for (int i = 0; i < n && owned; i++) owned = batch_owned_tiles(b->inst[i], ncu, &b->rtl[i]) == HV_OK;
     b->res32_is_owned = deep ? 2 : (owned ? 1 : 0);
+    b->res32_fold = deep;
     if (deep) {
-        kern = (const void *)kb_sor_resident_f32d<HV_DEEP_BATCH>;
+        kern = (const void *)kb_sor_resident_f32d<HV_DEEP_BATCH, true>;
         lds = DeepLayout<kDeepBatchP, kDeepBatchQJ, kDeepBatchIM>::bytes(kRes32Threads / kWave);
     } else if (owned) {
         kern = (const void *)kb_sor_resident_f32o<HV_OWNED_BATCH>;
@@ -779,6 +781,8 @@ int batch_res32_prepare(hv_batch *b, const hv_params *prm) {
         BRes32 &I = h[i];
         I.delta = c->sor_delta;
         I.R = c->sor_R;
+        I.phi = c->phi;
+        I.rho = (const float *)c->rho_s;
         I.sp = c->sp;
         I.tl = b->rtl[i];
         I.cf = make_sor_coef(c, prm[i].sor_omega, prm[i].sor_tol);
@@ -1022,12 +1026,18 @@ int batch_step_device(hv_batch *b, int end_kick, bool res, bool steer = false, b
             kb_sor_extrapolate<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n, b->xorder);
             HV_BHIP(b, hipGetLastError());
         }
-        kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
-        HV_BHIP(b, hipGetLastError());
+        // (the deep-halo resident kernel computes the residual and folds the
+        // correction itself)
+        if (!b->res32_fold) {
+            kb_sor_defect<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
+            HV_BHIP(b, hipGetLastError());
+        }
         int rc = batch_res32_enqueue(b, !steer);
         if (rc) return rc;
-        kb_sor_fold<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
-        HV_BHIP(b, hipGetLastError());
+        if (!b->res32_fold) {
+            kb_sor_fold<T><<<b->blocks[kBGh2], kThreads, 0, b->stream>>>(a, btab(b, kBGh2), n);
+            HV_BHIP(b, hipGetLastError());
+        }
         if (steer) {
             rc = batch_fin_enqueue<T>(b);
             if (rc) return rc;

@@ -26,6 +26,8 @@ namespace hv {
 struct BRes32 {
     float *delta;
     const float *R;
+    double *phi;       // (the deep-halo kernel's FUSE form: the residual from phi and rho,
+    const float *rho;  // the correction added to phi)
     Split sp;
     ResTiles tl;
     SorCoef cf;
@@ -90,8 +92,10 @@ kb_sor_resident_f32o(const BRes32 *__restrict__ inst, const int *__restrict__ jo
 
 // The same job loop with the deep-halo body <SI, SR, SH, P, QJ, IM> (tilings
 // with the deep exchange stride; the host launches it only when every
-// instance fits).
-template <int SI, int SR, int SH, int P, int QJ, int IM>
+// instance fits).  FUSE: each tile computes its residual from the instance's
+// phi and rho and adds its correction to phi (the defect and fold launches'
+// parts, the same bits).
+template <int SI, int SR, int SH, int P, int QJ, int IM, bool FUSE = false>
 __global__ void __launch_bounds__(kRes32Threads)
 kb_sor_resident_f32d(const BRes32 *__restrict__ inst, const int *__restrict__ job_first,
                      const int2 *__restrict__ jobs, int lag) {
@@ -103,7 +107,9 @@ kb_sor_resident_f32d(const BRes32 *__restrict__ inst, const int *__restrict__ jo
         if (threadIdx.x == 0) I.jtrace[2 * j] = wall_clock64();
 #endif
         if (deep_fits<SI, SR, SH, P, QJ, IM, kRes32Threads>(I.tl) && I.tl.xstride == deep_xstride(I.tl))
-            res32d_solve_tile<SI, SR, SH, P, QJ, IM>(I.delta, I.R, I.sp, I.tl, I.cf, I.max_it, lag, I.sy, jb.y);
+            res32d_solve_tile<SI, SR, SH, P, QJ, IM, true, kRes32Threads, FUSE>(I.delta, I.R, I.sp, I.tl, I.cf,
+                                                                               I.max_it, lag, I.sy, jb.y, I.phi,
+                                                                               I.rho);
         else if (threadIdx.x == 0)
             st_dev_i(I.sy.err, 1);
         __syncthreads();

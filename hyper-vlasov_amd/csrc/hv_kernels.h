@@ -1960,7 +1960,24 @@ k_sor_colour(double *__restrict__ phi, const RT *__restrict__ rho, Split sp, Sor
 // max |update|, carried in FP32 on the small correction delta (half the bytes
 // per sweep).  R is computed once per solve in FP64 and stored in FP32.
 
-// R (FP32, split) for the interior nodes, zero elsewhere; delta set to zero.
+// The FP32 residual at node qq of colour `colour` (split index; its (i, j, h)
+// given): R for an interior node, zero elsewhere.
+template <typename RT>
+__device__ __forceinline__ float sor_defect_at(const double *__restrict__ phi, const RT *__restrict__ rho, Split sp,
+                                               const SorCoef &cf, int colour, long long qq, int i, int j, int h) {
+    const int off = (i + j + colour) & 1;
+    const int k = 2 * h + off;
+    if (!(i >= 1 && i <= sp.nx - 2 && j >= 1 && j <= sp.ny - 2 && k >= 1 && k <= sp.nz - 2)) return 0.0f;
+    const long long q = qq + (colour ? sp.Gh : 0);
+    const double *oth = phi + (colour ? 0 : sp.Gh);
+    const long long sx = (long long)sp.ny * sp.nzh;
+    const double ax = (oth[qq + sx] + oth[qq - sx]) * cf.ydx2;
+    const double ay = (oth[qq + sp.nzh] + oth[qq - sp.nzh]) * cf.ydy2;
+    const double az = (oth[qq + off] + oth[qq + off - 1]) * cf.ydz2;
+    return (float)(ax + ay + az + cf.src * (double)rho[q] - phi[q] / cf.inv_dx2);
+}
+
+// R (FP32, split) for every node; delta set to zero.
 template <typename RT>
 __device__ __forceinline__ void sor_defect_body(const int bid, const double *__restrict__ phi,
                                                 const RT *__restrict__ rho, float *__restrict__ R,
@@ -1972,18 +1989,7 @@ __device__ __forceinline__ void sor_defect_body(const int bid, const double *__r
     const int h = (int)(qq % sp.nzh);
     const long long row = qq / sp.nzh;
     const int j = (int)(row % sp.ny), i = (int)(row / sp.ny);
-    const int off = (i + j + colour) & 1;
-    const int k = 2 * h + off;
-    float res = 0.0f;
-    if (i >= 1 && i <= sp.nx - 2 && j >= 1 && j <= sp.ny - 2 && k >= 1 && k <= sp.nz - 2) {
-        const double *oth = phi + (colour ? 0 : sp.Gh);
-        const long long sx = (long long)sp.ny * sp.nzh;
-        const double ax = (oth[qq + sx] + oth[qq - sx]) * cf.ydx2;
-        const double ay = (oth[qq + sp.nzh] + oth[qq - sp.nzh]) * cf.ydy2;
-        const double az = (oth[qq + off] + oth[qq + off - 1]) * cf.ydz2;
-        res = (float)(ax + ay + az + cf.src * (double)rho[q] - phi[q] / cf.inv_dx2);
-    }
-    R[q] = res;
+    R[q] = sor_defect_at<RT>(phi, rho, sp, cf, colour, qq, i, j, h);
     delta[q] = 0.0f;
 }
 
@@ -2175,7 +2181,14 @@ __device__ __forceinline__ void sor_extrapolate_body(const int bid, double *__re
 
 template <int NPTS>
 __global__ void __launch_bounds__(kBlock) k_sor_extrapolate(double *__restrict__ phi, SorHist hist, int id,
-                                                            long long n) {
+                                                            long long n, ResClears cl) {
+    // (with the LDS-resident correction that computes its own residual, this
+    // launch also clears that solve's words, as k_sor_defect_clear does)
+    const long long q0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = q0; e < cl.nslots; e += stride) cl.slots[e] = -1.0f;
+    for (long long e = q0; e < cl.nx; e += stride) cl.x[e] = 0ULL;
+    if (cl.err && q0 < 4) cl.err[q0] = 0;
     sor_extrapolate_body<NPTS>(blockIdx.x, phi, hist, id, n);
 }
 

@@ -471,10 +471,19 @@ __host__ __device__ inline bool deep_fits(const ResTiles &t) {
 // MP: the convergence poll rides in the rounds of the neighbour fetch (one
 // round trip fewer per iteration); without it wave 0 polls the slot row on
 // its own before the fetch
-template <int SI, int SR, int SH, int P, int QJ, int IM, bool MP = true, int NT = kRes32Threads>
+// FUSE: the solve also takes the defect launch's and the fold's part for its
+// tile: delta starts at zero in LDS, each slot's R is the FP32 residual of phi
+// and rho computed here (sor_defect_at, the defect launch's arithmetic), and
+// the result is added to phi (phi += (double) delta, the fold's arithmetic)
+// instead of being stored; the same bits as defect, solve and fold in three
+// launches.  No tile folds before every tile of the solve has computed its
+// residual: a tile stops only after reading every tile's maximum of an
+// iteration, which each tile writes after its set-up.
+template <int SI, int SR, int SH, int P, int QJ, int IM, bool MP = true, int NT = kRes32Threads, bool FUSE = false>
 __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, const float *__restrict__ R, Split sp,
                                                   ResTiles tl, SorCoef cf, int max_it, int lag, ResSync32 sy,
-                                                  const int t) {
+                                                  const int t, double *__restrict__ phi = nullptr,
+                                                  const float *__restrict__ rho = nullptr) {
     using L = DeepLayout<P, QJ, IM>;
     static_assert(L::E < (1 << 16), "the exchange words' LDS slots are 16-bit fields");
     constexpr int kFetchMaxD = fetch_max_d<NT>();
@@ -516,8 +525,13 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
         if (!(i >= 0 && i < nx && j >= 0 && j < ny)) continue;
         const long long q = ((long long)i * ny + j) * nzh + h;
         const int w = lrow(li, lj) + h;
-        if (own || ring1) x0[w] = delta[q];
-        if (black) x1[w] = delta[sp.Gh + q];
+        if constexpr (FUSE) {
+            if (own || ring1) x0[w] = 0.0f;
+            if (black) x1[w] = 0.0f;
+        } else {
+            if (own || ring1) x0[w] = delta[q];
+            if (black) x1[w] = delta[sp.Gh + q];
+        }
     }
 
     // Row sets: interior rows, the ring of the tile's face rows, and the ring
@@ -582,8 +596,13 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
             d = (lds0 + 4u * (unsigned)(w - XS)) | o0 << 20 | o1 << 21 | ok0 << 22 | ok1 << 23;
             if (i >= 0 && i < nx && j >= 0 && j < ny) {
                 const long long g = ((long long)i * ny + j) * nzh + h;
-                r0 = R[g];
-                if (set < 2) r1 = R[sp.Gh + g];
+                if constexpr (FUSE) {
+                    r0 = sor_defect_at<float>(phi, rho, sp, cf, 0, g, i, j, h);
+                    if (set < 2) r1 = sor_defect_at<float>(phi, rho, sp, cf, 1, g, i, j, h);
+                } else {
+                    r0 = R[g];
+                    if (set < 2) r1 = R[sp.Gh + g];
+                }
             }
         }
         D[s] = d;
@@ -931,8 +950,13 @@ __device__ __forceinline__ void res32d_solve_tile(float *__restrict__ delta, con
         if (i < nx && j < ny) {
             const long long q = ((long long)i * ny + j) * nzh + h;
             const int w = lrow(li, lj) + h;
-            delta[q] = x0[w];
-            delta[sp.Gh + q] = x1[w];
+            if constexpr (FUSE) {
+                phi[q] += (double)x0[w];
+                phi[sp.Gh + q] += (double)x1[w];
+            } else {
+                delta[q] = x0[w];
+                delta[sp.Gh + q] = x1[w];
+            }
         }
     }
     if (t == 0 && tid == 0) {
@@ -975,6 +999,16 @@ __global__ void __launch_bounds__(NT, kRes32Threads / NT)
 k_sor_resident_f32d(float *__restrict__ delta, const float *__restrict__ R, Split sp, ResTiles tl, SorCoef cf,
                     int max_it, int lag, ResSync32 sy) {
     res32d_solve_tile<SI, SR, SH, P, QJ, IM, true, NT>(delta, R, sp, tl, cf, max_it, lag, sy, (int)blockIdx.x);
+}
+
+// The same with the defect and the fold in the solve (FUSE above): phi is
+// read for the residual and receives the correction, rho is the split source.
+template <int SI, int SR, int SH, int P, int QJ, int IM, int NT = kRes32Threads>
+__global__ void __launch_bounds__(NT, kRes32Threads / NT)
+k_sor_resident_f32d_fold(double *__restrict__ phi, const float *__restrict__ rho, Split sp, ResTiles tl, SorCoef cf,
+                         int max_it, int lag, ResSync32 sy) {
+    res32d_solve_tile<SI, SR, SH, P, QJ, IM, true, NT, true>(nullptr, nullptr, sp, tl, cf, max_it, lag, sy,
+                                                              (int)blockIdx.x, phi, rho);
 }
 
 }  // namespace hv

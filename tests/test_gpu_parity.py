@@ -386,6 +386,34 @@ def test_determinism_large():
     _eq(outs[0][2], outs[1][2])
 
 
+def test_resident_solve_with_defect_and_fold_equals_three_launches():
+    """In the mixed loop (extrapolated starts) the deep-halo resident FP32
+    correction computes its residual and folds its correction itself; the
+    lane-owned body (HV_OPT_SOR_F32_OWNED 1) still runs the defect launch, the
+    solve and the fold launch.  Both give the same bits, step by step."""
+    w = configs.c1()
+    steps = 24
+    outs = []
+    for owned in (2, 1):
+        print("mixed loop, resident body", owned, flush=True)
+        sim = hvamd.Simulation(w.world(), w.n_protons, w.n_neutrons, hvamd.HV_PREC_MIXED)
+        sim.set_option(hvamd.HV_OPT_SOR_F32_OWNED, owned)
+        sim.upload_particles(*w.initial_state())
+        sim.run_init(w.sky(), w.params(steps))
+        stats, its = sim.run_steps(w.sky(), w.params(steps), 0, steps)
+        assert sim.get_option(hvamd.HV_OPT_SOR_RESIDENT_F32) == 1
+        outs.append((stats, its, sim.download_particles(), sim.download_field(hvamd.HV_FIELD_COULOMB),
+                     sim.get_option(hvamd.HV_OPT_SOR_F32_OWNED)))
+        sim.close()
+    print("bodies that ran:", outs[0][4], outs[1][4])
+    assert outs[0][4] == 2 and outs[1][4] != 2  # (1, or the per-row body where that does not fit)
+    _eq(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    for k in ("x", "y", "z", "kx", "ky", "kz"):
+        _eq(outs[0][2][k], outs[1][2][k])
+    _eq(outs[0][3], outs[1][3])
+
+
 @pytest.mark.parametrize("sort_every", [0, 1, 3])
 def test_sort_cadence_does_not_change_bits(oracle, sort_every):
     """Particle sorting (any cadence) leaves every result bit unchanged, and
