@@ -44,6 +44,7 @@ struct hv_ctx {
     // solve and overwrites it).  ev_copy: the read-back's completion.
     std::function<int()> spec;
     bool spec_valid = false;
+    bool rho_s_ready = false;           // the x pass wrote rho_s from the density just smeared
     hipEvent_t ev_copy = nullptr;
     char err[512] = {0};
 
@@ -531,10 +532,14 @@ int smear(hv_ctx *c, bool from_acc, double sigma_r, int ppn) {
         const long long nq = n2 / VW;
         if (c->smear_xs_vec && nz % VW == 0 && c->xs_rxm > 0) {
             const unsigned nb = (unsigned)((nq + kThreads - 1) / kThreads);
+            // a loop step's x pass also writes the SOR's colour-split source
+            // (the solve that follows then skips k_rho_split)
+            T *rs = from_acc ? (T *)c->rho_s : nullptr;
+            if (rs) c->rho_s_ready = true;
             switch (c->xs_rxm) {
-            case 4: k_smear_xs_vec<T, 4><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl); break;
-            case 9: k_smear_xs_vec<T, 9><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl); break;
-            default: k_smear_xs_vec<T, 14><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl); break;
+            case 4: k_smear_xs_vec<T, 4><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl, rs, c->sp); break;
+            case 9: k_smear_xs_vec<T, 9><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl, rs, c->sp); break;
+            default: k_smear_xs_vec<T, 14><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl, rs, c->sp); break;
             }
         } else {
             k_smear_xs<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl);
@@ -1046,8 +1051,11 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
     c->last_extrap = 0;
     const int nx = c->world.n[0], ny = c->world.n[1], nz = c->world.n[2];
     RT *rho = (RT *)c->rho_s;
-    k_rho_split<RT><<<blocks_for(c->G), kThreads, 0, c->stream>>>(rho_nat, rho, c->G, c->sp);
-    HV_CHECK_LAUNCH(c);
+    if (!(loop && c->rho_s_ready && rho_nat == (const RT *)c->density)) {
+        k_rho_split<RT><<<blocks_for(c->G), kThreads, 0, c->stream>>>(rho_nat, rho, c->G, c->sp);
+        HV_CHECK_LAUNCH(c);
+    }
+    c->rho_s_ready = false;
     int rc = ensure_maxd(c, max_it);
     if (rc) return rc;
     rc = ensure_pinned(c, (size_t)max_it + 8);

@@ -1018,8 +1018,10 @@ int batch_step_device(hv_batch *b, int end_kick, bool res, bool steer = false, b
         kb_zero_acc<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
         HV_BHIP(b, hipGetLastError());
     }
-    kb_rho_split<T><<<b->blocks[kBG], kThreads, 0, b->stream>>>(a, btab(b, kBG), n);
-    HV_BHIP(b, hipGetLastError());
+    if (b->smear_fused || !b->xs_vec) {  // (the vector x pass writes the split source itself)
+        kb_rho_split<T><<<b->blocks[kBG], kThreads, 0, b->stream>>>(a, btab(b, kBG), n);
+        HV_BHIP(b, hipGetLastError());
+    }
     if (res) {
         BStage st(b, HV_KCLASS_SOR_STAGE);
         if (b->extrap) {

@@ -217,15 +217,20 @@ kb_smear_xs(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_smear_xs_vec(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
+    // (the protons' densities also into the colour-split source: no
+    // kb_rho_split launch after it)
     HV_BATCH_LOCATE();
     SmearTabC<T> *tb = smear_tab_c(A.tab);
     const int rx = tb->rx;  // wave-uniform: one instance per block
     if (rx <= 4)
-        smear_xs_vec_body<T, 4>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2);
+        smear_xs_vec_body<T, 4>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2,
+                                 (T *)A.rho_s, A.sp);
     else if (rx <= 9)
-        smear_xs_vec_body<T, 9>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2);
+        smear_xs_vec_body<T, 9>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2,
+                                 (T *)A.rho_s, A.sp);
     else
-        smear_xs_vec_body<T, 14>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2);
+        smear_xs_vec_body<T, 14>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2,
+                                 (T *)A.rho_s, A.sp);
 }
 
 template <typename T>

@@ -1570,7 +1570,7 @@ __device__ __forceinline__ void smear_xs_vec_body(const int bid, const T *__rest
                                                   const unsigned *__restrict__ fmask,
                                                   const unsigned *__restrict__ cbox, int cw,
                                                   SmearTabC<T> *__restrict__ tb, int nx, int ny, int nz,
-                                                  SmearTiles st) {
+                                                  SmearTiles st, T *__restrict__ rs = nullptr, Split sps = Split{}) {
     constexpr int VW = smear_vw<T>(), NA = 2 * RXM + 1;
     static_assert(RXM <= 16, "two mask words cover the planes of a node");
     typedef T V __attribute__((ext_vector_type(VW)));
@@ -1587,6 +1587,14 @@ __device__ __forceinline__ void smear_xs_vec_body(const int bid, const T *__rest
     const long long q = sp * G + (long long)i * plane + (long long)j * nz + k0;
     const int rx = tb->rx;
     V *out = (V *)(rho + q);
+    // rs: the protons' densities also into the SOR's colour-split source (the
+    // values k_rho_split would copy there)
+    auto put = [&](const V &v) {
+        *out = v;
+        if (rs && sp == 0)
+#pragma unroll
+            for (int u = 0; u < VW; u++) rs[split_index(sps, i, j, k0 + u)] = v[u];
+    };
     int plo = 0, phi = nx - 1;  // planes that may hold counts
     if (cbox) {
         const unsigned *cb = cbox + (long long)sp * cw;
@@ -1595,7 +1603,7 @@ __device__ __forceinline__ void smear_xs_vec_body(const int bid, const T *__rest
         const int ci1 = cbox_hi(cb[3]), cj1 = cbox_hi(cb[4]), ck1 = cbox_hi(cb[5]);
         if (ci1 < ci0 || i < ci0 - rx || i > ci1 + rx || j < cj0 - ry || j > cj1 + ry || k0 + VW - 1 < ck0 - rz ||
             k0 > ck1 + rz) {
-            *out = V(T(0.0));
+            put(V(T(0.0)));
             return;
         }
         plo = ci0;
@@ -1625,15 +1633,15 @@ __device__ __forceinline__ void smear_xs_vec_body(const int bid, const T *__rest
     const T term = tb->term;
 #pragma unroll
     for (int v = 0; v < VW; v++) acc[v] = acc[v] * term;
-    *out = acc;
+    put(acc);
 }
 
 template <typename T, int RXM>
 __global__ void __launch_bounds__(kBlock)
 k_smear_xs_vec(const T *__restrict__ D, T *__restrict__ rho, const unsigned *__restrict__ fmask,
                const unsigned *__restrict__ cbox, int cw, const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz,
-               SmearTiles st) {
-    smear_xs_vec_body<T, RXM>(blockIdx.x, D, rho, fmask, cbox, cw, smear_tab_c(tb), nx, ny, nz, st);
+               SmearTiles st, T *__restrict__ rs, Split sps) {
+    smear_xs_vec_body<T, RXM>(blockIdx.x, D, rho, fmask, cbox, cw, smear_tab_c(tb), nx, ny, nz, st, rs, sps);
 }
 
 // ---- Skyrme potentials (include/physics.hpp:273-281), optionally + phi ------
