@@ -1015,7 +1015,8 @@ int batch_step_device(hv_batch *b, int end_kick, bool res, bool steer = false, b
             else
                 kb_smear_xs<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
         }
-        kb_zero_acc<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
+        if (b->smear_fused || !b->xs_vec)  // (the vector x pass clears the counts itself)
+            kb_zero_acc<T><<<b->blocks[kBN2], kThreads, 0, b->stream>>>(a, btab(b, kBN2), n);
         HV_BHIP(b, hipGetLastError());
     }
     if (b->smear_fused || !b->xs_vec) {  // (the vector x pass writes the split source itself)

@@ -217,20 +217,22 @@ kb_smear_xs(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 kb_smear_xs_vec(const BInst<T> *__restrict__ a, const int *__restrict__ first, int n) {
-    // (the protons' densities also into the colour-split source: no
-    // kb_rho_split launch after it)
+    // (the protons' densities also into the colour-split source, and the
+    // deposit's counts and box words cleared: no kb_rho_split or kb_zero_acc
+    // launch after it)
     HV_BATCH_LOCATE();
+    if (bid == 0 && threadIdx.x < cbox_acc_words(A.g.nx)) A.acc[2 * A.G + threadIdx.x] = 0ULL;
     SmearTabC<T> *tb = smear_tab_c(A.tab);
     const int rx = tb->rx;  // wave-uniform: one instance per block
     if (rx <= 4)
         smear_xs_vec_body<T, 4>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2,
-                                 (T *)A.rho_s, A.sp);
+                                 (T *)A.rho_s, A.sp, A.acc);
     else if (rx <= 9)
         smear_xs_vec_body<T, 9>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2,
-                                 (T *)A.rho_s, A.sp);
+                                 (T *)A.rho_s, A.sp, A.acc);
     else
         smear_xs_vec_body<T, 14>(bid, A.D, A.rho, A.fmask, nullptr, 0, tb, A.g.nx, A.g.ny, A.g.nz, A.st2,
-                                 (T *)A.rho_s, A.sp);
+                                 (T *)A.rho_s, A.sp, A.acc);
 }
 
 template <typename T>

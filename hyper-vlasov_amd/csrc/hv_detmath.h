@@ -216,6 +216,20 @@ HV_HD HV_INLINE double hv_skyrme_f64(double A, double B, double C, double gamma,
     return A * t + B * hv_det_pow(t, gamma) + 2.0 * tau * C * ((rho_n - rho_p) / HV_RHO0);
 }
 
+/* Both species' potentials at one node, with the parts they share (t, t^gamma,
+ * A t + B t^gamma and the isovector density) evaluated once: each result is the
+ * same operations in the same order as hv_skyrme_f64 for its species, so the
+ * same bits, for half the work. */
+HV_HD HV_INLINE void hv_skyrme_pair_f64(double A, double B, double C, double gamma, double rho_p, double rho_n,
+                                        double *u_p, double *u_n) {
+    double rho = rho_p + rho_n;
+    double t = rho / HV_RHO0;
+    double base = A * t + B * hv_det_pow(t, gamma);
+    double d = (rho_n - rho_p) / HV_RHO0;
+    *u_p = base + 2.0 * -1.0 * C * d;
+    *u_n = base + 2.0 * 1.0 * C * d;
+}
+
 /* ---- total energy (no counterpart in the reference) ----------------------- */
 /* Energy density [MeV fm^-3] of the mean field, split into its Skyrme and
  * Coulomb parts:

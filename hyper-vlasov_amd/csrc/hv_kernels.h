@@ -1570,7 +1570,8 @@ __device__ __forceinline__ void smear_xs_vec_body(const int bid, const T *__rest
                                                   const unsigned *__restrict__ fmask,
                                                   const unsigned *__restrict__ cbox, int cw,
                                                   SmearTabC<T> *__restrict__ tb, int nx, int ny, int nz,
-                                                  SmearTiles st, T *__restrict__ rs = nullptr, Split sps = Split{}) {
+                                                  SmearTiles st, T *__restrict__ rs = nullptr, Split sps = Split{},
+                                                  unsigned long long *__restrict__ acc_clear = nullptr) {
     constexpr int VW = smear_vw<T>(), NA = 2 * RXM + 1;
     static_assert(RXM <= 16, "two mask words cover the planes of a node");
     typedef T V __attribute__((ext_vector_type(VW)));
@@ -1589,11 +1590,18 @@ __device__ __forceinline__ void smear_xs_vec_body(const int bid, const T *__rest
     V *out = (V *)(rho + q);
     // rs: the protons' densities also into the SOR's colour-split source (the
     // values k_rho_split would copy there)
+    // acc_clear: the deposit's counts of these nodes (which the zy pass has
+    // read) cleared for the next deposit
     auto put = [&](const V &v) {
         *out = v;
         if (rs && sp == 0)
 #pragma unroll
             for (int u = 0; u < VW; u++) rs[split_index(sps, i, j, k0 + u)] = v[u];
+        if (acc_clear) {
+            typedef unsigned long long U2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+            for (int u = 0; u < VW; u += 2) *(U2 *)(acc_clear + q + u) = U2(0ULL);
+        }
     };
     int plo = 0, phi = nx - 1;  // planes that may hold counts
     if (cbox) {
@@ -1652,8 +1660,8 @@ __device__ __forceinline__ void skyrme_body(const int bid, const T *__restrict__
     long long i = (long long)bid * blockDim.x + threadIdx.x;
     if (i >= G) return;
     double rp = (double)rho[i], rn = (double)rho[i + G];
-    double up = hv_skyrme_f64(A, B, C, gamma, rp, rn, 0);
-    double un = hv_skyrme_f64(A, B, C, gamma, rp, rn, 1);
+    double up, un;
+    hv_skyrme_pair_f64(A, B, C, gamma, rp, rn, &up, &un);
     if (phi) {
         // merge_volumetric_potentials (include/tools.hpp:378-384), in the
         // grid precision like the separate pass
