@@ -71,9 +71,13 @@ def sor_roofline(w, res):
     ms, n = prof.get("sor_resident_f32", (0.0, 0))
     if n > 0:
         it = cnt["iters_f32"] / n
-        name = {2: "k_sor_resident_f32d", 1: "k_sor_resident_f32o"}.get(res.get("body"), "k_sor_resident_f32")
+        name = {2: "k_sor_resident_f32d_fold", 1: "k_sor_resident_f32o"}.get(res.get("body"), "k_sor_resident_f32")
+        # the deep-halo body of a loop step also computes the residual (read phi
+        # and rho) and folds the correction into phi (read and write phi) once
+        # per launch: 8 + 4 + 8 + 8 B per node
+        once = 28.0 * interior if res.get("body") == 2 else 0.0
         return {"kernel": name + " (the mixed form's FP32 SOR correction, one LDS-resident launch per solve)",
-                "bytes_per_launch": it * 12.0 * interior, "avg_launch_us": 1e3 * ms / n,
+                "bytes_per_launch": it * 12.0 * interior + once, "avg_launch_us": 1e3 * ms / n,
                 "iterations_per_launch": it, "launches": n,
                 "limiter": "neighbour exchange: the round trip of a tile face between CUs (agent-scope stores "
                            "and loads) each iteration; the tile's values stay in LDS"}
@@ -370,8 +374,12 @@ def run_batch(ws, prec: str, steps: int, warmup: int, mid_steps: int, full: bool
         ms, nl = prof["sor_resident_f32"]
         launches = nl
         avg_ms = ms / nl
-        achieved = node_iters * bpn / (ms * 1e-3) / 1e9
-        kname = {2: "kb_sor_resident_f32d", 1: "kb_sor_resident_f32o"}.get(body, "kb_sor_resident_f32") + \
+        # the deep-halo body also computes each instance's residual and folds
+        # its correction (28 B per interior node once per launch, as for c2)
+        once = 28.0 * sum((w.grid - 2) ** 3 for w in ws) * nl if body == 2 else 0.0
+        achieved = (node_iters * bpn + once) / (ms * 1e-3) / 1e9
+        kname = {2: "kb_sor_resident_f32d (FUSE: with its residual and fold)",
+                 1: "kb_sor_resident_f32o"}.get(body, "kb_sor_resident_f32") + \
             " (the FP32 correction of every instance, one LDS-resident launch per step)"
     else:
         kcls = "sor_sweep_f32" if prec == "mixed" else "sor_sweep"
@@ -379,6 +387,7 @@ def run_batch(ws, prec: str, steps: int, warmup: int, mid_steps: int, full: bool
         launches = cnt["sweeps_f32"] if prec == "mixed" else cnt["sweeps_f64"]
         avg_ms = ms / max(nl, 1)
         achieved = node_iters * bpn / (launches * avg_ms * 1e-3) / 1e9 if launches else 0.0
+        once = 0.0
         kname = ("kb_sor_colour_f32" if prec == "mixed" else "kb_sor_colour") + \
             " (one red or black sweep over every instance still iterating)"
     head = mid if mid is not None else early
@@ -397,7 +406,8 @@ def run_batch(ws, prec: str, steps: int, warmup: int, mid_steps: int, full: bool
         "stage_ms_per_step": {k: v[0] / prof_steps for k, v in prof.items() if v[1] and k in
                               ("particle_pass", "smear", "sor_stage", "potential")},
         "stage_steps": f"{prof_steps} profiled steps after the {'mid-run' if mid is not None else 'early'} window",
-        "roofline": sweep_or_resident_entry(prec, kname, achieved, avg_ms, launches, node_iters * bpn),
+        "roofline": sweep_or_resident_entry(prec, kname, achieved, avg_ms, launches,
+                                            node_iters * bpn + (once if prec == "mixed" else 0.0)),
     }
 
 

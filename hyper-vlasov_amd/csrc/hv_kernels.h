@@ -331,39 +331,93 @@ __device__ __forceinline__ unsigned sort_key_of(const Particles<T> &p, int i, in
 // and pid keeps each slot's original index for the download.
 
 // Particle q's key into skey[q] and its rank within the key into srank[q];
-// buck[k] counts key k.  Runs of equal keys in consecutive lanes (particles
-// are mostly in key order; those outside the grid share one key per species)
-// take one atomic per run: its head adds the run's length and the lanes rank
-// after it.  Called by every lane of the wave (valid: q is a particle).
+// buck[k] counts key k.  The block's keys go through a table in LDS (open
+// addressing, twice as many slots as threads, so it never fills): a particle
+// takes its rank among the block's particles of its key from an LDS atomic,
+// then one global atomic per key present adds the block's count and returns
+// the block's first rank (particles are mostly in key order, so a block holds
+// few keys: a handful of global atomics per block, not one per run of equal
+// keys in a wave).  Called by every thread of the block (valid: q is a
+// particle).
+constexpr int kSortHashBits = 9;
+constexpr int kSortHash = 1 << kSortHashBits;
+static_assert(kSortHash >= 2 * kBlock, "the key table must hold twice the block's particles");
+
 template <typename T>
 __device__ __forceinline__ void sort_count_body(const Particles<T> &p, int np, const Geom<T> &g, int q, bool valid,
                                                 unsigned *__restrict__ buck, unsigned *__restrict__ skey,
                                                 int *__restrict__ srank) {
-    const unsigned k = valid ? sort_key_of(p, q, np, g) : 0xffffffffu;
-    const int lane = (int)threadIdx.x & (kWave - 1);
-    const unsigned kp = __shfl_up(k, 1, kWave);
-    const unsigned long long heads = __ballot(lane == 0 || kp != k);
-    const unsigned long long upto = (2ULL << lane) - 1ULL;  // lanes 0..lane
-    const int h = 63 - __clzll((long long)(heads & upto));
-    const unsigned long long above = heads & ~upto;
-    const int next = above ? __ffsll((long long)above) - 1 : kWave;
-    unsigned base = 0;
-    if (valid && lane == h) base = atomicAdd(&buck[k], (unsigned)(next - h));
-    base = __shfl(base, h, kWave);
+    __shared__ unsigned hkey[kSortHash], hcnt[kSortHash], hbase[kSortHash];
+    constexpr unsigned kEmpty = 0xffffffffu;
+    for (int e = threadIdx.x; e < kSortHash; e += blockDim.x) {
+        hkey[e] = kEmpty;
+        hcnt[e] = 0u;
+    }
+    __syncthreads();
+    const unsigned k = valid ? sort_key_of(p, q, np, g) : kEmpty;
+    int slot = -1;
+    unsigned local = 0;
+    if (valid) {
+        unsigned h = (k * 2654435761u) >> (32 - kSortHashBits);  // in [0, kSortHash)
+        for (;;) {
+            const unsigned old = atomicCAS(&hkey[h], kEmpty, k);
+            if (old == kEmpty || old == k) break;
+            h = (h + 1) & (kSortHash - 1);
+        }
+        slot = (int)h;
+        local = atomicAdd(&hcnt[h], 1u);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kSortHash; e += blockDim.x)
+        if (hkey[e] != kEmpty) hbase[e] = atomicAdd(&buck[hkey[e]], hcnt[e]);
+    __syncthreads();
     if (!valid) return;
     skey[q] = k;
-    srank[q] = (int)(base + (unsigned)(lane - h));
+    srank[q] = (int)(hbase[slot] + local);
 }
 
 // The m key counts at b become the first slot of each key (exclusive scan),
-// by one block of kSortScanThreads: tiles of 4 kSortScanThreads counts, each
-// thread four consecutive ones (coalesced), scanned over the block with the
-// running total of the earlier tiles carried.
+// by one block of kSortScanThreads.  Up to kSortScanPer counts per thread:
+// each thread loads its run of consecutive counts at once, the runs' totals
+// are scanned over the block (one pass, two barriers), and each thread
+// writes its run's offsets.  Longer: tiles of 4 kSortScanThreads counts, each
+// thread four consecutive ones, scanned over the block with the running total
+// of the earlier tiles carried.
 constexpr int kSortScanThreads = 1024;
+constexpr int kSortScanPer = 64;
 
 __device__ __forceinline__ void sort_scan_body(unsigned *__restrict__ b, int m) {
     __shared__ unsigned wsum[kSortScanThreads / kWave];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int per = (m + kSortScanThreads - 1) / kSortScanThreads;
+    if (per <= kSortScanPer) {
+        // (the run is read twice, for its total and for its offsets: the
+        // second read hits the cache, and no run is held in registers)
+        const int e0 = (int)threadIdx.x * per;
+        unsigned s = 0;
+#pragma unroll 16
+        for (int u = 0; u < per; u++) s += e0 + u < m ? b[e0 + u] : 0u;
+        unsigned inc = s;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const unsigned t = __shfl_up(inc, d, kWave);
+            if (lane >= d) inc += t;
+        }
+        if (lane == kWave - 1) wsum[wave] = inc;
+        __syncthreads();
+        unsigned before = 0;
+        for (int w = 0; w < wave; w++) before += wsum[w];
+        unsigned run = before + inc - s;
+#pragma unroll 16
+        for (int u = 0; u < per; u++) {
+            if (e0 + u < m) {
+                const unsigned c = b[e0 + u];
+                b[e0 + u] = run;
+                run += c;
+            }
+        }
+        return;
+    }
     unsigned carry = 0;
     for (int base = 0; base < m; base += 4 * kSortScanThreads) {
         const int e0 = base + 4 * (int)threadIdx.x;
