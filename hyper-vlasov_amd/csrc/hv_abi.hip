@@ -45,6 +45,7 @@ struct hv_ctx {
     std::function<int()> spec;
     bool spec_valid = false;
     bool rho_s_ready = false;           // the x pass wrote rho_s from the density just smeared
+    bool cbox_ready = false;            // the count box of the last deposit is built (k_stats_cbox)
     hipEvent_t ev_copy = nullptr;
     char err[512] = {0};
 
@@ -1340,10 +1341,15 @@ int potential_from_density(hv_ctx *c, const hv_skyrme *s, bool with_phi) {
 // deposit already in c->acc -> rho -> SOR -> U (src/hyper_vlasov.cpp:94-98)
 template <typename T>
 int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, int *warn) {
-    // the count box of the deposit, from its per-block records
-    k_cbox_build<<<2, kCboxThreads, 0, c->stream>>>(c->brec, (int)blocks_for(c->total),
-                                                     (unsigned *)(c->acc + 2 * c->G));
-    HV_CHECK_LAUNCH(c);
+    // the count box of the deposit, from its per-block records (unless the
+    // particle pass's diagnostics launch built it)
+    const bool cbox_built = c->cbox_ready;
+    c->cbox_ready = false;
+    if (!cbox_built) {
+        k_cbox_build<<<2, kCboxThreads, 0, c->stream>>>(c->brec, (int)blocks_for(c->total),
+                                                         (unsigned *)(c->acc + 2 * c->G));
+        HV_CHECK_LAUNCH(c);
+    }
     int rc = smear<T>(c, true, p->sigma_r, p->part_per_nucleon);
     if (rc < 0) return rc;
     *warn |= rc;
@@ -1460,8 +1466,11 @@ int run_steps(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int first, int 
                 drift_fac, n > 0 ? 1 : 0, boost, (T)bp, (T)bn, c->partials, c->status);
         HV_CHECK_LAUNCH(c);
         fgrid_ok = false;
-        k_stats_finalize<<<1, kStatThreads, 0, c->stream>>>(c->partials, (int)nbp, c->slots + (n + 1));
+        // the pass's diagnostics and the count box of its deposit in one launch
+        k_stats_cbox<<<3, kCboxThreads, 0, c->stream>>>(c->partials, (int)nbp, c->slots + (n + 1), c->brec,
+                                                        (unsigned *)(c->acc + 2 * c->G));
         HV_CHECK_LAUNCH(c);
+        c->cbox_ready = true;
         prof_end(c, HV_KCLASS_PARTICLE, pa, 1);
         int it = 0;
         // F = -grad U at the cells the deposit above recorded: the nodes the

@@ -140,9 +140,8 @@ constexpr int kBoxRec = 8;
 // record loads in flight per thread.
 constexpr int kCboxThreads = 1024;
 
-__global__ void __launch_bounds__(kCboxThreads)
-k_cbox_build(const int *__restrict__ brec, int nrec, unsigned *__restrict__ cbox) {
-    const int sp = blockIdx.x;
+__device__ __forceinline__ void cbox_build_block(const int *__restrict__ brec, int nrec, unsigned *__restrict__ cbox,
+                                                 const int sp) {
     constexpr int kBig = 0x7fffffff;
     int lo[3] = {kBig, kBig, kBig}, hi[3] = {-1, -1, -1};
     for (int r0 = threadIdx.x; r0 < nrec; r0 += 4 * kCboxThreads) {
@@ -193,6 +192,67 @@ k_cbox_build(const int *__restrict__ brec, int nrec, unsigned *__restrict__ cbox
         const bool empty = red[3][0] < red[0][0];
         cbox[sp * kCboxWords + q] = empty ? 0u : q < 3 ? 0xffffu - (unsigned)red[q][0] : (unsigned)red[q][0] + 1u;
     }
+}
+
+__global__ void __launch_bounds__(kCboxThreads)
+k_cbox_build(const int *__restrict__ brec, int nrec, unsigned *__restrict__ cbox) {
+    cbox_build_block(brec, nrec, cbox, blockIdx.x);
+}
+
+// stats_finalize_block's sums by a block of kCboxThreads: each thread sums a
+// strided share, the lanes of a wave are summed by shuffles and the waves'
+// sums in LDS (128-bit integer sums: the same values in any order).
+__device__ __forceinline__ void stats_finalize_wide(const long long *__restrict__ partials, int nblocks,
+                                                    StatSlot *__restrict__ slot) {
+    __shared__ unsigned long long wlo[6][kCboxThreads / kWave], whi[6][kCboxThreads / kWave];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    unsigned long long l[6] = {0, 0, 0, 0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 4
+    for (int b = tid; b < nblocks; b += kCboxThreads)
+#pragma unroll
+        for (int c = 0; c < 6; c++) add_i128(l[c], h[c], partials[6 * b + c]);
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) {
+            const unsigned long long ol = __shfl_xor(l[c], d, kWave), oh = __shfl_xor(h[c], d, kWave);
+            const unsigned long long nl = l[c] + ol;
+            h[c] += oh + (nl < ol ? 1ULL : 0ULL);
+            l[c] = nl;
+        }
+        if (lane == 0) {
+            wlo[c][wave] = l[c];
+            whi[c][wave] = h[c];
+        }
+    }
+    __syncthreads();
+    if (tid < 6) {
+        const int c = tid;
+        unsigned long long L = 0, H = 0;
+        for (int w = 0; w < kCboxThreads / kWave; w++) {
+            const unsigned long long nl = L + wlo[c][w];
+            H += whi[c][w] + (nl < L ? 1ULL : 0ULL);
+            L = nl;
+        }
+        if (c < 4) {
+            slot->s[c][0] = L;
+            slot->s[c][1] = H;
+        } else {
+            slot->inbox[c - 4] = L;
+        }
+    }
+}
+
+// After a particle pass: the count box of its deposit (blocks 0, 1, as
+// k_cbox_build) and its diagnostics (block 2, as k_stats_finalize), in one
+// launch.
+__global__ void __launch_bounds__(kCboxThreads)
+k_stats_cbox(const long long *__restrict__ partials, int nblocks, StatSlot *__restrict__ slot,
+             const int *__restrict__ brec, unsigned *__restrict__ cbox) {
+    if (blockIdx.x < 2)
+        cbox_build_block(brec, nblocks, cbox, blockIdx.x);
+    else
+        stats_finalize_wide(partials, nblocks, slot);
 }
 
 // ---- block-local CIC deposit ----------------------------------------------------
