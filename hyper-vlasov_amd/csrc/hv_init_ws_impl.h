@@ -6,7 +6,9 @@
 // region solver (include/fit_algorithm.hpp:96-138).  Included once, at the end
 // of hv_abi.hip.
 
+#include <cstdint>
 #include <cstdlib>
+#include <cstring>
 
 #include "hv_init.h"
 
@@ -17,9 +19,39 @@ constexpr double kWsRelax = 0.6;             // relax_coef (include/physics.hpp:
 constexpr double kWsDeltaTol = 0.1;          // delta_epsilon_tolerance (include/sim_structs.hpp:46)
 constexpr int kWsMaxIterations = 32;         // MAX_INIT_ITERATIONS (include/sim_structs.hpp:32)
 
+// The generator of a whole initialisation (hv_init_woods_saxon): the C
+// library's rand() sequence from a private state (glibc's random_r on a
+// 128-byte state, the generator and state size behind rand(), seeded as
+// srand(seed) seeds it), so that rand() calls from other threads of the
+// process (the GPU runtime's, which starts during the first round) cannot
+// shift it.  Unset, the sampling takes the process's rand() itself.
+thread_local random_data *tl_ws_rng = nullptr;
+
+struct WsRngScope {
+    random_data d;
+    char state[128];
+    explicit WsRngScope(unsigned seed) {
+        std::memset(&d, 0, sizeof d);
+        std::memset(state, 0, sizeof state);
+        initstate_r(seed, state, sizeof state, &d);
+        tl_ws_rng = &d;
+    }
+    ~WsRngScope() { tl_ws_rng = nullptr; }
+    WsRngScope(const WsRngScope &) = delete;
+    WsRngScope &operator=(const WsRngScope &) = delete;
+};
+
 // rand_val / random_vec (include/math_functions.hpp:30-39)
 inline double ws_rand_val(double lo, double hi) {
-    const double s = std::rand() / double(RAND_MAX);
+    int r;
+    if (tl_ws_rng) {
+        int32_t v = 0;
+        random_r(tl_ws_rng, &v);
+        r = v;
+    } else {
+        r = std::rand();
+    }
+    const double s = r / double(RAND_MAX);
     return lo + s * (hi - lo);
 }
 
@@ -405,7 +437,7 @@ int hv_init_woods_saxon(int z, int n, int part_per_nucleon, const hv_skyrme *sky
     ws[0].a = ws[1].a = a;
     const int np = z * part_per_nucleon, nn = n * part_per_nucleon, total = np + nn;
     std::vector<double> rp((size_t)total), rn((size_t)total);
-    std::srand(seed);
+    const WsRngScope rng(seed);
     double total_delta = 0.0;
     int it = 0;
     do {

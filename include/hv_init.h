@@ -73,6 +73,10 @@ int hv_ws_chi_squared(const double *x, const double *y, const double *z, const d
                       int part_per_nucleon, double out[2]);
 /* initialize_particles (include/physics.hpp:35-97): srand(seed), then rounds
  * of sampling (hv_ws_sample_round), densities (GPU), well fit and relaxation
+ * -- the rounds draw the sequence rand() gives after srand(seed) from a
+ * private generator state (glibc random_r, the generator behind rand()), so
+ * other threads' rand() calls cannot shift it and the process's own rand()
+ * state is left as it was
  * (0.6 new + 0.4 old, include/tools.hpp:369-375) until the Fermi levels move
  * by less than 0.1 MeV together (at most 32 rounds); wells start at V0,
  * R12 = 0.8 r_max, a (include/init.hpp:333-334).  Outputs the particles (z*ppn

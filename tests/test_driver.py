@@ -138,3 +138,25 @@ def test_run_from_the_reference_initial_state(driver, tmp_path):
     assert data.shape[1] == 5 and np.isfinite(data).all()
     # sqrt<r^2> of a bound nucleus of A = 40: a few fm
     assert 2.0 < data[0, 1] < 5.0 and 2.0 < data[0, 2] < 5.0
+
+
+@pytest.mark.gpu
+def test_reference_initial_state_repeats_across_processes(driver, tmp_path):
+    """Two runs of the driver from the reference's initial state (seed 128)
+    in two processes print the same initialisation and the same stats.txt:
+    the rounds' rand() sequence is private to the initialisation, so the GPU
+    runtime's own threads cannot shift it."""
+    print("driver: reference initial state in two processes", flush=True)
+    runs = []
+    for rep in range(2):
+        d = tmp_path / f"run{rep}"
+        d.mkdir()
+        _write(d / "input.txt", INPUT, "vtk 0\n")
+        (d / "out").mkdir()
+        r = subprocess.run([str(DRIVER), str(d / "input.txt"), str(d / "out")], capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        lines = [ln for ln in r.stdout.splitlines() if not ln.startswith("Time taken")]
+        runs.append((lines, (d / "out" / "stats.txt").read_bytes()))
+    assert runs[0][0] == runs[1][0]
+    assert runs[0][1] == runs[1][1]
