@@ -498,7 +498,7 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
 // counts (fixed point in c->acc when from_acc, else T counts in c->density) ->
 // smeared density in c->density
 template <typename T>
-int smear(hv_ctx *c, bool from_acc, double sigma_r, int ppn) {
+int smear(hv_ctx *c, bool from_acc, double sigma_r, int ppn, bool *cleared = nullptr) {
     int rc = ensure_smear<T>(c, sigma_r, ppn);
     if (rc) return rc;
     const long long n2 = 2 * c->G;
@@ -537,10 +537,16 @@ int smear(hv_ctx *c, bool from_acc, double sigma_r, int ppn) {
             // (the solve that follows then skips k_rho_split)
             T *rs = from_acc ? (T *)c->rho_s : nullptr;
             if (rs) c->rho_s_ready = true;
+            // asked to (cleared given): the x pass also clears the counts of
+            // the box and keeps the boxes in fbox, in place of k_clear_counts
+            const bool clr = from_acc && cleared;
+            unsigned long long *ac = clr ? c->acc : nullptr;
+            unsigned *fb = clr ? c->fbox : nullptr;
+            if (clr) *cleared = true;
             switch (c->xs_rxm) {
-            case 4: k_smear_xs_vec<T, 4><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl, rs, c->sp); break;
-            case 9: k_smear_xs_vec<T, 9><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl, rs, c->sp); break;
-            default: k_smear_xs_vec<T, 14><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl, rs, c->sp); break;
+            case 4: k_smear_xs_vec<T, 4><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl, rs, c->sp, ac, fb); break;
+            case 9: k_smear_xs_vec<T, 9><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl, rs, c->sp, ac, fb); break;
+            default: k_smear_xs_vec<T, 14><<<nb, kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl, rs, c->sp, ac, fb); break;
             }
         } else {
             k_smear_xs<T><<<blocks_for(n2), kThreads, 0, c->stream>>>(D, rho, c->sflags, cbox, cw, tb, nx, ny, nz, c->stl);
@@ -1350,14 +1356,17 @@ int field_solve(hv_ctx *c, const hv_skyrme *s, const hv_params *p, int *iters, i
                                                          (unsigned *)(c->acc + 2 * c->G));
         HV_CHECK_LAUNCH(c);
     }
-    int rc = smear<T>(c, true, p->sigma_r, p->part_per_nucleon);
+    bool cleared = false;
+    int rc = smear<T>(c, true, p->sigma_r, p->part_per_nucleon, &cleared);
     if (rc < 0) return rc;
     *warn |= rc;
-    // counts and their box back to zero: the box's rows only (the global
-    // boxes kept in fbox for the force grid)
-    k_clear_counts<<<c->clear_blocks, 256, 0, c->stream>>>(c->acc, c->world.n[0], c->world.n[1], c->world.n[2],
-                                                           c->fbox);
-    HV_CHECK_LAUNCH(c);
+    // counts back to zero: the box's rows only (the global boxes kept in fbox
+    // for the force grid), unless the smearing's x pass did both
+    if (!cleared) {
+        k_clear_counts<<<c->clear_blocks, 256, 0, c->stream>>>(c->acc, c->world.n[0], c->world.n[1], c->world.n[2],
+                                                               c->fbox);
+        HV_CHECK_LAUNCH(c);
+    }
     double md;
     rc = sor_solve<T>(c, (const T *)c->density, p->sor_omega, p->sor_tol, p->sor_max_it, iters, &md, true);
     if (rc < 0) return rc;

@@ -1705,13 +1705,15 @@ __device__ __forceinline__ void smear_xs_vec_body(const int bid, const T *__rest
     // rs: the protons' densities also into the SOR's colour-split source (the
     // values k_rho_split would copy there)
     // acc_clear: the deposit's counts of these nodes (which the zy pass has
-    // read) cleared for the next deposit
+    // read) cleared for the next deposit; with a count box, only the nodes
+    // inside it (every count outside it is zero already)
+    bool clr = acc_clear != nullptr;
     auto put = [&](const V &v) {
         *out = v;
         if (rs && sp == 0)
 #pragma unroll
             for (int u = 0; u < VW; u++) rs[split_index(sps, i, j, k0 + u)] = v[u];
-        if (acc_clear) {
+        if (clr) {
             typedef unsigned long long U2 __attribute__((ext_vector_type(2)));
 #pragma unroll
             for (int u = 0; u < VW; u += 2) *(U2 *)(acc_clear + q + u) = U2(0ULL);
@@ -1725,9 +1727,11 @@ __device__ __forceinline__ void smear_xs_vec_body(const int bid, const T *__rest
         const int ci1 = cbox_hi(cb[3]), cj1 = cbox_hi(cb[4]), ck1 = cbox_hi(cb[5]);
         if (ci1 < ci0 || i < ci0 - rx || i > ci1 + rx || j < cj0 - ry || j > cj1 + ry || k0 + VW - 1 < ck0 - rz ||
             k0 > ck1 + rz) {
+            clr = false;
             put(V(T(0.0)));
             return;
         }
+        clr = clr && i >= ci0 && i <= ci1 && j >= cj0 && j <= cj1 && k0 + VW - 1 >= ck0 && k0 <= ck1;
         plo = ci0;
         phi = ci1;
     }
@@ -1762,8 +1766,13 @@ template <typename T, int RXM>
 __global__ void __launch_bounds__(kBlock)
 k_smear_xs_vec(const T *__restrict__ D, T *__restrict__ rho, const unsigned *__restrict__ fmask,
                const unsigned *__restrict__ cbox, int cw, const SmearTab<T> *__restrict__ tb, int nx, int ny, int nz,
-               SmearTiles st, T *__restrict__ rs, Split sps) {
-    smear_xs_vec_body<T, RXM>(blockIdx.x, D, rho, fmask, cbox, cw, smear_tab_c(tb), nx, ny, nz, st, rs, sps);
+               SmearTiles st, T *__restrict__ rs, Split sps, unsigned long long *__restrict__ acc_clear,
+               unsigned *__restrict__ fbox) {
+    // fbox: the count boxes kept for the force grid (as k_clear_counts keeps
+    // them when it clears the counts)
+    if (fbox && blockIdx.x == 0 && threadIdx.x < 12) fbox[threadIdx.x] = cbox[(threadIdx.x / 6) * cw + threadIdx.x % 6];
+    smear_xs_vec_body<T, RXM>(blockIdx.x, D, rho, fmask, cbox, cw, smear_tab_c(tb), nx, ny, nz, st, rs, sps,
+                              acc_clear);
 }
 
 // ---- Skyrme potentials (include/physics.hpp:273-281), optionally + phi ------
