@@ -478,36 +478,50 @@ __device__ __forceinline__ void sort_scan_body(unsigned *__restrict__ b, int m) 
         }
         return;
     }
+    // tiles of 4 counts per thread, four tiles' loads issued together (the
+    // scans of a group then wait on one load latency, not four)
+    constexpr int kTile = 4 * kSortScanThreads, kGroup = 4;
     unsigned carry = 0;
-    for (int base = 0; base < m; base += 4 * kSortScanThreads) {
-        const int e0 = base + 4 * (int)threadIdx.x;
-        unsigned v[4];
+    for (int gbase = 0; gbase < m; gbase += kGroup * kTile) {
+        unsigned vg[kGroup][4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) v[u] = e0 + u < m ? b[e0 + u] : 0u;
-        const unsigned s = v[0] + v[1] + v[2] + v[3];
-        // inclusive scan over the wave, then over the waves' totals
-        unsigned inc = s;
+        for (int gi = 0; gi < kGroup; gi++)
 #pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-            const unsigned t = __shfl_up(inc, d, kWave);
-            if (lane >= d) inc += t;
-        }
-        if (lane == kWave - 1) wsum[wave] = inc;
-        __syncthreads();
-        unsigned before = 0, total = 0;
-        for (int w = 0; w < kSortScanThreads / kWave; w++) {
-            const unsigned t = wsum[w];
-            before += w < wave ? t : 0u;
-            total += t;
-        }
-        unsigned run = carry + before + inc - s;
+            for (int u = 0; u < 4; u++) {
+                const int e = gbase + gi * kTile + 4 * (int)threadIdx.x + u;
+                vg[gi][u] = e < m ? b[e] : 0u;
+            }
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            if (e0 + u < m) b[e0 + u] = run;
-            run += v[u];
+        for (int gi = 0; gi < kGroup; gi++) {
+            const int base = gbase + gi * kTile;  // (the same in every thread)
+            if (base >= m) break;
+            const int e0 = base + 4 * (int)threadIdx.x;
+            const unsigned *v = vg[gi];
+            const unsigned s = v[0] + v[1] + v[2] + v[3];
+            // inclusive scan over the wave, then over the waves' totals
+            unsigned inc = s;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const unsigned t = __shfl_up(inc, d, kWave);
+                if (lane >= d) inc += t;
+            }
+            if (lane == kWave - 1) wsum[wave] = inc;
+            __syncthreads();
+            unsigned before = 0, total = 0;
+            for (int w = 0; w < kSortScanThreads / kWave; w++) {
+                const unsigned t = wsum[w];
+                before += w < wave ? t : 0u;
+                total += t;
+            }
+            unsigned run = carry + before + inc - s;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (e0 + u < m) b[e0 + u] = run;
+                run += v[u];
+            }
+            carry += total;
+            __syncthreads();  // wsum is rewritten by the next tile
         }
-        carry += total;
-        __syncthreads();  // wsum is rewritten by the next tile
     }
 }
 
