@@ -46,6 +46,7 @@ struct hv_ctx {
     bool spec_valid = false;
     bool rho_s_ready = false;           // the x pass wrote rho_s from the density just smeared
     bool cbox_ready = false;            // the count box of the last deposit is built (k_stats_cbox)
+    int sor_rows_cleared = 0;           // rows of maxd (and SorCtl) the extrapolation cleared for the next sor_iterate
     hipEvent_t ev_copy = nullptr;
     char err[512] = {0};
 
@@ -903,12 +904,16 @@ int sor_iterate(hv_ctx *c, const SorCoef &cf, int max_it, int *iters_out, double
     int it = 0, iters = -1, start = 0;
     int chunk = c->sor_pred > 2 ? c->sor_pred - 1 : 1;
     bool first = true;
+    // rows a launch earlier in the queue already cleared (with SorCtl)
+    const int pre_cleared = c->sor_rows_cleared;
+    c->sor_rows_cleared = 0;
     while (it < max_it) {
         start = it;
         int end = it + chunk < max_it ? it + chunk : max_it;
         // this chunk's rows of the slot maxima (and SorCtl at the start)
-        k_sor_chunk_reset<<<(unsigned)(((end - start) * kMaxSlots + 255) / 256), 256, 0, c->stream>>>(
-            c->maxd, start, end, c->ctl, first ? 1 : 0);
+        if (!(first && end <= pre_cleared))
+            k_sor_chunk_reset<<<(unsigned)(((end - start) * kMaxSlots + 255) / 256), 256, 0, c->stream>>>(
+                c->maxd, start, end, c->ctl, first ? 1 : 0);
         for (; it < end; it++) launch(it);
         HV_CHECK_LAUNCH(c);
         const float *xr = first ? xres : nullptr;
@@ -1056,6 +1061,7 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
               double *maxdiff_out, bool loop = false) {
     if (max_it <= 0) return set_err(c, HV_ERR_ARG, "sor max_it must be positive");
     c->last_extrap = 0;
+    c->sor_rows_cleared = 0;
     const int nx = c->world.n[0], ny = c->world.n[1], nz = c->world.n[2];
     RT *rho = (RT *)c->rho_s;
     if (!(loop && c->rho_s_ready && rho_nat == (const RT *)c->density)) {
@@ -1111,7 +1117,14 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
             double w[5];
             const unsigned nb = blocks_for(2 * c->sp.Gh);
             const long long nn = 2 * c->sp.Gh;
-            const ResClears xc = infold ? cl : ResClears{};
+            ResClears xc = infold ? cl : ResClears{};
+            // (in a fused solve the FP64 sweeps' first chunk, one iteration,
+            // finds its slot row and SorCtl cleared by this launch)
+            if (infold) {
+                xc.maxd = c->maxd;
+                xc.maxd_rows = 1;
+                xc.ctl = c->ctl;
+            }
             switch (sor_predictor(id, w)) {
             case 1: k_sor_extrapolate<1><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn, xc); break;
             case 2: k_sor_extrapolate<2><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn, xc); break;
@@ -1120,6 +1133,7 @@ int sor_solve(hv_ctx *c, const RT *rho_nat, double omega, double tol, int max_it
             default: k_sor_extrapolate<5><<<nb, kThreads, 0, c->stream>>>(c->phi, hs, id, nn, xc); break;
             }
             HV_CHECK_LAUNCH(c);
+            if (infold) c->sor_rows_cleared = 1;
             c->hist_pos = (c->hist_pos + 4) % 5;
             c->last_extrap = id;
         }

@@ -2162,6 +2162,12 @@ struct ResClears {
     unsigned long long *x;
     long long nx;
     int *err;
+    // (k_sor_extrapolate only) the first maxd_rows rows of the FP64 sweeps'
+    // slot maxima and their SorCtl, as k_sor_chunk_reset clears them at the
+    // start of a solve
+    unsigned long long *maxd;
+    int maxd_rows;
+    SorCtl *ctl;
 };
 
 template <typename RT>
@@ -2342,6 +2348,11 @@ __global__ void __launch_bounds__(kBlock) k_sor_extrapolate(double *__restrict__
     for (long long e = q0; e < cl.nslots; e += stride) cl.slots[e] = -1.0f;
     for (long long e = q0; e < cl.nx; e += stride) cl.x[e] = 0ULL;
     if (cl.err && q0 < 4) cl.err[q0] = 0;
+    for (long long e = q0; e < (long long)cl.maxd_rows * kMaxSlots; e += stride) cl.maxd[e * kSlotStride] = 0ULL;
+    if (cl.ctl && q0 == 0) {
+        cl.ctl->done = 0;
+        cl.ctl->iters = 0;
+    }
     sor_extrapolate_body<NPTS>(blockIdx.x, phi, hist, id, n);
 }
 
