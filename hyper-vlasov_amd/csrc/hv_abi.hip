@@ -124,6 +124,7 @@ struct hv_ctx {
     int *ord[2] = {};                   // [0]: ranks within a key
     int sort_every = 8;  // steps between sorts; 0 = never
     int smear_tk_max = 64;  // widest smearing tile (columns) tried
+    bool smear_tj16 = true; // two-pass vector tile of 16 rows when its LDS fits (else 8)
     bool own_stream = true; // false for the instances of a batch (the batch's stream)
     // LDS-resident SOR (hv_sor_resident.h)
     int sor_resident = 0;     // option: use it when the grid fits (off by default: slower than per-sweep launches at 64^3..128^3)
@@ -357,7 +358,8 @@ void prof_flush(hv_ctx *c) {
 template <typename T>
 int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
     if (!(sigma_r > 0.0) || ppn <= 0) return set_err(c, HV_ERR_ARG, "smearing needs sigma_r > 0 and ppn > 0");
-    if (c->smear_tab && c->tab_sigma == sigma_r && c->tab_ppn == ppn) return HV_OK;
+    if (c->smear_tab && c->tab_sigma == sigma_r && c->tab_ppn == ppn && (c->smear_tj16 || c->stl.TJ != 16))
+        return HV_OK;
     hv_smear_table tb;
     if (hv_smear_table_build(&tb, c->world.n, c->world.d_max, sigma_r, ppn) != 0)
         return set_err(c, HV_ERR_ARG, "smearing radius exceeds %d nodes", HV_SMEAR_MAXR);
@@ -372,12 +374,14 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
         h.gz[a] = (T)tb.gz[a];
     }
     h.term = (T)tb.term;
-    // tap offsets for the vector tile (8 + 2 ry rows of 8 VW columns)
-    const int vR = 8 + 2 * tb.ry, vTK = 8 * smear_vw<T>();
+    // tap offsets for the vector tiles (8 + 2 ry or 16 + 2 ry rows of 8 VW
+    // columns)
+    const int vR = 8 + 2 * tb.ry, vR16 = 16 + 2 * tb.ry, vTK = 8 * smear_vw<T>();
     for (int a = 0; a <= tb.rx; a++)
         for (int t = 0; t <= 2 * tb.kb[a]; t++) {
             const int ab = t > tb.kb[a] ? t - tb.kb[a] : tb.kb[a] - t;
             h.toff[a][t] = (tb.kc[a][ab] * vR + t) * vTK;
+            h.toff16[a][t] = (tb.kc[a][ab] * vR16 + t) * vTK;
             h.tg[a][t] = (T)tb.gy[ab];
         }
     // channels to the 4 waves of a smearing block: longest first (2 kb + 1
@@ -403,15 +407,21 @@ int ensure_smear(hv_ctx *c, double sigma_r, int ppn) {
     HV_HIP(c, hipMemcpyAsync(c->smear_tab, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HV_HIP(c, hipStreamSynchronize(c->stream));
     size_t n2 = (size_t)(2 * c->G);
-    // two-pass form: the vector tile (8 rows x 8 VW columns) when it fits
-    // kSmear2LdsVec and smear_tk_max allows it, else TJ = 8 rows and TK the
-    // widest of smear_tk_max, /2, ... columns whose counts and H levels fit 64 KB
+    // two-pass form: the vector tile (16 or 8 rows x 8 VW columns) when it
+    // fits kSmear2LdsVec and smear_tk_max allows it, else TJ = 8 rows and TK
+    // the widest of smear_tk_max, /2, ... columns whose counts and H levels
+    // fit 64 KB.  16 rows stage (16 + 2 ry) rows of counts and H levels for
+    // 16 output rows where 8 rows stage 8 + 2 ry for 8.
     constexpr int kVecTK = 8 * smear_vw<T>();
     SmearTiles stl{};
-    stl.TJ = 8;
+    stl.TJ = 16;
     size_t lds = 0;
     stl.TK = kVecTK;
-    if (c->smear_tk_max >= kVecTK && (lds = smear2_lds_bytes(stl, tb.ry, tb.rz, sizeof(T))) <= kSmear2LdsVec) {
+    if (c->smear_tj16 && c->smear_tk_max >= kVecTK &&
+        (lds = smear2_lds_bytes(stl, tb.ry, tb.rz, sizeof(T))) <= kSmear2LdsVec &&
+        (size_t)(stl.TJ + 2 * tb.ry) * (stl.TK + 2 * tb.rz) <= (size_t)kZyStaged * kSmearThreads) {
+    } else if ((stl.TJ = 8, c->smear_tk_max >= kVecTK) &&
+               (lds = smear2_lds_bytes(stl, tb.ry, tb.rz, sizeof(T))) <= kSmear2LdsVec) {
     } else {
         for (int tk = c->smear_tk_max; tk >= 8; tk /= 2) {
             stl.TK = tk;

@@ -809,6 +809,8 @@ struct SmearTab {
     // (consecutive, so that a run of taps loads with a few wide scalar loads)
     int toff[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
     T tg[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
+    // the same offsets for the 16-row vector tile (R = 16 + 2 ry)
+    int toff16[HV_SMEAR_MAXR + 1][2 * HV_SMEAR_MAXR + 1];
     // the channels wave w of a 256-thread block sums in the vector tile, -1
     // after the last (longest first to the least loaded wave: the channels'
     // tap counts differ, so a round robin leaves waves idle)
@@ -914,7 +916,7 @@ constexpr int smear_vw() { return 16 / (int)sizeof(T); }
 
 // (the vector tile: TK = 8 VW columns and R = 8 + 2 ry rows, the shape the
 // table's tap offsets are built for)
-template <typename T>
+template <typename T, bool T16 = false>
 __device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, SmearTabC<T> *__restrict__ tb, int a,
                                                   int h0, int R, int TK, T (&acc)[smear_vw<T>()]) {
     constexpr int VW = smear_vw<T>();
@@ -933,7 +935,7 @@ __device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, Smea
         V h[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            o[u] = tb->toff[a][t + u];
+            o[u] = (T16 ? tb->toff16[a] : tb->toff[a])[t + u];
             g[u] = tb->tg[a][t + u];
         }
 #pragma unroll
@@ -952,7 +954,7 @@ __device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, Smea
         T g[U8];
 #pragma unroll
         for (int u = 0; u < U8; u++) {
-            o[u] = tb->toff[a][u];
+            o[u] = (T16 ? tb->toff16[a] : tb->toff[a])[u];
             g[u] = tb->tg[a][u];
         }
         for (;;) {
@@ -965,7 +967,7 @@ __device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, Smea
             if (more) {
 #pragma unroll
                 for (int u = 0; u < U8; u++) {
-                    on[u] = tb->toff[a][t + U8 + u];
+                    on[u] = (T16 ? tb->toff16[a] : tb->toff[a])[t + U8 + u];
                     gn[u] = tb->tg[a][t + U8 + u];
                 }
             }
@@ -992,7 +994,9 @@ __device__ __forceinline__ void smear_channel_vec(const T *__restrict__ Hs, Smea
 // The vector D stage needs TJ = 8 rows of TK = 8 VW columns: one wave covers
 // the tile's 64 VW nodes of one channel.
 template <typename T>
-__host__ __device__ constexpr bool smear_vec_tile(int TJ, int TK) { return TJ == 8 && TK == 8 * smear_vw<T>(); }
+__host__ __device__ constexpr bool smear_vec_tile(int TJ, int TK) {
+    return (TJ == 8 || TJ == 16) && TK == 8 * smear_vw<T>();
+}
 
 // The VW sums of a lane into D at dp: one 16-byte store when the row length
 // nz is a multiple of VW (dp is then aligned), else element by element up to
@@ -1346,7 +1350,7 @@ __device__ __forceinline__ void smear_zy_coords(int R, int W, int (&ecoord)[NS])
 
 // The H levels and D channels of a staged item (counts in C, `any` set when
 // one of them is non-zero): sets or clears its plane bit.
-template <typename T>
+template <typename T, bool V16 = false>
 __device__ __forceinline__ void smear_zy_finish(bool any, int sp, int i, int jt, int kt, T *__restrict__ D,
                                                 unsigned *__restrict__ fmask, SmearTabC<T> *__restrict__ tb, int nx,
                                                 int ny, int nz, const SmearTiles &st) {
@@ -1374,15 +1378,22 @@ __device__ __forceinline__ void smear_zy_finish(bool any, int sp, int i, int jt,
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), nwaves = kSmearThreads / kWave;
     const int r = lane >> 3, c0 = (lane & 7) * VW;
-    const int j = j0 + r;
-    const long long q = base + (long long)j * nz + k0 + c0;
     (void)nwaves;
+    // a 16-row tile: its two 8-row halves in turn
     for (int n = 0; n <= HV_SMEAR_MAXR; n++) {
         const int a = tb->wch[wave][n];
         if (a < 0) break;
-        T acc[VW];
-        smear_channel_vec<T>(Hs, tb, a, (r + ry) * TK + c0, R, TK, acc);
-        if (j < ny) smear_store_vec<T>(D + (long long)a * n2 + q, acc, nz, nz - (k0 + c0));
+        // (V16: the caller's tiles may have 16 rows; the ensemble's have 8)
+        for (int hh = 0; hh < (V16 ? TJ : 8); hh += 8) {
+            const int j = j0 + hh + r;
+            const long long q = base + (long long)j * nz + k0 + c0;
+            T acc[VW];
+            if (V16 && TJ == 16)
+                smear_channel_vec<T, true>(Hs, tb, a, (hh + r + ry) * TK + c0, R, TK, acc);
+            else
+                smear_channel_vec<T>(Hs, tb, a, (r + ry) * TK + c0, R, TK, acc);
+            if (j < ny) smear_store_vec<T>(D + (long long)a * n2 + q, acc, nz, nz - (k0 + c0));
+        }
     }
 }
 
@@ -1476,15 +1487,21 @@ __device__ __forceinline__ void smear_zy_item(int sp, int i, int jt, int kt, con
         const int lane = threadIdx.x & (kWave - 1);
         const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), nwaves = kSmearThreads / kWave;
         const int r = lane >> 3, c0 = (lane & 7) * VW;
-        const int j = j0 + r;
-        const long long q = base + (long long)j * nz + k0 + c0;
         (void)nwaves;
+        // a 16-row tile: its two 8-row halves in turn
         for (int n = 0; n <= HV_SMEAR_MAXR; n++) {
             const int a = tb->wch[wave][n];
             if (a < 0) break;
-            T acc[VW];
-            smear_channel_vec<T>(Hs, tb, a, (r + ry) * TK + c0, R, TK, acc);
-            if (j < ny) smear_store_vec<T>(D + (long long)a * n2 + q, acc, nz, nz - (k0 + c0));
+            for (int hh = 0; hh < TJ; hh += 8) {
+                const int j = j0 + hh + r;
+                const long long q = base + (long long)j * nz + k0 + c0;
+                T acc[VW];
+                if (TJ == 16)
+                    smear_channel_vec<T, true>(Hs, tb, a, (hh + r + ry) * TK + c0, R, TK, acc);
+                else
+                    smear_channel_vec<T>(Hs, tb, a, (r + ry) * TK + c0, R, TK, acc);
+                if (j < ny) smear_store_vec<T>(D + (long long)a * n2 + q, acc, nz, nz - (k0 + c0));
+            }
         }
         return;
     }
@@ -1641,7 +1658,7 @@ k_smear_zy(const CT *__restrict__ cnt, T *__restrict__ D, unsigned *__restrict__
         const bool chit = hit;
         if (it + (int)gridDim.x < nall) stage(it + gridDim.x, sp, i, jt, kt, hit);
         const bool any = chit && __builtin_amdgcn_readfirstlane(block_or_lds(nonzero, orslot));
-        smear_zy_finish<T>(any, csp, ci, cjt, ckt, D, fmask, tb, nx, ny, nz, st);
+        smear_zy_finish<T, true>(any, csp, ci, cjt, ckt, D, fmask, tb, nx, ny, nz, st);
         lds_barrier();  // the next item's counts overwrite C and H
     }
 }
