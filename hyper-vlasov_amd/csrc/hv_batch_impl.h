@@ -252,7 +252,6 @@ int batch_build(hv_batch *b, const hv_skyrme *sk, const hv_params *prm, int nslo
     bool xs_vec = true;  // every instance fits the vector x pass
     for (int i = 0; i < b->n; i++) {
         hv_ctx *c = b->inst[i];
-        c->smear_tj16 = false;  // the ensemble's zy kernels take 8-row tiles
         int rc = ensure_smear<T>(c, prm[i].sigma_r, prm[i].part_per_nucleon);
         if (rc) return batch_fwd(b, c, rc);
         // one chunk of planes per tile: the ensemble has blocks enough

@@ -198,7 +198,7 @@ kb_smear_zy_loop(const BInst<T> *__restrict__ a, const int *__restrict__ first, 
         }
         if (it + (int)gridDim.x < nall) stage(it + (int)gridDim.x);
         const bool any = __builtin_amdgcn_readfirstlane(block_or_lds(nonzero, orslot));
-        smear_zy_finish<T>(any, sp, i, jt, kt, A.D, A.fmask, tb, A.g.nx, A.g.ny, A.g.nz, A.st2);
+        smear_zy_finish<T, true>(any, sp, i, jt, kt, A.D, A.fmask, tb, A.g.nx, A.g.ny, A.g.nz, A.st2);
         lds_barrier();  // the next item's counts overwrite C and H
     }
 }
